@@ -1,0 +1,249 @@
+/*
+ * dpg_capi.h -- C ABI of the MI355X-native DPG local-assembly library
+ * (libcamellia2_dpg.so).
+ *
+ * Plain C: fixed-size structs, plain pointers and sizes, no C++ or torch types.
+ * Every entry point returns an int status:
+ *     0   success
+ *    <0   argument / shape / runtime error (message in dpg_last_error())
+ *    >0   number of cells whose Gram matrix was not positive definite
+ *         (their per-cell info[] holds the LAPACK POTRF convention: the
+ *         1-based column at which the factorisation stopped).
+ * No exception crosses this boundary.
+ *
+ * Layouts follow Camellia's Intrepid::FieldContainer (row-major):
+ *   nodes      (C, V, d)       V = 4 (quad), 8 (hex, shards vertex order)
+ *   parities   (C, S)          int8, +1 / -1 (cellID < neighbourID -> +1, boundary +1)
+ *   K          (C, M, M)       optimal-test stiffness  B^T G^-1 B
+ *   rhs        (C, M)          B^T G^-1 l
+ *   G          (C, N, N)       test Gram matrix
+ *   schur      (C, T, T)       K_TT - K_TF K_FF^-1 K_FT   (trace dofs after the field dofs)
+ *   schur_rhs  (C, T)          rhs_T - K_TF K_FF^-1 rhs_F
+ * M = trial dofs (fields first, then traces/fluxes side by side), N = test dofs.
+ */
+#ifndef DPG_CAPI_H
+#define DPG_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DPG_MAX_VARS 16
+#define DPG_MAX_BTERMS 64
+#define DPG_MAX_IPTERMS 32
+#define DPG_MAX_LT_SUMMANDS 8
+#define DPG_MAX_RHS_TERMS 8
+#define DPG_MAX_LOAD_MODES 32
+#define DPG_MAX_COEFS 8
+
+/* cell topologies */
+enum { DPG_QUAD = 0, DPG_HEX = 1 };
+/* trial variable kinds (VarFactory::fieldVar / traceVar / fluxVar).
+ * With H1 order k: FIELD and FIELD_VECTOR use HGRAD Cn(k-1) (HVOL, identity
+ * transform), TRACE uses Cn(k) on every side (HGRAD trace), FLUX uses
+ * Cn(k-1) on every side (L2 flux, multiplied by the side parity). */
+enum { DPG_FIELD = 0, DPG_FIELD_VECTOR = 1, DPG_TRACE = 2, DPG_FLUX = 3 };
+/* test spaces: HGRAD Cn(kt), HDIV In(kt) */
+enum { DPG_HGRAD = 0, DPG_HDIV = 1 };
+/* scalar test operators */
+enum {
+  DPG_OP_VALUE = 0,      /* v (HGRAD)                         */
+  DPG_OP_DX = 1,         /* dv/dx                             */
+  DPG_OP_DY = 2,         /* dv/dy                             */
+  DPG_OP_DZ = 3,         /* dv/dz                             */
+  DPG_OP_X = 4,          /* tau_x (HDIV)                      */
+  DPG_OP_Y = 5,          /* tau_y                             */
+  DPG_OP_Z = 6,          /* tau_z                             */
+  DPG_OP_DIV = 7,        /* div tau                           */
+  DPG_OP_DOT_NORMAL = 8  /* tau . n  (boundary-only)          */
+};
+enum { DPG_FN_VALUE = 0, DPG_FN_SQRT = 1 };
+enum { DPG_PREC_FP64 = 0, DPG_PREC_FP32 = 1, DPG_PREC_BF16X3 = 2, DPG_PREC_FP16X3 = 3 };
+
+/* coefficient of a scalar product: scale * [fn(coef[c][slot])] * [n_comp] */
+typedef struct {
+  double scale;
+  int32_t coef_slot;   /* -1: none, else per-cell coefficient slot        */
+  int32_t coef_fn;     /* DPG_FN_VALUE or DPG_FN_SQRT                      */
+  int32_t normal_comp; /* -1: none, else outward unit normal component    */
+  int32_t pad;
+} dpg_coef;
+
+/* one scalar bilinear product  coef * (trial component) * (test operator).
+ * It is integrated over the sides when the trial variable is a trace/flux,
+ * the test operator is DOT_NORMAL or the coefficient holds a normal
+ * (LinearTerm boundary-only summands), otherwise over the cell volume. */
+typedef struct {
+  int32_t trial_var;
+  int32_t trial_comp; /* 0 for scalar variables, d for FIELD_VECTOR component d */
+  int32_t test_var;
+  int32_t test_op;
+  dpg_coef coef;
+} dpg_bterm;
+
+typedef struct {
+  int32_t test_var;
+  int32_t test_op;
+  dpg_coef coef;
+} dpg_lt_summand;
+
+/* scalar linear term of the test inner product: G += (lt, lt) */
+typedef struct {
+  int32_t n;
+  int32_t pad;
+  dpg_lt_summand s[DPG_MAX_LT_SUMMANDS];
+} dpg_ipterm;
+
+/* load functional term:  l_i += scale * (f_{load_comp}, op(psi_i)) */
+typedef struct {
+  int32_t test_var;
+  int32_t test_op;
+  int32_t load_comp;
+  int32_t pad;
+  double scale;
+} dpg_rhsterm;
+
+/* compiled BF + IP + RHS (the term tables of a Camellia formulation) */
+typedef struct {
+  int32_t spatial_dim;
+  int32_t n_trial;
+  int32_t n_test;
+  int32_t n_coefs; /* per-cell coefficients consumed by coef_slot */
+  int32_t trial_kind[DPG_MAX_VARS];
+  int32_t test_space[DPG_MAX_VARS];
+  int32_t n_bterms;
+  int32_t n_ipterms;
+  int32_t n_rhs;
+  int32_t pad;
+  dpg_bterm bterms[DPG_MAX_BTERMS];
+  dpg_ipterm ipterms[DPG_MAX_IPTERMS];
+  dpg_rhsterm rhs[DPG_MAX_RHS_TERMS];
+} dpg_form_desc;
+
+/* load f_c(x) = constant[c] + sum_m amp[c][m] prod_d sin(k[c][m][d] pi x_d) */
+typedef struct {
+  int32_t ncomp;
+  int32_t nmodes[3];
+  double constant[3];
+  double amp[3][DPG_MAX_LOAD_MODES];
+  int32_t k[3][DPG_MAX_LOAD_MODES][3];
+} dpg_load_desc;
+
+/* element type = bucket key (cell topology, H1 order, test degree) */
+typedef struct {
+  int32_t topology;
+  int32_t h1_order;    /* k: fields Cn(k-1), traces Cn(k), fluxes Cn(k-1) */
+  int32_t test_degree; /* kt: HGRAD Cn(kt), HDIV In(kt)                 */
+  int32_t pad;
+} dpg_bucket_desc;
+
+/* derived sizes of one bucket */
+typedef struct {
+  int32_t N;        /* test dofs                          */
+  int32_t M;        /* trial dofs                         */
+  int32_t F;        /* field dofs (first F trial dofs)    */
+  int32_t T;        /* trace/flux dofs  (M - F)           */
+  int32_t Q;        /* volume points, stiffness/rhs cache */
+  int32_t Qs;       /* points per side                    */
+  int32_t QG;       /* volume points, Gram (IP) cache     */
+  int32_t nsides;
+  int32_t nverts;
+  int32_t pad;
+  double flops_per_cell; /* algorithmic flop count used for GFLOP/s */
+} dpg_bucket_info;
+
+typedef struct {
+  int32_t precision;       /* DPG_PREC_*: arithmetic of the G/B/l contractions */
+  int32_t refine;          /* 1: one FP64 refinement step on G^-1 B            */
+  int32_t max_batch_cells; /* 0: derived from workspace_bytes                  */
+  int32_t pad;
+  double workspace_bytes;  /* device workspace budget, 0: 16 GiB               */
+} dpg_options;
+
+/* one call's inputs; all pointers are device pointers */
+typedef struct {
+  int32_t ncells;
+  int32_t pad;
+  const double* nodes;     /* (C, V, d)                */
+  const int8_t* parities;  /* (C, S)                   */
+  const double* coefs;     /* (C, n_coefs) or NULL     */
+  dpg_load_desc load;
+} dpg_batch;
+
+/* one call's outputs; device pointers, NULL = not requested */
+typedef struct {
+  double* K;         /* (C, M, M) */
+  double* rhs;       /* (C, M)    */
+  double* G;         /* (C, N, N) */
+  double* schur;     /* (C, T, T) */
+  double* schur_rhs; /* (C, T)    */
+  int32_t* info;     /* (C)       */
+} dpg_outputs;
+
+typedef struct dpg_plan dpg_plan;
+
+/* ---- library ----------------------------------------------------------- */
+const char* dpg_version(void);
+/* message of the last failing call on this thread */
+const char* dpg_last_error(void);
+
+/* ---- formulations (host only, no GPU) ------------------------------------
+ * Term tables of Camellia's ultraweak formulations with the graph-norm inner
+ * product (TBF::graphNorm) and the load functional. */
+/* PoissonFormulation(spaceDim, conforming, ULTRAWEAK)  PoissonFormulation.cpp:30-88 */
+int dpg_form_poisson_uw(int spatial_dim, dpg_form_desc* out);
+/* ConvectionDiffusionReactionFormulation(ULTRAWEAK) with per-cell
+ * coefficients [eps, beta_1..beta_d, alpha]; sqrt(eps) enters as in
+ * ConvectionDiffusionReactionFormulation.cpp:71-111 */
+int dpg_form_cdr_uw(int spatial_dim, dpg_form_desc* out);
+/* StokesVGPFormulation steady, conforming traces, StokesVGPFormulation.cpp:372-408, rhs :1015-1028 */
+int dpg_form_stokes_vgp_uw(int spatial_dim, double mu, dpg_form_desc* out);
+/* TBF::graphNorm (BF.cpp:1039-1141): replace out->ipterms from the bilinear terms */
+int dpg_form_graph_norm(dpg_form_desc* form);
+
+/* ---- seeded synthetic inputs (host only) -------------------------------- */
+/* nx*ny quads on [0,1]^2, interior vertices jittered by U(-jitter, jitter)*h,
+ * cells numbered x fastest; nodes (C,4,2) CCW; parities (C,4) */
+int dpg_mesh_quad(int nx, int ny, double jitter, uint64_t seed, double* nodes, int8_t* parities);
+/* nx*ny*nz hexes on [0,1]^3, shards Hexahedron<8> vertex order; nodes (C,8,3); parities (C,6) */
+int dpg_mesh_hex(int nx, int ny, int nz, double jitter, uint64_t seed, double* nodes, int8_t* parities);
+/* sine-mode load: nmodes modes per component, amp ~ N(0,1), k ~ U{1..kmax} */
+int dpg_load_sine_modes(int spatial_dim, int ncomp, int nmodes, int kmax, uint64_t seed, dpg_load_desc* out);
+/* xoshiro256** seeded by splitmix64: n uniforms on [lo,hi) / n normals (Box-Muller) */
+int dpg_rng_uniform(uint64_t seed, double lo, double hi, double* out, size_t n);
+int dpg_rng_normal(uint64_t seed, double* out, size_t n);
+
+/* sizes / algorithmic flop count of a bucket (host only) */
+int dpg_bucket_sizes(const dpg_form_desc* form, const dpg_bucket_desc* bucket, dpg_bucket_info* out);
+
+/* ---- GPU plan and assembly ---------------------------------------------- */
+/* Builds the device basis/cubature tables of every bucket (the BasisCache
+ * role, BasisCache.cpp:236-267) on the current HIP device. */
+int dpg_plan_create(const dpg_form_desc* form, const dpg_bucket_desc* buckets, int nbuckets,
+                    const dpg_options* options, dpg_plan** out);
+int dpg_plan_destroy(dpg_plan* plan);
+/* bytes of device workspace the plan holds */
+int dpg_plan_workspace(const dpg_plan* plan, size_t* bytes);
+
+/* Local assembly of one element type: replaces TBF::localStiffnessMatrixAndRHS
+ * (BF.cpp:1308-1496, FACTORED_CHOLESKY branch) for all cells of the batch,
+ * plus the static condensation of CondensedDofInterpreter::interpretLocalData
+ * (CondensedDofInterpreter.cpp:1063-1225) when outputs->schur is set.
+ * Stream-ordered on `stream` (a hipStream_t, NULL = default stream).
+ * Returns after the work is enqueued unless outputs->info is non-NULL, in
+ * which case it synchronises the stream and returns the non-SPD cell count. */
+int dpg_assemble(dpg_plan* plan, int bucket, const dpg_batch* batch, const dpg_outputs* outputs,
+                 void* stream);
+
+/* Energy-error indicator of TSolution::rankLocalEnergyError (Solution.cpp:2653-2798):
+ * r = l - B u,  G e = r,  eta2[c] = r^T e  (= e^T G e; Camellia reports sqrt). */
+int dpg_energy_error(dpg_plan* plan, int bucket, const dpg_batch* batch, const double* u,
+                     double* eta2, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DPG_CAPI_H */

@@ -1,0 +1,139 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes binding of oracle/liboracle.so.
+
+The oracle is the FP64 CPU restatement of the reference's DPG local assembly
+(see dpg_oracle.cpp for the reference file:line of every part). Only tests/,
+__graft_entry__.smoke() and the cpu_baseline leg of bench.py use it, and only
+as the checker / CPU baseline.
+"""
+import ctypes as ct
+import os
+
+import numpy as np
+
+from camellia2_amd import capi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
+        L = ct.CDLL(LIB_PATH)
+        P = ct.POINTER
+        vp = ct.c_void_p
+        L.oracle_factored_cholesky_solve.argtypes = [ct.c_int, ct.c_int, vp, vp, vp, vp, vp]
+        L.oracle_schur.argtypes = [ct.c_int, ct.c_int, vp, vp, vp, vp]
+        L.oracle_sizes.argtypes = [P(capi.FormDesc), P(capi.BucketDesc), vp]
+        L.oracle_gauss_rule.argtypes = [ct.c_int, vp, vp]
+        L.oracle_gll_nodes.argtypes = [ct.c_int, vp]
+        L.oracle_assemble.argtypes = [P(capi.FormDesc), P(capi.BucketDesc), ct.c_int, vp, vp, vp,
+                                      P(capi.LoadDesc), vp, vp, vp, vp, vp, vp, vp, vp, ct.c_int]
+        L.oracle_energy_error.argtypes = [P(capi.FormDesc), P(capi.BucketDesc), ct.c_int, vp, vp, vp,
+                                          P(capi.LoadDesc), vp, vp, ct.c_int]
+        L.oracle_graph_norm.argtypes = [P(capi.FormDesc)]
+        L.oracle_form_poisson_uw.argtypes = [ct.c_int, P(capi.FormDesc)]
+        L.oracle_form_cdr_uw.argtypes = [ct.c_int, P(capi.FormDesc)]
+        L.oracle_form_stokes_vgp_uw.argtypes = [ct.c_int, ct.c_double, P(capi.FormDesc)]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def factored_cholesky_solve(G_colmajor, B_colmajor, l, N, M):
+    G = np.array(G_colmajor, dtype=np.float64).copy()
+    B = np.array(B_colmajor, dtype=np.float64).copy()
+    ll = np.array(l, dtype=np.float64).copy()
+    K = np.zeros((M, M))
+    rhs = np.zeros(M)
+    info = lib().oracle_factored_cholesky_solve(N, M, _p(G), _p(B), _p(ll), _p(K), _p(rhs))
+    return K, rhs, info
+
+
+def sizes(form, bucket):
+    out = np.zeros(6, dtype=np.int32)
+    rc = lib().oracle_sizes(ct.byref(form), ct.byref(bucket), _p(out))
+    assert rc == 0
+    return dict(N=int(out[0]), M=int(out[1]), F=int(out[2]), Q1=int(out[3]), Qs1=int(out[4]),
+                QG1=int(out[5]))
+
+
+def gauss_rule(n):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    lib().oracle_gauss_rule(n, _p(x), _p(w))
+    return x, w
+
+
+def gll_nodes(n):
+    x = np.zeros(n + 1)
+    lib().oracle_gll_nodes(n, _p(x))
+    return x
+
+
+def assemble(form, bucket, nodes, parities, coefs=None, load=None, want=("K", "rhs"), nthreads=0):
+    """Oracle local assembly of C cells; returns a dict of the requested arrays
+    among B (C,M,N), G (C,N,N), l (C,N), K (C,M,M), rhs (C,M), S (C,T,T), g (C,T), info (C)."""
+    s = sizes(form, bucket)
+    N, M, F = s["N"], s["M"], s["F"]
+    T = M - F
+    C = nodes.shape[0]
+    nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+    parities = np.ascontiguousarray(parities, dtype=np.int8)
+    if coefs is not None:
+        coefs = np.ascontiguousarray(coefs, dtype=np.float64)
+    shapes = dict(B=(C, M, N), G=(C, N, N), l=(C, N), K=(C, M, M), rhs=(C, M), S=(C, T, T), g=(C, T))
+    out = {k: np.zeros(shapes[k]) for k in want if k in shapes}
+    info = np.zeros(C, dtype=np.int32)
+    if load is None:
+        load = capi.LoadDesc()
+    nbad = lib().oracle_assemble(ct.byref(form), ct.byref(bucket), C, _p(nodes), _p(parities), _p(coefs),
+                                 ct.byref(load), _p(out.get("B")), _p(out.get("G")), _p(out.get("l")),
+                                 _p(out.get("K")), _p(out.get("rhs")), _p(out.get("S")), _p(out.get("g")),
+                                 _p(info), nthreads)
+    assert nbad >= 0, "oracle_assemble rejected its arguments"
+    out["info"] = info
+    return out
+
+
+def energy_error(form, bucket, nodes, parities, u, coefs=None, load=None, nthreads=0):
+    C = nodes.shape[0]
+    eta2 = np.zeros(C)
+    nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+    parities = np.ascontiguousarray(parities, dtype=np.int8)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    if coefs is not None:
+        coefs = np.ascontiguousarray(coefs, dtype=np.float64)
+    if load is None:
+        load = capi.LoadDesc()
+    rc = lib().oracle_energy_error(ct.byref(form), ct.byref(bucket), C, _p(nodes), _p(parities), _p(coefs),
+                                   ct.byref(load), _p(u), _p(eta2), nthreads)
+    assert rc >= 0
+    return eta2
+
+
+def form(name, dim, mu=1.0):
+    f = capi.FormDesc()
+    if name == "poisson":
+        rc = lib().oracle_form_poisson_uw(dim, ct.byref(f))
+    elif name == "cdr":
+        rc = lib().oracle_form_cdr_uw(dim, ct.byref(f))
+    elif name == "stokes":
+        rc = lib().oracle_form_stokes_vgp_uw(dim, mu, ct.byref(f))
+    else:
+        raise ValueError(name)
+    assert rc == 0
+    return f
+
+
+def graph_norm(f):
+    rc = lib().oracle_graph_norm(ct.byref(f))
+    assert rc == 0
+    return f

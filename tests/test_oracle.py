@@ -1,0 +1,264 @@
+"""Pins the oracle (CPU restatement of the reference's local assembly) against
+the reference's own golden vectors and against what the mathematics fixes."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from camellia2_amd import capi
+from oracle import oracle
+
+
+def load(golden_dir, name):
+    with open(os.path.join(golden_dir, name)) as f:
+        return json.load(f)
+
+
+def coef(scale=1.0, slot=-1, fn=0, normal=-1):
+    c = capi.Coef()
+    c.scale, c.coef_slot, c.coef_fn, c.normal_comp = scale, slot, fn, normal
+    return c
+
+
+def add_ip(f, summands):
+    lt = f.ipterms[f.n_ipterms]
+    lt.n = len(summands)
+    for i, (var, op, c) in enumerate(summands):
+        lt.s[i].test_var, lt.s[i].test_op, lt.s[i].coef = var, op, c
+    f.n_ipterms += 1
+
+
+def analytic_boundary_form():
+    """b(uhat, v) = <uhat n, grad v>, IP = (v,v) + (grad v, grad v)
+    (TestBilinearFormAnalyticBoundaryIntegral.h:42-50, MathInnerProduct.cpp:17-60)."""
+    f = capi.FormDesc()
+    f.spatial_dim, f.n_trial, f.n_test = 2, 1, 1
+    f.trial_kind[0] = capi.TRACE
+    f.test_space[0] = capi.HGRAD
+    for d in range(2):
+        b = f.bterms[f.n_bterms]
+        b.trial_var, b.trial_comp, b.test_var, b.test_op = 0, 0, 0, capi.OP_DX + d
+        b.coef = coef(normal=d)
+        f.n_bterms += 1
+    for op in (capi.OP_VALUE, capi.OP_DX, capi.OP_DY):
+        add_ip(f, [(0, op, coef())])
+    return f
+
+
+REF_QUAD = np.array([[[-1.0, -1.0], [1.0, -1.0], [1.0, 1.0], [-1.0, 1.0]]])
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_factored_cholesky_kats(golden_dir, case):
+    """BFTests.cpp:28-72,136-381 (TEST_FLOATING_EQUALITY, tol 1e-14)."""
+    kats = load(golden_dir, "factored_cholesky_kats.json")
+    c = kats["cases"][case]
+    K, rhs, info = oracle.factored_cholesky_solve(c["G_colmajor"], c["B_colmajor"], c["l"], c["N"], c["M"])
+    assert info == 0
+    Ke, re = np.array(c["K"]), np.array(c["rhs"])
+    tol = kats["tol"]
+    mask = np.abs(Ke) > tol
+    assert np.all(np.abs(K[mask] - Ke[mask]) <= tol * np.abs(Ke[mask]) * 4)
+    assert np.all(np.abs(K[~mask]) < tol)
+    maskr = np.abs(re) > tol
+    assert np.all(np.abs(rhs[maskr] - re[maskr]) <= tol * np.abs(re[maskr]) * 4)
+    assert np.all(np.abs(rhs[~maskr]) < tol)
+
+
+def test_analytic_boundary_integral_cubic(golden_dir):
+    """DPGTests.cpp:966-1098 with the Mathematica values, tol 3.76e-14."""
+    fx = load(golden_dir, "analytic_boundary_integral_cubic.json")
+    f = analytic_boundary_form()
+    bk = capi.BucketDesc(capi.QUAD, 3, 3, 0)
+    o = oracle.assemble(f, bk, REF_QUAD, np.ones((1, 4), np.int8), want=("B", "G", "K"))
+    tol = fx["tol"]
+    B, G, K = o["B"][0], o["G"][0], o["K"][0]
+    assert np.abs(B.T - np.array(fx["pre_stiffness"]["nonconforming"])).max() < tol
+    assert np.abs(G - np.array(fx["ip_matrix"]["only"])).max() < tol * 1.01
+    W = np.linalg.solve(G, B.T).T
+    assert np.abs(W - np.array(fx["opt_test_weights"]["nonconforming"])).max() < tol
+    assert np.abs(K - np.array(fx["final_stiffness"]["nonconforming"])).max() < tol
+
+
+def test_analytic_boundary_integral_conforming(golden_dir):
+    """Conforming traces identify the vertex dofs of neighbouring sides
+    (TestBilinearFormAnalyticBoundaryIntegral.h:60-81): K_conf = P^T K P."""
+    fx = load(golden_dir, "analytic_boundary_integral_cubic.json")
+    f = analytic_boundary_form()
+    bk = capi.BucketDesc(capi.QUAD, 3, 3, 0)
+    o = oracle.assemble(f, bk, REF_QUAD, np.ones((1, 4), np.int8), want=("B", "K"))
+    # side s: local dofs 4s..4s+3; last dof of side s == first dof of side s+1
+    P = np.zeros((16, 12))
+    for s in range(4):
+        for a in range(4):
+            g = (3 * s + a) % 12
+            P[4 * s + a, g] = 1.0
+    Bc = P.T @ o["B"][0]
+    Kc = P.T @ o["K"][0] @ P
+    assert np.abs(Bc.T - np.array(fx["pre_stiffness"]["conforming"])).max() < fx["tol"]
+    assert np.abs(Kc - np.array(fx["final_stiffness"]["conforming"])).max() < fx["tol"]
+
+
+def test_math_inner_product_c1(golden_dir):
+    """DPGTests.cpp:755-813: C1 quad (vertex ordering) IP matrix."""
+    fx = load(golden_dir, "math_inner_product_c1.json")
+    f = analytic_boundary_form()
+    bk = capi.BucketDesc(capi.QUAD, 1, 1, 0)
+    o = oracle.assemble(f, bk, REF_QUAD, np.ones((1, 4), np.int8), want=("G",))
+    perm = [0, 1, 3, 2]  # vertex order -> tensor (x fastest) order
+    G = o["G"][0][np.ix_(perm, perm)]
+    assert np.abs(G - np.array(fx["ip_matrix"])).max() < fx["tol"]
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 8, 11])
+def test_gauss_rule_matches_numpy(n):
+    x, w = oracle.gauss_rule(n)
+    xr, wr = np.polynomial.legendre.leggauss(n)
+    assert np.abs(x - xr).max() < 1e-14 and np.abs(w - wr).max() < 1e-14
+    # exactness up to degree 2n-1
+    for d in range(2 * n):
+        exact = 0.0 if d % 2 else 2.0 / (d + 1)
+        assert abs(np.dot(w, x ** d) - exact) < 1e-13
+
+
+def test_gll_nodes_closed_form():
+    assert np.allclose(oracle.gll_nodes(2), [-1, 0, 1], atol=1e-15)
+    assert np.allclose(oracle.gll_nodes(3), [-1, -1 / np.sqrt(5), 1 / np.sqrt(5), 1], atol=1e-15)
+    assert np.allclose(oracle.gll_nodes(4), [-1, -np.sqrt(3 / 7), 0, np.sqrt(3 / 7), 1], atol=1e-15)
+    for n in range(2, 10):
+        x = oracle.gll_nodes(n)
+        dP = np.polynomial.legendre.Legendre.basis(n).deriv()
+        assert np.abs(dP(x[1:-1])).max() < 1e-10 * max(1.0, np.abs(dP.coef).max())
+
+
+def _quad_mesh(nx, ny, jitter, seed):
+    rng = np.random.default_rng(seed)
+    h = 1.0 / nx
+    X = np.zeros((ny + 1, nx + 1, 2))
+    for j in range(ny + 1):
+        for i in range(nx + 1):
+            X[j, i] = (i * h, j / ny)
+            if 0 < i < nx and 0 < j < ny:
+                X[j, i] += rng.uniform(-jitter, jitter, 2) * h
+    nodes = np.zeros((nx * ny, 4, 2))
+    for j in range(ny):
+        for i in range(nx):
+            nodes[j * nx + i] = [X[j, i], X[j, i + 1], X[j + 1, i + 1], X[j + 1, i]]
+    par = np.ones((nx * ny, 4), np.int8)
+    return nodes, par
+
+
+def test_graph_norm_poisson_terms():
+    """BF.cpp:1039-1141 on PoissonFormulation: (div tau)^2 + |tau - grad v|^2 + v^2 + |tau|^2."""
+    f = oracle.form("poisson", 2)
+    got = []
+    for i in range(f.n_ipterms):
+        lt = f.ipterms[i]
+        got.append(sorted((lt.s[k].test_var, lt.s[k].test_op, lt.s[k].coef.scale) for k in range(lt.n)))
+    V, TAU = 0, 1
+    want = [[(TAU, capi.OP_DIV, 1.0)],
+            sorted([(TAU, capi.OP_X, 1.0), (V, capi.OP_DX, -1.0)]),
+            sorted([(TAU, capi.OP_Y, 1.0), (V, capi.OP_DY, -1.0)]),
+            [(V, capi.OP_VALUE, 1.0)], [(TAU, capi.OP_X, 1.0)], [(TAU, capi.OP_Y, 1.0)]]
+    assert got == want
+
+
+def _nodal_trial_coefficients_linear(nodes, par, k, a):
+    """Trial coefficients of u = a0 + a1 x + a2 y (sigma = grad u, uhat = u,
+    sigma_n_hat = sigma.n * parity) on 2D Poisson UW, by nodal interpolation
+    at the GLL nodes of the field (Cn(k-1)), trace (Cn(k)) and flux (Cn(k-1)) bases."""
+    C = nodes.shape[0]
+    gf, gt = oracle.gll_nodes(k - 1), oracle.gll_nodes(k)
+    nf, nt = len(gf), len(gt)
+    M = 3 * nf * nf + 4 * nt + 4 * nf
+    u = np.zeros((C, M))
+    ref = REF_QUAD[0]
+    sides = [(0, 1), (1, 2), (2, 3), (3, 0)]
+
+    def phys(c, xi):
+        N = np.array([(1 + ref[v, 0] * xi[0]) * (1 + ref[v, 1] * xi[1]) / 4 for v in range(4)])
+        return N @ nodes[c]
+
+    for c in range(C):
+        fld = np.zeros(nf * nf)
+        for j in range(nf):
+            for i in range(nf):
+                x = phys(c, (gf[i], gf[j]))
+                fld[j * nf + i] = a[0] + a[1] * x[0] + a[2] * x[1]
+        u[c, :nf * nf] = fld
+        u[c, nf * nf:2 * nf * nf] = a[1]
+        u[c, 2 * nf * nf:3 * nf * nf] = a[2]
+        off = 3 * nf * nf
+        for s, (p, q) in enumerate(sides):
+            for i, t in enumerate(gt):
+                xi = 0.5 * (ref[p] + ref[q]) + 0.5 * (ref[q] - ref[p]) * t
+                x = phys(c, xi)
+                u[c, off + s * nt + i] = a[0] + a[1] * x[0] + a[2] * x[1]
+        off += 4 * nt
+        for s, (p, q) in enumerate(sides):
+            T = nodes[c, q] - nodes[c, p]
+            n = np.array([T[1], -T[0]]) / np.hypot(*T)
+            u[c, off + s * nf:off + (s + 1) * nf] = (a[1] * n[0] + a[2] * n[1]) * par[c, s]
+    return u
+
+
+@pytest.mark.parametrize("k,kt", [(2, 3), (3, 4)])
+def test_poisson_linear_solution_has_zero_residual(k, kt):
+    """A linear u lies in the trial space of a bilinear mesh; with f = lap u = 0
+    its DPG residual l - B u and energy error vanish (the TracesUltraweak
+    invariant of ConvectionDiffusionReactionFormulationTests.cpp:116-235)."""
+    nodes, par = _quad_mesh(3, 3, 0.1, 7)
+    par[:, 0] = -1  # parities flip the flux sign; the residual must still vanish
+    f = oracle.form("poisson", 2)
+    bk = capi.BucketDesc(capi.QUAD, k, kt, 0)
+    a = (0.3, -1.2, 0.7)
+    u = _nodal_trial_coefficients_linear(nodes, par, k, a)
+    o = oracle.assemble(f, bk, nodes, par, want=("B", "l"))
+    r = o["l"] - np.einsum("cmn,cm->cn", o["B"], u)
+    scale = np.abs(o["B"]).max() * np.abs(u).max()
+    assert np.abs(r).max() < 1e-13 * scale
+    eta2 = oracle.energy_error(f, bk, nodes, par, u)
+    assert np.abs(eta2).max() < 1e-24 * scale ** 2 * 1e4
+
+
+def test_poisson_residual_detects_wrong_parity():
+    """The same check fails when the flux coefficients ignore the parity."""
+    nodes, par = _quad_mesh(2, 2, 0.1, 3)
+    par[:, 1] = -1
+    f = oracle.form("poisson", 2)
+    bk = capi.BucketDesc(capi.QUAD, 2, 3, 0)
+    u = _nodal_trial_coefficients_linear(nodes, np.ones_like(par), 2, (0.0, 1.0, 0.5))
+    o = oracle.assemble(f, bk, nodes, par, want=("B", "l"))
+    r = o["l"] - np.einsum("cmn,cm->cn", o["B"], u)
+    assert np.abs(r).max() > 1e-3
+
+
+def test_K_symmetric_spd_and_schur_closed_form():
+    nodes, par = _quad_mesh(2, 2, 0.1, 11)
+    f = oracle.form("poisson", 2)
+    bk = capi.BucketDesc(capi.QUAD, 3, 4, 0)
+    o = oracle.assemble(f, bk, nodes, par, want=("B", "G", "l", "K", "rhs", "S", "g"))
+    s = oracle.sizes(f, bk)
+    F = s["F"]
+    for c in range(nodes.shape[0]):
+        K = o["K"][c]
+        assert np.array_equal(K, K.T)
+        G, B, l = o["G"][c], o["B"][c], o["l"][c]
+        Kr = B @ np.linalg.solve(G, B.T)
+        assert np.abs(K - Kr).max() < 1e-10 * np.abs(K).max()
+        assert np.abs(o["rhs"][c] - B @ np.linalg.solve(G, l)).max() < 1e-10 * np.abs(K).max()
+        KFF, KFT, KTT = K[:F, :F], K[:F, F:], K[F:, F:]
+        S = KTT - KFT.T @ np.linalg.solve(KFF, KFT)
+        g = o["rhs"][c][F:] - KFT.T @ np.linalg.solve(KFF, o["rhs"][c][:F])
+        assert np.abs(o["S"][c] - S).max() < 1e-9 * np.abs(S).max()
+        assert np.abs(o["g"][c] - g).max() < 1e-9 * max(1.0, np.abs(g).max())
+        assert np.linalg.eigvalsh(G).min() > 0
+
+
+def test_indefinite_gram_reports_info():
+    """POTRF INFO semantics (BF.cpp:963-969): a non-SPD G reports the failing column."""
+    G = np.eye(3)
+    G[1, 1] = -1.0
+    K, rhs, info = oracle.factored_cholesky_solve(G.ravel(order="F"), np.eye(3).ravel(), np.ones(3), 3, 3)
+    assert info == 2
