@@ -1,0 +1,53 @@
+// device_common.hpp -- types shared by the HIP kernels and the host orchestration.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace dpg {
+
+// One summand of a compiled term: coef(scale, slot, fn, normal) * op(test var).
+struct DSummand {
+  int32_t var, op, slot, fn, normal, aux;  // aux: load component (rhs) / flux flag (side blocks)
+  double scale;
+};
+
+constexpr int kMaxSummands = 256;
+constexpr int kMaxGroups = 40;
+
+// Compiled per-form "term program" evaluated at every cubature point.
+// Groups: IP scalar terms (lt), volume trial blocks (field components),
+// side trial variables (traces/fluxes), rhs (one group).
+struct DevProgram {
+  int32_t dim, nvar, npr;  // npr = dim + 1 reference prims per test dof
+  int32_t n_lt, n_vb, n_sb, n_rhs;
+  int32_t pad;
+  int32_t lt_beg[kMaxGroups + 1];
+  int32_t vb_beg[kMaxGroups + 1];
+  int32_t sb_beg[kMaxGroups + 1];
+  int32_t sb_flux[kMaxGroups];
+  int32_t rhs_beg[2];
+  DSummand s[kMaxSummands];
+};
+
+// per-bucket device tables (reference data shared by all cells)
+struct DevTables {
+  const double* R_vol;    // (Q, npr, N)
+  const double* R_ip;     // (QG, npr, N)
+  const double* R_side;   // (S*Qs, npr, N)
+  const double* dN_vol;   // (Q, V, dim)
+  const double* dN_ip;    // (QG, V, dim)
+  const double* dN_side;  // (S*Qs, V, dim)
+  const double* Nv_vol;   // (Q, V)
+  const double* w_vol;    // (Q)
+  const double* w_ip;     // (QG)
+  const double* w_side;   // (S*Qs)
+  const double* field_vol;  // (Q, nf)
+  const double* trace;    // (Qs, ntr)
+  const double* flux;     // (Qs, nfl)
+  const double* side_tan;  // (S, 2, dim) reference side tangents
+  const int32_t* var_of_dof;  // (N)
+};
+
+}  // namespace dpg

@@ -1,0 +1,617 @@
+// kernels.hip -- gfx950 kernels of the DPG local-assembly pipeline.
+//
+// Per batch of cells of one element type the pipeline is
+//   point_kernel      per (cell, cubature point): Jacobian, detJ, J^-1,
+//                     normals, measures, coefficients -> small per-point
+//                     weight tensors W[group][test var][prim]
+//   operand_kernel    per (cell, point, test dof): sum_r W[..][r] * R[q][r][i]
+//                     (physical operator values times weights; R are the
+//                     reference prims shared by all cells)
+//   gemm_tn_f64       batched FP64 MFMA contraction C(i,j) = a*sum_k X(k,i)Y(k,j) + b*C
+//                     (Gram matrix, B blocks, and the Cholesky trailing updates)
+//   panel_f64         per 64-column panel: LDS Cholesky of the diagonal block
+//                     and the triangular solve of the rows below it
+//   small kernels     load vector, POEQU scaling, residual row, extraction
+//
+// All per-cell matrices live in one column-major "augmented" matrix
+//   A = [[S G S, S B, S l, S r], [.., 0]]   (lower triangle used)
+// of size D = N + M + 2; a partial Cholesky over its first N columns leaves
+// -[B l r]^T G^-1 [B l r] in the trailing block, i.e. -K, -rhs and -eta^2.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "../../include/dpg_capi.h"
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace dpg {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// ----------------------------------------------------------------- geometry
+template <int DIM>
+struct PointGeo {
+  double J[DIM][DIM], Jinv[DIM][DIM], detJ;
+};
+
+template <int DIM>
+__device__ inline void point_geometry(const double* __restrict__ X, const double* __restrict__ dN, int V,
+                                      PointGeo<DIM>& g) {
+#pragma unroll
+  for (int d = 0; d < DIM; ++d)
+#pragma unroll
+    for (int e = 0; e < DIM; ++e) g.J[d][e] = 0.0;
+  for (int v = 0; v < V; ++v) {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      double xv = X[v * DIM + d];
+#pragma unroll
+      for (int e = 0; e < DIM; ++e) g.J[d][e] = fma(xv, dN[v * DIM + e], g.J[d][e]);
+    }
+  }
+  if constexpr (DIM == 2) {
+    g.detJ = g.J[0][0] * g.J[1][1] - g.J[0][1] * g.J[1][0];
+    double id = 1.0 / g.detJ;
+    g.Jinv[0][0] = g.J[1][1] * id;
+    g.Jinv[0][1] = -g.J[0][1] * id;
+    g.Jinv[1][0] = -g.J[1][0] * id;
+    g.Jinv[1][1] = g.J[0][0] * id;
+  } else {
+    const double(*A)[DIM] = g.J;
+    double c00 = A[1][1] * A[2][2] - A[1][2] * A[2][1];
+    double c01 = A[1][2] * A[2][0] - A[1][0] * A[2][2];
+    double c02 = A[1][0] * A[2][1] - A[1][1] * A[2][0];
+    g.detJ = A[0][0] * c00 + A[0][1] * c01 + A[0][2] * c02;
+    double id = 1.0 / g.detJ;
+    g.Jinv[0][0] = c00 * id;
+    g.Jinv[1][0] = c01 * id;
+    g.Jinv[2][0] = c02 * id;
+    g.Jinv[0][1] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) * id;
+    g.Jinv[1][1] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) * id;
+    g.Jinv[2][1] = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) * id;
+    g.Jinv[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) * id;
+    g.Jinv[1][2] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) * id;
+    g.Jinv[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) * id;
+  }
+}
+
+// coefficients t[r] expressing physical operator `op` of a test dof in its
+// reference prims (HGRAD: v, dv/dxi; HDIV: tau_hat, div_hat) -- the transforms
+// of BasisEvaluation.cpp:391-650 (grad: J^-T, HDIV value: J/detJ, div: 1/detJ)
+template <int DIM>
+__device__ inline void op_to_prims(int op, const PointGeo<DIM>& g, const double* n, double* t) {
+#pragma unroll
+  for (int r = 0; r <= DIM; ++r) t[r] = 0.0;
+  if (op == DPG_OP_VALUE) {
+    t[0] = 1.0;
+  } else if (op >= DPG_OP_DX && op < DPG_OP_DX + DIM) {
+    int d = op - DPG_OP_DX;
+#pragma unroll
+    for (int e = 0; e < DIM; ++e) t[1 + e] = g.Jinv[e][d];
+  } else if (op >= DPG_OP_X && op < DPG_OP_X + DIM) {
+    int d = op - DPG_OP_X;
+    double id = 1.0 / g.detJ;
+#pragma unroll
+    for (int e = 0; e < DIM; ++e) t[e] = g.J[d][e] * id;
+  } else if (op == DPG_OP_DIV) {
+    t[DIM] = 1.0 / g.detJ;
+  } else if (op == DPG_OP_DOT_NORMAL && n) {
+    double id = 1.0 / g.detJ;
+#pragma unroll
+    for (int e = 0; e < DIM; ++e) {
+      double s = 0.0;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) s += n[d] * g.J[d][e];
+      t[e] = s * id;
+    }
+  }
+}
+
+__device__ inline double coef_value(const DSummand& s, const double* cc, const double* n) {
+  double v = s.scale;
+  if (s.slot >= 0) {
+    double a = cc[s.slot];
+    v *= (s.fn == DPG_FN_SQRT) ? sqrt(a) : a;
+  }
+  if (s.normal >= 0) v *= n ? n[s.normal] : 0.0;
+  return v;
+}
+
+__device__ inline double load_value(const dpg_load_desc& L, int comp, const double* x, int dim) {
+  if (comp >= L.ncomp) return 0.0;
+  double f = L.constant[comp];
+  for (int m = 0; m < L.nmodes[comp]; ++m) {
+    double p = L.amp[comp][m];
+    for (int d = 0; d < dim; ++d) p *= sin(L.k[comp][m][d] * M_PI * x[d]);
+    f += p;
+  }
+  return f;
+}
+
+// ------------------------------------------------------------- point kernel
+// kind 0: Gram points, weight sqrt(w detJ), groups = IP terms
+// kind 1: stiffness points, weight w detJ, groups = field blocks (+ rhs into Wrhs)
+// kind 2: side points, weight w |J t|, unit outward normal, groups = side variables
+template <int DIM>
+__global__ __launch_bounds__(128) void point_kernel(int kind, const DevProgram* __restrict__ prog,
+                                                    DevTables tab, int npts, int Qs, int ncells,
+                                                    const double* __restrict__ nodes,
+                                                    const int8_t* __restrict__ parities,
+                                                    const double* __restrict__ coefs, int ncoef, int nsides,
+                                                    dpg_load_desc load, double* __restrict__ W,
+                                                    double* __restrict__ Wrhs) {
+  int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= ncells * npts) return;
+  int c = gid / npts, p = gid % npts;
+  const int V = DIM == 2 ? 4 : 8;
+  const int npr = DIM + 1;
+  const int nvar = prog->nvar;
+  const double* X = nodes + (size_t)c * V * DIM;
+  const double* cc = coefs ? coefs + (size_t)c * ncoef : nullptr;
+  const double* dN = kind == 0 ? tab.dN_ip : (kind == 1 ? tab.dN_vol : tab.dN_side);
+  PointGeo<DIM> g;
+  point_geometry<DIM>(X, dN + (size_t)p * V * DIM, V, g);
+  double n[3] = {0.0, 0.0, 0.0};
+  double fac;
+  int s = 0;
+  if (kind == 0) {
+    fac = sqrt(tab.w_ip[p] * g.detJ);
+  } else if (kind == 1) {
+    fac = tab.w_vol[p] * g.detJ;
+  } else {
+    s = p / Qs;
+    double T[2][3];
+    for (int a = 0; a < DIM - 1; ++a)
+      for (int d = 0; d < DIM; ++d) {
+        double acc = 0.0;
+        for (int e = 0; e < DIM; ++e) acc += g.J[d][e] * tab.side_tan[((size_t)s * 2 + a) * DIM + e];
+        T[a][d] = acc;
+      }
+    if constexpr (DIM == 2) {
+      n[0] = T[0][1];
+      n[1] = -T[0][0];
+    } else {
+      n[0] = T[0][1] * T[1][2] - T[0][2] * T[1][1];
+      n[1] = T[0][2] * T[1][0] - T[0][0] * T[1][2];
+      n[2] = T[0][0] * T[1][1] - T[0][1] * T[1][0];
+    }
+    double len = 0.0;
+    for (int d = 0; d < DIM; ++d) len += n[d] * n[d];
+    len = sqrt(len);
+    for (int d = 0; d < DIM; ++d) n[d] /= len;
+    fac = tab.w_side[p] * len;
+  }
+  int ng;
+  const int32_t* gbeg;
+  if (kind == 0) {
+    ng = prog->n_lt;
+    gbeg = prog->lt_beg;
+  } else if (kind == 1) {
+    ng = prog->n_vb;
+    gbeg = prog->vb_beg;
+  } else {
+    ng = prog->n_sb;
+    gbeg = prog->sb_beg;
+  }
+  const int slice = ng * nvar * npr;
+  double* Wp = W + ((size_t)c * npts + p) * slice;
+  for (int e = 0; e < slice; ++e) Wp[e] = 0.0;
+  double t[DIM + 1];
+  const double* nn = kind == 2 ? n : nullptr;
+  for (int gi = 0; gi < ng; ++gi) {
+    double gfac = fac;
+    if (kind == 2 && prog->sb_flux[gi]) gfac *= (double)parities[(size_t)c * nsides + s];
+    for (int si = gbeg[gi]; si < gbeg[gi + 1]; ++si) {
+      const DSummand& sm = prog->s[si];
+      double cv = coef_value(sm, cc, nn) * gfac;
+      op_to_prims<DIM>(sm.op, g, nn, t);
+      double* w = Wp + ((size_t)gi * nvar + sm.var) * npr;
+      for (int r = 0; r < npr; ++r) w[r] += cv * t[r];
+    }
+  }
+  if (kind == 1 && Wrhs) {
+    double x[3] = {0.0, 0.0, 0.0};
+    const double* Nv = tab.Nv_vol + (size_t)p * V;
+    for (int v = 0; v < V; ++v)
+      for (int d = 0; d < DIM; ++d) x[d] += Nv[v] * X[v * DIM + d];
+    double* Wr = Wrhs + ((size_t)c * npts + p) * nvar * npr;
+    for (int e = 0; e < nvar * npr; ++e) Wr[e] = 0.0;
+    for (int si = prog->rhs_beg[0]; si < prog->rhs_beg[1]; ++si) {
+      const DSummand& sm = prog->s[si];
+      double cv = load_value(load, sm.aux, x, DIM) * sm.scale * fac;
+      op_to_prims<DIM>(sm.op, g, nullptr, t);
+      for (int r = 0; r < npr; ++r) Wr[sm.var * npr + r] += cv * t[r];
+    }
+  }
+}
+
+// ----------------------------------------------------------- operand kernel
+// out[c*sc + b*sb + p*sp + i] = sum_r W[c][p][b][var(i)][r] * R[p][r][i]
+template <typename T, int NPR>
+__global__ __launch_bounds__(256) void operand_kernel(const double* __restrict__ W, int ng, int nvar,
+                                                      const double* __restrict__ R, const int32_t* __restrict__ vod,
+                                                      int N, int npts, T* __restrict__ out, int64_t sc, int64_t sb,
+                                                      int64_t sp) {
+  extern __shared__ double Ws[];
+  int p = blockIdx.y, c = blockIdx.z;
+  int slice = ng * nvar * NPR;
+  const double* Wp = W + ((size_t)c * npts + p) * slice;
+  for (int e = threadIdx.x; e < slice; e += blockDim.x) Ws[e] = Wp[e];
+  __syncthreads();
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double r[NPR];
+#pragma unroll
+  for (int k = 0; k < NPR; ++k) r[k] = R[((size_t)p * NPR + k) * N + i];
+  int var = vod[i];
+  T* o = out + (size_t)c * sc + (size_t)p * sp + i;
+  for (int b = 0; b < ng; ++b) {
+    const double* w = Ws + ((size_t)b * nvar + var) * NPR;
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < NPR; ++k) acc = fma(w[k], r[k], acc);
+    o[(size_t)b * sb] = (T)acc;
+  }
+}
+
+// load vector: A[c][row_l + i*lda] = sum_q sum_r Wrhs[c][q][var(i)][r] R_vol[q][r][i]
+template <int NPR>
+__global__ __launch_bounds__(256) void rhs_kernel(const double* __restrict__ Wrhs, int nvar,
+                                                  const double* __restrict__ R, const int32_t* __restrict__ vod,
+                                                  int N, int Q, double* __restrict__ A, int64_t lda, int64_t sA,
+                                                  int row_l) {
+  int c = blockIdx.y;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int var = vod[i];
+  double acc = 0.0;
+  for (int q = 0; q < Q; ++q) {
+    const double* w = Wrhs + ((size_t)c * Q + q) * nvar * NPR + var * NPR;
+#pragma unroll
+    for (int k = 0; k < NPR; ++k) acc = fma(w[k], R[((size_t)q * NPR + k) * N + i], acc);
+  }
+  A[(size_t)c * sA + row_l + (size_t)i * lda] = acc;
+}
+
+// ------------------------------------------------------------------- GEMM
+// C(i,j) = alpha * sum_k X(k,i) Y(k,j) + beta * C(i,j), X(k,i) = X[k*ldx + i],
+// C col-major (C[i + j*ldc]); 64x64 tiles, 4 waves, v_mfma_f64_16x16x4.
+constexpr int GT = 64, GKC = 16, GLD = GT + 4;
+
+__device__ inline int xcd_remap(int b, int nb) {
+  // consecutive logical tiles on one XCD (blocks are dealt round-robin to 8 XCDs)
+  int per = nb / 8, rem = nb % 8, x = b % 8, idx = b / 8;
+  return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + idx;
+}
+
+__device__ inline void tile_coords(int tile, int lower, int tiles_m, int& ti, int& tj) {
+  if (lower) {
+    int t = (int)((sqrt(8.0 * tile + 1.0) - 1.0) * 0.5);
+    while ((t + 1) * (t + 2) / 2 <= tile) ++t;
+    while (t * (t + 1) / 2 > tile) --t;
+    ti = t;
+    tj = tile - t * (t + 1) / 2;
+  } else {
+    ti = tile % tiles_m;
+    tj = tile / tiles_m;
+  }
+}
+
+__global__ __launch_bounds__(256) void gemm_tn_f64(GemmArgs g) {
+  int b = xcd_remap(blockIdx.x, gridDim.x);
+  int cell = b / g.ntiles, tile = b % g.ntiles;
+  if (cell >= g.ncells) return;
+  int ti, tj;
+  tile_coords(tile, g.lower, g.tiles_m, ti, tj);
+  const int i0 = ti * GT, j0 = tj * GT;
+  const double* __restrict__ X = g.X + (size_t)cell * g.sx;
+  const double* __restrict__ Y = g.Y + (size_t)cell * g.sy;
+  double* __restrict__ C = g.C + (size_t)cell * g.sc;
+  __shared__ double Xs[GKC][GLD];
+  __shared__ double Ys[GKC][GLD];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
+  d4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int bb = 0; bb < 2; ++bb) acc[a][bb] = d4{0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < g.k; k0 += GKC) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int e = t + 256 * r, kk = e >> 6, ii = e & 63, kg = k0 + kk;
+      double xv = 0.0, yv = 0.0;
+      if (kg < g.k) {
+        if (i0 + ii < g.m) xv = X[(size_t)kg * g.ldx + i0 + ii];
+        if (j0 + ii < g.n) yv = Y[(size_t)kg * g.ldy + j0 + ii];
+      }
+      Xs[kk][ii] = xv;
+      Ys[kk][ii] = yv;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k4 = 0; k4 < GKC; k4 += 4) {
+      const int kr = k4 + (lane >> 4), cc = lane & 15;
+      double xa[2], yb[2];
+      xa[0] = Xs[kr][wr * 32 + cc];
+      xa[1] = Xs[kr][wr * 32 + 16 + cc];
+      yb[0] = Ys[kr][wc * 32 + cc];
+      yb[1] = Ys[kr][wc * 32 + 16 + cc];
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+          acc[bj][bi] = __builtin_amdgcn_mfma_f64_16x16x4f64(yb[bj], xa[bi], acc[bj][bi], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // D[j][i] layout: i = lane%16 (C row), j = 4*(lane/16) + v (C column)
+#pragma unroll
+  for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        int i = i0 + wr * 32 + bi * 16 + (lane & 15);
+        int j = j0 + wc * 32 + bj * 16 + 4 * (lane >> 4) + v;
+        if (i < g.m && j < g.n && (!g.lower || i >= j)) {
+          double* cp = C + (size_t)i + (size_t)j * g.ldc;
+          double val = g.alpha * acc[bj][bi][v];
+          if (g.beta != 0.0) val = fma(g.beta, *cp, val);
+          *cp = val;
+        }
+      }
+}
+
+// ------------------------------------------------------------ panel kernel
+// Columns [k0, k0+kb) (kb <= 64) of the lower-stored matrix: factor the
+// diagonal block in LDS (POTRF), then X <- X L^-T for the rows below.
+constexpr int PNB = 64;
+
+__global__ __launch_bounds__(256) void panel_f64(double* __restrict__ A, int64_t lda, int64_t sA, int D, int k0,
+                                                 int kb, int32_t* __restrict__ info, int info_base) {
+  __shared__ double L[PNB][PNB + 1];
+  __shared__ int fail;
+  const int c = blockIdx.y, chunk = blockIdx.x, t = threadIdx.x;
+  double* Ac = A + (size_t)c * sA;
+  if (t == 0) fail = -1;
+  for (int e = t; e < PNB * PNB; e += 256) {
+    int r = e % PNB, cc = e / PNB;
+    double v;
+    if (r < kb && cc < kb) v = (r >= cc) ? Ac[(size_t)(k0 + r) + (size_t)(k0 + cc) * lda] : 0.0;
+    else v = (r == cc) ? 1.0 : 0.0;
+    L[r][cc] = v;
+  }
+  __syncthreads();
+  for (int j = 0; j < PNB; ++j) {
+    if (t == 0) {
+      double d = L[j][j];
+      if (!(d > 0.0)) {
+        if (fail < 0) fail = j;
+        d = __builtin_nan("");
+      }
+      L[j][j] = sqrt(d);
+    }
+    __syncthreads();
+    double djj = L[j][j];
+    for (int r = j + 1 + t; r < PNB; r += 256) L[r][j] /= djj;
+    __syncthreads();
+    int rem = PNB - j - 1;
+    for (int e = t; e < rem * rem; e += 256) {
+      int r = j + 1 + e % rem, cc = j + 1 + e / rem;
+      if (r >= cc) L[r][cc] -= L[r][j] * L[cc][j];
+    }
+    __syncthreads();
+  }
+  if (chunk == 0) {
+    for (int e = t; e < kb * kb; e += 256) {
+      int r = e % kb, cc = e / kb;
+      if (r >= cc) Ac[(size_t)(k0 + r) + (size_t)(k0 + cc) * lda] = L[r][cc];
+    }
+    if (t == 0 && fail >= 0 && fail < kb && info) atomicCAS(&info[c], 0, info_base + k0 + fail + 1);
+  }
+  const int row = k0 + kb + chunk * 256 + t;
+  if (row >= D) return;
+  double x[PNB];
+#pragma unroll
+  for (int j = 0; j < PNB; ++j) x[j] = (j < kb) ? Ac[(size_t)row + (size_t)(k0 + j) * lda] : 0.0;
+#pragma unroll
+  for (int j = 0; j < PNB; ++j) {
+    double s = x[j];
+#pragma unroll
+    for (int m = 0; m < j; ++m) s = fma(-x[m], L[j][m], s);
+    x[j] = s / L[j][j];
+  }
+#pragma unroll
+  for (int j = 0; j < PNB; ++j)
+    if (j < kb) Ac[(size_t)row + (size_t)(k0 + j) * lda] = x[j];
+}
+
+// ---------------------------------------------------------- small kernels
+// zero the trailing block rows/cols [r0, D) (lower part and the diagonal)
+__global__ void zero_trailing(double* __restrict__ A, int64_t lda, int64_t sA, int r0, int D) {
+  int c = blockIdx.z, col = r0 + blockIdx.y;
+  int n = D - r0;
+  double* Ac = A + (size_t)c * sA + (size_t)col * lda;
+  for (int r = col + blockIdx.x * blockDim.x + threadIdx.x; r < D; r += gridDim.x * blockDim.x) Ac[r] = 0.0;
+  (void)n;
+}
+
+// POEQU: s_i = 1 / sqrt(A_ii), i < n
+__global__ void diag_scale_factors(const double* __restrict__ A, int64_t lda, int64_t sA, int r0, int n,
+                                   double* __restrict__ s, int64_t ss, int sign) {
+  int c = blockIdx.y, i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double d = sign * A[(size_t)c * sA + (size_t)(r0 + i) * (lda + 1)];
+  s[(size_t)c * ss + i] = 1.0 / sqrt(d);
+}
+
+// A[r, j] *= sign * s'_r * s_j for the lower part of the square block [r0, D):
+// s'_r = s[r - r0] if r - r0 < n else 1; only columns j - r0 < ncols are scaled by s_j
+__global__ void apply_scaling(double* __restrict__ A, int64_t lda, int64_t sA, int r0, int D, int n,
+                              const double* __restrict__ s, int64_t ss, double sign) {
+  int c = blockIdx.z, jc = blockIdx.y;
+  int col = r0 + jc;
+  const double* sc = s + (size_t)c * ss;
+  double sj = jc < n ? sc[jc] : 1.0;
+  double* Ac = A + (size_t)c * sA + (size_t)col * lda;
+  for (int r = col + blockIdx.x * blockDim.x + threadIdx.x; r < D; r += gridDim.x * blockDim.x) {
+    int ir = r - r0;
+    double si = ir < n ? sc[ir] : 1.0;
+    Ac[r] = sign * Ac[r] * si * sj;
+  }
+}
+
+// residual row: A[row_r, i] = A[row_l, i] - sum_j u[c][j] A[r0 + j, i], one wave per column i
+__global__ __launch_bounds__(256) void residual_row(double* __restrict__ A, int64_t lda, int64_t sA, int N, int M,
+                                                    int r0, int row_l, int row_r, const double* __restrict__ u) {
+  int c = blockIdx.y;
+  int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  int lane = threadIdx.x & 63;
+  if (i >= N) return;
+  const double* col = A + (size_t)c * sA + (size_t)i * lda;
+  double acc = 0.0;
+  if (u) {
+    const double* uc = u + (size_t)c * M;
+    for (int j = lane; j < M; j += 64) acc = fma(uc[j], col[r0 + j], acc);
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  }
+  // without u the row is cleared (it must not carry stale workspace values)
+  if (lane == 0) A[(size_t)c * sA + row_r + (size_t)i * lda] = u ? col[row_l] - acc : 0.0;
+}
+
+// out[c][a][b] = sign * sym(A)[r0 + a][r0 + b] (row-major n x n), 32x32 tiles
+__global__ __launch_bounds__(256) void extract_sym(const double* __restrict__ A, int64_t lda, int64_t sA, int r0,
+                                                   int n, double sign, double* __restrict__ out) {
+  __shared__ double tile[32][33];
+  int c = blockIdx.z, ta = blockIdx.y, tb = blockIdx.x;
+  const double* Ac = A + (size_t)c * sA;
+  double* oc = out + (size_t)c * n * n;
+  int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  if (ta < tb) {
+    // strictly upper tile: value(a,b) = A[r0+b, r0+a] with b > a: read along b, write along b
+    for (int yy = ty; yy < 32; yy += 8) {
+      int a = ta * 32 + yy, b = tb * 32 + tx;
+      if (a < n && b < n) oc[(size_t)a * n + b] = sign * Ac[(size_t)(r0 + b) + (size_t)(r0 + a) * lda];
+    }
+    return;
+  }
+  for (int yy = ty; yy < 32; yy += 8) {
+    int a = ta * 32 + tx, b = tb * 32 + yy;
+    double v = 0.0;
+    if (a < n && b < n)
+      v = (a >= b) ? Ac[(size_t)(r0 + a) + (size_t)(r0 + b) * lda] : Ac[(size_t)(r0 + b) + (size_t)(r0 + a) * lda];
+    tile[tx][yy] = v;
+  }
+  __syncthreads();
+  for (int yy = ty; yy < 32; yy += 8) {
+    int a = ta * 32 + yy, b = tb * 32 + tx;
+    if (a < n && b < n) oc[(size_t)a * n + b] = sign * tile[yy][tx];
+  }
+}
+
+// out[c][a] = sign * A[row, r0 + a]
+__global__ void extract_row(const double* __restrict__ A, int64_t lda, int64_t sA, int row, int r0, int n,
+                            double sign, double* __restrict__ out) {
+  int c = blockIdx.y, a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  out[(size_t)c * n + a] = sign * A[(size_t)c * sA + row + (size_t)(r0 + a) * lda];
+}
+
+// ------------------------------------------------------------ host launchers
+static void check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void launch_point(int dim, int kind, const DevProgram* prog, const DevTables& tab, int npts, int Qs, int ncells,
+                  const double* nodes, const int8_t* par, const double* coefs, int ncoef, int nsides,
+                  const dpg_load_desc& load, double* W, double* Wrhs, hipStream_t st) {
+  int total = ncells * npts;
+  dim3 grid((total + 127) / 128);
+  if (dim == 2)
+    point_kernel<2><<<grid, 128, 0, st>>>(kind, prog, tab, npts, Qs, ncells, nodes, par, coefs, ncoef, nsides, load,
+                                           W, Wrhs);
+  else
+    point_kernel<3><<<grid, 128, 0, st>>>(kind, prog, tab, npts, Qs, ncells, nodes, par, coefs, ncoef, nsides, load,
+                                           W, Wrhs);
+  check_launch("point_kernel");
+}
+
+void launch_operand(int dim, const double* W, int ng, int nvar, const double* R, const int32_t* vod, int N, int npts,
+                    int ncells, double* out, int64_t sc, int64_t sb, int64_t sp, hipStream_t st) {
+  if (ng == 0 || npts == 0) return;
+  dim3 grid((N + 255) / 256, npts, ncells);
+  size_t shm = sizeof(double) * ng * nvar * (dim + 1);
+  if (dim == 2)
+    operand_kernel<double, 3><<<grid, 256, shm, st>>>(W, ng, nvar, R, vod, N, npts, out, sc, sb, sp);
+  else
+    operand_kernel<double, 4><<<grid, 256, shm, st>>>(W, ng, nvar, R, vod, N, npts, out, sc, sb, sp);
+  check_launch("operand_kernel");
+}
+
+void launch_rhs(int dim, const double* Wrhs, int nvar, const double* R, const int32_t* vod, int N, int Q, int ncells,
+                double* A, int64_t lda, int64_t sA, int row_l, hipStream_t st) {
+  dim3 grid((N + 255) / 256, ncells);
+  if (dim == 2)
+    rhs_kernel<3><<<grid, 256, 0, st>>>(Wrhs, nvar, R, vod, N, Q, A, lda, sA, row_l);
+  else
+    rhs_kernel<4><<<grid, 256, 0, st>>>(Wrhs, nvar, R, vod, N, Q, A, lda, sA, row_l);
+  check_launch("rhs_kernel");
+}
+
+void launch_gemm(GemmArgs g, hipStream_t st) {
+  if (g.m <= 0 || g.n <= 0 || g.ncells <= 0) return;
+  g.tiles_m = (g.m + GT - 1) / GT;
+  int tiles_n = (g.n + GT - 1) / GT;
+  g.ntiles = g.lower ? g.tiles_m * (g.tiles_m + 1) / 2 : g.tiles_m * tiles_n;
+  int64_t nblocks = (int64_t)g.ntiles * g.ncells;
+  if (nblocks > 0x7fffffff) throw DeviceError("gemm grid too large");
+  gemm_tn_f64<<<dim3((unsigned)nblocks), 256, 0, st>>>(g);
+  check_launch("gemm_tn_f64");
+}
+
+void launch_panel(double* A, int64_t lda, int64_t sA, int D, int k0, int kb, int ncells, int32_t* info, int info_base,
+                  hipStream_t st) {
+  int rows = D - k0 - kb;
+  int chunks = rows > 0 ? (rows + 255) / 256 : 1;
+  panel_f64<<<dim3(chunks, ncells), 256, 0, st>>>(A, lda, sA, D, k0, kb, info, info_base);
+  check_launch("panel_f64");
+}
+
+void launch_zero_trailing(double* A, int64_t lda, int64_t sA, int r0, int D, int ncells, hipStream_t st) {
+  int n = D - r0;
+  if (n <= 0) return;
+  zero_trailing<<<dim3((n + 255) / 256, n, ncells), 256, 0, st>>>(A, lda, sA, r0, D);
+  check_launch("zero_trailing");
+}
+
+void launch_scaling(double* A, int64_t lda, int64_t sA, int r0, int D, int n, int ncol, double* s, int64_t ss,
+                    int ncells, double sign, hipStream_t st) {
+  diag_scale_factors<<<dim3((n + 255) / 256, ncells), 256, 0, st>>>(A, lda, sA, r0, n, s, ss, sign > 0 ? 1 : -1);
+  check_launch("diag_scale_factors");
+  apply_scaling<<<dim3((ncol + 255) / 256, ncol, ncells), 256, 0, st>>>(A, lda, sA, r0, D, n, s, ss, sign);
+  check_launch("apply_scaling");
+}
+
+void launch_residual_row(double* A, int64_t lda, int64_t sA, int N, int M, int r0, int row_l, int row_r,
+                         const double* u, int ncells, hipStream_t st) {
+  residual_row<<<dim3((N + 3) / 4, ncells), 256, 0, st>>>(A, lda, sA, N, M, r0, row_l, row_r, u);
+  check_launch("residual_row");
+}
+
+void launch_extract_sym(const double* A, int64_t lda, int64_t sA, int r0, int n, double sign, double* out, int ncells,
+                        hipStream_t st) {
+  int nt = (n + 31) / 32;
+  extract_sym<<<dim3(nt, nt, ncells), 256, 0, st>>>(A, lda, sA, r0, n, sign, out);
+  check_launch("extract_sym");
+}
+
+void launch_extract_row(const double* A, int64_t lda, int64_t sA, int row, int r0, int n, double sign, double* out,
+                        int ncells, hipStream_t st) {
+  extract_row<<<dim3((n + 255) / 256, ncells), 256, 0, st>>>(A, lda, sA, row, r0, n, sign, out);
+  check_launch("extract_row");
+}
+
+}  // namespace dpg

@@ -1,0 +1,52 @@
+// kernels.hpp -- host launchers of the gfx950 kernels (kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/dpg_capi.h"
+#include "device_common.hpp"
+
+namespace dpg {
+
+struct DeviceError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+// C(i,j) = alpha * sum_k X(k,i) Y(k,j) + beta * C(i,j) for every cell of a batch
+struct GemmArgs {
+  const double* X;
+  int64_t ldx, sx;  // X(k,i) = X[k*ldx + i], per-cell stride sx (0: shared)
+  const double* Y;
+  int64_t ldy, sy;
+  double* C;
+  int64_t ldc, sc;  // C(i,j) = C[i + j*ldc]
+  int32_t m, n, k, lower;
+  double alpha, beta;
+  int32_t tiles_m, ntiles, ncells, pad;
+};
+
+void launch_point(int dim, int kind, const DevProgram* prog, const DevTables& tab, int npts, int Qs, int ncells,
+                  const double* nodes, const int8_t* par, const double* coefs, int ncoef, int nsides,
+                  const dpg_load_desc& load, double* W, double* Wrhs, hipStream_t st);
+void launch_operand(int dim, const double* W, int ng, int nvar, const double* R, const int32_t* vod, int N, int npts,
+                    int ncells, double* out, int64_t sc, int64_t sb, int64_t sp, hipStream_t st);
+void launch_rhs(int dim, const double* Wrhs, int nvar, const double* R, const int32_t* vod, int N, int Q, int ncells,
+                double* A, int64_t lda, int64_t sA, int row_l, hipStream_t st);
+void launch_gemm(GemmArgs g, hipStream_t st);
+void launch_panel(double* A, int64_t lda, int64_t sA, int D, int k0, int kb, int ncells, int32_t* info, int info_base,
+                  hipStream_t st);
+void launch_zero_trailing(double* A, int64_t lda, int64_t sA, int r0, int D, int ncells, hipStream_t st);
+void launch_scaling(double* A, int64_t lda, int64_t sA, int r0, int D, int n, int ncol, double* s, int64_t ss,
+                    int ncells, double sign, hipStream_t st);
+void launch_residual_row(double* A, int64_t lda, int64_t sA, int N, int M, int r0, int row_l, int row_r,
+                         const double* u, int ncells, hipStream_t st);
+void launch_extract_sym(const double* A, int64_t lda, int64_t sA, int r0, int n, double sign, double* out, int ncells,
+                        hipStream_t st);
+void launch_extract_row(const double* A, int64_t lda, int64_t sA, int row, int r0, int n, double sign, double* out,
+                        int ncells, hipStream_t st);
+
+}  // namespace dpg

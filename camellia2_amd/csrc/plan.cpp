@@ -1,0 +1,600 @@
+// plan.cpp -- C-ABI entry points of the device path: plan creation (device
+// basis/cubature tables per element type, the compiled term program) and the
+// batched local-assembly pipeline that replaces TBF::localStiffnessMatrixAndRHS.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/dpg_capi.h"
+#include "element.hpp"
+#include "errors.hpp"
+#include "kernels.hpp"
+
+namespace dpg {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+#define HIP_CHECK(expr)                                                                     \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess) throw ::dpg::DeviceError(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+namespace {
+
+int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+struct DeviceBuffer {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DeviceBuffer() = default;
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  ~DeviceBuffer() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void ensure(size_t n) {
+    if (n <= bytes) return;
+    release();
+    HIP_CHECK(hipMalloc(&p, n));
+    bytes = n;
+  }
+};
+
+template <class T>
+T* upload(std::vector<std::unique_ptr<DeviceBuffer>>& owner, const std::vector<T>& host) {
+  owner.emplace_back(new DeviceBuffer());
+  size_t n = std::max<size_t>(1, host.size()) * sizeof(T);
+  owner.back()->ensure(n);
+  if (!host.empty()) HIP_CHECK(hipMemcpy(owner.back()->p, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice));
+  return (T*)owner.back()->p;
+}
+
+DSummand summand(int var, int op, const dpg_coef& c, int aux) {
+  DSummand s;
+  s.var = var;
+  s.op = op;
+  s.slot = c.coef_slot;
+  s.fn = c.coef_fn;
+  s.normal = c.normal_comp;
+  s.aux = aux;
+  s.scale = c.scale;
+  return s;
+}
+
+bool bterm_on_side(const dpg_form_desc& f, const dpg_bterm& t) {
+  int kind = f.trial_kind[t.trial_var];
+  return kind == DPG_TRACE || kind == DPG_FLUX || t.test_op == DPG_OP_DOT_NORMAL || t.coef.normal_comp >= 0;
+}
+
+void validate_form(const dpg_form_desc& f) {
+  if (f.spatial_dim != 2 && f.spatial_dim != 3) throw std::invalid_argument("spatial_dim must be 2 or 3");
+  if (f.n_bterms < 0 || f.n_bterms > DPG_MAX_BTERMS || f.n_ipterms <= 0 || f.n_ipterms > DPG_MAX_IPTERMS ||
+      f.n_rhs < 0 || f.n_rhs > DPG_MAX_RHS_TERMS || f.n_coefs < 0 || f.n_coefs > DPG_MAX_COEFS)
+    throw std::invalid_argument("term counts out of range");
+  auto check_op = [&](int var, int op, bool side_ok) {
+    if (var < 0 || var >= f.n_test) throw std::invalid_argument("test variable index out of range");
+    bool hgrad = f.test_space[var] == DPG_HGRAD;
+    bool ok = hgrad ? (op == DPG_OP_VALUE || (op >= DPG_OP_DX && op < DPG_OP_DX + f.spatial_dim))
+                    : ((op >= DPG_OP_X && op < DPG_OP_X + f.spatial_dim) || op == DPG_OP_DIV ||
+                       (side_ok && op == DPG_OP_DOT_NORMAL));
+    if (!ok) throw std::invalid_argument("test operator does not apply to the test space");
+  };
+  auto check_coef = [&](const dpg_coef& c, bool side_ok) {
+    if (c.coef_slot >= f.n_coefs || c.coef_slot < -1) throw std::invalid_argument("coefficient slot out of range");
+    if (c.normal_comp >= f.spatial_dim || (!side_ok && c.normal_comp >= 0))
+      throw std::invalid_argument("normal component out of range");
+  };
+  for (int i = 0; i < f.n_bterms; ++i) {
+    const dpg_bterm& t = f.bterms[i];
+    if (t.trial_var < 0 || t.trial_var >= f.n_trial) throw std::invalid_argument("trial variable index out of range");
+    int kind = f.trial_kind[t.trial_var];
+    int ncomp = kind == DPG_FIELD_VECTOR ? f.spatial_dim : 1;
+    if (t.trial_comp < 0 || t.trial_comp >= ncomp) throw std::invalid_argument("trial component out of range");
+    bool side = bterm_on_side(f, t);
+    if (side && (kind == DPG_FIELD || kind == DPG_FIELD_VECTOR))
+      throw std::invalid_argument("field variables on the mesh skeleton are not supported");
+    check_op(t.test_var, t.test_op, side);
+    check_coef(t.coef, side);
+  }
+  for (int i = 0; i < f.n_ipterms; ++i) {
+    if (f.ipterms[i].n <= 0 || f.ipterms[i].n > DPG_MAX_LT_SUMMANDS) throw std::invalid_argument("bad IP term");
+    for (int s = 0; s < f.ipterms[i].n; ++s) {
+      check_op(f.ipterms[i].s[s].test_var, f.ipterms[i].s[s].test_op, false);
+      check_coef(f.ipterms[i].s[s].coef, false);
+    }
+  }
+  for (int i = 0; i < f.n_rhs; ++i) {
+    check_op(f.rhs[i].test_var, f.rhs[i].test_op, false);
+    if (f.rhs[i].load_comp < 0 || f.rhs[i].load_comp > 2) throw std::invalid_argument("bad load component");
+  }
+}
+
+}  // namespace
+
+struct BucketPlan {
+  dpg_bucket_desc desc{};
+  ElementLayout L;
+  DevProgram prog{};
+  DevProgram* prog_d = nullptr;
+  DevTables tab{};
+  std::vector<int> vb_block;  // volume group -> index in L.blocks
+  std::vector<int> sb_var;    // side group -> trial variable
+  int npr = 0, D = 0;
+  int64_t lda = 0, Npad = 0;
+  // per-cell element counts (doubles) of the workspace regions
+  int64_t nA = 0, nAt = 0, nTv = 0, nTs = 0, nWip = 0, nWvol = 0, nWrhs = 0, nWside = 0, nS = 0;
+  int64_t per_cell_bytes = 0;
+  double flops = 0.0;
+};
+
+}  // namespace dpg
+
+struct dpg_plan {
+  dpg_form_desc form{};
+  dpg_options opt{};
+  int device = 0;
+  std::vector<dpg::BucketPlan> buckets;
+  std::vector<std::unique_ptr<dpg::DeviceBuffer>> tables;
+  dpg::DeviceBuffer ws;
+  dpg::DeviceBuffer info;
+};
+
+namespace dpg {
+namespace {
+
+void compile_program(const dpg_form_desc& f, BucketPlan& B) {
+  DevProgram& P = B.prog;
+  std::memset(&P, 0, sizeof(P));
+  P.dim = f.spatial_dim;
+  P.nvar = f.n_test;
+  P.npr = f.spatial_dim + 1;
+  int ns = 0;
+  auto push = [&](const DSummand& s) {
+    if (ns >= kMaxSummands) throw std::invalid_argument("form has too many summands");
+    P.s[ns++] = s;
+  };
+  // IP terms
+  P.n_lt = f.n_ipterms;
+  if (P.n_lt > kMaxGroups) throw std::invalid_argument("too many IP terms");
+  for (int i = 0; i < f.n_ipterms; ++i) {
+    P.lt_beg[i] = ns;
+    for (int s = 0; s < f.ipterms[i].n; ++s)
+      push(summand(f.ipterms[i].s[s].test_var, f.ipterms[i].s[s].test_op, f.ipterms[i].s[s].coef, 0));
+  }
+  P.lt_beg[P.n_lt] = ns;
+  // volume trial blocks: field components
+  B.vb_block.clear();
+  for (int bi = 0; bi < (int)B.L.blocks.size(); ++bi)
+    if (B.L.blocks[bi].side < 0) B.vb_block.push_back(bi);
+  P.n_vb = (int)B.vb_block.size();
+  if (P.n_vb > kMaxGroups) throw std::invalid_argument("too many field blocks");
+  for (int g = 0; g < P.n_vb; ++g) {
+    const TrialBlock& blk = B.L.blocks[B.vb_block[g]];
+    P.vb_beg[g] = ns;
+    for (int i = 0; i < f.n_bterms; ++i) {
+      const dpg_bterm& t = f.bterms[i];
+      if (t.trial_var != blk.var || t.trial_comp != blk.comp || bterm_on_side(f, t)) continue;
+      push(summand(t.test_var, t.test_op, t.coef, 0));
+    }
+  }
+  P.vb_beg[P.n_vb] = ns;
+  // side groups: trace / flux variables
+  B.sb_var.clear();
+  for (int v = 0; v < f.n_trial; ++v)
+    if (f.trial_kind[v] == DPG_TRACE || f.trial_kind[v] == DPG_FLUX) B.sb_var.push_back(v);
+  P.n_sb = (int)B.sb_var.size();
+  if (P.n_sb > kMaxGroups) throw std::invalid_argument("too many side variables");
+  for (int g = 0; g < P.n_sb; ++g) {
+    int v = B.sb_var[g];
+    P.sb_flux[g] = f.trial_kind[v] == DPG_FLUX ? 1 : 0;
+    P.sb_beg[g] = ns;
+    for (int i = 0; i < f.n_bterms; ++i) {
+      const dpg_bterm& t = f.bterms[i];
+      if (t.trial_var != v) continue;
+      push(summand(t.test_var, t.test_op, t.coef, 0));
+    }
+  }
+  P.sb_beg[P.n_sb] = ns;
+  P.rhs_beg[0] = ns;
+  for (int i = 0; i < f.n_rhs; ++i) {
+    dpg_coef c{};
+    c.scale = f.rhs[i].scale;
+    c.coef_slot = -1;
+    c.normal_comp = -1;
+    push(summand(f.rhs[i].test_var, f.rhs[i].test_op, c, f.rhs[i].load_comp));
+  }
+  P.rhs_beg[1] = ns;
+}
+
+void build_bucket(dpg_plan* plan, const dpg_bucket_desc& desc, BucketPlan& B) {
+  const dpg_form_desc& f = plan->form;
+  B.desc = desc;
+  B.L = make_layout(f, desc);
+  const ElementLayout& L = B.L;
+  B.npr = L.dim + 1;
+  B.flops = flops_per_cell(f, L);
+  compile_program(f, B);
+  RefTables T = make_tables(f, L);
+  auto& own = plan->tables;
+  B.tab.R_vol = upload(own, T.vol.R);
+  B.tab.R_ip = upload(own, T.ip.R);
+  B.tab.R_side = upload(own, T.side.R);
+  B.tab.dN_vol = upload(own, T.vol.dN);
+  B.tab.dN_ip = upload(own, T.ip.dN);
+  B.tab.dN_side = upload(own, T.side.dN);
+  B.tab.Nv_vol = upload(own, T.vol.Nv);
+  B.tab.w_vol = upload(own, T.vol.w);
+  B.tab.w_ip = upload(own, T.ip.w);
+  B.tab.w_side = upload(own, T.side.w);
+  B.tab.field_vol = upload(own, T.vol.field);
+  B.tab.trace = upload(own, T.trace);
+  B.tab.flux = upload(own, T.flux);
+  std::vector<double> tan((size_t)L.rc.nsides * 2 * L.dim, 0.0);
+  for (int s = 0; s < L.rc.nsides; ++s)
+    for (int a = 0; a < L.dim - 1; ++a)
+      for (int d = 0; d < L.dim; ++d)
+        tan[((size_t)s * 2 + a) * L.dim + d] = L.rc.side_param[((size_t)s * 3 + 1 + a) * L.dim + d];
+  B.tab.side_tan = upload(own, tan);
+  std::vector<int32_t> vod(L.var_of_test_dof.begin(), L.var_of_test_dof.end());
+  B.tab.var_of_dof = upload(own, vod);
+  std::vector<DevProgram> pv(1, B.prog);
+  B.prog_d = upload(own, pv);
+  // workspace layout (doubles per cell)
+  B.D = L.N + L.M + 2;
+  B.lda = round_up(B.D, 8);
+  B.Npad = round_up(L.N, 8);
+  int S = L.rc.nsides, nvar = f.n_test;
+  B.nA = B.lda * B.D;
+  B.nAt = (int64_t)B.prog.n_lt * L.QG * B.Npad;
+  B.nTv = (int64_t)B.prog.n_vb * L.Q * B.Npad;
+  B.nTs = (int64_t)B.prog.n_sb * S * L.Qs * B.Npad;
+  B.nWip = (int64_t)L.QG * B.prog.n_lt * nvar * B.npr;
+  B.nWvol = (int64_t)L.Q * B.prog.n_vb * nvar * B.npr;
+  B.nWrhs = (int64_t)L.Q * nvar * B.npr;
+  B.nWside = (int64_t)S * L.Qs * B.prog.n_sb * nvar * B.npr;
+  B.nS = round_up(std::max(L.N, L.M), 8);
+  int64_t tot = B.nA + B.nAt + B.nTv + B.nTs + B.nWip + B.nWvol + B.nWrhs + B.nWside + B.nS;
+  B.per_cell_bytes = tot * (int64_t)sizeof(double);
+}
+
+struct Work {
+  double *A, *At, *Tv, *Ts, *Wip, *Wvol, *Wrhs, *Wside, *s;
+};
+
+Work carve(dpg_plan* plan, const BucketPlan& B, int nb) {
+  Work w;
+  double* p = (double*)plan->ws.p;
+  auto take = [&](int64_t per) {
+    double* r = p;
+    p += per * nb;
+    return r;
+  };
+  w.A = take(B.nA);
+  w.At = take(B.nAt);
+  w.Tv = take(B.nTv);
+  w.Ts = take(B.nTs);
+  w.Wip = take(B.nWip);
+  w.Wvol = take(B.nWvol);
+  w.Wrhs = take(B.nWrhs);
+  w.Wside = take(B.nWside);
+  w.s = take(B.nS);
+  return w;
+}
+
+int batch_cells(dpg_plan* plan, const BucketPlan& B, int ncells) {
+  if (plan->opt.max_batch_cells > 0) return std::min(ncells, plan->opt.max_batch_cells);
+  double budget = plan->opt.workspace_bytes > 0 ? plan->opt.workspace_bytes : 16.0 * (1ull << 30);
+  int64_t nb = (int64_t)(budget / (double)B.per_cell_bytes);
+  nb = std::max<int64_t>(1, std::min<int64_t>(nb, ncells));
+  return (int)nb;
+}
+
+// Stage 1 of the pipeline for cells [c0, c0+nb): contractions into A, POEQU
+// scaling and the partial Cholesky over the first N columns.
+void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int c0, int nb, const Work& w,
+                  const double* u, double* Gout, hipStream_t st) {
+  const ElementLayout& L = B.L;
+  const int dim = L.dim, V = L.rc.nverts, S = L.rc.nsides, N = L.N, M = L.M, nvar = plan->form.n_test;
+  const double* nodes = bt.nodes + (size_t)c0 * V * dim;
+  const int8_t* par = bt.parities + (size_t)c0 * S;
+  const double* coefs = bt.coefs ? bt.coefs + (size_t)c0 * plan->form.n_coefs : nullptr;
+  const int64_t sA = B.nA, lda = B.lda;
+  // per-point weights
+  launch_point(dim, 0, B.prog_d, B.tab, L.QG, L.Qs, nb, nodes, par, coefs, plan->form.n_coefs, S, bt.load, w.Wip,
+               nullptr, st);
+  launch_point(dim, 1, B.prog_d, B.tab, L.Q, L.Qs, nb, nodes, par, coefs, plan->form.n_coefs, S, bt.load, w.Wvol,
+               w.Wrhs, st);
+  if (B.prog.n_sb > 0)
+    launch_point(dim, 2, B.prog_d, B.tab, S * L.Qs, L.Qs, nb, nodes, par, coefs, plan->form.n_coefs, S, bt.load,
+                 w.Wside, nullptr, st);
+  // operands
+  launch_operand(dim, w.Wip, B.prog.n_lt, nvar, B.tab.R_ip, B.tab.var_of_dof, N, L.QG, nb, w.At, B.nAt, B.Npad,
+                 (int64_t)B.prog.n_lt * B.Npad, st);
+  launch_operand(dim, w.Wvol, B.prog.n_vb, nvar, B.tab.R_vol, B.tab.var_of_dof, N, L.Q, nb, w.Tv, B.nTv,
+                 (int64_t)L.Q * B.Npad, B.Npad, st);
+  if (B.prog.n_sb > 0)
+    launch_operand(dim, w.Wside, B.prog.n_sb, nvar, B.tab.R_side, B.tab.var_of_dof, N, S * L.Qs, nb, w.Ts, B.nTs,
+                   (int64_t)S * L.Qs * B.Npad, B.Npad, st);
+  launch_zero_trailing(w.A, lda, sA, N, B.D, nb, st);
+  // G = At^T At (lower)
+  GemmArgs g{};
+  g.X = w.At;
+  g.ldx = B.Npad;
+  g.sx = B.nAt;
+  g.Y = w.At;
+  g.ldy = B.Npad;
+  g.sy = B.nAt;
+  g.C = w.A;
+  g.ldc = lda;
+  g.sc = sA;
+  g.m = g.n = N;
+  g.k = B.prog.n_lt * L.QG;
+  g.lower = 1;
+  g.alpha = 1.0;
+  g.beta = 0.0;
+  g.ncells = nb;
+  launch_gemm(g, st);
+  if (Gout) launch_extract_sym(w.A, lda, sA, 0, N, 1.0, Gout + (size_t)c0 * N * N, nb, st);
+  // B^T blocks: rows N + trial offset, columns = test dofs
+  for (int vb = 0; vb < B.prog.n_vb; ++vb) {
+    const TrialBlock& blk = L.blocks[B.vb_block[vb]];
+    GemmArgs b{};
+    b.X = B.tab.field_vol;
+    b.ldx = L.nf;
+    b.sx = 0;
+    b.Y = w.Tv + (size_t)vb * L.Q * B.Npad;
+    b.ldy = B.Npad;
+    b.sy = B.nTv;
+    b.C = w.A + N + blk.offset;
+    b.ldc = lda;
+    b.sc = sA;
+    b.m = blk.card;
+    b.n = N;
+    b.k = L.Q;
+    b.alpha = 1.0;
+    b.ncells = nb;
+    launch_gemm(b, st);
+  }
+  for (int sb = 0; sb < B.prog.n_sb; ++sb) {
+    int v = B.sb_var[sb];
+    for (const TrialBlock& blk : L.blocks) {
+      if (blk.var != v || blk.side < 0) continue;
+      GemmArgs b{};
+      b.X = blk.kind == DPG_TRACE ? B.tab.trace : B.tab.flux;
+      b.ldx = blk.card;
+      b.sx = 0;
+      b.Y = w.Ts + ((size_t)sb * S + blk.side) * L.Qs * B.Npad;
+      b.ldy = B.Npad;
+      b.sy = B.nTs;
+      b.C = w.A + N + blk.offset;
+      b.ldc = lda;
+      b.sc = sA;
+      b.m = blk.card;
+      b.n = N;
+      b.k = L.Qs;
+      b.alpha = 1.0;
+      b.ncells = nb;
+      launch_gemm(b, st);
+    }
+  }
+  // load vector row, residual row
+  launch_rhs(dim, w.Wrhs, nvar, B.tab.R_vol, B.tab.var_of_dof, N, L.Q, nb, w.A, lda, sA, N + M, st);
+  launch_residual_row(w.A, lda, sA, N, M, N, N + M, N + M + 1, u ? u + (size_t)c0 * M : nullptr, nb, st);
+  // DPOEQU scaling of G (and of the rows of B, l, r)
+  launch_scaling(w.A, lda, sA, 0, B.D, N, N, w.s, B.nS, nb, 1.0, st);
+  int32_t* info = (int32_t*)plan->info.p;
+  HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int32_t) * nb, st));
+  for (int k0 = 0; k0 < N; k0 += 64) {
+    int kb = std::min(64, N - k0);
+    launch_panel(w.A, lda, sA, B.D, k0, kb, nb, info, 0, st);
+    int k1 = k0 + kb;
+    if (B.D - k1 > 0) {
+      GemmArgs t{};
+      t.X = w.A + k1 + (size_t)k0 * lda;
+      t.ldx = lda;
+      t.sx = sA;
+      t.Y = t.X;
+      t.ldy = lda;
+      t.sy = sA;
+      t.C = w.A + k1 + (size_t)k1 * lda;
+      t.ldc = lda;
+      t.sc = sA;
+      t.m = t.n = B.D - k1;
+      t.k = kb;
+      t.lower = 1;
+      t.alpha = -1.0;
+      t.beta = 1.0;
+      t.ncells = nb;
+      launch_gemm(t, st);
+    }
+  }
+}
+
+// Stage 2: static condensation of the field dofs of K = -A[N:, N:]
+void stage_schur(dpg_plan* plan, const BucketPlan& B, int nb, const Work& w, hipStream_t st) {
+  const ElementLayout& L = B.L;
+  const int N = L.N, F = L.F;
+  const int64_t sA = B.nA, lda = B.lda;
+  // negate the trailing block and equilibrate its field rows/columns
+  launch_scaling(w.A, lda, sA, N, B.D, F, B.D - N, w.s, B.nS, nb, -1.0, st);
+  int32_t* info = (int32_t*)plan->info.p;
+  for (int k0 = N; k0 < N + F; k0 += 64) {
+    int kb = std::min(64, N + F - k0);
+    launch_panel(w.A, lda, sA, B.D, k0, kb, nb, info, 0, st);
+    int k1 = k0 + kb;
+    if (B.D - k1 > 0) {
+      GemmArgs t{};
+      t.X = w.A + k1 + (size_t)k0 * lda;
+      t.ldx = lda;
+      t.sx = sA;
+      t.Y = t.X;
+      t.ldy = lda;
+      t.sy = sA;
+      t.C = w.A + k1 + (size_t)k1 * lda;
+      t.ldc = lda;
+      t.sc = sA;
+      t.m = t.n = B.D - k1;
+      t.k = kb;
+      t.lower = 1;
+      t.alpha = -1.0;
+      t.beta = 1.0;
+      t.ncells = nb;
+      launch_gemm(t, st);
+    }
+  }
+}
+
+void check_batch(const dpg_plan* plan, int bucket, const dpg_batch* batch) {
+  if (!plan) throw std::invalid_argument("plan is NULL");
+  if (bucket < 0 || bucket >= (int)plan->buckets.size()) throw std::invalid_argument("bucket index out of range");
+  if (!batch) throw std::invalid_argument("batch is NULL");
+  if (batch->ncells < 0) throw std::invalid_argument("ncells < 0");
+  if (batch->ncells > 0 && (!batch->nodes || !batch->parities))
+    throw std::invalid_argument("nodes and parities are required");
+  if (plan->form.n_coefs > 0 && batch->ncells > 0 && !batch->coefs)
+    throw std::invalid_argument("the form needs per-cell coefficients");
+  if (batch->load.ncomp < 0 || batch->load.ncomp > 3) throw std::invalid_argument("bad load");
+  for (int c = 0; c < batch->load.ncomp; ++c)
+    if (batch->load.nmodes[c] < 0 || batch->load.nmodes[c] > DPG_MAX_LOAD_MODES)
+      throw std::invalid_argument("bad load mode count");
+}
+
+void prepare_workspace(dpg_plan* plan, const BucketPlan& B, int nb) {
+  plan->ws.ensure((size_t)B.per_cell_bytes * nb);
+  plan->info.ensure(sizeof(int32_t) * std::max(nb, 1));
+}
+
+int count_failures(dpg_plan* plan, int nb, int32_t* out_info, hipStream_t st) {
+  std::vector<int32_t> h(nb);
+  HIP_CHECK(hipMemcpyAsync(h.data(), plan->info.p, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, st));
+  HIP_CHECK(hipStreamSynchronize(st));
+  int bad = 0;
+  for (int i = 0; i < nb; ++i) bad += h[i] != 0;
+  if (out_info) HIP_CHECK(hipMemcpy(out_info, h.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice));
+  return bad;
+}
+
+}  // namespace
+}  // namespace dpg
+
+extern "C" {
+
+const char* dpg_version(void) { return "camellia2_amd-dpg 0.1 (gfx950)"; }
+
+const char* dpg_last_error(void) { return dpg::g_last_error.c_str(); }
+
+int dpg_bucket_sizes(const dpg_form_desc* form, const dpg_bucket_desc* bucket, dpg_bucket_info* out) {
+  return dpg::guarded([&] {
+    if (!form || !bucket || !out) throw std::invalid_argument("NULL argument");
+    dpg::validate_form(*form);
+    dpg::ElementLayout L = dpg::make_layout(*form, *bucket);
+    std::memset(out, 0, sizeof(*out));
+    out->N = L.N;
+    out->M = L.M;
+    out->F = L.F;
+    out->T = L.T;
+    out->Q = L.Q;
+    out->Qs = L.Qs;
+    out->QG = L.QG;
+    out->nsides = L.rc.nsides;
+    out->nverts = L.rc.nverts;
+    out->flops_per_cell = dpg::flops_per_cell(*form, L);
+  });
+}
+
+int dpg_plan_create(const dpg_form_desc* form, const dpg_bucket_desc* buckets, int nbuckets,
+                    const dpg_options* options, dpg_plan** out) {
+  return dpg::guarded([&] {
+    if (!form || !buckets || nbuckets <= 0 || !out) throw std::invalid_argument("NULL argument or no buckets");
+    *out = nullptr;
+    dpg::validate_form(*form);
+    std::unique_ptr<dpg_plan> plan(new dpg_plan());
+    plan->form = *form;
+    if (options) plan->opt = *options;
+    if (plan->opt.precision != DPG_PREC_FP64)
+      throw std::invalid_argument("this build assembles in FP64 only (precision must be DPG_PREC_FP64)");
+    HIP_CHECK(hipGetDevice(&plan->device));
+    plan->buckets.resize(nbuckets);
+    for (int i = 0; i < nbuckets; ++i) dpg::build_bucket(plan.get(), buckets[i], plan->buckets[i]);
+    *out = plan.release();
+  });
+}
+
+int dpg_plan_destroy(dpg_plan* plan) {
+  return dpg::guarded([&] { delete plan; });
+}
+
+int dpg_plan_workspace(const dpg_plan* plan, size_t* bytes) {
+  return dpg::guarded([&] {
+    if (!plan || !bytes) throw std::invalid_argument("NULL argument");
+    *bytes = plan->ws.bytes;
+  });
+}
+
+int dpg_assemble(dpg_plan* plan, int bucket, const dpg_batch* batch, const dpg_outputs* outputs, void* stream) {
+  return dpg::guarded_status([&] {
+    dpg::check_batch(plan, bucket, batch);
+    if (!outputs) throw std::invalid_argument("outputs is NULL");
+    if (batch->ncells == 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    const dpg::BucketPlan& B = plan->buckets[bucket];
+    const int N = B.L.N, M = B.L.M, F = B.L.F, T = B.L.T;
+    int C = batch->ncells;
+    int maxb = dpg::batch_cells(plan, B, C);
+    dpg::prepare_workspace(plan, B, maxb);
+    int bad = 0;
+    for (int c0 = 0; c0 < C; c0 += maxb) {
+      int nb = std::min(maxb, C - c0);
+      dpg::Work w = dpg::carve(plan, B, nb);
+      dpg::stage_factor(plan, B, *batch, c0, nb, w, nullptr, outputs->G, st);
+      if (outputs->K) dpg::launch_extract_sym(w.A, B.lda, B.nA, N, M, -1.0, outputs->K + (size_t)c0 * M * M, nb, st);
+      if (outputs->rhs)
+        dpg::launch_extract_row(w.A, B.lda, B.nA, N + M, N, M, -1.0, outputs->rhs + (size_t)c0 * M, nb, st);
+      if (outputs->schur || outputs->schur_rhs) {
+        dpg::stage_schur(plan, B, nb, w, st);
+        if (outputs->schur)
+          dpg::launch_extract_sym(w.A, B.lda, B.nA, N + F, T, 1.0, outputs->schur + (size_t)c0 * T * T, nb, st);
+        if (outputs->schur_rhs)
+          dpg::launch_extract_row(w.A, B.lda, B.nA, N + M, N + F, T, 1.0, outputs->schur_rhs + (size_t)c0 * T, nb,
+                                  st);
+      }
+      // the next batch reuses the workspace in stream order
+      if (outputs->info) bad += dpg::count_failures(plan, nb, outputs->info + c0, st);
+    }
+    return bad;
+  });
+}
+
+int dpg_energy_error(dpg_plan* plan, int bucket, const dpg_batch* batch, const double* u, double* eta2,
+                     void* stream) {
+  return dpg::guarded_status([&] {
+    dpg::check_batch(plan, bucket, batch);
+    if (!u || !eta2) throw std::invalid_argument("u and eta2 are required");
+    if (batch->ncells == 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    const dpg::BucketPlan& B = plan->buckets[bucket];
+    int C = batch->ncells;
+    int maxb = dpg::batch_cells(plan, B, C);
+    dpg::prepare_workspace(plan, B, maxb);
+    int bad = 0;
+    for (int c0 = 0; c0 < C; c0 += maxb) {
+      int nb = std::min(maxb, C - c0);
+      dpg::Work w = dpg::carve(plan, B, nb);
+      dpg::stage_factor(plan, B, *batch, c0, nb, w, u, nullptr, st);
+      dpg::launch_extract_row(w.A, B.lda, B.nA, B.D - 1, B.D - 1, 1, -1.0, eta2 + c0, nb, st);
+      bad += dpg::count_failures(plan, nb, nullptr, st);
+    }
+    return bad;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,173 @@
+"""Host-side API over the C ABI (libcamellia2_dpg.so).
+
+Names follow the reference's call surface: a form is the compiled BF/IP/RHS
+of a formulation (PoissonFormulation, ConvectionDiffusionReactionFormulation,
+StokesVGPFormulation with BF::graphNorm), `LocalAssembler.local_stiffness_and_rhs`
+is TBF::localStiffnessMatrixAndRHS for a batch of cells of one element type
+(plus the CondensedDofInterpreter Schur complement), `energy_error` is the
+per-cell indicator of TSolution::rankLocalEnergyError.
+
+Device memory and streams come from torch (ROCm); every computation runs in
+the HIP library, there is no Python or CPU fallback.
+"""
+import ctypes as ct
+
+import numpy as np
+
+from . import capi
+from .capi import check, lib
+
+
+def form(name, dim=2, mu=1.0):
+    """Term tables of a reference formulation (ultraweak, graph norm, f*v load)."""
+    f = capi.FormDesc()
+    L = lib()
+    if name == "poisson":
+        check(L.dpg_form_poisson_uw(dim, ct.byref(f)))
+    elif name == "cdr":
+        check(L.dpg_form_cdr_uw(dim, ct.byref(f)))
+    elif name == "stokes":
+        check(L.dpg_form_stokes_vgp_uw(dim, mu, ct.byref(f)))
+    else:
+        raise ValueError(f"unknown formulation {name}")
+    return f
+
+
+def graph_norm(f):
+    check(lib().dpg_form_graph_norm(ct.byref(f)))
+    return f
+
+
+def bucket(topology, h1_order, test_degree):
+    return capi.BucketDesc(topology, h1_order, test_degree, 0)
+
+
+def sizes(f, b):
+    info = capi.BucketInfo()
+    check(lib().dpg_bucket_sizes(ct.byref(f), ct.byref(b), ct.byref(info)))
+    return info
+
+
+def mesh_quad(nx, ny, jitter, seed):
+    C = nx * ny
+    nodes = np.zeros((C, 4, 2))
+    par = np.zeros((C, 4), dtype=np.int8)
+    check(lib().dpg_mesh_quad(nx, ny, jitter, seed, nodes.ctypes.data, par.ctypes.data))
+    return nodes, par
+
+
+def mesh_hex(nx, ny, nz, jitter, seed):
+    C = nx * ny * nz
+    nodes = np.zeros((C, 8, 3))
+    par = np.zeros((C, 6), dtype=np.int8)
+    check(lib().dpg_mesh_hex(nx, ny, nz, jitter, seed, nodes.ctypes.data, par.ctypes.data))
+    return nodes, par
+
+
+def load_sine_modes(dim, ncomp, nmodes, kmax, seed):
+    L = capi.LoadDesc()
+    check(lib().dpg_load_sine_modes(dim, ncomp, nmodes, kmax, seed, ct.byref(L)))
+    return L
+
+
+def rng_uniform(seed, lo, hi, n):
+    out = np.zeros(n)
+    check(lib().dpg_rng_uniform(seed, lo, hi, out.ctypes.data, n))
+    return out
+
+
+def rng_normal(seed, n):
+    out = np.zeros(n)
+    check(lib().dpg_rng_normal(seed, out.ctypes.data, n))
+    return out
+
+
+class LocalAssembler:
+    """A plan: device basis/cubature tables of the given element types."""
+
+    def __init__(self, f, buckets, precision=capi.PREC_FP64, refine=0, workspace_bytes=0.0, max_batch_cells=0):
+        import torch  # device memory only
+
+        self._torch = torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("LocalAssembler needs a ROCm GPU")
+        self.form = f
+        self.buckets = list(buckets)
+        arr = (capi.BucketDesc * len(self.buckets))(*self.buckets)
+        opt = capi.Options(precision, refine, max_batch_cells, 0, float(workspace_bytes))
+        plan = ct.c_void_p()
+        check(lib().dpg_plan_create(ct.byref(f), arr, len(self.buckets), ct.byref(opt), ct.byref(plan)))
+        self._plan = plan
+        self.info = [sizes(f, b) for b in self.buckets]
+
+    def close(self):
+        if self._plan:
+            lib().dpg_plan_destroy(self._plan)
+            self._plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def workspace_bytes(self):
+        n = ct.c_size_t()
+        check(lib().dpg_plan_workspace(self._plan, ct.byref(n)))
+        return n.value
+
+    def _batch(self, nodes, parities, coefs, load):
+        t = self._torch
+        b = capi.Batch()
+        b.ncells = nodes.shape[0]
+        b.nodes = nodes.data_ptr()
+        b.parities = parities.data_ptr()
+        b.coefs = coefs.data_ptr() if coefs is not None else None
+        if load is not None:
+            b.load = load
+        assert nodes.is_cuda and nodes.dtype == t.float64 and nodes.is_contiguous()
+        assert parities.is_cuda and parities.dtype == t.int8 and parities.is_contiguous()
+        if coefs is not None:
+            assert coefs.is_cuda and coefs.dtype == t.float64 and coefs.is_contiguous()
+        return b
+
+    def local_stiffness_and_rhs(self, bucket, nodes, parities, coefs=None, load=None, want_G=False,
+                                want_schur=False, want_info=True, out=None, stream=None):
+        """K (C,M,M), rhs (C,M) [, G (C,N,N), schur (C,T,T), schur_rhs (C,T)] on the device.
+
+        `nodes`, `parities`, `coefs` are device tensors. Returns (outputs dict, #non-SPD cells)."""
+        t = self._torch
+        info = self.info[bucket]
+        C = nodes.shape[0]
+        dev = nodes.device
+        o = out if out is not None else {}
+        if "K" not in o:
+            o["K"] = t.empty((C, info.M, info.M), dtype=t.float64, device=dev)
+        if "rhs" not in o:
+            o["rhs"] = t.empty((C, info.M), dtype=t.float64, device=dev)
+        if want_G and "G" not in o:
+            o["G"] = t.empty((C, info.N, info.N), dtype=t.float64, device=dev)
+        if want_schur:
+            o.setdefault("schur", t.empty((C, info.T, info.T), dtype=t.float64, device=dev))
+            o.setdefault("schur_rhs", t.empty((C, info.T), dtype=t.float64, device=dev))
+        if want_info and "info" not in o:
+            o["info"] = t.zeros((C,), dtype=t.int32, device=dev)
+        outs = capi.Outputs()
+        for key in ("K", "rhs", "G", "schur", "schur_rhs", "info"):
+            setattr(outs, key, o[key].data_ptr() if key in o else None)
+        b = self._batch(nodes, parities, coefs, load)
+        s = stream.cuda_stream if stream is not None else t.cuda.current_stream().cuda_stream
+        nbad = check(lib().dpg_assemble(self._plan, bucket, ct.byref(b), ct.byref(outs), ct.c_void_p(s)))
+        return o, nbad
+
+    def energy_error(self, bucket, nodes, parities, u, coefs=None, load=None, stream=None):
+        """eta2[c] = r^T G^-1 r with r = l - B u (TSolution::rankLocalEnergyError reports sqrt)."""
+        t = self._torch
+        C = nodes.shape[0]
+        eta2 = t.empty((C,), dtype=t.float64, device=nodes.device)
+        assert u.is_cuda and u.dtype == t.float64 and u.is_contiguous()
+        b = self._batch(nodes, parities, coefs, load)
+        s = stream.cuda_stream if stream is not None else t.cuda.current_stream().cuda_stream
+        check(lib().dpg_energy_error(self._plan, bucket, ct.byref(b), ct.c_void_p(u.data_ptr()),
+                                     ct.c_void_p(eta2.data_ptr()), ct.c_void_p(s)))
+        return eta2

@@ -137,3 +137,10 @@ def graph_norm(f):
     rc = lib().oracle_graph_norm(ct.byref(f))
     assert rc == 0
     return f
+
+
+def form_like(f):
+    """A copy of a form descriptor for the oracle (term tables are plain data)."""
+    g = capi.FormDesc()
+    ct.pointer(g)[0] = f
+    return g

@@ -1,0 +1,143 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle on the same
+seeded inputs. FP64 tolerance: relative Frobenius error per cell <= 1e-12 for
+K and rhs (the bar of the plain FP64 form), looser only where stated."""
+import numpy as np
+import pytest
+
+from camellia2_amd import capi, dpg
+from oracle import oracle
+
+from helpers import cdr2d, poisson2d, poisson3d, rel_fro, stokes2d
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+TOL_K = 1e-12
+
+
+def dev(a):
+    if a is None:
+        return None
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
+
+
+def gpu_assemble(f, b, nodes, par, coefs, load, want_G=False, want_schur=False, **plan_kw):
+    A = dpg.LocalAssembler(f, [b], **plan_kw)
+    out, nbad = A.local_stiffness_and_rhs(0, dev(nodes), dev(par), dev(coefs), load, want_G=want_G,
+                                          want_schur=want_schur)
+    torch.cuda.synchronize()
+    res = {k: v.cpu().numpy() for k, v in out.items()}
+    A.close()
+    return res, nbad
+
+
+def check_against_oracle(case, want_G=False, want_schur=False, tol_schur=1e-10, **plan_kw):
+    f, b, nodes, par, coefs, load = case
+    g, nbad = gpu_assemble(f, b, nodes, par, coefs, load, want_G, want_schur, **plan_kw)
+    want = ["K", "rhs"] + (["G"] if want_G else []) + (["S", "g"] if want_schur else [])
+    o = oracle.assemble(oracle.form_like(f), b, nodes, par, coefs, load, want=want)
+    assert nbad == 0 and np.all(g["info"] == 0)
+    errs = {"K": rel_fro(g["K"], o["K"]), "rhs": rel_fro(g["rhs"], o["rhs"])}
+    if want_G:
+        errs["G"] = rel_fro(g["G"], o["G"])
+    if want_schur:
+        errs["schur"] = rel_fro(g["schur"], o["S"])
+        errs["schur_rhs"] = rel_fro(g["schur_rhs"], o["g"])
+    print("relative Frobenius errors:", {k: f"{v:.2e}" for k, v in errs.items()})
+    assert errs["K"] <= TOL_K and errs["rhs"] <= TOL_K
+    if want_G:
+        assert errs["G"] <= 1e-13
+    if want_schur:
+        assert errs["schur"] <= tol_schur and errs["schur_rhs"] <= tol_schur
+    assert np.array_equal(g["K"], np.transpose(g["K"], (0, 2, 1)))
+    return g, o
+
+
+def test_c1_poisson_2d_full_mesh():
+    """C1: 16x16 jittered quads, p=2 (H1 order 3), test order 4; K, rhs, G and the Schur complement."""
+    check_against_oracle(poisson2d(16, 16), want_G=True, want_schur=True)
+
+
+def test_c5_cdr_2d_per_cell_coefficients():
+    """C5 coefficients (eps log-uniform [1e-3,1], |beta| in [0.5,2], c in [0,1]) on an 8x8 patch."""
+    check_against_oracle(cdr2d(8, 8), want_schur=True)
+
+
+def test_c2_stokes_2d():
+    """C2 orders (p=3, traces 4, test 5) on a 4x4 patch."""
+    check_against_oracle(stokes2d(4, 4), want_schur=True)
+
+
+def test_c4_poisson_3d_hexes():
+    """C4 orders (p=3, test 5: N=666, M=502) on a 2x2x2 patch of jittered hexes."""
+    check_against_oracle(poisson3d(2, 2, 2), want_schur=True)
+
+
+@pytest.mark.parametrize("k,kt", [(2, 3), (2, 4), (4, 6)])
+def test_poisson_2d_other_orders(k, kt):
+    check_against_oracle(poisson2d(3, 3, k=k, kt=kt), want_G=True, want_schur=True)
+
+
+def test_batches_give_identical_results():
+    case = poisson2d(6, 5)
+    a, _ = gpu_assemble(*case, want_schur=True)
+    b, _ = gpu_assemble(*case, want_schur=True, max_batch_cells=7)
+    for key in ("K", "rhs", "schur", "schur_rhs"):
+        assert np.array_equal(a[key], b[key]), key
+
+
+def test_energy_error_matches_oracle():
+    f, b, nodes, par, coefs, load = cdr2d(4, 4)
+    s = dpg.sizes(f, b)
+    u = dpg.rng_normal(77, nodes.shape[0] * s.M).reshape(nodes.shape[0], s.M)
+    A = dpg.LocalAssembler(f, [b])
+    eta2 = A.energy_error(0, dev(nodes), dev(par), dev(u), dev(coefs), load).cpu().numpy()
+    ref = oracle.energy_error(oracle.form_like(f), b, nodes, par, u, coefs, load)
+    print("eta2 rel err", np.max(np.abs(eta2 - ref) / np.abs(ref)))
+    assert np.max(np.abs(eta2 - ref) / np.abs(ref)) < 1e-11
+
+
+def test_energy_error_vanishes_for_exact_linear_solution():
+    """u = a + b x + c y is in the trial space of a bilinear mesh and solves the
+    Poisson UW equations with f = 0, so its energy error is zero."""
+    from test_oracle import _nodal_trial_coefficients_linear
+
+    f, b, nodes, par, _, _ = poisson2d(4, 4, k=2, kt=3)
+    u = _nodal_trial_coefficients_linear(nodes, par, 2, (0.2, 1.1, -0.4))
+    A = dpg.LocalAssembler(f, [b])
+    eta2 = A.energy_error(0, dev(nodes), dev(par), dev(u), None, capi.LoadDesc()).cpu().numpy()
+    assert np.max(np.abs(eta2)) < 1e-20
+
+
+def test_non_spd_gram_reports_lapack_info():
+    """An IP without the tau terms leaves G singular in the tau block: info is
+    the 1-based column of the first tau dof (POTRF convention), nothing aborts."""
+    f, b, nodes, par, _, load = poisson2d(2, 2)
+    f.n_ipterms = 0
+    lt = f.ipterms[0]
+    lt.n = 1
+    lt.s[0].test_var, lt.s[0].test_op = 0, capi.OP_VALUE
+    lt.s[0].coef.scale, lt.s[0].coef.coef_slot, lt.s[0].coef.normal_comp = 1.0, -1, -1
+    f.n_ipterms = 1
+    g, nbad = gpu_assemble(f, b, nodes, par, None, load)
+    nv = 25  # Cn(4) on the quad
+    assert nbad == 4 and np.all(g["info"] == nv + 1)
+
+
+def test_c2_full_mesh_properties_and_sampled_parity():
+    """All 16,384 cells of C2: every cell SPD-factorable, K symmetric, and a
+    seeded sample of cells equal to the oracle."""
+    f, b, nodes, par, coefs, load = stokes2d(128, 128)
+    A = dpg.LocalAssembler(f, [b])
+    out, nbad = A.local_stiffness_and_rhs(0, dev(nodes), dev(par), None, load, want_schur=True)
+    torch.cuda.synchronize()
+    assert nbad == 0
+    K = out["K"]
+    assert torch.equal(K, K.transpose(1, 2))
+    assert torch.isfinite(out["schur"]).all()
+    sample = np.sort(np.random.default_rng(2).choice(nodes.shape[0], 12, replace=False))
+    o = oracle.assemble(oracle.form_like(f), b, nodes[sample], par[sample], None, load, want=("K", "rhs", "S"))
+    assert rel_fro(K[sample].cpu().numpy(), o["K"]) <= TOL_K
+    assert rel_fro(out["rhs"][sample].cpu().numpy(), o["rhs"]) <= TOL_K
+    assert rel_fro(out["schur"][sample].cpu().numpy(), o["S"]) <= 1e-10
