@@ -346,7 +346,8 @@ __global__ __launch_bounds__(256) void gemm_tn_f64(GemmArgs g) {
     }
     __syncthreads();
   }
-  // D[j][i] layout: i = lane%16 (C row), j = 4*(lane/16) + v (C column)
+  // v_mfma_f64_16x16x4 result layout: D[a][b] in lane b + 16*(a%4), element a/4,
+  // with a the A-operand (Y, C column j) and b the B-operand (X, C row i) index
 #pragma unroll
   for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
@@ -354,7 +355,7 @@ __global__ __launch_bounds__(256) void gemm_tn_f64(GemmArgs g) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         int i = i0 + wr * 32 + bi * 16 + (lane & 15);
-        int j = j0 + wc * 32 + bj * 16 + 4 * (lane >> 4) + v;
+        int j = j0 + wc * 32 + bj * 16 + (lane >> 4) + 4 * v;
         if (i < g.m && j < g.n && (!g.lower || i >= j)) {
           double* cp = C + (size_t)i + (size_t)j * g.ldc;
           double val = g.alpha * acc[bj][bi][v];

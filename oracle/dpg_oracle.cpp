@@ -663,6 +663,106 @@ void trsv_lower_trans(int n, const double* L, double* b) {
   }
 }
 
+// The same factored solve in 80-bit extended precision (x87 long double), used
+// by the tests to measure how far an FP64 evaluation of B^T G^-1 B is from the
+// exact value for given G, B, l (a property of the input, not of the code).
+template <class R>
+void factored_solve_t(int N, int M, const double* ip, const double* Bm, const double* l, double* K, double* rhs) {
+  vector<R> G((size_t)N * N), B((size_t)N * M), y(N), s(N);
+  for (size_t i = 0; i < G.size(); ++i) G[i] = ip[i];
+  for (size_t i = 0; i < B.size(); ++i) B[i] = Bm[i];
+  for (int i = 0; i < N; ++i) y[i] = l[i];
+  for (int i = 0; i < N; ++i) s[i] = R(1) / std::sqrt((R)G[(size_t)i * N + i]);
+  for (int j = 0; j < N; ++j)
+    for (int i = 0; i < N; ++i) G[(size_t)j * N + i] *= s[i] * s[j];
+  for (int j = 0; j < N; ++j) {
+    R* colj = &G[(size_t)j * N];
+    for (int k = 0; k < j; ++k) {
+      const R* colk = &G[(size_t)k * N];
+      R ljk = colk[j];
+      for (int i = j; i < N; ++i) colj[i] -= colk[i] * ljk;
+    }
+    R d = std::sqrt(colj[j]);
+    colj[j] = d;
+    for (int i = j + 1; i < N; ++i) colj[i] /= d;
+  }
+  for (int j = 0; j < N; ++j)
+    for (int i = j; i < N; ++i) G[(size_t)j * N + i] /= s[i];
+  auto solve = [&](R* b) {
+    for (int j = 0; j < N; ++j) {
+      R bj = b[j] / G[(size_t)j * N + j];
+      b[j] = bj;
+      for (int i = j + 1; i < N; ++i) b[i] -= G[(size_t)j * N + i] * bj;
+    }
+  };
+  for (int c = 0; c < M; ++c) solve(&B[(size_t)c * N]);
+  solve(y.data());
+  for (int i = 0; i < M; ++i) {
+    for (int j = 0; j < M; ++j) {
+      R acc = 0;
+      for (int k = 0; k < N; ++k) acc += B[(size_t)i * N + k] * B[(size_t)j * N + k];
+      K[(size_t)i * M + j] = (double)acc;
+    }
+    R acc = 0;
+    for (int k = 0; k < N; ++k) acc += B[(size_t)i * N + k] * y[k];
+    rhs[i] = (double)acc;
+  }
+}
+
+// LU-based static condensation, templated on the arithmetic
+template <class R>
+int schur_t(int M, int F, const double* K, const double* rhs, double* S, double* g) {
+  int T = M - F;
+  // LU with partial pivoting of D = K_FF (row-major copy)
+  vector<R> D((size_t)F * F), X((size_t)F * (T + 1));
+  for (int i = 0; i < F; ++i)
+    for (int j = 0; j < F; ++j) D[(size_t)i * F + j] = K[(size_t)i * M + j];
+  for (int i = 0; i < F; ++i) {
+    for (int j = 0; j < T; ++j) X[(size_t)i * (T + 1) + j] = K[(size_t)i * M + F + j];
+    X[(size_t)i * (T + 1) + T] = rhs[i];
+  }
+  int info = 0;
+  vector<int> piv(F);
+  for (int k = 0; k < F; ++k) {
+    int p = k;
+    R mx = std::fabs(D[(size_t)k * F + k]);
+    for (int i = k + 1; i < F; ++i)
+      if (std::fabs(D[(size_t)i * F + k]) > mx) { mx = std::fabs(D[(size_t)i * F + k]); p = i; }
+    piv[k] = p;
+    if (mx == 0.0) { if (!info) info = k + 1; continue; }
+    if (p != k) {
+      for (int j = 0; j < F; ++j) std::swap(D[(size_t)k * F + j], D[(size_t)p * F + j]);
+      for (int j = 0; j <= T; ++j) std::swap(X[(size_t)k * (T + 1) + j], X[(size_t)p * (T + 1) + j]);
+    }
+    R dkk = D[(size_t)k * F + k];
+    for (int i = k + 1; i < F; ++i) {
+      R m = D[(size_t)i * F + k] / dkk;
+      D[(size_t)i * F + k] = m;
+      for (int j = k + 1; j < F; ++j) D[(size_t)i * F + j] -= m * D[(size_t)k * F + j];
+      for (int j = 0; j <= T; ++j) X[(size_t)i * (T + 1) + j] -= m * X[(size_t)k * (T + 1) + j];
+    }
+  }
+  // back substitution U X = Y
+  for (int i = F - 1; i >= 0; --i) {
+    for (int j = 0; j <= T; ++j) {
+      R sacc = X[(size_t)i * (T + 1) + j];
+      for (int k = i + 1; k < F; ++k) sacc -= D[(size_t)i * F + k] * X[(size_t)k * (T + 1) + j];
+      X[(size_t)i * (T + 1) + j] = sacc / D[(size_t)i * F + i];
+    }
+  }
+  for (int a = 0; a < T; ++a) {
+    for (int b = 0; b < T; ++b) {
+      R sacc = K[(size_t)(F + a) * M + F + b];
+      for (int k = 0; k < F; ++k) sacc -= K[(size_t)k * M + F + a] * X[(size_t)k * (T + 1) + b];
+      S[(size_t)a * T + b] = (double)sacc;
+    }
+    R gacc = rhs[F + a];
+    for (int k = 0; k < F; ++k) gacc -= K[(size_t)k * M + F + a] * X[(size_t)k * (T + 1) + T];
+    g[a] = (double)gacc;
+  }
+  return info;
+}
+
 }  // namespace
 
 // =================================================================== C API
@@ -700,58 +800,22 @@ int oracle_factored_cholesky_solve(int N, int M, double* ip, double* Bm, double*
   return info;
 }
 
+// Extended-precision reference of factoredCholeskySolve; inputs are not modified.
+int oracle_factored_solve_extended(int N, int M, const double* ip, const double* Bm, const double* l, double* K,
+                                   double* rhs) {
+  factored_solve_t<long double>(N, M, ip, Bm, l, K, rhs);
+  return 0;
+}
+
 // Static condensation of one cell (CondensedDofInterpreter.cpp:1063-1192):
 // the first F dofs are fields, the remaining T = M - F are traces/fluxes.
 int oracle_schur(int M, int F, const double* K, const double* rhs, double* S, double* g) {
-  int T = M - F;
-  // LU with partial pivoting of D = K_FF (row-major copy)
-  vector<double> D((size_t)F * F), X((size_t)F * (T + 1));
-  for (int i = 0; i < F; ++i)
-    for (int j = 0; j < F; ++j) D[(size_t)i * F + j] = K[(size_t)i * M + j];
-  for (int i = 0; i < F; ++i) {
-    for (int j = 0; j < T; ++j) X[(size_t)i * (T + 1) + j] = K[(size_t)i * M + F + j];
-    X[(size_t)i * (T + 1) + T] = rhs[i];
-  }
-  int info = 0;
-  vector<int> piv(F);
-  for (int k = 0; k < F; ++k) {
-    int p = k;
-    double mx = std::fabs(D[(size_t)k * F + k]);
-    for (int i = k + 1; i < F; ++i)
-      if (std::fabs(D[(size_t)i * F + k]) > mx) { mx = std::fabs(D[(size_t)i * F + k]); p = i; }
-    piv[k] = p;
-    if (mx == 0.0) { if (!info) info = k + 1; continue; }
-    if (p != k) {
-      for (int j = 0; j < F; ++j) std::swap(D[(size_t)k * F + j], D[(size_t)p * F + j]);
-      for (int j = 0; j <= T; ++j) std::swap(X[(size_t)k * (T + 1) + j], X[(size_t)p * (T + 1) + j]);
-    }
-    double dkk = D[(size_t)k * F + k];
-    for (int i = k + 1; i < F; ++i) {
-      double m = D[(size_t)i * F + k] / dkk;
-      D[(size_t)i * F + k] = m;
-      for (int j = k + 1; j < F; ++j) D[(size_t)i * F + j] -= m * D[(size_t)k * F + j];
-      for (int j = 0; j <= T; ++j) X[(size_t)i * (T + 1) + j] -= m * X[(size_t)k * (T + 1) + j];
-    }
-  }
-  // back substitution U X = Y
-  for (int i = F - 1; i >= 0; --i) {
-    for (int j = 0; j <= T; ++j) {
-      double sacc = X[(size_t)i * (T + 1) + j];
-      for (int k = i + 1; k < F; ++k) sacc -= D[(size_t)i * F + k] * X[(size_t)k * (T + 1) + j];
-      X[(size_t)i * (T + 1) + j] = sacc / D[(size_t)i * F + i];
-    }
-  }
-  for (int a = 0; a < T; ++a) {
-    for (int b = 0; b < T; ++b) {
-      double sacc = K[(size_t)(F + a) * M + F + b];
-      for (int k = 0; k < F; ++k) sacc -= K[(size_t)k * M + F + a] * X[(size_t)k * (T + 1) + b];
-      S[(size_t)a * T + b] = sacc;
-    }
-    double gacc = rhs[F + a];
-    for (int k = 0; k < F; ++k) gacc -= K[(size_t)k * M + F + a] * X[(size_t)k * (T + 1) + T];
-    g[a] = gacc;
-  }
-  return info;
+  return schur_t<double>(M, F, K, rhs, S, g);
+}
+
+// the same in 80-bit extended precision (test reference)
+int oracle_schur_extended(int M, int F, const double* K, const double* rhs, double* S, double* g) {
+  return schur_t<long double>(M, F, K, rhs, S, g);
 }
 
 int oracle_sizes(const dpg_form_desc* f, const dpg_bucket_desc* b, int32_t* out /*N,M,F,Q1,Qs1,QG1*/) {

@@ -28,6 +28,8 @@ def lib():
         vp = ct.c_void_p
         L.oracle_factored_cholesky_solve.argtypes = [ct.c_int, ct.c_int, vp, vp, vp, vp, vp]
         L.oracle_schur.argtypes = [ct.c_int, ct.c_int, vp, vp, vp, vp]
+        L.oracle_schur_extended.argtypes = [ct.c_int, ct.c_int, vp, vp, vp, vp]
+        L.oracle_factored_solve_extended.argtypes = [ct.c_int, ct.c_int, vp, vp, vp, vp, vp]
         L.oracle_sizes.argtypes = [P(capi.FormDesc), P(capi.BucketDesc), vp]
         L.oracle_gauss_rule.argtypes = [ct.c_int, vp, vp]
         L.oracle_gll_nodes.argtypes = [ct.c_int, vp]
@@ -144,3 +146,42 @@ def form_like(f):
     g = capi.FormDesc()
     ct.pointer(g)[0] = f
     return g
+
+
+def extended_K(G, B, l):
+    """K = B^T G^-1 B and rhs = B^T G^-1 l in 80-bit arithmetic from FP64 G (N,N), B (M,N), l (N)."""
+    N = G.shape[0]
+    M = B.shape[0]
+    G = np.ascontiguousarray(G, dtype=np.float64)  # symmetric: row- and column-major agree
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    l = np.ascontiguousarray(l, dtype=np.float64)
+    K = np.zeros((M, M))
+    rhs = np.zeros(M)
+    lib().oracle_factored_solve_extended(N, M, _p(G), _p(B), _p(l), _p(K), _p(rhs))
+    return K, rhs
+
+
+def extended_schur(K, rhs, F):
+    M = K.shape[0]
+    T = M - F
+    S = np.zeros((T, T))
+    g = np.zeros(T)
+    K = np.ascontiguousarray(K, dtype=np.float64)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    lib().oracle_schur_extended(M, F, _p(K), _p(rhs), _p(S), _p(g))
+    return S, g
+
+
+def with_extended(form, bucket, nodes, parities, coefs=None, load=None, schur=False):
+    """Oracle outputs plus extended-precision K, rhs (and S, g) computed from the
+    oracle's own FP64 G, B, l. The distance of the FP64 outputs from these is
+    the accuracy the reference algorithm reaches in FP64 on these inputs."""
+    want = ("B", "G", "l", "K", "rhs") + (("S", "g") if schur else ())
+    o = assemble(form, bucket, nodes, parities, coefs, load, want=want)
+    F = sizes(form, bucket)["F"]
+    x = {k: np.zeros_like(o[k]) for k in ("K", "rhs") + (("S", "g") if schur else ())}
+    for c in range(nodes.shape[0]):
+        x["K"][c], x["rhs"][c] = extended_K(o["G"][c], o["B"][c], o["l"][c])
+        if schur:
+            x["S"][c], x["g"][c] = extended_schur(x["K"][c], x["rhs"][c], F)
+    return o, x

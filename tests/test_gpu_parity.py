@@ -32,26 +32,30 @@ def gpu_assemble(f, b, nodes, par, coefs, load, want_G=False, want_schur=False, 
     return res, nbad
 
 
-def check_against_oracle(case, want_G=False, want_schur=False, tol_schur=1e-10, **plan_kw):
+def check_against_oracle(case, want_G=False, want_schur=False, **plan_kw):
+    """GPU vs oracle on the same inputs. The reference algorithm's own FP64
+    accuracy on these inputs (`floor`: oracle vs the 80-bit evaluation from the
+    oracle's G, B, l) can exceed 1e-12 when G is ill conditioned, so the bound
+    is max(1e-12, 4 * floor) on the GPU's distance from the 80-bit result."""
     f, b, nodes, par, coefs, load = case
     g, nbad = gpu_assemble(f, b, nodes, par, coefs, load, want_G, want_schur, **plan_kw)
-    want = ["K", "rhs"] + (["G"] if want_G else []) + (["S", "g"] if want_schur else [])
-    o = oracle.assemble(oracle.form_like(f), b, nodes, par, coefs, load, want=want)
+    o, x = oracle.with_extended(oracle.form_like(f), b, nodes, par, coefs, load, schur=want_schur)
     assert nbad == 0 and np.all(g["info"] == 0)
-    errs = {"K": rel_fro(g["K"], o["K"]), "rhs": rel_fro(g["rhs"], o["rhs"])}
+    pairs = [("K", "K"), ("rhs", "rhs")] + ([("schur", "S"), ("schur_rhs", "g")] if want_schur else [])
+    report = {}
+    for gk, ok in pairs:
+        floor = rel_fro(o[ok], x[ok])
+        err_x = rel_fro(g[gk], x[ok])
+        err_o = rel_fro(g[gk], o[ok])
+        report[gk] = (err_o, err_x, floor)
+        print(f"{gk:10s} gpu-vs-oracle {err_o:.2e}  gpu-vs-80bit {err_x:.2e}  oracle-vs-80bit {floor:.2e}")
+        assert err_x <= max(TOL_K, 4.0 * floor), gk
     if want_G:
-        errs["G"] = rel_fro(g["G"], o["G"])
-    if want_schur:
-        errs["schur"] = rel_fro(g["schur"], o["S"])
-        errs["schur_rhs"] = rel_fro(g["schur_rhs"], o["g"])
-    print("relative Frobenius errors:", {k: f"{v:.2e}" for k, v in errs.items()})
-    assert errs["K"] <= TOL_K and errs["rhs"] <= TOL_K
-    if want_G:
-        assert errs["G"] <= 1e-13
-    if want_schur:
-        assert errs["schur"] <= tol_schur and errs["schur_rhs"] <= tol_schur
+        eg = rel_fro(g["G"], o["G"])
+        print(f"G          gpu-vs-oracle {eg:.2e}")
+        assert eg <= 1e-13
     assert np.array_equal(g["K"], np.transpose(g["K"], (0, 2, 1)))
-    return g, o
+    return g, o, report
 
 
 def test_c1_poisson_2d_full_mesh():
@@ -137,7 +141,9 @@ def test_c2_full_mesh_properties_and_sampled_parity():
     assert torch.equal(K, K.transpose(1, 2))
     assert torch.isfinite(out["schur"]).all()
     sample = np.sort(np.random.default_rng(2).choice(nodes.shape[0], 12, replace=False))
-    o = oracle.assemble(oracle.form_like(f), b, nodes[sample], par[sample], None, load, want=("K", "rhs", "S"))
-    assert rel_fro(K[sample].cpu().numpy(), o["K"]) <= TOL_K
-    assert rel_fro(out["rhs"][sample].cpu().numpy(), o["rhs"]) <= TOL_K
-    assert rel_fro(out["schur"][sample].cpu().numpy(), o["S"]) <= 1e-10
+    o, x = oracle.with_extended(oracle.form_like(f), b, nodes[sample], par[sample], None, load, schur=True)
+    for gk, ok in (("K", "K"), ("rhs", "rhs"), ("schur", "S")):
+        floor = rel_fro(o[ok], x[ok])
+        err = rel_fro(out[gk][sample].cpu().numpy(), x[ok])
+        print(f"{gk}: gpu-vs-80bit {err:.2e} oracle-vs-80bit {floor:.2e}")
+        assert err <= max(TOL_K, 4.0 * floor)
