@@ -1,0 +1,185 @@
+"""Throughput of batched DPG local assembly on one MI355X.
+
+One step = local assembly of every cell of the workload: K, rhs and the
+static-condensation Schur complement (plus G for C1), inputs resident in HBM.
+
+Default workload (the configuration the metric's target is quoted on):
+C4, 3D Poisson ultraweak on 32^3 jittered hexes (+-5% h), p=3 (H1 order 4),
+test order 5 (N=666, M=502), 16-mode sine load, FP64.
+
+Prints one JSON line (see the field list in `main`). The CPU baseline is the
+oracle (the FP64 C++ restatement of the reference algorithm, OpenMP over
+cells) timed on this host on a bounded sample of the same cells; the same
+sample gives the error of the GPU result against the FP64 oracle.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2.5e15, "hbm": 8.0e12}
+
+CONFIGS = {
+    "c4": dict(name="poisson3d_hex32_p3_test5", form=("poisson", 3), topo="hex", n=(32, 32, 32), jitter=0.05,
+               k=4, kt=5, modes=16, seed=0xC0FFEE + 4),
+    "c2": dict(name="stokes2d_quad128_p3_test5", form=("stokes", 2), topo="quad", n=(128, 128), jitter=0.10,
+               k=4, kt=5, modes=8, seed=0xC0FFEE + 2),
+    "c1": dict(name="poisson2d_quad16_p2_test4", form=("poisson", 2), topo="quad", n=(16, 16), jitter=0.10,
+               k=3, kt=4, modes=8, seed=0xC0FFEE + 1),
+}
+
+
+def build_inputs(cfg):
+    from camellia2_amd import capi, dpg
+
+    f = dpg.form(cfg["form"][0], cfg["form"][1])
+    if cfg["topo"] == "hex":
+        b = dpg.bucket(capi.HEX, cfg["k"], cfg["kt"])
+        nodes, par = dpg.mesh_hex(*cfg["n"], cfg["jitter"], cfg["seed"])
+        load = dpg.load_sine_modes(3, 1, cfg["modes"], 6, cfg["seed"] + 100)
+    else:
+        b = dpg.bucket(capi.QUAD, cfg["k"], cfg["kt"])
+        nodes, par = dpg.mesh_quad(*cfg["n"], cfg["jitter"], cfg["seed"])
+        ncomp = 2 if cfg["form"][0] == "stokes" else 1
+        load = dpg.load_sine_modes(2, ncomp, cfg["modes"], 6, cfg["seed"] + 100)
+    return f, b, nodes, par, load
+
+
+def cpu_baseline(f, b, nodes, par, load, gpu_K, target_s=12.0):
+    """Oracle on a bounded sample of the workload's cells (all host threads),
+    plus the GPU-vs-oracle error on that sample."""
+    from oracle import oracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    ff = oracle.form_like(f)
+    # calibrate on one cell per thread, then size the sample to ~target_s
+    n0 = max(1, threads)
+    t0 = time.perf_counter()
+    oracle.assemble(ff, b, nodes[:n0], par[:n0], None, load, want=("K", "rhs", "S", "g"), nthreads=threads)
+    dt0 = time.perf_counter() - t0
+    per_cell = dt0 / n0
+    n = int(min(nodes.shape[0], max(n0, target_s / max(per_cell, 1e-9))))
+    n = max(n0, (n // threads) * threads)
+    t0 = time.perf_counter()
+    o = oracle.assemble(ff, b, nodes[:n], par[:n], None, load, want=("K", "rhs", "S", "g"), nthreads=threads)
+    dt = time.perf_counter() - t0
+    Kg = gpu_K[:n].cpu().numpy()
+    err = float(np.max(np.linalg.norm((Kg - o["K"]).reshape(n, -1), axis=1) /
+                       np.linalg.norm(o["K"].reshape(n, -1), axis=1)))
+    return {"value": n / dt, "unit": "cells/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} cells of the same mesh, oracle K+rhs+Schur, {dt:.1f} s wall, {threads} OpenMP threads"}, err
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c4", choices=sorted(CONFIGS))
+    ap.add_argument("--precision", default="fp64", choices=["fp64"])
+    ap.add_argument("--workspace-gib", type=float, default=64.0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-schur", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (local assembly shards by "
+              "cells with no exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from camellia2_amd import dpg
+
+    cfg = CONFIGS[args.config]
+    f, b, nodes, par, load = build_inputs(cfg)
+    info = dpg.sizes(f, b)
+    C = nodes.shape[0]
+    dev = torch.device("cuda:0")
+    d_nodes = torch.from_numpy(nodes).to(dev)
+    d_par = torch.from_numpy(par).to(dev)
+    ws = args.workspace_gib * 2 ** 30
+    A = dpg.LocalAssembler(f, [b], workspace_bytes=ws)
+    out = {
+        "K": torch.empty((C, info.M, info.M), dtype=torch.float64, device=dev),
+        "rhs": torch.empty((C, info.M), dtype=torch.float64, device=dev),
+    }
+    want_schur = not args.no_schur
+    if want_schur:
+        out["schur"] = torch.empty((C, info.T, info.T), dtype=torch.float64, device=dev)
+        out["schur_rhs"] = torch.empty((C, info.T), dtype=torch.float64, device=dev)
+
+    def step(asm):
+        return asm.local_stiffness_and_rhs(0, d_nodes, d_par, None, load, want_schur=want_schur, want_info=False,
+                                           out=out)
+
+    for _ in range(args.warmup):
+        step(A)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(A)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_per_step = dt / args.steps * 1e3
+    cells_per_s = C * args.steps / dt
+    A.close()
+
+    # one more step on a plan that times every launch with HIP events on the
+    # launch stream (kept out of the timed steps above)
+    Ap = dpg.LocalAssembler(f, [b], workspace_bytes=ws, profile=True)
+    step(Ap)
+    torch.cuda.synchronize()
+    tm, _ = Ap.timings(reset=True)
+    Ap.close()
+    stages = {k: {"ms": round(v["ms"], 3), "launches": v["launches"], "gflop": round(v["flops"] / 1e9, 3),
+                  "gbyte": round(v["bytes"] / 1e9, 3)} for k, v in tm.items()}
+    dom = max(["gram", "update"], key=lambda k: tm.get(k, {"ms": 0.0})["ms"])
+    d = tm[dom]
+    achieved = d["flops"] / (d["ms"] * 1e-3)
+    roofline = {"bound": "mfma", "kernel": {"gram": "gemm_gram_f64", "update": "gemm_update_f64"}[dom],
+                "achieved": round(achieved / 1e12, 3), "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK["fp64"], 4), "traffic": None,
+                "avg_launch_ms": round(d["ms"] / d["launches"], 4),
+                "flop_per_launch": d["flops"] / d["launches"]}
+    traffic_file = os.path.join(ROOT, "profiles", f"traffic_{args.config}_{args.precision}.json")
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as fh:
+            tr = json.load(fh)
+        roofline["traffic"] = tr.get(roofline["kernel"])
+
+    result = {
+        "metric": "cells/s local DPG assembly, mixed-prec vs FP64 GPU form at ≤1e-10 rel err",
+        "value": round(cells_per_s, 2),
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded jittered mesh and sine-mode load)",
+        "config": {"workload": cfg["name"], "cells": C, "N": info.N, "M": info.M, "T": info.T,
+                   "outputs": "K,rhs" + (",schur,schur_rhs" if want_schur else ""),
+                   "precision": args.precision},
+        "gflops": round(info.flops_per_cell * cells_per_s / 1e9, 1),
+        "algorithmic_flop_per_cell": info.flops_per_cell,
+        "roofline": roofline,
+        "stages": stages,
+    }
+    if not args.no_cpu:
+        cpu, err = cpu_baseline(f, b, nodes, par, load, out["K"])
+        result["cpu_baseline"] = cpu
+        result["rel_err_K_vs_fp64_oracle"] = err
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

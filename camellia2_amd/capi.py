@@ -75,7 +75,16 @@ class BucketInfo(ct.Structure):
 
 class Options(ct.Structure):
     _fields_ = [("precision", ct.c_int32), ("refine", ct.c_int32), ("max_batch_cells", ct.c_int32),
-                ("pad", ct.c_int32), ("workspace_bytes", ct.c_double)]
+                ("profile", ct.c_int32), ("workspace_bytes", ct.c_double)]
+
+
+STAGES = ["point", "operand", "gram", "bmat", "scale", "panel", "update", "extract"]
+NSTAGES = len(STAGES)
+
+
+class Timings(ct.Structure):
+    _fields_ = [("ms", ct.c_double * NSTAGES), ("flops", ct.c_double * NSTAGES), ("bytes", ct.c_double * NSTAGES),
+                ("launches", ct.c_int64 * NSTAGES), ("cells", ct.c_int64)]
 
 
 class Batch(ct.Structure):
@@ -96,7 +105,7 @@ EXPORTS = [
     "dpg_version", "dpg_last_error", "dpg_form_poisson_uw", "dpg_form_cdr_uw",
     "dpg_form_stokes_vgp_uw", "dpg_form_graph_norm", "dpg_mesh_quad", "dpg_mesh_hex",
     "dpg_load_sine_modes", "dpg_rng_uniform", "dpg_rng_normal", "dpg_bucket_sizes",
-    "dpg_plan_create", "dpg_plan_destroy", "dpg_plan_workspace", "dpg_assemble",
+    "dpg_plan_create", "dpg_plan_destroy", "dpg_plan_workspace", "dpg_plan_timings", "dpg_assemble",
     "dpg_energy_error",
 ]
 
@@ -127,6 +136,7 @@ def lib():
         L.dpg_plan_create.argtypes = [P(FormDesc), P(BucketDesc), ct.c_int, P(Options), P(ct.c_void_p)]
         L.dpg_plan_destroy.argtypes = [ct.c_void_p]
         L.dpg_plan_workspace.argtypes = [ct.c_void_p, P(ct.c_size_t)]
+        L.dpg_plan_timings.argtypes = [ct.c_void_p, P(Timings), ct.c_int]
         L.dpg_assemble.argtypes = [ct.c_void_p, ct.c_int, P(Batch), P(Outputs), ct.c_void_p]
         L.dpg_energy_error.argtypes = [ct.c_void_p, ct.c_int, P(Batch), ct.c_void_p, ct.c_void_p,
                                        ct.c_void_p]

@@ -278,7 +278,7 @@ __global__ __launch_bounds__(256) void rhs_kernel(const double* __restrict__ Wrh
 // ------------------------------------------------------------------- GEMM
 // C(i,j) = alpha * sum_k X(k,i) Y(k,j) + beta * C(i,j), X(k,i) = X[k*ldx + i],
 // C col-major (C[i + j*ldc]); 64x64 tiles, 4 waves, v_mfma_f64_16x16x4.
-constexpr int GT = 64, GKC = 16, GLD = GT + 4;
+constexpr int GT = 64, GKC = 16, GLD = GT + 4;  // 64x64 tiles, k-chunks of 16
 
 __device__ inline int xcd_remap(int b, int nb) {
   // consecutive logical tiles on one XCD (blocks are dealt round-robin to 8 XCDs)
@@ -299,7 +299,7 @@ __device__ inline void tile_coords(int tile, int lower, int tiles_m, int& ti, in
   }
 }
 
-__global__ __launch_bounds__(256) void gemm_tn_f64(GemmArgs g) {
+__device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
   int b = xcd_remap(blockIdx.x, gridDim.x);
   int cell = b / g.ntiles, tile = b % g.ntiles;
   if (cell >= g.ncells) return;
@@ -317,19 +317,28 @@ __global__ __launch_bounds__(256) void gemm_tn_f64(GemmArgs g) {
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int bb = 0; bb < 2; ++bb) acc[a][bb] = d4{0.0, 0.0, 0.0, 0.0};
-  for (int k0 = 0; k0 < g.k; k0 += GKC) {
+  // each thread stages 4 elements of X and of Y per k-chunk; the next chunk is
+  // fetched into registers while the current one is multiplied from LDS
+  double xr[4], yr[4];
+  auto fetch = [&](int k0) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       int e = t + 256 * r, kk = e >> 6, ii = e & 63, kg = k0 + kk;
-      double xv = 0.0, yv = 0.0;
-      if (kg < g.k) {
-        if (i0 + ii < g.m) xv = X[(size_t)kg * g.ldx + i0 + ii];
-        if (j0 + ii < g.n) yv = Y[(size_t)kg * g.ldy + j0 + ii];
-      }
-      Xs[kk][ii] = xv;
-      Ys[kk][ii] = yv;
+      xr[r] = (kg < g.k && i0 + ii < g.m) ? X[(size_t)kg * g.ldx + i0 + ii] : 0.0;
+      yr[r] = (kg < g.k && j0 + ii < g.n) ? Y[(size_t)kg * g.ldy + j0 + ii] : 0.0;
+    }
+  };
+  fetch(0);
+  for (int k0 = 0; k0 < g.k; k0 += GKC) {
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int e = t + 256 * r;
+      Xs[e >> 6][e & 63] = xr[r];
+      Ys[e >> 6][e & 63] = yr[r];
     }
     __syncthreads();
+    if (k0 + GKC < g.k) fetch(k0 + GKC);
 #pragma unroll
     for (int k4 = 0; k4 < GKC; k4 += 4) {
       const int kr = k4 + (lane >> 4), cc = lane & 15;
@@ -344,7 +353,6 @@ __global__ __launch_bounds__(256) void gemm_tn_f64(GemmArgs g) {
         for (int bi = 0; bi < 2; ++bi)
           acc[bj][bi] = __builtin_amdgcn_mfma_f64_16x16x4f64(yb[bj], xa[bi], acc[bj][bi], 0, 0, 0);
     }
-    __syncthreads();
   }
   // v_mfma_f64_16x16x4 result layout: D[a][b] in lane b + 16*(a%4), element a/4,
   // with a the A-operand (Y, C column j) and b the B-operand (X, C row i) index
@@ -365,16 +373,23 @@ __global__ __launch_bounds__(256) void gemm_tn_f64(GemmArgs g) {
       }
 }
 
+// one kernel symbol per role so that profiles separate them
+__global__ __launch_bounds__(256) void gemm_gram_f64(GemmArgs g) { gemm_tn_f64_body(g); }
+__global__ __launch_bounds__(256) void gemm_bmat_f64(GemmArgs g) { gemm_tn_f64_body(g); }
+__global__ __launch_bounds__(256) void gemm_update_f64(GemmArgs g) { gemm_tn_f64_body(g); }
+
 // ------------------------------------------------------------ panel kernel
-// Columns [k0, k0+kb) (kb <= 64) of the lower-stored matrix: factor the
-// diagonal block in LDS (POTRF), then X <- X L^-T for the rows below.
-constexpr int PNB = 64;
+// Columns [k0, k0+kb) (kb <= 64) of one cell's lower-stored matrix per
+// workgroup: POTRF of the 64x64 diagonal block in LDS, its triangular inverse
+// in LDS, then X <- X L^-T for the rows below in 64-row tiles on MFMA.
+constexpr int PNB = 64, PLD = PNB + 4;
 
 __global__ __launch_bounds__(256) void panel_f64(double* __restrict__ A, int64_t lda, int64_t sA, int D, int k0,
                                                  int kb, int32_t* __restrict__ info, int info_base) {
-  __shared__ double L[PNB][PNB + 1];
+  __shared__ double L[PNB][PLD];   // factor, later the row tile X (k-major: X[m][r])
+  __shared__ double Yt[PNB][PLD];  // Yt[m][j] = Linv[j][m]
   __shared__ int fail;
-  const int c = blockIdx.y, chunk = blockIdx.x, t = threadIdx.x;
+  const int c = blockIdx.x, t = threadIdx.x;
   double* Ac = A + (size_t)c * sA;
   if (t == 0) fail = -1;
   for (int e = t; e < PNB * PNB; e += 256) {
@@ -385,6 +400,7 @@ __global__ __launch_bounds__(256) void panel_f64(double* __restrict__ A, int64_t
     L[r][cc] = v;
   }
   __syncthreads();
+  // right-looking Cholesky of the diagonal block
   for (int j = 0; j < PNB; ++j) {
     if (t == 0) {
       double d = L[j][j];
@@ -405,28 +421,63 @@ __global__ __launch_bounds__(256) void panel_f64(double* __restrict__ A, int64_t
     }
     __syncthreads();
   }
-  if (chunk == 0) {
-    for (int e = t; e < kb * kb; e += 256) {
-      int r = e % kb, cc = e / kb;
-      if (r >= cc) Ac[(size_t)(k0 + r) + (size_t)(k0 + cc) * lda] = L[r][cc];
+  for (int e = t; e < kb * kb; e += 256) {
+    int r = e % kb, cc = e / kb;
+    if (r >= cc) Ac[(size_t)(k0 + r) + (size_t)(k0 + cc) * lda] = L[r][cc];
+  }
+  if (t == 0 && fail >= 0 && fail < kb && info) atomicCAS(&info[c], 0, info_base + k0 + fail + 1);
+  // triangular inverse: thread j forms column j of Linv by forward substitution
+  if (t < PNB) {
+    const int j = t;
+    for (int i = 0; i < j; ++i) Yt[j][i] = 0.0;
+    Yt[j][j] = 1.0 / L[j][j];
+    for (int i = j + 1; i < PNB; ++i) {
+      double sacc = 0.0;
+      for (int m = j; m < i; ++m) sacc = fma(L[i][m], Yt[j][m], sacc);
+      Yt[j][i] = -sacc / L[i][i];
     }
-    if (t == 0 && fail >= 0 && fail < kb && info) atomicCAS(&info[c], 0, info_base + k0 + fail + 1);
   }
-  const int row = k0 + kb + chunk * 256 + t;
-  if (row >= D) return;
-  double x[PNB];
+  __syncthreads();
+  // Yt[a][b] = Linv[b][a], so  X_new(r, j) = sum_m X(r, m) Linv(j, m) = sum_m X[m][r] Yt[m][j]
+  // with the row tile stored k-major in L (L[m][r] = X(r, m))
+  const int lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
+  for (int r0 = k0 + kb; r0 < D; r0 += PNB) {
+    __syncthreads();
+    for (int e = t; e < PNB * PNB; e += 256) {
+      int r = e % PNB, m = e / PNB;
+      L[m][r] = (m < kb && r0 + r < D) ? Ac[(size_t)(r0 + r) + (size_t)(k0 + m) * lda] : 0.0;
+    }
+    __syncthreads();
+    d4 acc[2][2];
 #pragma unroll
-  for (int j = 0; j < PNB; ++j) x[j] = (j < kb) ? Ac[(size_t)row + (size_t)(k0 + j) * lda] : 0.0;
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-  for (int j = 0; j < PNB; ++j) {
-    double s = x[j];
+      for (int bb = 0; bb < 2; ++bb) acc[a][bb] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int k4 = 0; k4 < PNB; k4 += 4) {
+      const int kr = k4 + (lane >> 4), cc = lane & 15;
+      double xa[2], yb[2];
+      xa[0] = L[kr][wr * 32 + cc];
+      xa[1] = L[kr][wr * 32 + 16 + cc];
+      yb[0] = Yt[kr][wc * 32 + cc];
+      yb[1] = Yt[kr][wc * 32 + 16 + cc];
 #pragma unroll
-    for (int m = 0; m < j; ++m) s = fma(-x[m], L[j][m], s);
-    x[j] = s / L[j][j];
+      for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+          acc[bj][bi] = __builtin_amdgcn_mfma_f64_16x16x4f64(yb[bj], xa[bi], acc[bj][bi], 0, 0, 0);
+    }
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          int r = r0 + wr * 32 + bi * 16 + (lane & 15);
+          int j = wc * 32 + bj * 16 + (lane >> 4) + 4 * v;
+          if (r < D && j < kb) Ac[(size_t)r + (size_t)(k0 + j) * lda] = acc[bj][bi][v];
+        }
   }
-#pragma unroll
-  for (int j = 0; j < PNB; ++j)
-    if (j < kb) Ac[(size_t)row + (size_t)(k0 + j) * lda] = x[j];
 }
 
 // ---------------------------------------------------------- small kernels
@@ -562,22 +613,23 @@ void launch_rhs(int dim, const double* Wrhs, int nvar, const double* R, const in
   check_launch("rhs_kernel");
 }
 
-void launch_gemm(GemmArgs g, hipStream_t st) {
+void launch_gemm(GemmArgs g, int role, hipStream_t st) {
   if (g.m <= 0 || g.n <= 0 || g.ncells <= 0) return;
   g.tiles_m = (g.m + GT - 1) / GT;
   int tiles_n = (g.n + GT - 1) / GT;
   g.ntiles = g.lower ? g.tiles_m * (g.tiles_m + 1) / 2 : g.tiles_m * tiles_n;
   int64_t nblocks = (int64_t)g.ntiles * g.ncells;
   if (nblocks > 0x7fffffff) throw DeviceError("gemm grid too large");
-  gemm_tn_f64<<<dim3((unsigned)nblocks), 256, 0, st>>>(g);
+  dim3 grid((unsigned)nblocks);
+  if (role == kGemmGram) gemm_gram_f64<<<grid, 256, 0, st>>>(g);
+  else if (role == kGemmBmat) gemm_bmat_f64<<<grid, 256, 0, st>>>(g);
+  else gemm_update_f64<<<grid, 256, 0, st>>>(g);
   check_launch("gemm_tn_f64");
 }
 
 void launch_panel(double* A, int64_t lda, int64_t sA, int D, int k0, int kb, int ncells, int32_t* info, int info_base,
                   hipStream_t st) {
-  int rows = D - k0 - kb;
-  int chunks = rows > 0 ? (rows + 255) / 256 : 1;
-  panel_f64<<<dim3(chunks, ncells), 256, 0, st>>>(A, lda, sA, D, k0, kb, info, info_base);
+  panel_f64<<<dim3(ncells), 256, 0, st>>>(A, lda, sA, D, k0, kb, info, info_base);
   check_launch("panel_f64");
 }
 

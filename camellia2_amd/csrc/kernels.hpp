@@ -36,7 +36,8 @@ void launch_operand(int dim, const double* W, int ng, int nvar, const double* R,
                     int ncells, double* out, int64_t sc, int64_t sb, int64_t sp, hipStream_t st);
 void launch_rhs(int dim, const double* Wrhs, int nvar, const double* R, const int32_t* vod, int N, int Q, int ncells,
                 double* A, int64_t lda, int64_t sA, int row_l, hipStream_t st);
-void launch_gemm(GemmArgs g, hipStream_t st);
+enum { kGemmGram = 0, kGemmBmat = 1, kGemmUpdate = 2 };
+void launch_gemm(GemmArgs g, int role, hipStream_t st);
 void launch_panel(double* A, int64_t lda, int64_t sA, int D, int k0, int kb, int ncells, int32_t* info, int info_base,
                   hipStream_t st);
 void launch_zero_trailing(double* A, int64_t lda, int64_t sA, int r0, int D, int ncells, hipStream_t st);

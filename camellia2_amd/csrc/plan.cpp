@@ -120,6 +120,69 @@ void validate_form(const dpg_form_desc& f) {
 
 }  // namespace
 
+// Per-launch kernel timing with HIP events on the call's stream (only when
+// the plan was created with options.profile = 1).
+class Profiler {
+ public:
+  bool on = false;
+  ~Profiler() {
+    for (auto e : pool_) (void)hipEventDestroy(e);
+    for (auto& r : pending_) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+  }
+  hipEvent_t start(hipStream_t st) {
+    if (!on) return nullptr;
+    hipEvent_t e = get();
+    HIP_CHECK(hipEventRecord(e, st));
+    return e;
+  }
+  void stop(hipEvent_t a, hipStream_t st, int stage, double flops, double bytes) {
+    if (!on) return;
+    hipEvent_t b = get();
+    HIP_CHECK(hipEventRecord(b, st));
+    pending_.push_back({stage, a, b, flops, bytes});
+  }
+  // waits for the stream and folds the pending launches into the totals
+  void collect(hipStream_t st, int64_t cells) {
+    if (!on) return;
+    HIP_CHECK(hipStreamSynchronize(st));
+    for (auto& r : pending_) {
+      float ms = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
+      acc.ms[r.stage] += ms;
+      acc.flops[r.stage] += r.flops;
+      acc.bytes[r.stage] += r.bytes;
+      acc.launches[r.stage] += 1;
+      pool_.push_back(r.a);
+      pool_.push_back(r.b);
+    }
+    pending_.clear();
+    acc.cells += cells;
+  }
+  dpg_timings acc{};
+
+ private:
+  struct Rec {
+    int stage;
+    hipEvent_t a, b;
+    double flops, bytes;
+  };
+  hipEvent_t get() {
+    if (!pool_.empty()) {
+      hipEvent_t e = pool_.back();
+      pool_.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIP_CHECK(hipEventCreate(&e));
+    return e;
+  }
+  std::vector<hipEvent_t> pool_;
+  std::vector<Rec> pending_;
+};
+
 struct BucketPlan {
   dpg_bucket_desc desc{};
   ElementLayout L;
@@ -134,6 +197,7 @@ struct BucketPlan {
   int64_t nA = 0, nAt = 0, nTv = 0, nTs = 0, nWip = 0, nWvol = 0, nWrhs = 0, nWside = 0, nS = 0;
   int64_t per_cell_bytes = 0;
   double flops = 0.0;
+  double gram_flops = 0.0;  // sum over IP terms of QG * n(n+1), n = dofs the term touches
 };
 
 }  // namespace dpg
@@ -146,6 +210,7 @@ struct dpg_plan {
   std::vector<std::unique_ptr<dpg::DeviceBuffer>> tables;
   dpg::DeviceBuffer ws;
   dpg::DeviceBuffer info;
+  dpg::Profiler prof;
 };
 
 namespace dpg {
@@ -222,6 +287,15 @@ void build_bucket(dpg_plan* plan, const dpg_bucket_desc& desc, BucketPlan& B) {
   const ElementLayout& L = B.L;
   B.npr = L.dim + 1;
   B.flops = flops_per_cell(f, L);
+  B.gram_flops = 0.0;
+  for (int i = 0; i < f.n_ipterms; ++i) {
+    bool used[DPG_MAX_VARS] = {false};
+    for (int k = 0; k < f.ipterms[i].n; ++k) used[f.ipterms[i].s[k].test_var] = true;
+    double n = 0;
+    for (int v = 0; v < f.n_test; ++v)
+      if (used[v]) n += L.test_card[v];
+    B.gram_flops += (double)L.QG * n * (n + 1.0);
+  }
   compile_program(f, B);
   RefTables T = make_tables(f, L);
   auto& own = plan->tables;
@@ -291,24 +365,96 @@ Work carve(dpg_plan* plan, const BucketPlan& B, int nb) {
 }
 
 int batch_cells(dpg_plan* plan, const BucketPlan& B, int ncells) {
-  if (plan->opt.max_batch_cells > 0) return std::min(ncells, plan->opt.max_batch_cells);
+  if (plan->opt.max_batch_cells > 0) return std::min(std::min(ncells, plan->opt.max_batch_cells), 65535);
   double budget = plan->opt.workspace_bytes > 0 ? plan->opt.workspace_bytes : 16.0 * (1ull << 30);
   int64_t nb = (int64_t)(budget / (double)B.per_cell_bytes);
   nb = std::max<int64_t>(1, std::min<int64_t>(nb, ncells));
-  return (int)nb;
+  return (int)std::min<int64_t>(nb, 65535);  // cells are a grid y/z dimension
 }
 
-// Stage 1 of the pipeline for cells [c0, c0+nb): contractions into A, POEQU
-// scaling and the partial Cholesky over the first N columns.
+// Left-looking partial Cholesky of columns [kbeg, kend) of the augmented
+// matrices of a batch (columns before kbeg already hold their final values,
+// columns [kbeg, kend) have not been touched by the factorisation yet):
+//   panel update  A[k0:D, k0:k1] -= A[k0:D, kbeg:k0] A[k0:k1, kbeg:k0]^T   (MFMA)
+//   panel         POTRF of the diagonal block, X <- X L^-T below it
+// then the trailing block A[kend:D, kend:D] -= A[kend:D, kbeg:kend] (...)^T.
+void factor_columns(dpg_plan* plan, const BucketPlan& B, int nb, const Work& w, int kbeg, int kend, double beta_trailing,
+                    hipStream_t st) {
+  Profiler& P = plan->prof;
+  const int64_t sA = B.nA, lda = B.lda;
+  int32_t* info = (int32_t*)plan->info.p;
+  for (int k0 = kbeg; k0 < kend; k0 += 64) {
+    int kb = std::min(64, kend - k0);
+    int k1 = k0 + kb;
+    double rows = B.D - k0;
+    hipEvent_t e;
+    if (k0 > kbeg) {
+      GemmArgs t{};
+      t.X = w.A + k0 + (size_t)kbeg * lda;
+      t.ldx = lda;
+      t.sx = sA;
+      t.Y = t.X;
+      t.ldy = lda;
+      t.sy = sA;
+      t.C = w.A + k0 + (size_t)k0 * lda;
+      t.ldc = lda;
+      t.sc = sA;
+      t.m = B.D - k0;
+      t.n = kb;
+      t.k = k0 - kbeg;
+      t.alpha = -1.0;
+      t.beta = 1.0;
+      t.ncells = nb;
+      double kk = k0 - kbeg;
+      e = P.start(st);
+      launch_gemm(t, kGemmUpdate, st);
+      P.stop(e, st, DPG_STAGE_UPDATE, nb * 2.0 * (rows * kb - kb * (kb - 1) / 2.0) * kk,
+             nb * (rows * kk + rows * kb * 2.0) * 8.0);
+    }
+    e = P.start(st);
+    launch_panel(w.A, lda, sA, B.D, k0, kb, nb, info, 0, st);
+    P.stop(e, st, DPG_STAGE_PANEL, nb * ((double)kb * kb * kb / 3.0 + (rows - kb) * kb * kb),
+           nb * rows * kb * 16.0);
+  }
+  int n = B.D - kend;
+  if (n > 0) {
+    GemmArgs t{};
+    t.X = w.A + kend + (size_t)kbeg * lda;
+    t.ldx = lda;
+    t.sx = sA;
+    t.Y = t.X;
+    t.ldy = lda;
+    t.sy = sA;
+    t.C = w.A + kend + (size_t)kend * lda;
+    t.ldc = lda;
+    t.sc = sA;
+    t.m = t.n = n;
+    t.k = kend - kbeg;
+    t.lower = 1;
+    t.alpha = -1.0;
+    t.beta = beta_trailing;
+    t.ncells = nb;
+    hipEvent_t e = P.start(st);
+    launch_gemm(t, kGemmUpdate, st);
+    P.stop(e, st, DPG_STAGE_UPDATE, nb * (double)n * (n + 1.0) * t.k,
+           nb * ((double)n * t.k + n * (n + 1.0) / 2.0 * (beta_trailing != 0.0 ? 2.0 : 1.0)) * 8.0);
+  }
+}
+
+// Stage 1 for cells [c0, c0+nb): contractions into A, DPOEQU scaling and the
+// partial Cholesky over the first N columns.
 void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int c0, int nb, const Work& w,
                   const double* u, double* Gout, hipStream_t st) {
+  Profiler& P = plan->prof;
   const ElementLayout& L = B.L;
   const int dim = L.dim, V = L.rc.nverts, S = L.rc.nsides, N = L.N, M = L.M, nvar = plan->form.n_test;
   const double* nodes = bt.nodes + (size_t)c0 * V * dim;
   const int8_t* par = bt.parities + (size_t)c0 * S;
   const double* coefs = bt.coefs ? bt.coefs + (size_t)c0 * plan->form.n_coefs : nullptr;
   const int64_t sA = B.nA, lda = B.lda;
+  const double D8 = 8.0;
   // per-point weights
+  hipEvent_t e = P.start(st);
   launch_point(dim, 0, B.prog_d, B.tab, L.QG, L.Qs, nb, nodes, par, coefs, plan->form.n_coefs, S, bt.load, w.Wip,
                nullptr, st);
   launch_point(dim, 1, B.prog_d, B.tab, L.Q, L.Qs, nb, nodes, par, coefs, plan->form.n_coefs, S, bt.load, w.Wvol,
@@ -316,7 +462,9 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
   if (B.prog.n_sb > 0)
     launch_point(dim, 2, B.prog_d, B.tab, S * L.Qs, L.Qs, nb, nodes, par, coefs, plan->form.n_coefs, S, bt.load,
                  w.Wside, nullptr, st);
+  P.stop(e, st, DPG_STAGE_POINT, 0.0, nb * D8 * (B.nWip + B.nWvol + B.nWrhs + B.nWside + V * dim));
   // operands
+  e = P.start(st);
   launch_operand(dim, w.Wip, B.prog.n_lt, nvar, B.tab.R_ip, B.tab.var_of_dof, N, L.QG, nb, w.At, B.nAt, B.Npad,
                  (int64_t)B.prog.n_lt * B.Npad, st);
   launch_operand(dim, w.Wvol, B.prog.n_vb, nvar, B.tab.R_vol, B.tab.var_of_dof, N, L.Q, nb, w.Tv, B.nTv,
@@ -324,7 +472,8 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
   if (B.prog.n_sb > 0)
     launch_operand(dim, w.Wside, B.prog.n_sb, nvar, B.tab.R_side, B.tab.var_of_dof, N, S * L.Qs, nb, w.Ts, B.nTs,
                    (int64_t)S * L.Qs * B.Npad, B.Npad, st);
-  launch_zero_trailing(w.A, lda, sA, N, B.D, nb, st);
+  double opnd = (double)B.prog.n_lt * L.QG + (double)B.prog.n_vb * L.Q + (double)B.prog.n_sb * S * L.Qs;
+  P.stop(e, st, DPG_STAGE_OPERAND, nb * opnd * N * 2.0 * B.npr, nb * opnd * N * D8);
   // G = At^T At (lower)
   GemmArgs g{};
   g.X = w.At;
@@ -342,9 +491,17 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
   g.alpha = 1.0;
   g.beta = 0.0;
   g.ncells = nb;
-  launch_gemm(g, st);
-  if (Gout) launch_extract_sym(w.A, lda, sA, 0, N, 1.0, Gout + (size_t)c0 * N * N, nb, st);
+  e = P.start(st);
+  launch_gemm(g, kGemmGram, st);
+  P.stop(e, st, DPG_STAGE_GRAM, nb * B.gram_flops, nb * ((double)g.k * N + N * (N + 1.0) / 2.0) * D8);
+  if (Gout) {
+    e = P.start(st);
+    launch_extract_sym(w.A, lda, sA, 0, N, 1.0, Gout + (size_t)c0 * N * N, nb, st);
+    P.stop(e, st, DPG_STAGE_EXTRACT, 0.0, nb * (double)N * N * 2.0 * D8);
+  }
   // B^T blocks: rows N + trial offset, columns = test dofs
+  e = P.start(st);
+  double bfl = 0.0, bby = 0.0;
   for (int vb = 0; vb < B.prog.n_vb; ++vb) {
     const TrialBlock& blk = L.blocks[B.vb_block[vb]];
     GemmArgs b{};
@@ -362,7 +519,9 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
     b.k = L.Q;
     b.alpha = 1.0;
     b.ncells = nb;
-    launch_gemm(b, st);
+    launch_gemm(b, kGemmBmat, st);
+    bfl += 2.0 * b.m * b.n * b.k;
+    bby += ((double)b.k * b.n + (double)b.m * b.n) * D8;
   }
   for (int sb = 0; sb < B.prog.n_sb; ++sb) {
     int v = B.sb_var[sb];
@@ -383,74 +542,35 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
       b.k = L.Qs;
       b.alpha = 1.0;
       b.ncells = nb;
-      launch_gemm(b, st);
+      launch_gemm(b, kGemmBmat, st);
+      bfl += 2.0 * b.m * b.n * b.k;
+      bby += ((double)b.k * b.n + (double)b.m * b.n) * D8;
     }
   }
   // load vector row, residual row
   launch_rhs(dim, w.Wrhs, nvar, B.tab.R_vol, B.tab.var_of_dof, N, L.Q, nb, w.A, lda, sA, N + M, st);
   launch_residual_row(w.A, lda, sA, N, M, N, N + M, N + M + 1, u ? u + (size_t)c0 * M : nullptr, nb, st);
+  bfl += 2.0 * L.Q * N * B.npr + (u ? 2.0 * M * N : 0.0);
+  P.stop(e, st, DPG_STAGE_BMAT, nb * bfl, nb * bby);
   // DPOEQU scaling of G (and of the rows of B, l, r)
+  e = P.start(st);
   launch_scaling(w.A, lda, sA, 0, B.D, N, N, w.s, B.nS, nb, 1.0, st);
-  int32_t* info = (int32_t*)plan->info.p;
-  HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int32_t) * nb, st));
-  for (int k0 = 0; k0 < N; k0 += 64) {
-    int kb = std::min(64, N - k0);
-    launch_panel(w.A, lda, sA, B.D, k0, kb, nb, info, 0, st);
-    int k1 = k0 + kb;
-    if (B.D - k1 > 0) {
-      GemmArgs t{};
-      t.X = w.A + k1 + (size_t)k0 * lda;
-      t.ldx = lda;
-      t.sx = sA;
-      t.Y = t.X;
-      t.ldy = lda;
-      t.sy = sA;
-      t.C = w.A + k1 + (size_t)k1 * lda;
-      t.ldc = lda;
-      t.sc = sA;
-      t.m = t.n = B.D - k1;
-      t.k = kb;
-      t.lower = 1;
-      t.alpha = -1.0;
-      t.beta = 1.0;
-      t.ncells = nb;
-      launch_gemm(t, st);
-    }
-  }
+  P.stop(e, st, DPG_STAGE_SCALE, 0.0, nb * ((double)N * (N + 1) / 2.0 + (double)(B.D - N) * N) * 16.0);
+  HIP_CHECK(hipMemsetAsync(plan->info.p, 0, sizeof(int32_t) * nb, st));
+  // the trailing block starts as 0 and is written (beta = 0) by the closing SYRK
+  factor_columns(plan, B, nb, w, 0, N, 0.0, st);
 }
 
 // Stage 2: static condensation of the field dofs of K = -A[N:, N:]
 void stage_schur(dpg_plan* plan, const BucketPlan& B, int nb, const Work& w, hipStream_t st) {
   const ElementLayout& L = B.L;
   const int N = L.N, F = L.F;
-  const int64_t sA = B.nA, lda = B.lda;
+  Profiler& P = plan->prof;
   // negate the trailing block and equilibrate its field rows/columns
-  launch_scaling(w.A, lda, sA, N, B.D, F, B.D - N, w.s, B.nS, nb, -1.0, st);
-  int32_t* info = (int32_t*)plan->info.p;
-  for (int k0 = N; k0 < N + F; k0 += 64) {
-    int kb = std::min(64, N + F - k0);
-    launch_panel(w.A, lda, sA, B.D, k0, kb, nb, info, 0, st);
-    int k1 = k0 + kb;
-    if (B.D - k1 > 0) {
-      GemmArgs t{};
-      t.X = w.A + k1 + (size_t)k0 * lda;
-      t.ldx = lda;
-      t.sx = sA;
-      t.Y = t.X;
-      t.ldy = lda;
-      t.sy = sA;
-      t.C = w.A + k1 + (size_t)k1 * lda;
-      t.ldc = lda;
-      t.sc = sA;
-      t.m = t.n = B.D - k1;
-      t.k = kb;
-      t.lower = 1;
-      t.alpha = -1.0;
-      t.beta = 1.0;
-      t.ncells = nb;
-      launch_gemm(t, st);
-    }
-  }
+  hipEvent_t e = P.start(st);
+  launch_scaling(w.A, B.lda, B.nA, N, B.D, F, B.D - N, w.s, B.nS, nb, -1.0, st);
+  P.stop(e, st, DPG_STAGE_SCALE, 0.0, nb * (double)(B.D - N) * (B.D - N + 1) / 2.0 * 16.0);
+  factor_columns(plan, B, nb, w, N, N + F, 1.0, st);
 }
 
 void check_batch(const dpg_plan* plan, int bucket, const dpg_batch* batch) {
@@ -523,6 +643,7 @@ int dpg_plan_create(const dpg_form_desc* form, const dpg_bucket_desc* buckets, i
     if (plan->opt.precision != DPG_PREC_FP64)
       throw std::invalid_argument("this build assembles in FP64 only (precision must be DPG_PREC_FP64)");
     HIP_CHECK(hipGetDevice(&plan->device));
+    plan->prof.on = plan->opt.profile != 0;
     plan->buckets.resize(nbuckets);
     for (int i = 0; i < nbuckets; ++i) dpg::build_bucket(plan.get(), buckets[i], plan->buckets[i]);
     *out = plan.release();
@@ -537,6 +658,14 @@ int dpg_plan_workspace(const dpg_plan* plan, size_t* bytes) {
   return dpg::guarded([&] {
     if (!plan || !bytes) throw std::invalid_argument("NULL argument");
     *bytes = plan->ws.bytes;
+  });
+}
+
+int dpg_plan_timings(dpg_plan* plan, dpg_timings* out, int reset) {
+  return dpg::guarded([&] {
+    if (!plan || !out) throw std::invalid_argument("NULL argument");
+    *out = plan->prof.acc;
+    if (reset) plan->prof.acc = dpg_timings{};
   });
 }
 
@@ -556,17 +685,22 @@ int dpg_assemble(dpg_plan* plan, int bucket, const dpg_batch* batch, const dpg_o
       int nb = std::min(maxb, C - c0);
       dpg::Work w = dpg::carve(plan, B, nb);
       dpg::stage_factor(plan, B, *batch, c0, nb, w, nullptr, outputs->G, st);
+      hipEvent_t e = plan->prof.start(st);
       if (outputs->K) dpg::launch_extract_sym(w.A, B.lda, B.nA, N, M, -1.0, outputs->K + (size_t)c0 * M * M, nb, st);
       if (outputs->rhs)
         dpg::launch_extract_row(w.A, B.lda, B.nA, N + M, N, M, -1.0, outputs->rhs + (size_t)c0 * M, nb, st);
+      plan->prof.stop(e, st, DPG_STAGE_EXTRACT, 0.0, nb * ((double)M * M * 2.0 + M * 2.0) * 8.0);
       if (outputs->schur || outputs->schur_rhs) {
         dpg::stage_schur(plan, B, nb, w, st);
+        e = plan->prof.start(st);
         if (outputs->schur)
           dpg::launch_extract_sym(w.A, B.lda, B.nA, N + F, T, 1.0, outputs->schur + (size_t)c0 * T * T, nb, st);
         if (outputs->schur_rhs)
           dpg::launch_extract_row(w.A, B.lda, B.nA, N + M, N + F, T, 1.0, outputs->schur_rhs + (size_t)c0 * T, nb,
                                   st);
+        plan->prof.stop(e, st, DPG_STAGE_EXTRACT, 0.0, nb * ((double)T * T * 2.0 + T * 2.0) * 8.0);
       }
+      plan->prof.collect(st, nb);
       // the next batch reuses the workspace in stream order
       if (outputs->info) bad += dpg::count_failures(plan, nb, outputs->info + c0, st);
     }
@@ -591,6 +725,7 @@ int dpg_energy_error(dpg_plan* plan, int bucket, const dpg_batch* batch, const d
       dpg::Work w = dpg::carve(plan, B, nb);
       dpg::stage_factor(plan, B, *batch, c0, nb, w, u, nullptr, st);
       dpg::launch_extract_row(w.A, B.lda, B.nA, B.D - 1, B.D - 1, 1, -1.0, eta2 + c0, nb, st);
+      plan->prof.collect(st, nb);
       bad += dpg::count_failures(plan, nb, nullptr, st);
     }
     return bad;

@@ -85,7 +85,8 @@ def rng_normal(seed, n):
 class LocalAssembler:
     """A plan: device basis/cubature tables of the given element types."""
 
-    def __init__(self, f, buckets, precision=capi.PREC_FP64, refine=0, workspace_bytes=0.0, max_batch_cells=0):
+    def __init__(self, f, buckets, precision=capi.PREC_FP64, refine=0, workspace_bytes=0.0, max_batch_cells=0,
+                 profile=False):
         import torch  # device memory only
 
         self._torch = torch
@@ -94,7 +95,7 @@ class LocalAssembler:
         self.form = f
         self.buckets = list(buckets)
         arr = (capi.BucketDesc * len(self.buckets))(*self.buckets)
-        opt = capi.Options(precision, refine, max_batch_cells, 0, float(workspace_bytes))
+        opt = capi.Options(precision, refine, max_batch_cells, 1 if profile else 0, float(workspace_bytes))
         plan = ct.c_void_p()
         check(lib().dpg_plan_create(ct.byref(f), arr, len(self.buckets), ct.byref(opt), ct.byref(plan)))
         self._plan = plan
@@ -115,6 +116,16 @@ class LocalAssembler:
         n = ct.c_size_t()
         check(lib().dpg_plan_workspace(self._plan, ct.byref(n)))
         return n.value
+
+    def timings(self, reset=True):
+        """Per-stage kernel times / algorithmic flops / bytes since the last reset (profile=True plans)."""
+        t = capi.Timings()
+        check(lib().dpg_plan_timings(self._plan, ct.byref(t), 1 if reset else 0))
+        out = {}
+        for i, name in enumerate(capi.STAGES):
+            if t.launches[i]:
+                out[name] = {"ms": t.ms[i], "launches": int(t.launches[i]), "flops": t.flops[i], "bytes": t.bytes[i]}
+        return out, int(t.cells)
 
     def _batch(self, nodes, parities, coefs, load):
         t = self._torch
@@ -147,9 +158,10 @@ class LocalAssembler:
             o["rhs"] = t.empty((C, info.M), dtype=t.float64, device=dev)
         if want_G and "G" not in o:
             o["G"] = t.empty((C, info.N, info.N), dtype=t.float64, device=dev)
-        if want_schur:
-            o.setdefault("schur", t.empty((C, info.T, info.T), dtype=t.float64, device=dev))
-            o.setdefault("schur_rhs", t.empty((C, info.T), dtype=t.float64, device=dev))
+        if want_schur and "schur" not in o:
+            o["schur"] = t.empty((C, info.T, info.T), dtype=t.float64, device=dev)
+        if want_schur and "schur_rhs" not in o:
+            o["schur_rhs"] = t.empty((C, info.T), dtype=t.float64, device=dev)
         if want_info and "info" not in o:
             o["info"] = t.zeros((C,), dtype=t.int32, device=dev)
         outs = capi.Outputs()

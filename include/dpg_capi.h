@@ -159,9 +159,33 @@ typedef struct {
   int32_t precision;       /* DPG_PREC_*: arithmetic of the G/B/l contractions */
   int32_t refine;          /* 1: one FP64 refinement step on G^-1 B            */
   int32_t max_batch_cells; /* 0: derived from workspace_bytes                  */
-  int32_t pad;
+  int32_t profile;         /* 1: time every kernel launch (dpg_plan_timings)   */
   double workspace_bytes;  /* device workspace budget, 0: 16 GiB               */
 } dpg_options;
+
+/* pipeline stages timed when dpg_options.profile is set */
+enum {
+  DPG_STAGE_POINT = 0,   /* per-point geometry and weights            */
+  DPG_STAGE_OPERAND = 1, /* weighted operator values                  */
+  DPG_STAGE_GRAM = 2,    /* G contraction (MFMA)                      */
+  DPG_STAGE_BMAT = 3,    /* B and l contractions                      */
+  DPG_STAGE_SCALE = 4,   /* DPOEQU scaling                            */
+  DPG_STAGE_PANEL = 5,   /* diagonal-block Cholesky + panel solve     */
+  DPG_STAGE_UPDATE = 6,  /* trailing updates (MFMA)                   */
+  DPG_STAGE_EXTRACT = 7, /* K, rhs, G, Schur copies to the outputs    */
+  DPG_NSTAGES = 8
+};
+
+/* accumulated per-stage kernel times (HIP events on the call's stream) with
+ * the algorithmic flops and HBM bytes of the timed launches; the analogue of
+ * the reference's timing callbacks (BF.h:40-41,190-191) */
+typedef struct {
+  double ms[DPG_NSTAGES];
+  double flops[DPG_NSTAGES];
+  double bytes[DPG_NSTAGES];
+  int64_t launches[DPG_NSTAGES];
+  int64_t cells;
+} dpg_timings;
 
 /* one call's inputs; all pointers are device pointers */
 typedef struct {
@@ -227,6 +251,8 @@ int dpg_plan_create(const dpg_form_desc* form, const dpg_bucket_desc* buckets, i
 int dpg_plan_destroy(dpg_plan* plan);
 /* bytes of device workspace the plan holds */
 int dpg_plan_workspace(const dpg_plan* plan, size_t* bytes);
+/* timings accumulated since the last reset (needs options.profile = 1) */
+int dpg_plan_timings(dpg_plan* plan, dpg_timings* out, int reset);
 
 /* Local assembly of one element type: replaces TBF::localStiffnessMatrixAndRHS
  * (BF.cpp:1308-1496, FACTORED_CHOLESKY branch) for all cells of the batch,
