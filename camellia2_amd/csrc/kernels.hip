@@ -261,7 +261,7 @@ template <int NPR>
 __global__ __launch_bounds__(256) void rhs_kernel(const double* __restrict__ Wrhs, int nvar,
                                                   const double* __restrict__ R, const int32_t* __restrict__ vod,
                                                   int N, int Q, double* __restrict__ A, int64_t lda, int64_t sA,
-                                                  int row_l) {
+                                                  int row_l, const double* __restrict__ s, int64_t ss) {
   int c = blockIdx.y;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
@@ -272,7 +272,46 @@ __global__ __launch_bounds__(256) void rhs_kernel(const double* __restrict__ Wrh
 #pragma unroll
     for (int k = 0; k < NPR; ++k) acc = fma(w[k], R[((size_t)q * NPR + k) * N + i], acc);
   }
-  A[(size_t)c * sA + row_l + (size_t)i * lda] = acc;
+  A[(size_t)c * sA + row_l + (size_t)i * lda] = s ? acc * s[(size_t)c * ss + i] : acc;
+}
+
+// Gram operand (one thread per test dof j, loop over the points in a fixed
+// order): out[c][q*n_lt + lt][j] = sum_r Wip[c][q][lt][var j][r] R[q][r][j]
+// (the weighted values of IP term lt; G = out^T out), and the DPOEQU factors
+// s[c][j] = 1/sqrt(G_jj) with G_jj = sum over rows of out[..][j]^2.
+template <int NPR>
+__global__ __launch_bounds__(256) void ip_operand_kernel(const double* __restrict__ Wip, int n_lt, int nvar,
+                                                         const double* __restrict__ R,
+                                                         const int32_t* __restrict__ vod, int N, int QG,
+                                                         double* __restrict__ out, int64_t sc, int64_t ld,
+                                                         double* __restrict__ s, int64_t ss) {
+  extern __shared__ double Wl[];
+  const int wslice = n_lt * nvar * NPR;
+  const int c = blockIdx.y, t = threadIdx.x;
+  const int j = blockIdx.x * blockDim.x + t;
+  const bool active = j < N;
+  const int var = active ? vod[j] : 0;
+  double diag = 0.0;
+  double* oc = out + (size_t)c * sc + j;
+  for (int q = 0; q < QG; ++q) {
+    __syncthreads();
+    const double* Wp = Wip + ((size_t)c * QG + q) * wslice;
+    for (int e = t; e < wslice; e += blockDim.x) Wl[e] = Wp[e];
+    __syncthreads();
+    if (!active) continue;
+    double r[NPR];
+#pragma unroll
+    for (int k = 0; k < NPR; ++k) r[k] = R[((size_t)q * NPR + k) * N + j];
+    for (int lt = 0; lt < n_lt; ++lt) {
+      const double* w = Wl + (lt * nvar + var) * NPR;
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < NPR; ++k) acc = fma(w[k], r[k], acc);
+      oc[((size_t)q * n_lt + lt) * ld] = acc;
+      diag = fma(acc, acc, diag);
+    }
+  }
+  if (active) s[(size_t)c * ss + j] = 1.0 / sqrt(diag);
 }
 
 // ------------------------------------------------------------------- GEMM
@@ -317,6 +356,7 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int bb = 0; bb < 2; ++bb) acc[a][bb] = d4{0.0, 0.0, 0.0, 0.0};
+  const int kbeg = 0, kend = g.k;
   // each thread stages 4 elements of X and of Y per k-chunk; the next chunk is
   // fetched into registers while the current one is multiplied from LDS
   double xr[4], yr[4];
@@ -324,12 +364,12 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       int e = t + 256 * r, kk = e >> 6, ii = e & 63, kg = k0 + kk;
-      xr[r] = (kg < g.k && i0 + ii < g.m) ? X[(size_t)kg * g.ldx + i0 + ii] : 0.0;
-      yr[r] = (kg < g.k && j0 + ii < g.n) ? Y[(size_t)kg * g.ldy + j0 + ii] : 0.0;
+      xr[r] = (kg < kend && i0 + ii < g.m) ? X[(size_t)kg * g.ldx + i0 + ii] : 0.0;
+      yr[r] = (kg < kend && j0 + ii < g.n) ? Y[(size_t)kg * g.ldy + j0 + ii] : 0.0;
     }
   };
-  fetch(0);
-  for (int k0 = 0; k0 < g.k; k0 += GKC) {
+  fetch(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += GKC) {
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -338,7 +378,7 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
       Ys[e >> 6][e & 63] = yr[r];
     }
     __syncthreads();
-    if (k0 + GKC < g.k) fetch(k0 + GKC);
+    if (k0 + GKC < kend) fetch(k0 + GKC);
 #pragma unroll
     for (int k4 = 0; k4 < GKC; k4 += 4) {
       const int kr = k4 + (lane >> 4), cc = lane & 15;
@@ -367,6 +407,8 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
         if (i < g.m && j < g.n && (!g.lower || i >= j)) {
           double* cp = C + (size_t)i + (size_t)j * g.ldc;
           double val = g.alpha * acc[bj][bi][v];
+          if (g.srow) val *= g.srow[(size_t)cell * g.ss + i];
+          if (g.scol) val *= g.scol[(size_t)cell * g.ss + j];
           if (g.beta != 0.0) val = fma(g.beta, *cp, val);
           *cp = val;
         }
@@ -378,20 +420,24 @@ __global__ __launch_bounds__(256) void gemm_gram_f64(GemmArgs g) { gemm_tn_f64_b
 __global__ __launch_bounds__(256) void gemm_bmat_f64(GemmArgs g) { gemm_tn_f64_body(g); }
 __global__ __launch_bounds__(256) void gemm_update_f64(GemmArgs g) { gemm_tn_f64_body(g); }
 
-// ------------------------------------------------------------ panel kernel
-// Columns [k0, k0+kb) (kb <= 64) of one cell's lower-stored matrix per
-// workgroup: POTRF of the 64x64 diagonal block in LDS, its triangular inverse
-// in LDS, then X <- X L^-T for the rows below in 64-row tiles on MFMA.
-constexpr int PNB = 64, PLD = PNB + 4;
+// ------------------------------------------------------------ panel kernels
+// Columns [k0, k0+kb) (kb <= 64) of each cell's lower-stored matrix:
+//   potrf_block  one workgroup per cell: POTRF of the 64x64 diagonal block in
+//                LDS, the factor to Ld, its in-place triangular inverse
+//                (LAPACK DTRTI2 order) to Li as Li[m][j] = Linv[j][m]
+//   panel_trsm   workgroup (chunk, cell): X <- X L^-T = X Li for rows
+//                [k0+kb + chunk*PCH, +PCH) in 64-row tiles on MFMA.
+// The diagonal block of A itself is left untouched (no later stage reads it).
+constexpr int PNB = 64, PLD = PNB + 1, PCH = 256;
 
-__global__ __launch_bounds__(256) void panel_f64(double* __restrict__ A, int64_t lda, int64_t sA, int D, int k0,
-                                                 int kb, int32_t* __restrict__ info, int info_base) {
-  __shared__ double L[PNB][PLD];   // factor, later the row tile X (k-major: X[m][r])
-  __shared__ double Yt[PNB][PLD];  // Yt[m][j] = Linv[j][m]
-  __shared__ int fail;
+__global__ __launch_bounds__(256) void potrf_block(const double* __restrict__ A, int64_t lda, int64_t sA, int k0,
+                                                   int kb, double* __restrict__ Ld, int64_t sLd,
+                                                   double* __restrict__ Li, int32_t* __restrict__ info,
+                                                   int info_base) {
+  __shared__ double L[PNB][PLD];
+  __shared__ double piv[PNB];
   const int c = blockIdx.x, t = threadIdx.x;
-  double* Ac = A + (size_t)c * sA;
-  if (t == 0) fail = -1;
+  const double* Ac = A + (size_t)c * sA;
   for (int e = t; e < PNB * PNB; e += 256) {
     int r = e % PNB, cc = e / PNB;
     double v;
@@ -399,74 +445,113 @@ __global__ __launch_bounds__(256) void panel_f64(double* __restrict__ A, int64_t
     else v = (r == cc) ? 1.0 : 0.0;
     L[r][cc] = v;
   }
-  __syncthreads();
-  // right-looking Cholesky of the diagonal block
-  for (int j = 0; j < PNB; ++j) {
-    if (t == 0) {
-      double d = L[j][j];
+  // packed lower-triangle elements owned by this thread (2080 = 64*65/2 of them)
+  int own_r[9], own_c[9];
+#pragma unroll
+  for (int m = 0; m < 9; ++m) {
+    int e = t + 256 * m;
+    if (e < PNB * (PNB + 1) / 2) {
+      int r = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+      while ((r + 1) * (r + 2) / 2 <= e) ++r;
+      while (r * (r + 1) / 2 > e) --r;
+      own_r[m] = r;
+      own_c[m] = e - r * (r + 1) / 2;
+    } else {
+      own_r[m] = -1;
+      own_c[m] = -1;
+    }
+  }
+  // right-looking Cholesky, one barrier per column: the update of step j uses
+  // the unscaled column j (l_rj l_cj = L_rj L_cj / d_j); column j is scaled by
+  // 1/sqrt(d_j) one step later, when no update reads it any more
+  int fail = -1;
+  for (int j = 0; j <= PNB; ++j) {
+    __syncthreads();
+    double d = 0.0;
+    if (j < PNB) {
+      d = L[j][j];
       if (!(d > 0.0)) {
         if (fail < 0) fail = j;
         d = __builtin_nan("");
       }
-      L[j][j] = sqrt(d);
+      if (t == 0) piv[j] = d;
     }
-    __syncthreads();
-    double djj = L[j][j];
-    for (int r = j + 1 + t; r < PNB; r += 256) L[r][j] /= djj;
-    __syncthreads();
-    int rem = PNB - j - 1;
-    for (int e = t; e < rem * rem; e += 256) {
-      int r = j + 1 + e % rem, cc = j + 1 + e / rem;
-      if (r >= cc) L[r][cc] -= L[r][j] * L[cc][j];
-    }
-    __syncthreads();
-  }
-  for (int e = t; e < kb * kb; e += 256) {
-    int r = e % kb, cc = e / kb;
-    if (r >= cc) Ac[(size_t)(k0 + r) + (size_t)(k0 + cc) * lda] = L[r][cc];
-  }
-  if (t == 0 && fail >= 0 && fail < kb && info) atomicCAS(&info[c], 0, info_base + k0 + fail + 1);
-  // triangular inverse: thread j forms column j of Linv by forward substitution
-  if (t < PNB) {
-    const int j = t;
-    for (int i = 0; i < j; ++i) Yt[j][i] = 0.0;
-    Yt[j][j] = 1.0 / L[j][j];
-    for (int i = j + 1; i < PNB; ++i) {
-      double sacc = 0.0;
-      for (int m = j; m < i; ++m) sacc = fma(L[i][m], Yt[j][m], sacc);
-      Yt[j][i] = -sacc / L[i][i];
+    const double inv = 1.0 / d;
+#pragma unroll
+    for (int m = 0; m < 9; ++m) {
+      const int r = own_r[m], cc = own_c[m];
+      if (r < 0) continue;
+      if (j < PNB && cc > j) L[r][cc] = fma(-L[r][j] * inv, L[cc][j], L[r][cc]);
+      if (j > 0 && cc == j - 1) {
+        const double sd = sqrt(piv[j - 1]);
+        L[r][cc] = (r == cc) ? sd : L[r][cc] / sd;
+      }
     }
   }
   __syncthreads();
-  // Yt[a][b] = Linv[b][a], so  X_new(r, j) = sum_m X(r, m) Linv(j, m) = sum_m X[m][r] Yt[m][j]
-  // with the row tile stored k-major in L (L[m][r] = X(r, m))
-  const int lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
-  for (int r0 = k0 + kb; r0 < D; r0 += PNB) {
+  double* Lc = Ld + (size_t)c * sLd;
+  for (int e = t; e < kb * PNB; e += 256) {
+    int r = e / PNB, cc = e % PNB;
+    Lc[(size_t)(k0 + r) * PNB + cc] = (cc <= r && cc < kb) ? L[r][cc] : 0.0;
+  }
+  if (t == 0 && fail >= 0 && fail < kb && info) atomicCAS(&info[c], 0, info_base + k0 + fail + 1);
+  // in-place inverse, columns from the last: L[j][j] <- 1/L[j][j];
+  // L[j+1:, j] <- -L[j][j]^-1 * Linv[j+1:, j+1:] L[j+1:, j]
+  for (int j = PNB - 1; j >= 0; --j) {
     __syncthreads();
-    for (int e = t; e < PNB * PNB; e += 256) {
-      int r = e % PNB, m = e / PNB;
-      L[m][r] = (m < kb && r0 + r < D) ? Ac[(size_t)(r0 + r) + (size_t)(k0 + m) * lda] : 0.0;
+    double x = 0.0;
+    const int i = j + 1 + t;
+    if (t < PNB - 1 - j) {
+      for (int m = j + 1; m <= i; ++m) x = fma(L[i][m], L[m][j], x);
     }
+    const double ajj = L[j][j];
     __syncthreads();
+    if (t < PNB - 1 - j) L[i][j] = -x / ajj;
+    if (t == 0) L[j][j] = 1.0 / ajj;
+  }
+  __syncthreads();
+  double* Lic = Li + (size_t)c * PNB * PNB;
+  for (int e = t; e < PNB * PNB; e += 256) {
+    int m = e / PNB, jj = e % PNB;  // Li[m][jj] = Linv[jj][m]
+    Lic[e] = (m <= jj) ? L[jj][m] : 0.0;
+  }
+}
+
+__global__ __launch_bounds__(256) void panel_trsm(double* __restrict__ A, int64_t lda, int64_t sA, int D, int k0,
+                                                  int kb, const double* __restrict__ Li) {
+  __shared__ double Ys[PNB][PNB + 4];
+  const int chunk = blockIdx.x, c = blockIdx.y, t = threadIdx.x;
+  double* Ac = A + (size_t)c * sA;
+  const double* Lic = Li + (size_t)c * PNB * PNB;
+  for (int e = t; e < PNB * PNB; e += 256) Ys[e / PNB][e % PNB] = Lic[e];
+  __syncthreads();
+  const int lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
+  const int rbeg = k0 + kb + chunk * PCH;
+  const int rend = min(D, rbeg + PCH);
+  for (int r0 = rbeg; r0 < rend; r0 += PNB) {
     d4 acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
       for (int bb = 0; bb < 2; ++bb) acc[a][bb] = d4{0.0, 0.0, 0.0, 0.0};
+    const int ra = r0 + wr * 32 + (lane & 15), rb = ra + 16;
 #pragma unroll 4
     for (int k4 = 0; k4 < PNB; k4 += 4) {
-      const int kr = k4 + (lane >> 4), cc = lane & 15;
+      const int m = k4 + (lane >> 4);
+      // X(r, m) straight from HBM: 16 consecutive rows per 16 lanes
       double xa[2], yb[2];
-      xa[0] = L[kr][wr * 32 + cc];
-      xa[1] = L[kr][wr * 32 + 16 + cc];
-      yb[0] = Yt[kr][wc * 32 + cc];
-      yb[1] = Yt[kr][wc * 32 + 16 + cc];
+      xa[0] = (m < kb && ra < rend) ? Ac[(size_t)ra + (size_t)(k0 + m) * lda] : 0.0;
+      xa[1] = (m < kb && rb < rend) ? Ac[(size_t)rb + (size_t)(k0 + m) * lda] : 0.0;
+      yb[0] = Ys[m][wc * 32 + (lane & 15)];
+      yb[1] = Ys[m][wc * 32 + 16 + (lane & 15)];
 #pragma unroll
       for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
         for (int bi = 0; bi < 2; ++bi)
           acc[bj][bi] = __builtin_amdgcn_mfma_f64_16x16x4f64(yb[bj], xa[bi], acc[bj][bi], 0, 0, 0);
     }
+    // all waves read their rows of this tile before any wave overwrites them
+    __syncthreads();
 #pragma unroll
     for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
@@ -475,7 +560,7 @@ __global__ __launch_bounds__(256) void panel_f64(double* __restrict__ A, int64_t
         for (int v = 0; v < 4; ++v) {
           int r = r0 + wr * 32 + bi * 16 + (lane & 15);
           int j = wc * 32 + bj * 16 + (lane >> 4) + 4 * v;
-          if (r < D && j < kb) Ac[(size_t)r + (size_t)(k0 + j) * lda] = acc[bj][bi][v];
+          if (r < rend && j < kb) Ac[(size_t)r + (size_t)(k0 + j) * lda] = acc[bj][bi][v];
         }
   }
 }
@@ -604,13 +689,25 @@ void launch_operand(int dim, const double* W, int ng, int nvar, const double* R,
 }
 
 void launch_rhs(int dim, const double* Wrhs, int nvar, const double* R, const int32_t* vod, int N, int Q, int ncells,
-                double* A, int64_t lda, int64_t sA, int row_l, hipStream_t st) {
+                double* A, int64_t lda, int64_t sA, int row_l, const double* s, int64_t ss, hipStream_t st) {
   dim3 grid((N + 255) / 256, ncells);
   if (dim == 2)
-    rhs_kernel<3><<<grid, 256, 0, st>>>(Wrhs, nvar, R, vod, N, Q, A, lda, sA, row_l);
+    rhs_kernel<3><<<grid, 256, 0, st>>>(Wrhs, nvar, R, vod, N, Q, A, lda, sA, row_l, s, ss);
   else
-    rhs_kernel<4><<<grid, 256, 0, st>>>(Wrhs, nvar, R, vod, N, Q, A, lda, sA, row_l);
+    rhs_kernel<4><<<grid, 256, 0, st>>>(Wrhs, nvar, R, vod, N, Q, A, lda, sA, row_l, s, ss);
   check_launch("rhs_kernel");
+}
+
+void launch_ip_operand(int dim, const double* Wip, int n_lt, int nvar, const double* R, const int32_t* vod, int N,
+                       int QG, int ncells, double* out, int64_t sc, int64_t ld, double* s, int64_t ss,
+                       hipStream_t st) {
+  size_t shm = sizeof(double) * (size_t)n_lt * nvar * (dim + 1);
+  dim3 grid((N + 255) / 256, ncells);
+  if (dim == 2)
+    ip_operand_kernel<3><<<grid, 256, shm, st>>>(Wip, n_lt, nvar, R, vod, N, QG, out, sc, ld, s, ss);
+  else
+    ip_operand_kernel<4><<<grid, 256, shm, st>>>(Wip, n_lt, nvar, R, vod, N, QG, out, sc, ld, s, ss);
+  check_launch("ip_operand_kernel");
 }
 
 void launch_gemm(GemmArgs g, int role, hipStream_t st) {
@@ -627,10 +724,15 @@ void launch_gemm(GemmArgs g, int role, hipStream_t st) {
   check_launch("gemm_tn_f64");
 }
 
-void launch_panel(double* A, int64_t lda, int64_t sA, int D, int k0, int kb, int ncells, int32_t* info, int info_base,
-                  hipStream_t st) {
-  panel_f64<<<dim3(ncells), 256, 0, st>>>(A, lda, sA, D, k0, kb, info, info_base);
-  check_launch("panel_f64");
+void launch_panel(double* A, int64_t lda, int64_t sA, int D, int k0, int kb, double* Ld, int64_t sLd, double* Li,
+                  int ncells, int32_t* info, int info_base, hipStream_t st) {
+  potrf_block<<<dim3(ncells), 256, 0, st>>>(A, lda, sA, k0, kb, Ld, sLd, Li, info, info_base);
+  check_launch("potrf_block");
+  int rows = D - k0 - kb;
+  if (rows <= 0) return;
+  int chunks = (rows + PCH - 1) / PCH;
+  panel_trsm<<<dim3(chunks, ncells), 256, 0, st>>>(A, lda, sA, D, k0, kb, Li);
+  check_launch("panel_trsm");
 }
 
 void launch_zero_trailing(double* A, int64_t lda, int64_t sA, int r0, int D, int ncells, hipStream_t st) {

@@ -27,6 +27,10 @@ struct GemmArgs {
   int32_t m, n, k, lower;
   double alpha, beta;
   int32_t tiles_m, ntiles, ncells, pad;
+  // optional: scale C(i,j) by srow[i] * scol[j] (per-cell vectors, stride ss)
+  const double* srow;
+  const double* scol;
+  int64_t ss;
 };
 
 void launch_point(int dim, int kind, const DevProgram* prog, const DevTables& tab, int npts, int Qs, int ncells,
@@ -35,11 +39,18 @@ void launch_point(int dim, int kind, const DevProgram* prog, const DevTables& ta
 void launch_operand(int dim, const double* W, int ng, int nvar, const double* R, const int32_t* vod, int N, int npts,
                     int ncells, double* out, int64_t sc, int64_t sb, int64_t sp, hipStream_t st);
 void launch_rhs(int dim, const double* Wrhs, int nvar, const double* R, const int32_t* vod, int N, int Q, int ncells,
-                double* A, int64_t lda, int64_t sA, int row_l, hipStream_t st);
+                double* A, int64_t lda, int64_t sA, int row_l, const double* s, int64_t ss, hipStream_t st);
+// Gram operand: out[c][q*n_lt + lt][j] = weighted value of IP term lt at point q
+// for test dof j (G = out^T out); s[c][j] = 1/sqrt(G_jj)
+void launch_ip_operand(int dim, const double* Wip, int n_lt, int nvar, const double* R, const int32_t* vod, int N,
+                       int QG, int ncells, double* out, int64_t sc, int64_t ld, double* s, int64_t ss,
+                       hipStream_t st);
 enum { kGemmGram = 0, kGemmBmat = 1, kGemmUpdate = 2 };
 void launch_gemm(GemmArgs g, int role, hipStream_t st);
-void launch_panel(double* A, int64_t lda, int64_t sA, int D, int k0, int kb, int ncells, int32_t* info, int info_base,
-                  hipStream_t st);
+// Ld: per-cell (rows, 64) store of the factored diagonal blocks (row k0 + r, column r');
+// Li: per-cell 64x64 scratch for the inverse of the current diagonal block
+void launch_panel(double* A, int64_t lda, int64_t sA, int D, int k0, int kb, double* Ld, int64_t sLd, double* Li,
+                  int ncells, int32_t* info, int info_base, hipStream_t st);
 void launch_zero_trailing(double* A, int64_t lda, int64_t sA, int r0, int D, int ncells, hipStream_t st);
 void launch_scaling(double* A, int64_t lda, int64_t sA, int r0, int D, int n, int ncol, double* s, int64_t ss,
                     int ncells, double sign, hipStream_t st);

@@ -194,7 +194,7 @@ struct BucketPlan {
   int npr = 0, D = 0;
   int64_t lda = 0, Npad = 0;
   // per-cell element counts (doubles) of the workspace regions
-  int64_t nA = 0, nAt = 0, nTv = 0, nTs = 0, nWip = 0, nWvol = 0, nWrhs = 0, nWside = 0, nS = 0;
+  int64_t nA = 0, nAt = 0, nTv = 0, nTs = 0, nWip = 0, nWvol = 0, nWrhs = 0, nWside = 0, nS = 0, nLd = 0, nLi = 0;
   int64_t per_cell_bytes = 0;
   double flops = 0.0;
   double gram_flops = 0.0;  // sum over IP terms of QG * n(n+1), n = dofs the term touches
@@ -322,13 +322,14 @@ void build_bucket(dpg_plan* plan, const dpg_bucket_desc& desc, BucketPlan& B) {
   B.tab.var_of_dof = upload(own, vod);
   std::vector<DevProgram> pv(1, B.prog);
   B.prog_d = upload(own, pv);
+
   // workspace layout (doubles per cell)
   B.D = L.N + L.M + 2;
   B.lda = round_up(B.D, 8);
   B.Npad = round_up(L.N, 8);
   int S = L.rc.nsides, nvar = f.n_test;
   B.nA = B.lda * B.D;
-  B.nAt = (int64_t)B.prog.n_lt * L.QG * B.Npad;
+  B.nAt = (int64_t)B.prog.n_lt * L.QG * B.Npad;  // weighted IP-term values
   B.nTv = (int64_t)B.prog.n_vb * L.Q * B.Npad;
   B.nTs = (int64_t)B.prog.n_sb * S * L.Qs * B.Npad;
   B.nWip = (int64_t)L.QG * B.prog.n_lt * nvar * B.npr;
@@ -336,12 +337,14 @@ void build_bucket(dpg_plan* plan, const dpg_bucket_desc& desc, BucketPlan& B) {
   B.nWrhs = (int64_t)L.Q * nvar * B.npr;
   B.nWside = (int64_t)S * L.Qs * B.prog.n_sb * nvar * B.npr;
   B.nS = round_up(std::max(L.N, L.M), 8);
-  int64_t tot = B.nA + B.nAt + B.nTv + B.nTs + B.nWip + B.nWvol + B.nWrhs + B.nWside + B.nS;
+  B.nLd = (int64_t)round_up(L.N + L.F, 64) * 64;  // factored diagonal blocks of both stages
+  B.nLi = 64 * 64;                                 // inverse of the current diagonal block
+  int64_t tot = B.nA + B.nAt + B.nTv + B.nTs + B.nWip + B.nWvol + B.nWrhs + B.nWside + B.nS + B.nLd + B.nLi;
   B.per_cell_bytes = tot * (int64_t)sizeof(double);
 }
 
 struct Work {
-  double *A, *At, *Tv, *Ts, *Wip, *Wvol, *Wrhs, *Wside, *s;
+  double *A, *At, *Tv, *Ts, *Wip, *Wvol, *Wrhs, *Wside, *s, *Ld, *Li;
 };
 
 Work carve(dpg_plan* plan, const BucketPlan& B, int nb) {
@@ -361,6 +364,8 @@ Work carve(dpg_plan* plan, const BucketPlan& B, int nb) {
   w.Wrhs = take(B.nWrhs);
   w.Wside = take(B.nWside);
   w.s = take(B.nS);
+  w.Ld = take(B.nLd);
+  w.Li = take(B.nLi);
   return w;
 }
 
@@ -412,7 +417,7 @@ void factor_columns(dpg_plan* plan, const BucketPlan& B, int nb, const Work& w, 
              nb * (rows * kk + rows * kb * 2.0) * 8.0);
     }
     e = P.start(st);
-    launch_panel(w.A, lda, sA, B.D, k0, kb, nb, info, 0, st);
+    launch_panel(w.A, lda, sA, B.D, k0, kb, w.Ld, B.nLd, w.Li, nb, info, 0, st);
     P.stop(e, st, DPG_STAGE_PANEL, nb * ((double)kb * kb * kb / 3.0 + (rows - kb) * kb * kb),
            nb * rows * kb * 16.0);
   }
@@ -463,18 +468,22 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
     launch_point(dim, 2, B.prog_d, B.tab, S * L.Qs, L.Qs, nb, nodes, par, coefs, plan->form.n_coefs, S, bt.load,
                  w.Wside, nullptr, st);
   P.stop(e, st, DPG_STAGE_POINT, 0.0, nb * D8 * (B.nWip + B.nWvol + B.nWrhs + B.nWside + V * dim));
-  // operands
+  // operands: weighted IP-term values (with the DPOEQU factors), B operands
   e = P.start(st);
-  launch_operand(dim, w.Wip, B.prog.n_lt, nvar, B.tab.R_ip, B.tab.var_of_dof, N, L.QG, nb, w.At, B.nAt, B.Npad,
-                 (int64_t)B.prog.n_lt * B.Npad, st);
+  const int KG = B.prog.n_lt * L.QG;
+  launch_ip_operand(dim, w.Wip, B.prog.n_lt, nvar, B.tab.R_ip, B.tab.var_of_dof, N, L.QG, nb, w.At, B.nAt, B.Npad,
+                    w.s, B.nS, st);
   launch_operand(dim, w.Wvol, B.prog.n_vb, nvar, B.tab.R_vol, B.tab.var_of_dof, N, L.Q, nb, w.Tv, B.nTv,
                  (int64_t)L.Q * B.Npad, B.Npad, st);
   if (B.prog.n_sb > 0)
     launch_operand(dim, w.Wside, B.prog.n_sb, nvar, B.tab.R_side, B.tab.var_of_dof, N, S * L.Qs, nb, w.Ts, B.nTs,
                    (int64_t)S * L.Qs * B.Npad, B.Npad, st);
-  double opnd = (double)B.prog.n_lt * L.QG + (double)B.prog.n_vb * L.Q + (double)B.prog.n_sb * S * L.Qs;
+  double opnd = (double)KG + (double)B.prog.n_vb * L.Q + (double)B.prog.n_sb * S * L.Qs;
   P.stop(e, st, DPG_STAGE_OPERAND, nb * opnd * N * 2.0 * B.npr, nb * opnd * N * D8);
-  // G = At^T At (lower)
+  // DPOEQU scaling is applied in the GEMM epilogues, except when G itself is an
+  // output: then G is written unscaled, copied out, and scaled in place
+  const double* s_ep = Gout ? nullptr : w.s;
+  // G = At^T At (lower), scaled by s_i s_j in the epilogue
   GemmArgs g{};
   g.X = w.At;
   g.ldx = B.Npad;
@@ -486,14 +495,17 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
   g.ldc = lda;
   g.sc = sA;
   g.m = g.n = N;
-  g.k = B.prog.n_lt * L.QG;
+  g.k = KG;
   g.lower = 1;
   g.alpha = 1.0;
   g.beta = 0.0;
   g.ncells = nb;
+  g.srow = s_ep;
+  g.scol = s_ep;
+  g.ss = B.nS;
   e = P.start(st);
   launch_gemm(g, kGemmGram, st);
-  P.stop(e, st, DPG_STAGE_GRAM, nb * B.gram_flops, nb * ((double)g.k * N + N * (N + 1.0) / 2.0) * D8);
+  P.stop(e, st, DPG_STAGE_GRAM, nb * B.gram_flops, nb * ((double)KG * N + N * (N + 1.0) / 2.0) * D8);
   if (Gout) {
     e = P.start(st);
     launch_extract_sym(w.A, lda, sA, 0, N, 1.0, Gout + (size_t)c0 * N * N, nb, st);
@@ -519,6 +531,8 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
     b.k = L.Q;
     b.alpha = 1.0;
     b.ncells = nb;
+    b.scol = s_ep;
+    b.ss = B.nS;
     launch_gemm(b, kGemmBmat, st);
     bfl += 2.0 * b.m * b.n * b.k;
     bby += ((double)b.k * b.n + (double)b.m * b.n) * D8;
@@ -542,20 +556,23 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
       b.k = L.Qs;
       b.alpha = 1.0;
       b.ncells = nb;
+      b.scol = s_ep;
+      b.ss = B.nS;
       launch_gemm(b, kGemmBmat, st);
       bfl += 2.0 * b.m * b.n * b.k;
       bby += ((double)b.k * b.n + (double)b.m * b.n) * D8;
     }
   }
-  // load vector row, residual row
-  launch_rhs(dim, w.Wrhs, nvar, B.tab.R_vol, B.tab.var_of_dof, N, L.Q, nb, w.A, lda, sA, N + M, st);
+  // load vector row, residual row (from the already scaled rows)
+  launch_rhs(dim, w.Wrhs, nvar, B.tab.R_vol, B.tab.var_of_dof, N, L.Q, nb, w.A, lda, sA, N + M, s_ep, B.nS, st);
   launch_residual_row(w.A, lda, sA, N, M, N, N + M, N + M + 1, u ? u + (size_t)c0 * M : nullptr, nb, st);
   bfl += 2.0 * L.Q * N * B.npr + (u ? 2.0 * M * N : 0.0);
   P.stop(e, st, DPG_STAGE_BMAT, nb * bfl, nb * bby);
-  // DPOEQU scaling of G (and of the rows of B, l, r)
-  e = P.start(st);
-  launch_scaling(w.A, lda, sA, 0, B.D, N, N, w.s, B.nS, nb, 1.0, st);
-  P.stop(e, st, DPG_STAGE_SCALE, 0.0, nb * ((double)N * (N + 1) / 2.0 + (double)(B.D - N) * N) * 16.0);
+  if (Gout) {
+    e = P.start(st);
+    launch_scaling(w.A, lda, sA, 0, B.D, N, N, w.s, B.nS, nb, 1.0, st);
+    P.stop(e, st, DPG_STAGE_SCALE, 0.0, nb * ((double)N * (N + 1) / 2.0 + (double)(B.D - N) * N) * 16.0);
+  }
   HIP_CHECK(hipMemsetAsync(plan->info.p, 0, sizeof(int32_t) * nb, st));
   // the trailing block starts as 0 and is written (beta = 0) by the closing SYRK
   factor_columns(plan, B, nb, w, 0, N, 0.0, st);
