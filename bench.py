@@ -24,6 +24,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2.5e15, "hbm": 8.0e12}
+PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2, "fp16x3": 3}
 
 CONFIGS = {
     "c4": dict(name="poisson3d_hex32_p3_test5", form=("poisson", 3), topo="hex", n=(32, 32, 32), jitter=0.05,
@@ -82,10 +83,14 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c4", choices=sorted(CONFIGS))
-    ap.add_argument("--precision", default="fp64", choices=["fp64"])
+    ap.add_argument("--precision", default="fp64", choices=sorted(PRECISIONS))
+    ap.add_argument("--compare", default="fp32,bf16x3,fp16x3",
+                    help="mixed precisions timed against the FP64 form (empty: none)")
     ap.add_argument("--workspace-gib", type=float, default=64.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-schur", action="store_true")
+    ap.add_argument("--calibrate-copy-gib", type=float, default=0.0,
+                    help="also run a plain device copy of this size (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (local assembly shards by "
@@ -104,7 +109,7 @@ def main():
     d_nodes = torch.from_numpy(nodes).to(dev)
     d_par = torch.from_numpy(par).to(dev)
     ws = args.workspace_gib * 2 ** 30
-    A = dpg.LocalAssembler(f, [b], workspace_bytes=ws)
+    A = dpg.LocalAssembler(f, [b], workspace_bytes=ws, precision=PRECISIONS[args.precision])
     out = {
         "K": torch.empty((C, info.M, info.M), dtype=torch.float64, device=dev),
         "rhs": torch.empty((C, info.M), dtype=torch.float64, device=dev),
@@ -132,7 +137,7 @@ def main():
 
     # one more step on a plan that times every launch with HIP events on the
     # launch stream (kept out of the timed steps above)
-    Ap = dpg.LocalAssembler(f, [b], workspace_bytes=ws, profile=True)
+    Ap = dpg.LocalAssembler(f, [b], workspace_bytes=ws, profile=True, precision=PRECISIONS[args.precision])
     step(Ap)
     torch.cuda.synchronize()
     tm, _ = Ap.timings(reset=True)
@@ -151,7 +156,8 @@ def main():
     if os.path.exists(traffic_file):
         with open(traffic_file) as fh:
             tr = json.load(fh)
-        roofline["traffic"] = tr.get(roofline["kernel"])
+        k = tr.get(roofline["kernel"])
+        roofline["traffic"] = None if k is None else k["bytes"]
 
     result = {
         "metric": "cells/s local DPG assembly, mixed-prec vs FP64 GPU form at ≤1e-10 rel err",
@@ -164,7 +170,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else f"{args.precision} contractions, f64 factorisation",
         "data": "synthetic (seeded jittered mesh and sine-mode load)",
         "config": {"workload": cfg["name"], "cells": C, "N": info.N, "M": info.M, "T": info.T,
                    "outputs": "K,rhs" + (",schur,schur_rhs" if want_schur else ""),
@@ -174,10 +180,50 @@ def main():
         "roofline": roofline,
         "stages": stages,
     }
+    # mixed-precision forms of the same step, timed the same way, with their
+    # distance from the FP64 GPU result over all cells
+    comps = [p for p in args.compare.split(",") if p and p != args.precision]
+    if comps:
+        ref_K = out["K"].clone() if args.precision == "fp64" else None
+        mixed = {}
+        for p in comps:
+            Am = dpg.LocalAssembler(f, [b], workspace_bytes=ws, precision=PRECISIONS[p])
+            step(Am)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                step(Am)
+            torch.cuda.synchronize()
+            dtm = time.perf_counter() - t0
+            Am.close()
+            entry = {"cells_per_s": round(C * args.steps / dtm, 2), "ms_per_step": round(dtm / args.steps * 1e3, 3),
+                     "speedup_vs_fp64": round(ms_per_step / (dtm / args.steps * 1e3), 3)}
+            if ref_K is not None:
+                rel = []
+                for c0 in range(0, C, 1024):  # chunks: no full-size temporary
+                    a, r = out["K"][c0:c0 + 1024], ref_K[c0:c0 + 1024]
+                    rel.append(torch.linalg.matrix_norm(a - r) / torch.linalg.matrix_norm(r))
+                rel = torch.cat(rel)
+                entry["rel_err_K_vs_fp64_gpu_max"] = float(rel.max())
+                entry["rel_err_K_vs_fp64_gpu_median"] = float(rel.median())
+            mixed[p] = entry
+        if ref_K is not None:
+            out["K"].copy_(ref_K)
+            del ref_K
+        result["mixed_precision"] = mixed
     if not args.no_cpu:
         cpu, err = cpu_baseline(f, b, nodes, par, load, out["K"])
         result["cpu_baseline"] = cpu
         result["rel_err_K_vs_fp64_oracle"] = err
+    if args.calibrate_copy_gib > 0:
+        n = int(args.calibrate_copy_gib * 2 ** 30 // 8)
+        src = torch.ones(n, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        result["calibration_copy_bytes"] = n * 8
+        del src, dst
     print(json.dumps(result))
 
 

@@ -420,6 +420,171 @@ __global__ __launch_bounds__(256) void gemm_gram_f64(GemmArgs g) { gemm_tn_f64_b
 __global__ __launch_bounds__(256) void gemm_bmat_f64(GemmArgs g) { gemm_tn_f64_body(g); }
 __global__ __launch_bounds__(256) void gemm_update_f64(GemmArgs g) { gemm_tn_f64_body(g); }
 
+// --------------------------------------------------- mixed-precision GEMM
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int MKC = 32;  // k per chunk
+
+template <int MODE>
+struct MixedOps;
+
+template <>
+struct MixedOps<DPG_PREC_FP32> {
+  static constexpr int kSplit = 0;
+};
+template <>
+struct MixedOps<DPG_PREC_BF16X3> {
+  static constexpr int kSplit = 1;
+  typedef __bf16 T;
+  typedef bf16x8 V;
+  static __device__ inline f4 mma(V a, V b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+};
+template <>
+struct MixedOps<DPG_PREC_FP16X3> {
+  static constexpr int kSplit = 1;
+  typedef _Float16 T;
+  typedef f16x8 V;
+  static __device__ inline f4 mma(V a, V b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
+};
+
+template <int MODE>
+__device__ inline void gemm_mixed_body(const GemmArgs& g) {
+  int b = xcd_remap(blockIdx.x, gridDim.x);
+  int cell = b / g.ntiles, tile = b % g.ntiles;
+  if (cell >= g.ncells) return;
+  int ti, tj;
+  tile_coords(tile, g.lower, g.tiles_m, ti, tj);
+  const int i0 = ti * GT, j0 = tj * GT;
+  const double* __restrict__ X = g.X + (size_t)cell * g.sx;
+  const double* __restrict__ Y = g.Y + (size_t)cell * g.sy;
+  double* __restrict__ C = g.C + (size_t)cell * g.sc;
+  const double* px = g.px ? g.px + (size_t)cell * g.ss : nullptr;
+  const double* py = g.py ? g.py + (size_t)cell * g.ss : nullptr;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
+  // per-thread column scale factors of the 8 elements it stages (its columns are fixed)
+  double sxv[8], syv[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    int ii = (t + 256 * r) & 63;
+    sxv[r] = (px && i0 + ii < g.m) ? px[i0 + ii] : 1.0;
+    syv[r] = (py && j0 + ii < g.n) ? py[j0 + ii] : 1.0;
+  }
+  double xr[8], yr[8];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      int e = t + 256 * r, kk = e >> 6, ii = e & 63, kg = k0 + kk;
+      xr[r] = (kg < g.k && i0 + ii < g.m) ? X[(size_t)kg * g.ldx + i0 + ii] * sxv[r] : 0.0;
+      yr[r] = (kg < g.k && j0 + ii < g.n) ? Y[(size_t)kg * g.ldy + j0 + ii] * syv[r] : 0.0;
+    }
+  };
+  f4 acc[2][2], accx[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int bb = 0; bb < 2; ++bb) {
+      acc[a][bb] = f4{0.f, 0.f, 0.f, 0.f};
+      accx[a][bb] = f4{0.f, 0.f, 0.f, 0.f};
+    }
+  if constexpr (MixedOps<MODE>::kSplit == 0) {
+    __shared__ float Xs[MKC][GT + 4];
+    __shared__ float Ys[MKC][GT + 4];
+    fetch(0);
+    for (int k0 = 0; k0 < g.k; k0 += MKC) {
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        int e = t + 256 * r;
+        Xs[e >> 6][e & 63] = (float)xr[r];
+        Ys[e >> 6][e & 63] = (float)yr[r];
+      }
+      __syncthreads();
+      if (k0 + MKC < g.k) fetch(k0 + MKC);
+#pragma unroll
+      for (int k4 = 0; k4 < MKC; k4 += 4) {
+        const int kr = k4 + (lane >> 4), cc = lane & 15;
+        float xa[2], yb[2];
+        xa[0] = Xs[kr][wr * 32 + cc];
+        xa[1] = Xs[kr][wr * 32 + 16 + cc];
+        yb[0] = Ys[kr][wc * 32 + cc];
+        yb[1] = Ys[kr][wc * 32 + 16 + cc];
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+          for (int bi = 0; bi < 2; ++bi)
+            acc[bj][bi] = __builtin_amdgcn_mfma_f32_16x16x4f32(yb[bj], xa[bi], acc[bj][bi], 0, 0, 0);
+      }
+    }
+  } else {
+    typedef typename MixedOps<MODE>::T T;
+    typedef typename MixedOps<MODE>::V V;
+    // row-major tiles (row, k) so that a lane reads its 8 consecutive k in one access
+    __shared__ __attribute__((aligned(16))) T Xh[GT][MKC + 8];
+    __shared__ __attribute__((aligned(16))) T Xl[GT][MKC + 8];
+    __shared__ __attribute__((aligned(16))) T Yh[GT][MKC + 8];
+    __shared__ __attribute__((aligned(16))) T Yl[GT][MKC + 8];
+    fetch(0);
+    for (int k0 = 0; k0 < g.k; k0 += MKC) {
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        int e = t + 256 * r, kk = e >> 6, ii = e & 63;
+        T xh = (T)xr[r], yh = (T)yr[r];
+        Xh[ii][kk] = xh;
+        Xl[ii][kk] = (T)(xr[r] - (double)xh);
+        Yh[ii][kk] = yh;
+        Yl[ii][kk] = (T)(yr[r] - (double)yh);
+      }
+      __syncthreads();
+      if (k0 + MKC < g.k) fetch(k0 + MKC);
+      const int kq = 8 * (lane >> 4), cc = lane & 15;
+      V xah[2], xal[2], ybh[2], ybl[2];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        xah[q] = *reinterpret_cast<const V*>(&Xh[wr * 32 + q * 16 + cc][kq]);
+        xal[q] = *reinterpret_cast<const V*>(&Xl[wr * 32 + q * 16 + cc][kq]);
+        ybh[q] = *reinterpret_cast<const V*>(&Yh[wc * 32 + q * 16 + cc][kq]);
+        ybl[q] = *reinterpret_cast<const V*>(&Yl[wc * 32 + q * 16 + cc][kq]);
+      }
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi) {
+          acc[bj][bi] = MixedOps<MODE>::mma(ybh[bj], xah[bi], acc[bj][bi]);
+          accx[bj][bi] = MixedOps<MODE>::mma(ybh[bj], xal[bi], accx[bj][bi]);
+          accx[bj][bi] = MixedOps<MODE>::mma(ybl[bj], xah[bi], accx[bj][bi]);
+        }
+    }
+  }
+  // f32 16x16 result layout: D[a][b] in lane b + 16*(a/4), element a%4
+#pragma unroll
+  for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        int i = i0 + wr * 32 + bi * 16 + (lane & 15);
+        int j = j0 + wc * 32 + bj * 16 + 4 * (lane >> 4) + v;
+        if (i < g.m && j < g.n && (!g.lower || i >= j)) {
+          double* cp = C + (size_t)i + (size_t)j * g.ldc;
+          double val = g.alpha * ((double)acc[bj][bi][v] + (double)accx[bj][bi][v]);
+          if (g.srow) val *= g.srow[(size_t)cell * g.ss + i];
+          if (g.scol) val *= g.scol[(size_t)cell * g.ss + j];
+          if (g.beta != 0.0) val = fma(g.beta, *cp, val);
+          *cp = val;
+        }
+      }
+}
+
+__global__ __launch_bounds__(256) void gemm_gram_f32(GemmArgs g) { gemm_mixed_body<DPG_PREC_FP32>(g); }
+__global__ __launch_bounds__(256) void gemm_bmat_f32(GemmArgs g) { gemm_mixed_body<DPG_PREC_FP32>(g); }
+__global__ __launch_bounds__(256) void gemm_gram_bf16x3(GemmArgs g) { gemm_mixed_body<DPG_PREC_BF16X3>(g); }
+__global__ __launch_bounds__(256) void gemm_bmat_bf16x3(GemmArgs g) { gemm_mixed_body<DPG_PREC_BF16X3>(g); }
+__global__ __launch_bounds__(256) void gemm_gram_f16x3(GemmArgs g) { gemm_mixed_body<DPG_PREC_FP16X3>(g); }
+__global__ __launch_bounds__(256) void gemm_bmat_f16x3(GemmArgs g) { gemm_mixed_body<DPG_PREC_FP16X3>(g); }
+
 // ------------------------------------------------------------ panel kernels
 // Columns [k0, k0+kb) (kb <= 64) of each cell's lower-stored matrix:
 //   potrf_block  one workgroup per cell: POTRF of the 64x64 diagonal block in
@@ -722,6 +887,30 @@ void launch_gemm(GemmArgs g, int role, hipStream_t st) {
   else if (role == kGemmBmat) gemm_bmat_f64<<<grid, 256, 0, st>>>(g);
   else gemm_update_f64<<<grid, 256, 0, st>>>(g);
   check_launch("gemm_tn_f64");
+}
+
+void launch_gemm_mixed(GemmArgs g, int precision, int role, hipStream_t st) {
+  if (g.m <= 0 || g.n <= 0 || g.ncells <= 0) return;
+  g.tiles_m = (g.m + GT - 1) / GT;
+  int tiles_n = (g.n + GT - 1) / GT;
+  g.ntiles = g.lower ? g.tiles_m * (g.tiles_m + 1) / 2 : g.tiles_m * tiles_n;
+  int64_t nblocks = (int64_t)g.ntiles * g.ncells;
+  if (nblocks > 0x7fffffff) throw DeviceError("gemm grid too large");
+  dim3 grid((unsigned)nblocks);
+  bool gram = role == kGemmGram;
+  if (precision == DPG_PREC_FP32) {
+    if (gram) gemm_gram_f32<<<grid, 256, 0, st>>>(g);
+    else gemm_bmat_f32<<<grid, 256, 0, st>>>(g);
+  } else if (precision == DPG_PREC_BF16X3) {
+    if (gram) gemm_gram_bf16x3<<<grid, 256, 0, st>>>(g);
+    else gemm_bmat_bf16x3<<<grid, 256, 0, st>>>(g);
+  } else if (precision == DPG_PREC_FP16X3) {
+    if (gram) gemm_gram_f16x3<<<grid, 256, 0, st>>>(g);
+    else gemm_bmat_f16x3<<<grid, 256, 0, st>>>(g);
+  } else {
+    throw DeviceError("unknown precision");
+  }
+  check_launch("gemm_mixed");
 }
 
 void launch_panel(double* A, int64_t lda, int64_t sA, int D, int k0, int kb, double* Ld, int64_t sLd, double* Li,

@@ -31,6 +31,10 @@ struct GemmArgs {
   const double* srow;
   const double* scol;
   int64_t ss;
+  // optional (mixed-precision GEMMs): scale X(k,i) by px[i] and Y(k,j) by py[j]
+  // before the conversion to the low-precision operand format
+  const double* px;
+  const double* py;
 };
 
 void launch_point(int dim, int kind, const DevProgram* prog, const DevTables& tab, int npts, int Qs, int ncells,
@@ -47,6 +51,10 @@ void launch_ip_operand(int dim, const double* Wip, int n_lt, int nvar, const dou
                        hipStream_t st);
 enum { kGemmGram = 0, kGemmBmat = 1, kGemmUpdate = 2 };
 void launch_gemm(GemmArgs g, int role, hipStream_t st);
+// the same contraction with FP64 operands converted on load to FP32 (precision
+// DPG_PREC_FP32) or to hi+lo BF16/FP16 pairs (DPG_PREC_BF16X3/FP16X3, products
+// hi*hi + hi*lo + lo*hi), FP32 MFMA accumulation, FP64 epilogue
+void launch_gemm_mixed(GemmArgs g, int precision, int role, hipStream_t st);
 // Ld: per-cell (rows, 64) store of the factored diagonal blocks (row k0 + r, column r');
 // Li: per-cell 64x64 scratch for the inverse of the current diagonal block
 void launch_panel(double* A, int64_t lda, int64_t sA, int D, int k0, int kb, double* Ld, int64_t sLd, double* Li,

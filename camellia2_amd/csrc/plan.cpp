@@ -503,8 +503,15 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
   g.srow = s_ep;
   g.scol = s_ep;
   g.ss = B.nS;
+  const int prec = plan->opt.precision;
+  if (prec != DPG_PREC_FP64) {
+    // low-precision operands are pre-scaled (keeps FP16 in range): X(k,i) s_i, Y(k,j) s_j
+    g.srow = g.scol = nullptr;
+    g.px = g.py = s_ep;
+  }
   e = P.start(st);
-  launch_gemm(g, kGemmGram, st);
+  if (prec == DPG_PREC_FP64) launch_gemm(g, kGemmGram, st);
+  else launch_gemm_mixed(g, prec, kGemmGram, st);
   P.stop(e, st, DPG_STAGE_GRAM, nb * B.gram_flops, nb * ((double)KG * N + N * (N + 1.0) / 2.0) * D8);
   if (Gout) {
     e = P.start(st);
@@ -533,7 +540,13 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
     b.ncells = nb;
     b.scol = s_ep;
     b.ss = B.nS;
-    launch_gemm(b, kGemmBmat, st);
+    if (prec == DPG_PREC_FP64) {
+      launch_gemm(b, kGemmBmat, st);
+    } else {
+      b.scol = nullptr;
+      b.py = s_ep;
+      launch_gemm_mixed(b, prec, kGemmBmat, st);
+    }
     bfl += 2.0 * b.m * b.n * b.k;
     bby += ((double)b.k * b.n + (double)b.m * b.n) * D8;
   }
@@ -558,7 +571,13 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
       b.ncells = nb;
       b.scol = s_ep;
       b.ss = B.nS;
-      launch_gemm(b, kGemmBmat, st);
+      if (prec == DPG_PREC_FP64) {
+        launch_gemm(b, kGemmBmat, st);
+      } else {
+        b.scol = nullptr;
+        b.py = s_ep;
+        launch_gemm_mixed(b, prec, kGemmBmat, st);
+      }
       bfl += 2.0 * b.m * b.n * b.k;
       bby += ((double)b.k * b.n + (double)b.m * b.n) * D8;
     }
@@ -657,8 +676,11 @@ int dpg_plan_create(const dpg_form_desc* form, const dpg_bucket_desc* buckets, i
     std::unique_ptr<dpg_plan> plan(new dpg_plan());
     plan->form = *form;
     if (options) plan->opt = *options;
-    if (plan->opt.precision != DPG_PREC_FP64)
-      throw std::invalid_argument("this build assembles in FP64 only (precision must be DPG_PREC_FP64)");
+    if (plan->opt.precision < DPG_PREC_FP64 || plan->opt.precision > DPG_PREC_FP16X3)
+      throw std::invalid_argument("unknown precision");
+    if (plan->opt.refine != 0)
+      throw std::invalid_argument("refine: a refinement of G^-1 B cannot correct an error that sits in G itself "
+                                  "(the factorisation is FP64); not supported");
     HIP_CHECK(hipGetDevice(&plan->device));
     plan->prof.on = plan->opt.profile != 0;
     plan->buckets.resize(nbuckets);

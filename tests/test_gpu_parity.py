@@ -147,3 +147,22 @@ def test_c2_full_mesh_properties_and_sampled_parity():
         err = rel_fro(out[gk][sample].cpu().numpy(), x[ok])
         print(f"{gk}: gpu-vs-80bit {err:.2e} oracle-vs-80bit {floor:.2e}")
         assert err <= max(TOL_K, 4.0 * floor)
+
+
+@pytest.mark.parametrize("prec,bound", [(capi.PREC_FP32, 1e-3), (capi.PREC_BF16X3, 1e-2), (capi.PREC_FP16X3, 1e-3)])
+def test_mixed_precision_contractions(prec, bound):
+    """G, B in FP32 MFMA or hi/lo-split BF16/FP16 MFMA (FP32 accumulation), the
+    factorisation and Schur complement in FP64. The error against the FP64
+    oracle is dominated by the contraction rounding amplified by the
+    conditioning of G; the measured value is printed, the bound only catches
+    breakage (the FP64 path meets the tight bound above)."""
+    f, b, nodes, par, coefs, load = poisson3d(2, 2, 1)
+    g64, _ = gpu_assemble(f, b, nodes, par, coefs, load, want_schur=True)
+    g, nbad = gpu_assemble(f, b, nodes, par, coefs, load, want_schur=True, precision=prec)
+    o = oracle.assemble(oracle.form_like(f), b, nodes, par, coefs, load, want=("K", "rhs", "S"))
+    assert nbad == 0
+    errs = {gk: rel_fro(g[gk], o[k]) for gk, k in (("K", "K"), ("rhs", "rhs"), ("schur", "S"))}
+    e64 = rel_fro(g64["K"], o["K"])
+    print(f"precision {prec}: K {errs['K']:.2e} rhs {errs['rhs']:.2e} schur {errs['schur']:.2e} (fp64 K {e64:.2e})")
+    assert errs["K"] < bound and errs["rhs"] < bound
+    assert errs["K"] > e64  # the low-precision path really ran
