@@ -106,7 +106,7 @@ EXPORTS = [
     "dpg_form_stokes_vgp_uw", "dpg_form_graph_norm", "dpg_mesh_quad", "dpg_mesh_hex",
     "dpg_load_sine_modes", "dpg_rng_uniform", "dpg_rng_normal", "dpg_bucket_sizes",
     "dpg_plan_create", "dpg_plan_destroy", "dpg_plan_workspace", "dpg_plan_timings", "dpg_assemble",
-    "dpg_energy_error",
+    "dpg_energy_error", "dpg_adjoint",
 ]
 
 _lib = None
@@ -140,6 +140,8 @@ def lib():
         L.dpg_assemble.argtypes = [ct.c_void_p, ct.c_int, P(Batch), P(Outputs), ct.c_void_p]
         L.dpg_energy_error.argtypes = [ct.c_void_p, ct.c_int, P(Batch), ct.c_void_p, ct.c_void_p,
                                        ct.c_void_p]
+        L.dpg_adjoint.argtypes = [ct.c_void_p, ct.c_int, P(Batch), ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                  ct.c_void_p]
         _lib = L
     return _lib
 

@@ -28,6 +28,13 @@ struct DevProgram {
   int32_t sb_beg[kMaxGroups + 1];
   int32_t sb_flux[kMaxGroups];
   int32_t rhs_beg[2];
+  // trial-dof offsets of the groups (adjoint): volume group -> block offset,
+  // side group -> offset of its side-0 block (sides follow with stride card)
+  int32_t vb_off[kMaxGroups];
+  int32_t sb_off[kMaxGroups];
+  int32_t sb_card[kMaxGroups];
+  int32_t nf;
+  int32_t ncoef;
   DSummand s[kMaxSummands];
 };
 

@@ -821,6 +821,184 @@ __global__ void extract_row(const double* __restrict__ A, int64_t lda, int64_t s
   out[(size_t)c * n + a] = sign * A[(size_t)c * sA + row + (size_t)(r0 + a) * lda];
 }
 
+// ------------------------------------------------------------------ adjoint
+// e = G^-1 r from the stored factor of S G S = Lt Lt^T: the residual row of A
+// holds y = Lt^-1 S r after the partial Cholesky, so e = S Lt^-T y by one
+// back substitution (diagonal blocks in Ld, the rest below them in A).
+__global__ __launch_bounds__(256) void backsolve_kernel(const double* __restrict__ A, int64_t lda, int64_t sA,
+                                                        int N, int row_r, const double* __restrict__ Ld,
+                                                        int64_t sLd, const double* __restrict__ s, int64_t ss,
+                                                        double* __restrict__ e, int64_t se) {
+  extern __shared__ double z[];
+  const int c = blockIdx.x, t = threadIdx.x;
+  const double* Ac = A + (size_t)c * sA;
+  const double* Lc = Ld + (size_t)c * sLd;
+  for (int i = t; i < N; i += blockDim.x) z[i] = Ac[(size_t)row_r + (size_t)i * lda];
+  for (int j = N - 1; j >= 0; --j) {
+    __syncthreads();
+    const int kj = j & ~63;
+    const double zj = z[j] / Lc[(size_t)j * 64 + (j - kj)];
+    __syncthreads();
+    if (t == 0) z[j] = zj;
+    // y_i -= L[j][i] z_j for i < j
+    for (int i = t; i < j; i += blockDim.x) {
+      double lji = (i >= kj) ? Lc[(size_t)j * 64 + (i - kj)] : Ac[(size_t)j + (size_t)i * lda];
+      z[i] = fma(-lji, zj, z[i]);
+    }
+  }
+  __syncthreads();
+  for (int i = t; i < N; i += blockDim.x) e[(size_t)c * se + i] = s[(size_t)c * ss + i] * z[i];
+}
+
+__device__ inline double dcoef_value(const DSummand& sm, int k, const double* cc, const double* n) {
+  if (sm.slot != k) return 0.0;
+  double v = sm.scale;
+  double a = cc[k];
+  v *= (sm.fn == DPG_FN_SQRT) ? 0.5 / sqrt(a) : 1.0;
+  if (sm.normal >= 0) v *= n ? n[sm.normal] : 0.0;
+  return v;
+}
+
+// Per-point contributions to d(r^T G^-1 r)/d(coef k) = -2 e^T dB/dk u - e^T dG/dk e:
+//   kind 0 (Gram points):  -w detJ * 2 * lt(e_h) * dlt/dk(e_h)        for every IP term
+//   kind 1 (volume):       -2 w detJ * dcoef/dk * op(e_h) * u_h|block  for every volume term
+//   kind 2 (sides):        -2 w |J t| * dcoef/dk * op(e_h) * u_h|side (x parity for fluxes)
+// e_h's operators come from the reference prims (recomputed, not stored) and
+// the transforms at the point; partial[c][p0 + p][k] are reduced afterwards.
+template <int DIM>
+__global__ __launch_bounds__(128) void adjoint_point_kernel(int kind, const DevProgram* __restrict__ prog,
+                                                            DevTables tab, int npts, int Qs, int ncells, int N,
+                                                            const double* __restrict__ nodes,
+                                                            const int8_t* __restrict__ parities,
+                                                            const double* __restrict__ coefs, int nsides,
+                                                            const double* __restrict__ e, int64_t se,
+                                                            const double* __restrict__ u, int M,
+                                                            double* __restrict__ partial, int64_t sp, int p0) {
+  int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= ncells * npts) return;
+  const int c = gid / npts, p = gid % npts;
+  const int V = DIM == 2 ? 4 : 8;
+  constexpr int NPR = DIM + 1;
+  const int nvar = prog->nvar, ncoef = prog->ncoef;
+  const double* X = nodes + (size_t)c * V * DIM;
+  const double* cc = coefs + (size_t)c * ncoef;
+  const double* dN = kind == 0 ? tab.dN_ip : (kind == 1 ? tab.dN_vol : tab.dN_side);
+  const double* R = kind == 0 ? tab.R_ip : (kind == 1 ? tab.R_vol : tab.R_side);
+  PointGeo<DIM> g;
+  point_geometry<DIM>(X, dN + (size_t)p * V * DIM, V, g);
+  // reference prims of the test function e_h: E[var][r]
+  double E[DPG_MAX_VARS][NPR];
+  for (int v = 0; v < nvar; ++v)
+#pragma unroll
+    for (int r = 0; r < NPR; ++r) E[v][r] = 0.0;
+  const double* ec = e + (size_t)c * se;
+  for (int i = 0; i < N; ++i) {
+    const int v = tab.var_of_dof[i];
+    const double ei = ec[i];
+#pragma unroll
+    for (int r = 0; r < NPR; ++r) E[v][r] = fma(ei, R[((size_t)p * NPR + r) * N + i], E[v][r]);
+  }
+  const double* uc = u + (size_t)c * M;
+  double n[3] = {0.0, 0.0, 0.0};
+  double wm;
+  int s = 0;
+  if (kind == 0) {
+    wm = tab.w_ip[p] * g.detJ;
+  } else if (kind == 1) {
+    wm = tab.w_vol[p] * g.detJ;
+  } else {
+    s = p / Qs;
+    double T[2][3];
+    for (int a = 0; a < DIM - 1; ++a)
+      for (int d = 0; d < DIM; ++d) {
+        double acc = 0.0;
+        for (int ee = 0; ee < DIM; ++ee) acc += g.J[d][ee] * tab.side_tan[((size_t)s * 2 + a) * DIM + ee];
+        T[a][d] = acc;
+      }
+    if constexpr (DIM == 2) {
+      n[0] = T[0][1];
+      n[1] = -T[0][0];
+    } else {
+      n[0] = T[0][1] * T[1][2] - T[0][2] * T[1][1];
+      n[1] = T[0][2] * T[1][0] - T[0][0] * T[1][2];
+      n[2] = T[0][0] * T[1][1] - T[0][1] * T[1][0];
+    }
+    double len = 0.0;
+    for (int d = 0; d < DIM; ++d) len += n[d] * n[d];
+    len = sqrt(len);
+    for (int d = 0; d < DIM; ++d) n[d] /= len;
+    wm = tab.w_side[p] * len;
+  }
+  const double* nn = kind == 2 ? n : nullptr;
+  double grad[DPG_MAX_COEFS];
+  for (int k = 0; k < ncoef; ++k) grad[k] = 0.0;
+  double t[NPR];
+  auto op_of_e = [&](const DSummand& sm) {
+    op_to_prims<DIM>(sm.op, g, nn, t);
+    double v = 0.0;
+#pragma unroll
+    for (int r = 0; r < NPR; ++r) v = fma(t[r], E[sm.var][r], v);
+    return v;
+  };
+  if (kind == 0) {
+    for (int lt = 0; lt < prog->n_lt; ++lt) {
+      double val = 0.0;
+      double dval[DPG_MAX_COEFS];
+      for (int k = 0; k < ncoef; ++k) dval[k] = 0.0;
+      for (int si = prog->lt_beg[lt]; si < prog->lt_beg[lt + 1]; ++si) {
+        const DSummand& sm = prog->s[si];
+        const double o = op_of_e(sm);
+        val = fma(coef_value(sm, cc, nullptr), o, val);
+        for (int k = 0; k < ncoef; ++k) dval[k] = fma(dcoef_value(sm, k, cc, nullptr), o, dval[k]);
+      }
+      for (int k = 0; k < ncoef; ++k) grad[k] -= 2.0 * wm * val * dval[k];
+    }
+  } else if (kind == 1) {
+    const double* phi = tab.field_vol + (size_t)p * prog->nf;
+    for (int gi = 0; gi < prog->n_vb; ++gi) {
+      double ug = 0.0;
+      const double* ub = uc + prog->vb_off[gi];
+      for (int j = 0; j < prog->nf; ++j) ug = fma(ub[j], phi[j], ug);
+      for (int si = prog->vb_beg[gi]; si < prog->vb_beg[gi + 1]; ++si) {
+        const DSummand& sm = prog->s[si];
+        if (sm.slot < 0) continue;
+        const double o = op_of_e(sm);
+        for (int k = 0; k < ncoef; ++k) grad[k] -= 2.0 * wm * dcoef_value(sm, k, cc, nullptr) * o * ug;
+      }
+    }
+  } else {
+    const int q = p % Qs;
+    for (int gi = 0; gi < prog->n_sb; ++gi) {
+      const int card = prog->sb_card[gi];
+      const double* tb = (prog->sb_flux[gi] ? tab.flux : tab.trace) + (size_t)q * card;
+      const double* ub = uc + prog->sb_off[gi] + (size_t)s * card;
+      double ug = 0.0;
+      for (int j = 0; j < card; ++j) ug = fma(ub[j], tb[j], ug);
+      if (prog->sb_flux[gi]) ug *= (double)parities[(size_t)c * nsides + s];
+      for (int si = prog->sb_beg[gi]; si < prog->sb_beg[gi + 1]; ++si) {
+        const DSummand& sm = prog->s[si];
+        if (sm.slot < 0) continue;
+        const double o = op_of_e(sm);
+        for (int k = 0; k < ncoef; ++k) grad[k] -= 2.0 * wm * dcoef_value(sm, k, cc, nn) * o * ug;
+      }
+    }
+  }
+  double* out = partial + (size_t)c * sp + (size_t)(p0 + p) * ncoef;
+  for (int k = 0; k < ncoef; ++k) out[k] = grad[k];
+}
+
+// grad[c][k] = sum over all points of partial[c][p][k], in point order
+__global__ void grad_reduce_kernel(const double* __restrict__ partial, int64_t sp, int npts, int ncoef, int ncells,
+                                   double* __restrict__ grad) {
+  int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= ncells * ncoef) return;
+  int c = gid / ncoef, k = gid % ncoef;
+  const double* pc = partial + (size_t)c * sp;
+  double acc = 0.0;
+  for (int p = 0; p < npts; ++p) acc += pc[(size_t)p * ncoef + k];
+  grad[(size_t)c * ncoef + k] = acc;
+}
+
 // ------------------------------------------------------------ host launchers
 static void check_launch(const char* what) {
   hipError_t e = hipGetLastError();
@@ -950,6 +1128,33 @@ void launch_extract_sym(const double* A, int64_t lda, int64_t sA, int r0, int n,
   int nt = (n + 31) / 32;
   extract_sym<<<dim3(nt, nt, ncells), 256, 0, st>>>(A, lda, sA, r0, n, sign, out);
   check_launch("extract_sym");
+}
+
+void launch_backsolve(const double* A, int64_t lda, int64_t sA, int N, int row_r, const double* Ld, int64_t sLd,
+                      const double* s, int64_t ss, double* e, int64_t se, int ncells, hipStream_t st) {
+  backsolve_kernel<<<dim3(ncells), 256, sizeof(double) * N, st>>>(A, lda, sA, N, row_r, Ld, sLd, s, ss, e, se);
+  check_launch("backsolve_kernel");
+}
+
+void launch_adjoint_points(int dim, int kind, const DevProgram* prog, const DevTables& tab, int npts, int Qs,
+                           int ncells, int N, const double* nodes, const int8_t* par, const double* coefs, int nsides,
+                           const double* e, int64_t se, const double* u, int M, double* partial, int64_t sp, int p0,
+                           hipStream_t st) {
+  if (npts <= 0) return;
+  dim3 grid((ncells * npts + 127) / 128);
+  if (dim == 2)
+    adjoint_point_kernel<2><<<grid, 128, 0, st>>>(kind, prog, tab, npts, Qs, ncells, N, nodes, par, coefs, nsides, e,
+                                                  se, u, M, partial, sp, p0);
+  else
+    adjoint_point_kernel<3><<<grid, 128, 0, st>>>(kind, prog, tab, npts, Qs, ncells, N, nodes, par, coefs, nsides, e,
+                                                  se, u, M, partial, sp, p0);
+  check_launch("adjoint_point_kernel");
+}
+
+void launch_grad_reduce(const double* partial, int64_t sp, int npts, int ncoef, int ncells, double* grad,
+                        hipStream_t st) {
+  grad_reduce_kernel<<<dim3((ncells * ncoef + 255) / 256), 256, 0, st>>>(partial, sp, npts, ncoef, ncells, grad);
+  check_launch("grad_reduce_kernel");
 }
 
 void launch_extract_row(const double* A, int64_t lda, int64_t sA, int row, int r0, int n, double sign, double* out,

@@ -66,6 +66,15 @@ void launch_residual_row(double* A, int64_t lda, int64_t sA, int N, int M, int r
                          const double* u, int ncells, hipStream_t st);
 void launch_extract_sym(const double* A, int64_t lda, int64_t sA, int r0, int n, double sign, double* out, int ncells,
                         hipStream_t st);
+// adjoint: e = S Lt^-T y (y = residual row of A), per-point gradient partials, reduction
+void launch_backsolve(const double* A, int64_t lda, int64_t sA, int N, int row_r, const double* Ld, int64_t sLd,
+                      const double* s, int64_t ss, double* e, int64_t se, int ncells, hipStream_t st);
+void launch_adjoint_points(int dim, int kind, const DevProgram* prog, const DevTables& tab, int npts, int Qs,
+                           int ncells, int N, const double* nodes, const int8_t* par, const double* coefs, int nsides,
+                           const double* e, int64_t se, const double* u, int M, double* partial, int64_t sp, int p0,
+                           hipStream_t st);
+void launch_grad_reduce(const double* partial, int64_t sp, int npts, int ncoef, int ncells, double* grad,
+                        hipStream_t st);
 void launch_extract_row(const double* A, int64_t lda, int64_t sA, int row, int r0, int n, double sign, double* out,
                         int ncells, hipStream_t st);
 

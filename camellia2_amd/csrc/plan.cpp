@@ -194,7 +194,7 @@ struct BucketPlan {
   int npr = 0, D = 0;
   int64_t lda = 0, Npad = 0;
   // per-cell element counts (doubles) of the workspace regions
-  int64_t nA = 0, nAt = 0, nTv = 0, nTs = 0, nWip = 0, nWvol = 0, nWrhs = 0, nWside = 0, nS = 0, nLd = 0, nLi = 0;
+  int64_t nA = 0, nAt = 0, nTv = 0, nTs = 0, nWip = 0, nWvol = 0, nWrhs = 0, nWside = 0, nS = 0, nLd = 0, nLi = 0, nE = 0, nP = 0;
   int64_t per_cell_bytes = 0;
   double flops = 0.0;
   double gram_flops = 0.0;  // sum over IP terms of QG * n(n+1), n = dofs the term touches
@@ -252,6 +252,9 @@ void compile_program(const dpg_form_desc& f, BucketPlan& B) {
     }
   }
   P.vb_beg[P.n_vb] = ns;
+  P.nf = B.L.nf;
+  P.ncoef = f.n_coefs;
+  for (int g = 0; g < P.n_vb; ++g) P.vb_off[g] = B.L.blocks[B.vb_block[g]].offset;
   // side groups: trace / flux variables
   B.sb_var.clear();
   for (int v = 0; v < f.n_trial; ++v)
@@ -261,6 +264,12 @@ void compile_program(const dpg_form_desc& f, BucketPlan& B) {
   for (int g = 0; g < P.n_sb; ++g) {
     int v = B.sb_var[g];
     P.sb_flux[g] = f.trial_kind[v] == DPG_FLUX ? 1 : 0;
+    P.sb_off[g] = -1;
+    for (const TrialBlock& blk : B.L.blocks)
+      if (blk.var == v && blk.side == 0) {
+        P.sb_off[g] = blk.offset;
+        P.sb_card[g] = blk.card;
+      }
     P.sb_beg[g] = ns;
     for (int i = 0; i < f.n_bterms; ++i) {
       const dpg_bterm& t = f.bterms[i];
@@ -339,12 +348,15 @@ void build_bucket(dpg_plan* plan, const dpg_bucket_desc& desc, BucketPlan& B) {
   B.nS = round_up(std::max(L.N, L.M), 8);
   B.nLd = (int64_t)round_up(L.N + L.F, 64) * 64;  // factored diagonal blocks of both stages
   B.nLi = 64 * 64;                                 // inverse of the current diagonal block
-  int64_t tot = B.nA + B.nAt + B.nTv + B.nTs + B.nWip + B.nWvol + B.nWrhs + B.nWside + B.nS + B.nLd + B.nLi;
+  B.nE = round_up(L.N, 8);                          // adjoint: e = G^-1 r
+  B.nP = (int64_t)(L.QG + L.Q + S * L.Qs) * std::max(1, f.n_coefs);  // adjoint: per-point partials
+  int64_t tot = B.nA + B.nAt + B.nTv + B.nTs + B.nWip + B.nWvol + B.nWrhs + B.nWside + B.nS + B.nLd + B.nLi +
+                B.nE + B.nP;
   B.per_cell_bytes = tot * (int64_t)sizeof(double);
 }
 
 struct Work {
-  double *A, *At, *Tv, *Ts, *Wip, *Wvol, *Wrhs, *Wside, *s, *Ld, *Li;
+  double *A, *At, *Tv, *Ts, *Wip, *Wvol, *Wrhs, *Wside, *s, *Ld, *Li, *E, *Pt;
 };
 
 Work carve(dpg_plan* plan, const BucketPlan& B, int nb) {
@@ -366,6 +378,8 @@ Work carve(dpg_plan* plan, const BucketPlan& B, int nb) {
   w.s = take(B.nS);
   w.Ld = take(B.nLd);
   w.Li = take(B.nLi);
+  w.E = take(B.nE);
+  w.Pt = take(B.nP);
   return w;
 }
 
@@ -764,6 +778,48 @@ int dpg_energy_error(dpg_plan* plan, int bucket, const dpg_batch* batch, const d
       dpg::Work w = dpg::carve(plan, B, nb);
       dpg::stage_factor(plan, B, *batch, c0, nb, w, u, nullptr, st);
       dpg::launch_extract_row(w.A, B.lda, B.nA, B.D - 1, B.D - 1, 1, -1.0, eta2 + c0, nb, st);
+      plan->prof.collect(st, nb);
+      bad += dpg::count_failures(plan, nb, nullptr, st);
+    }
+    return bad;
+  });
+}
+
+int dpg_adjoint(dpg_plan* plan, int bucket, const dpg_batch* batch, const double* u, double* eta2, double* grad,
+                void* stream) {
+  return dpg::guarded_status([&] {
+    dpg::check_batch(plan, bucket, batch);
+    if (!u || !eta2) throw std::invalid_argument("u and eta2 are required");
+    if (grad && plan->form.n_coefs <= 0) throw std::invalid_argument("the form has no per-cell coefficients");
+    if (plan->opt.precision != DPG_PREC_FP64) throw std::invalid_argument("the adjoint runs in FP64 only");
+    if (batch->ncells == 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    const dpg::BucketPlan& B = plan->buckets[bucket];
+    const dpg::ElementLayout& L = B.L;
+    const int N = L.N, M = L.M, S = L.rc.nsides, dim = L.dim, V = L.rc.nverts, nc = plan->form.n_coefs;
+    int C = batch->ncells;
+    int maxb = dpg::batch_cells(plan, B, C);
+    dpg::prepare_workspace(plan, B, maxb);
+    int bad = 0;
+    for (int c0 = 0; c0 < C; c0 += maxb) {
+      int nb = std::min(maxb, C - c0);
+      dpg::Work w = dpg::carve(plan, B, nb);
+      dpg::stage_factor(plan, B, *batch, c0, nb, w, u, nullptr, st);
+      dpg::launch_extract_row(w.A, B.lda, B.nA, B.D - 1, B.D - 1, 1, -1.0, eta2 + c0, nb, st);
+      if (grad) {
+        const double* nodes = batch->nodes + (size_t)c0 * V * dim;
+        const int8_t* par = batch->parities + (size_t)c0 * S;
+        const double* coefs = batch->coefs + (size_t)c0 * nc;
+        const double* uc = u + (size_t)c0 * M;
+        dpg::launch_backsolve(w.A, B.lda, B.nA, N, B.D - 1, w.Ld, B.nLd, w.s, B.nS, w.E, B.nE, nb, st);
+        dpg::launch_adjoint_points(dim, 0, B.prog_d, B.tab, L.QG, L.Qs, nb, N, nodes, par, coefs, S, w.E, B.nE, uc, M,
+                                   w.Pt, B.nP, 0, st);
+        dpg::launch_adjoint_points(dim, 1, B.prog_d, B.tab, L.Q, L.Qs, nb, N, nodes, par, coefs, S, w.E, B.nE, uc, M,
+                                   w.Pt, B.nP, L.QG, st);
+        dpg::launch_adjoint_points(dim, 2, B.prog_d, B.tab, S * L.Qs, L.Qs, nb, N, nodes, par, coefs, S, w.E, B.nE,
+                                   uc, M, w.Pt, B.nP, L.QG + L.Q, st);
+        dpg::launch_grad_reduce(w.Pt, B.nP, L.QG + L.Q + S * L.Qs, nc, nb, grad + (size_t)c0 * nc, st);
+      }
       plan->prof.collect(st, nb);
       bad += dpg::count_failures(plan, nb, nullptr, st);
     }

@@ -183,3 +183,41 @@ class LocalAssembler:
         check(lib().dpg_energy_error(self._plan, bucket, ct.byref(b), ct.c_void_p(u.data_ptr()),
                                      ct.c_void_p(eta2.data_ptr()), ct.c_void_p(s)))
         return eta2
+
+    def energy_error_and_gradient(self, bucket, nodes, parities, u, coefs, load=None, stream=None):
+        """(eta2 (C,), d eta2 / d coefs (C, n_coefs)) from the adjoint kernels (dpg_adjoint)."""
+        t = self._torch
+        C = nodes.shape[0]
+        eta2 = t.empty((C,), dtype=t.float64, device=nodes.device)
+        grad = t.empty((C, self.form.n_coefs), dtype=t.float64, device=nodes.device)
+        assert u.is_cuda and u.dtype == t.float64 and u.is_contiguous()
+        b = self._batch(nodes, parities, coefs, load)
+        s = stream.cuda_stream if stream is not None else t.cuda.current_stream().cuda_stream
+        check(lib().dpg_adjoint(self._plan, bucket, ct.byref(b), ct.c_void_p(u.data_ptr()),
+                                ct.c_void_p(eta2.data_ptr()), ct.c_void_p(grad.data_ptr()), ct.c_void_p(s)))
+        return eta2, grad
+
+
+class EnergyErrorFunction:
+    """torch.autograd.Function over the adjoint: sum_c eta2_c as a function of the
+    per-cell coefficients (C, n_coefs). Used to check the adjoint against finite
+    differences; the forward and backward both run in the HIP library."""
+
+    @staticmethod
+    def apply(assembler, bucket, nodes, parities, u, coefs, load=None):
+        import torch
+
+        class _F(torch.autograd.Function):
+            @staticmethod
+            def forward(ctx, c):
+                eta2, grad = assembler.energy_error_and_gradient(bucket, nodes, parities, u, c.detach().contiguous(),
+                                                                 load)
+                ctx.save_for_backward(grad)
+                return eta2
+
+            @staticmethod
+            def backward(ctx, g):
+                (grad,) = ctx.saved_tensors
+                return g[:, None] * grad
+
+        return _F.apply(coefs)

@@ -269,6 +269,15 @@ int dpg_assemble(dpg_plan* plan, int bucket, const dpg_batch* batch, const dpg_o
 int dpg_energy_error(dpg_plan* plan, int bucket, const dpg_batch* batch, const double* u,
                      double* eta2, void* stream);
 
+/* Adjoint of the energy-error indicator with respect to the per-cell PDE
+ * coefficients (no counterpart in the reference, which has no derivatives):
+ * eta2[c] = r^T G^-1 r as dpg_energy_error, and grad[c][k] = d eta2[c] / d coefs[c][k]
+ *   = -2 e^T (dB/dk) u - e^T (dG/dk) e,  e = G^-1 r,
+ * with e from the factor of the same pass (no refactorisation) and the basis
+ * values recomputed at the quadrature points. grad is (C, n_coefs) or NULL. */
+int dpg_adjoint(dpg_plan* plan, int bucket, const dpg_batch* batch, const double* u, double* eta2, double* grad,
+                void* stream);
+
 #ifdef __cplusplus
 }
 #endif

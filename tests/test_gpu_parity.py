@@ -166,3 +166,67 @@ def test_mixed_precision_contractions(prec, bound):
     print(f"precision {prec}: K {errs['K']:.2e} rhs {errs['rhs']:.2e} schur {errs['schur']:.2e} (fp64 K {e64:.2e})")
     assert errs["K"] < bound and errs["rhs"] < bound
     assert errs["K"] > e64  # the low-precision path really ran
+
+
+def _cdr_adjoint_case(n=6):
+    f, b, nodes, par, coefs, load = cdr2d(n, n, k=3, kt=4)  # C5 orders: p=2, test 4
+    s = dpg.sizes(f, b)
+    u = dpg.rng_normal(0xC0FFEE + 55, nodes.shape[0] * s.M).reshape(nodes.shape[0], s.M)
+    return f, b, nodes, par, coefs, load, u
+
+
+def test_adjoint_gradient_matches_central_differences():
+    """d(eta2)/d(eps, beta_x, beta_y, c) per cell from the adjoint kernels vs
+    central differences of the GPU forward (step 1e-6, all cells perturbed at
+    once: cells are independent) and vs central differences of the oracle."""
+    f, b, nodes, par, coefs, load, u = _cdr_adjoint_case()
+    A = dpg.LocalAssembler(f, [b])
+    dn, dp, du = dev(nodes), dev(par), dev(u)
+    eta2, grad = A.energy_error_and_gradient(0, dn, dp, du, dev(coefs), load)
+    eta2, grad = eta2.cpu().numpy(), grad.cpu().numpy()
+    eta_e = A.energy_error(0, dn, dp, du, dev(coefs), load).cpu().numpy()
+    assert np.array_equal(eta2, eta_e)
+    ref = oracle.energy_error(oracle.form_like(f), b, nodes, par, u, coefs, load)
+    assert np.max(np.abs(eta2 - ref) / np.abs(ref)) < 1e-11
+    h = 1e-6
+    fd = np.zeros_like(grad)
+    fdo = np.zeros_like(grad)
+    for k in range(coefs.shape[1]):
+        cp, cm = coefs.copy(), coefs.copy()
+        cp[:, k] += h
+        cm[:, k] -= h
+        ep = A.energy_error(0, dn, dp, du, dev(cp), load).cpu().numpy()
+        em = A.energy_error(0, dn, dp, du, dev(cm), load).cpu().numpy()
+        fd[:, k] = (ep - em) / (2 * h)
+        fdo[:, k] = (oracle.energy_error(oracle.form_like(f), b, nodes, par, u, cp, load) -
+                     oracle.energy_error(oracle.form_like(f), b, nodes, par, u, cm, load)) / (2 * h)
+    def central(k, hh):
+        cp, cm = coefs.copy(), coefs.copy()
+        cp[:, k] += hh
+        cm[:, k] -= hh
+        return (A.energy_error(0, dn, dp, du, dev(cp), load).cpu().numpy() -
+                A.energy_error(0, dn, dp, du, dev(cm), load).cpu().numpy()) / (2 * hh)
+
+    for k in range(coefs.shape[1]):
+        # Richardson extrapolation of central differences: O(h^4) truncation,
+        # rounding ~1e-12/h: a reference far sharper than the 1e-6 step
+        rich = (4.0 * central(k, 5e-4) - central(k, 1e-3)) / 3.0
+        e_gpu = np.linalg.norm(grad[:, k] - fd[:, k]) / np.linalg.norm(fd[:, k])
+        e_orc = np.linalg.norm(grad[:, k] - fdo[:, k]) / np.linalg.norm(fdo[:, k])
+        e_fd = np.linalg.norm(fd[:, k] - fdo[:, k]) / np.linalg.norm(fdo[:, k])
+        e_rich = np.linalg.norm(grad[:, k] - rich) / np.linalg.norm(rich)
+        print(f"coef {k}: adjoint vs central differences (h=1e-6) GPU {e_gpu:.2e} oracle {e_orc:.2e} "
+              f"(the two difference quotients differ by {e_fd:.2e}); adjoint vs Richardson {e_rich:.2e}")
+        # step-1e-6 quotients carry ~1e-12/(2h) relative rounding noise of eta2: bound 5e-6
+        assert e_gpu < 5e-6 and e_orc < 5e-6
+        assert e_rich < 1e-7
+
+
+def test_adjoint_through_torch_autograd():
+    f, b, nodes, par, coefs, load, u = _cdr_adjoint_case(4)
+    A = dpg.LocalAssembler(f, [b])
+    c = dev(coefs).requires_grad_(True)
+    eta2 = dpg.EnergyErrorFunction.apply(A, 0, dev(nodes), dev(par), dev(u), c, load)
+    eta2.sum().backward()
+    _, grad = A.energy_error_and_gradient(0, dev(nodes), dev(par), dev(u), dev(coefs), load)
+    assert torch.equal(c.grad, grad)
