@@ -33,6 +33,7 @@ CONFIGS = {
                k=4, kt=5, modes=8, seed=0xC0FFEE + 2),
     "c1": dict(name="poisson2d_quad16_p2_test4", form=("poisson", 2), topo="quad", n=(16, 16), jitter=0.10,
                k=3, kt=4, modes=8, seed=0xC0FFEE + 1),
+    "c3": dict(name="cd2d_hp_graded_quads49360_p1-6_dp1-3", hp=True, n0=16, levels=10),
 }
 
 
@@ -77,6 +78,155 @@ def cpu_baseline(f, b, nodes, par, load, gpu_K, target_s=12.0):
             "sample": f"first {n} cells of the same mesh, oracle K+rhs+Schur, {dt:.1f} s wall, {threads} OpenMP threads"}, err
 
 
+def cpu_baseline_hp(f, buckets, groups, nodes, par, coefs, load, gpu_K, target_s=12.0):
+    """Oracle on a stratified sample of the hp mesh (the same share of every
+    element type), plus the GPU-vs-oracle error of K on that sample."""
+    from oracle import oracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    ff = oracle.form_like(f)
+    per = []
+    for b, idx in zip(buckets, groups.values()):
+        sel = idx[:threads]
+        t0 = time.perf_counter()
+        oracle.assemble(ff, b, nodes[sel], par[sel], coefs[sel], load, want=("K", "rhs", "S", "g"), nthreads=threads)
+        per.append((time.perf_counter() - t0) / len(sel))
+    full = sum(t * len(idx) for t, idx in zip(per, groups.values()))
+    frac = min(1.0, target_s / full)
+    n_tot, t_tot, err = 0, 0.0, 0.0
+    for bi, (b, idx) in enumerate(zip(buckets, groups.values())):
+        n = max(threads, int(round(frac * len(idx) / threads)) * threads)
+        sel = idx[:n]
+        t0 = time.perf_counter()
+        o = oracle.assemble(ff, b, nodes[sel], par[sel], coefs[sel], load, want=("K", "rhs", "S", "g"),
+                            nthreads=threads)
+        t_tot += time.perf_counter() - t0
+        n_tot += len(sel)
+        Kg = gpu_K[bi][:len(sel)].cpu().numpy()
+        err = max(err, float(np.max(np.linalg.norm((Kg - o["K"]).reshape(len(sel), -1), axis=1) /
+                                    np.linalg.norm(o["K"].reshape(len(sel), -1), axis=1))))
+    return {"value": n_tot / t_tot, "unit": "cells/s", "cores": threads, "kind": "port",
+            "sample": f"{n_tot} cells, the first {frac:.1%} of every element type, oracle K+rhs+Schur, "
+                      f"{t_tot:.1f} s wall, {threads} OpenMP threads"}, err
+
+
+def bench_hp(args, cfg):
+    """hp mesh: every element type assembled by one captured graph launch,
+    against per-bucket plain launches (default and autotuned tiles)."""
+    import torch
+
+    from camellia2_amd import dpg
+    from camellia2_amd.workloads import cd_hp, hp_buckets, hp_groups
+
+    f, nodes, par, coefs, load, p, dp, lvl = cd_hp(cfg["n0"], cfg["levels"])
+    groups = hp_groups(p, dp)
+    buckets = hp_buckets(groups)
+    C = nodes.shape[0]
+    dev = torch.device("cuda:0")
+    ws = args.workspace_gib * 2 ** 30
+    infos = [dpg.sizes(f, b) for b in buckets]
+    flops = sum(len(idx) * i.flops_per_cell for idx, i in zip(groups.values(), infos))
+    ins = [(torch.from_numpy(np.ascontiguousarray(nodes[idx])).to(dev),
+            torch.from_numpy(np.ascontiguousarray(par[idx])).to(dev),
+            torch.from_numpy(np.ascontiguousarray(coefs[idx])).to(dev)) for idx in groups.values()]
+
+    def alloc():
+        return [{"K": torch.empty((len(idx), i.M, i.M), dtype=torch.float64, device=dev),
+                 "rhs": torch.empty((len(idx), i.M), dtype=torch.float64, device=dev),
+                 "schur": torch.empty((len(idx), i.T, i.T), dtype=torch.float64, device=dev),
+                 "schur_rhs": torch.empty((len(idx), i.T), dtype=torch.float64, device=dev)}
+                for idx, i in zip(groups.values(), infos)]
+
+    out_e, out_g = alloc(), alloc()
+
+    def eager(asm):
+        for bi, (n_, p_, c_) in enumerate(ins):
+            asm.local_stiffness_and_rhs(bi, n_, p_, c_, load, want_schur=True, want_info=False, out=out_e[bi])
+
+    def timed(fn, steps):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    A = dpg.LocalAssembler(f, buckets, workspace_bytes=ws)
+    dt_plain = timed(lambda: eager(A), args.steps)
+    t0 = time.perf_counter()
+    tiles = [A.autotune(bi, n_, p_, c_, load, sample=512) for bi, (n_, p_, c_) in enumerate(ins)]
+    t_tune = time.perf_counter() - t0
+    dt_tuned = timed(lambda: eager(A), args.steps)
+    items = [dict(bucket=bi, nodes=n_, parities=p_, coefs=c_, load=load, want_schur=True, want_info=True,
+                  out=out_g[bi]) for bi, (n_, p_, c_) in enumerate(ins)]
+    G = dpg.AssemblyGraph(A, items, lanes=args.lanes)
+    dt_graph = timed(G.launch, args.steps)
+    same = all(torch.equal(out_g[bi][k], out_e[bi][k]) for bi in range(len(buckets))
+               for k in ("K", "rhs", "schur", "schur_rhs"))
+    nbad = int(sum(int((o["info"] != 0).sum()) for o in G.outputs()))
+    gws = G.workspace_bytes()
+    G.close()
+    A.close()
+
+    # per-stage kernel times of the tuned per-bucket launches (HIP events)
+    Ap = dpg.LocalAssembler(f, buckets, workspace_bytes=ws, profile=True)
+    for bi, t in enumerate(tiles):
+        Ap.set_tiles(bi, *t)
+    eager(Ap)
+    torch.cuda.synchronize()
+    tm, _ = Ap.timings(reset=True)
+    Ap.close()
+    stages = {k: {"ms": round(v["ms"], 3), "launches": v["launches"], "gflop": round(v["flops"] / 1e9, 3),
+                  "gbyte": round(v["bytes"] / 1e9, 3)} for k, v in tm.items()}
+    dom = max(["gram", "update", "panel"], key=lambda k: tm.get(k, {"ms": 0.0})["ms"])
+    d = tm[dom]
+    achieved = d["flops"] / (d["ms"] * 1e-3)
+    roofline = {"bound": "mfma", "kernel": {"gram": "gemm_gram_f64", "update": "gemm_update_f64",
+                                            "panel": "potrf_block+panel_trsm"}[dom],
+                "achieved": round(achieved / 1e12, 3), "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK["fp64"], 4), "traffic": None,
+                "avg_launch_ms": round(d["ms"] / d["launches"], 4), "flop_per_launch": d["flops"] / d["launches"],
+                "note": "summed over the element types' launches of this kernel (per-bucket plain launches)"}
+    cps = C / dt_graph
+    result = {
+        "metric": "cells/s local DPG assembly, mixed-prec vs FP64 GPU form at ≤1e-10 rel err",
+        "value": round(cps, 2),
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt_graph * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded graded hp mesh, per-cell p, dp, eps; sine-mode load)",
+        "config": {"workload": cfg["name"], "cells": C, "element_types": len(buckets), "lanes": args.lanes,
+                   "outputs": "K,rhs,schur,schur_rhs", "precision": "fp64",
+                   "mode": "one hipGraph launch over all element types"},
+        "gflops": round(flops / dt_graph / 1e9, 1),
+        "algorithmic_flop_per_step": flops,
+        "per_bucket_plain": {"cells_per_s_default_tiles": round(C / dt_plain, 2),
+                             "cells_per_s_autotuned_tiles": round(C / dt_tuned, 2),
+                             "graph_speedup_vs_default": round(dt_plain / dt_graph, 3),
+                             "graph_speedup_vs_autotuned": round(dt_tuned / dt_graph, 3)},
+        "graph_equals_plain_bitwise": bool(same),
+        "non_spd_cells": nbad,
+        "autotune": {"seconds": round(t_tune, 2),
+                     "tiles": {f"k{k}_kt{kt}": list(t) for (k, kt), t in zip(groups, tiles)}},
+        "graph_workspace_gib": round(gws / 2 ** 30, 2),
+        "roofline": roofline,
+        "stages": stages,
+    }
+    if not args.no_cpu:
+        cpu, err = cpu_baseline_hp(f, buckets, groups, nodes, par, coefs, load, [o["K"] for o in out_e])
+        result["cpu_baseline"] = cpu
+        result["rel_err_K_vs_fp64_oracle"] = err
+    print(json.dumps(result))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,12 +239,14 @@ def main():
     ap.add_argument("--workspace-gib", type=float, default=64.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-schur", action="store_true")
+    ap.add_argument("--lanes", type=int, default=4, help="hp config: concurrent streams inside the graph")
     ap.add_argument("--calibrate-copy-gib", type=float, default=0.0,
                     help="also run a plain device copy of this size (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (local assembly shards by "
-              "cells with no exchange; see DESIGN.md)", file=sys.stderr)
+              "cells with no exchange between GPUs; a multi-GPU run is one independent process per GPU)",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -102,6 +254,8 @@ def main():
     from camellia2_amd import dpg
 
     cfg = CONFIGS[args.config]
+    if cfg.get("hp"):
+        return bench_hp(args, cfg)
     f, b, nodes, par, load = build_inputs(cfg)
     info = dpg.sizes(f, b)
     C = nodes.shape[0]

@@ -20,7 +20,7 @@ QUAD, HEX = 0, 1
 FIELD, FIELD_VECTOR, TRACE, FLUX = 0, 1, 2, 3
 HGRAD, HDIV = 0, 1
 OP_VALUE, OP_DX, OP_DY, OP_DZ, OP_X, OP_Y, OP_Z, OP_DIV, OP_DOT_NORMAL = range(9)
-FN_VALUE, FN_SQRT = 0, 1
+FN_VALUE, FN_SQRT, FN_INV = 0, 1, 2
 PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_FP16X3 = 0, 1, 2, 3
 
 
@@ -97,16 +97,25 @@ class Outputs(ct.Structure):
                 ("schur_rhs", ct.c_void_p), ("info", ct.c_void_p)]
 
 
+class Tiles(ct.Structure):
+    _fields_ = [("gram", ct.c_int32), ("bmat", ct.c_int32), ("update", ct.c_int32), ("pad", ct.c_int32)]
+
+
+class WorkItem(ct.Structure):
+    _fields_ = [("bucket", ct.c_int32), ("pad", ct.c_int32), ("batch", Batch), ("out", Outputs)]
+
+
 LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
 LIB_PATH = os.path.join(LIB_DIR, "libcamellia2_dpg.so")
 
 # every entry point declared in include/dpg_capi.h
 EXPORTS = [
     "dpg_version", "dpg_last_error", "dpg_form_poisson_uw", "dpg_form_cdr_uw",
-    "dpg_form_stokes_vgp_uw", "dpg_form_graph_norm", "dpg_mesh_quad", "dpg_mesh_hex",
+    "dpg_form_stokes_vgp_uw", "dpg_form_convection_diffusion_uw", "dpg_form_graph_norm", "dpg_mesh_quad", "dpg_mesh_hex",
     "dpg_load_sine_modes", "dpg_rng_uniform", "dpg_rng_normal", "dpg_bucket_sizes",
     "dpg_plan_create", "dpg_plan_destroy", "dpg_plan_workspace", "dpg_plan_timings", "dpg_assemble",
-    "dpg_energy_error", "dpg_adjoint",
+    "dpg_energy_error", "dpg_adjoint", "dpg_mesh_quad_graded", "dpg_plan_get_tiles", "dpg_plan_set_tiles",
+    "dpg_plan_autotune", "dpg_graph_create", "dpg_graph_launch", "dpg_graph_workspace", "dpg_graph_destroy",
 ]
 
 _lib = None
@@ -122,13 +131,15 @@ def lib():
         P = ct.POINTER
         L.dpg_version.restype = ct.c_char_p
         L.dpg_last_error.restype = ct.c_char_p
-        for name in ("dpg_form_poisson_uw", "dpg_form_cdr_uw"):
+        for name in ("dpg_form_poisson_uw", "dpg_form_cdr_uw", "dpg_form_convection_diffusion_uw"):
             getattr(L, name).argtypes = [ct.c_int, P(FormDesc)]
         L.dpg_form_stokes_vgp_uw.argtypes = [ct.c_int, ct.c_double, P(FormDesc)]
         L.dpg_form_graph_norm.argtypes = [P(FormDesc)]
         L.dpg_mesh_quad.argtypes = [ct.c_int, ct.c_int, ct.c_double, ct.c_uint64, ct.c_void_p, ct.c_void_p]
         L.dpg_mesh_hex.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_double, ct.c_uint64, ct.c_void_p,
                                    ct.c_void_p]
+        L.dpg_mesh_quad_graded.argtypes = [ct.c_int, ct.c_int, ct.c_double, ct.c_uint64, ct.c_void_p,
+                                           ct.c_void_p, ct.c_void_p]
         L.dpg_load_sine_modes.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_uint64, P(LoadDesc)]
         L.dpg_rng_uniform.argtypes = [ct.c_uint64, ct.c_double, ct.c_double, ct.c_void_p, ct.c_size_t]
         L.dpg_rng_normal.argtypes = [ct.c_uint64, ct.c_void_p, ct.c_size_t]
@@ -142,6 +153,14 @@ def lib():
                                        ct.c_void_p]
         L.dpg_adjoint.argtypes = [ct.c_void_p, ct.c_int, P(Batch), ct.c_void_p, ct.c_void_p, ct.c_void_p,
                                   ct.c_void_p]
+        L.dpg_plan_get_tiles.argtypes = [ct.c_void_p, ct.c_int, P(Tiles)]
+        L.dpg_plan_set_tiles.argtypes = [ct.c_void_p, ct.c_int, P(Tiles)]
+        L.dpg_plan_autotune.argtypes = [ct.c_void_p, ct.c_int, P(Batch), ct.c_int, ct.c_void_p]
+        L.dpg_graph_create.argtypes = [ct.c_void_p, P(WorkItem), ct.c_int, ct.c_int, ct.c_void_p,
+                                       P(ct.c_void_p)]
+        L.dpg_graph_launch.argtypes = [ct.c_void_p, ct.c_void_p]
+        L.dpg_graph_workspace.argtypes = [ct.c_void_p, P(ct.c_size_t)]
+        L.dpg_graph_destroy.argtypes = [ct.c_void_p]
         _lib = L
     return _lib
 

@@ -189,6 +189,25 @@ dpg_form_desc cdr_uw(int dim) {
   return bf.compile(bf.graphNorm(), rhs, dim + 2);
 }
 
+// ConvectionDiffusionFormulation, ConvectionDiffusionFormulation.cpp:23-96, per-cell [eps, beta_1..beta_d]:
+//   b = (sigma/eps, tau) + (u, div tau) - <uhat, tau.n> + (-beta u + sigma, grad v) + <tc, v>
+dpg_form_desc convection_diffusion_uw(int dim) {
+  VarFactory vf;
+  int u = vf.fieldVar(), sigma = vf.fieldVar(true), uhat = vf.traceVar(), tc = vf.fluxVar();
+  int v = vf.testVar(DPG_HGRAD), tau = vf.testVar(DPG_HDIV);
+  const int EPS = 0, BETA = 1;
+  BF bf(vf, dim);
+  for (int d = 0; d < dim; ++d) bf.addTerm({sigma, d}, {tau, DPG_OP_X + d}, coef(1.0, EPS, DPG_FN_INV));
+  bf.addTerm({u, 0}, {tau, DPG_OP_DIV});
+  bf.addTerm({uhat, 0}, {tau, DPG_OP_DOT_NORMAL}, coef(-1.0));
+  for (int d = 0; d < dim; ++d) bf.addTerm({u, 0}, {v, DPG_OP_DX + d}, coef(-1.0, BETA + d));
+  for (int d = 0; d < dim; ++d) bf.addTerm({sigma, d}, {v, DPG_OP_DX + d});
+  bf.addTerm({tc, 0}, {v, DPG_OP_VALUE});
+  RHS rhs;
+  rhs.addTerm(0, {v, DPG_OP_VALUE});
+  return bf.compile(bf.graphNorm(), rhs, dim + 1);
+}
+
 // StokesVGPFormulation (steady, conforming traces), StokesVGPFormulation.cpp:156-408; rhs :1015-1028
 dpg_form_desc stokes_vgp_uw(int dim, double mu) {
   if (dim != 2) throw std::invalid_argument("stokes: only spaceDim 2 is built");
@@ -249,6 +268,14 @@ int dpg_form_cdr_uw(int spatial_dim, dpg_form_desc* out) {
     if (!out) throw std::invalid_argument("out is NULL");
     if (spatial_dim != 2 && spatial_dim != 3) throw std::invalid_argument("spatial_dim must be 2 or 3");
     *out = dpg::cdr_uw(spatial_dim);
+  });
+}
+
+int dpg_form_convection_diffusion_uw(int spatial_dim, dpg_form_desc* out) {
+  return dpg::guarded([&] {
+    if (!out) throw std::invalid_argument("out is NULL");
+    if (spatial_dim != 2 && spatial_dim != 3) throw std::invalid_argument("spatial_dim must be 2 or 3");
+    *out = dpg::convection_diffusion_uw(spatial_dim);
   });
 }
 

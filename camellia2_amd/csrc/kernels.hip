@@ -113,7 +113,7 @@ __device__ inline double coef_value(const DSummand& s, const double* cc, const d
   double v = s.scale;
   if (s.slot >= 0) {
     double a = cc[s.slot];
-    v *= (s.fn == DPG_FN_SQRT) ? sqrt(a) : a;
+    v *= (s.fn == DPG_FN_SQRT) ? sqrt(a) : (s.fn == DPG_FN_INV ? 1.0 / a : a);
   }
   if (s.normal >= 0) v *= n ? n[s.normal] : 0.0;
   return v;
@@ -317,7 +317,7 @@ __global__ __launch_bounds__(256) void ip_operand_kernel(const double* __restric
 // ------------------------------------------------------------------- GEMM
 // C(i,j) = alpha * sum_k X(k,i) Y(k,j) + beta * C(i,j), X(k,i) = X[k*ldx + i],
 // C col-major (C[i + j*ldc]); 64x64 tiles, 4 waves, v_mfma_f64_16x16x4.
-constexpr int GT = 64, GKC = 16, GLD = GT + 4;  // 64x64 tiles, k-chunks of 16
+constexpr int GT = 64, GKC = 16;  // 64x64 tiles, k-chunks of 16
 
 __device__ inline int xcd_remap(int b, int nb) {
   // consecutive logical tiles on one XCD (blocks are dealt round-robin to 8 XCDs)
@@ -338,32 +338,39 @@ __device__ inline void tile_coords(int tile, int lower, int tiles_m, int& ti, in
   }
 }
 
+// TT x TT output tile per workgroup of (TT/32)^2 waves; each wave owns a 32x32
+// sub-tile (2x2 MFMA blocks). TT = 64 for large operands, TT = 32 for buckets
+// whose dimensions would leave most of a 64-tile empty.
+template <int TT>
 __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
+  constexpr int NT = (TT / 32) * (TT / 32) * 64;  // threads
+  constexpr int PER = GKC * TT / NT;              // staged elements per thread and operand
+  constexpr int LD = TT + 4;
   int b = xcd_remap(blockIdx.x, gridDim.x);
   int cell = b / g.ntiles, tile = b % g.ntiles;
   if (cell >= g.ncells) return;
   int ti, tj;
   tile_coords(tile, g.lower, g.tiles_m, ti, tj);
-  const int i0 = ti * GT, j0 = tj * GT;
+  const int i0 = ti * TT, j0 = tj * TT;
   const double* __restrict__ X = g.X + (size_t)cell * g.sx;
   const double* __restrict__ Y = g.Y + (size_t)cell * g.sy;
   double* __restrict__ C = g.C + (size_t)cell * g.sc;
-  __shared__ double Xs[GKC][GLD];
-  __shared__ double Ys[GKC][GLD];
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
+  __shared__ double Xs[GKC][LD];
+  __shared__ double Ys[GKC][LD];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, wr = w / (TT / 32), wc = w % (TT / 32);
   d4 acc[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int bb = 0; bb < 2; ++bb) acc[a][bb] = d4{0.0, 0.0, 0.0, 0.0};
   const int kbeg = 0, kend = g.k;
-  // each thread stages 4 elements of X and of Y per k-chunk; the next chunk is
-  // fetched into registers while the current one is multiplied from LDS
-  double xr[4], yr[4];
+  // each thread stages PER elements of X and of Y per k-chunk; the next chunk
+  // is fetched into registers while the current one is multiplied from LDS
+  double xr[PER], yr[PER];
   auto fetch = [&](int k0) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      int e = t + 256 * r, kk = e >> 6, ii = e & 63, kg = k0 + kk;
+    for (int r = 0; r < PER; ++r) {
+      int e = t + NT * r, kk = e / TT, ii = e % TT, kg = k0 + kk;
       xr[r] = (kg < kend && i0 + ii < g.m) ? X[(size_t)kg * g.ldx + i0 + ii] : 0.0;
       yr[r] = (kg < kend && j0 + ii < g.n) ? Y[(size_t)kg * g.ldy + j0 + ii] : 0.0;
     }
@@ -372,10 +379,10 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
   for (int k0 = kbeg; k0 < kend; k0 += GKC) {
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      int e = t + 256 * r;
-      Xs[e >> 6][e & 63] = xr[r];
-      Ys[e >> 6][e & 63] = yr[r];
+    for (int r = 0; r < PER; ++r) {
+      int e = t + NT * r;
+      Xs[e / TT][e % TT] = xr[r];
+      Ys[e / TT][e % TT] = yr[r];
     }
     __syncthreads();
     if (k0 + GKC < kend) fetch(k0 + GKC);
@@ -415,10 +422,13 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
       }
 }
 
-// one kernel symbol per role so that profiles separate them
-__global__ __launch_bounds__(256) void gemm_gram_f64(GemmArgs g) { gemm_tn_f64_body(g); }
-__global__ __launch_bounds__(256) void gemm_bmat_f64(GemmArgs g) { gemm_tn_f64_body(g); }
-__global__ __launch_bounds__(256) void gemm_update_f64(GemmArgs g) { gemm_tn_f64_body(g); }
+// one kernel symbol per role and tile so that profiles separate them
+__global__ __launch_bounds__(256) void gemm_gram_f64(GemmArgs g) { gemm_tn_f64_body<64>(g); }
+__global__ __launch_bounds__(256) void gemm_bmat_f64(GemmArgs g) { gemm_tn_f64_body<64>(g); }
+__global__ __launch_bounds__(256) void gemm_update_f64(GemmArgs g) { gemm_tn_f64_body<64>(g); }
+__global__ __launch_bounds__(64) void gemm_gram_f64_t32(GemmArgs g) { gemm_tn_f64_body<32>(g); }
+__global__ __launch_bounds__(64) void gemm_bmat_f64_t32(GemmArgs g) { gemm_tn_f64_body<32>(g); }
+__global__ __launch_bounds__(64) void gemm_update_f64_t32(GemmArgs g) { gemm_tn_f64_body<32>(g); }
 
 // --------------------------------------------------- mixed-precision GEMM
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -854,7 +864,7 @@ __device__ inline double dcoef_value(const DSummand& sm, int k, const double* cc
   if (sm.slot != k) return 0.0;
   double v = sm.scale;
   double a = cc[k];
-  v *= (sm.fn == DPG_FN_SQRT) ? 0.5 / sqrt(a) : 1.0;
+  v *= (sm.fn == DPG_FN_SQRT) ? 0.5 / sqrt(a) : (sm.fn == DPG_FN_INV ? -1.0 / (a * a) : 1.0);
   if (sm.normal >= 0) v *= n ? n[sm.normal] : 0.0;
   return v;
 }
@@ -1053,17 +1063,24 @@ void launch_ip_operand(int dim, const double* Wip, int n_lt, int nvar, const dou
   check_launch("ip_operand_kernel");
 }
 
-void launch_gemm(GemmArgs g, int role, hipStream_t st) {
+void launch_gemm(GemmArgs g, int role, hipStream_t st, int tile) {
   if (g.m <= 0 || g.n <= 0 || g.ncells <= 0) return;
-  g.tiles_m = (g.m + GT - 1) / GT;
-  int tiles_n = (g.n + GT - 1) / GT;
+  if (tile != 64 && tile != 32) throw DeviceError("gemm tile must be 64 or 32");
+  g.tiles_m = (g.m + tile - 1) / tile;
+  int tiles_n = (g.n + tile - 1) / tile;
   g.ntiles = g.lower ? g.tiles_m * (g.tiles_m + 1) / 2 : g.tiles_m * tiles_n;
   int64_t nblocks = (int64_t)g.ntiles * g.ncells;
   if (nblocks > 0x7fffffff) throw DeviceError("gemm grid too large");
   dim3 grid((unsigned)nblocks);
-  if (role == kGemmGram) gemm_gram_f64<<<grid, 256, 0, st>>>(g);
-  else if (role == kGemmBmat) gemm_bmat_f64<<<grid, 256, 0, st>>>(g);
-  else gemm_update_f64<<<grid, 256, 0, st>>>(g);
+  if (tile == 64) {
+    if (role == kGemmGram) gemm_gram_f64<<<grid, 256, 0, st>>>(g);
+    else if (role == kGemmBmat) gemm_bmat_f64<<<grid, 256, 0, st>>>(g);
+    else gemm_update_f64<<<grid, 256, 0, st>>>(g);
+  } else {
+    if (role == kGemmGram) gemm_gram_f64_t32<<<grid, 64, 0, st>>>(g);
+    else if (role == kGemmBmat) gemm_bmat_f64_t32<<<grid, 64, 0, st>>>(g);
+    else gemm_update_f64_t32<<<grid, 64, 0, st>>>(g);
+  }
   check_launch("gemm_tn_f64");
 }
 
@@ -1100,6 +1117,18 @@ void launch_panel(double* A, int64_t lda, int64_t sA, int D, int k0, int kb, dou
   int chunks = (rows + PCH - 1) / PCH;
   panel_trsm<<<dim3(chunks, ncells), 256, 0, st>>>(A, lda, sA, D, k0, kb, Li);
   check_launch("panel_trsm");
+}
+
+__global__ void fill_i32(int32_t* __restrict__ p, int n, int32_t v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// a kernel rather than hipMemsetAsync, so that captured graphs replay it reliably
+void launch_fill_i32(int32_t* p, int n, int32_t v, hipStream_t st) {
+  if (n <= 0) return;
+  fill_i32<<<(n + 255) / 256, 256, 0, st>>>(p, n, v);
+  check_launch("fill_i32");
 }
 
 void launch_zero_trailing(double* A, int64_t lda, int64_t sA, int r0, int D, int ncells, hipStream_t st) {

@@ -50,7 +50,8 @@ void launch_ip_operand(int dim, const double* Wip, int n_lt, int nvar, const dou
                        int QG, int ncells, double* out, int64_t sc, int64_t ld, double* s, int64_t ss,
                        hipStream_t st);
 enum { kGemmGram = 0, kGemmBmat = 1, kGemmUpdate = 2 };
-void launch_gemm(GemmArgs g, int role, hipStream_t st);
+void launch_gemm(GemmArgs g, int role, hipStream_t st, int tile = 64);
+void launch_fill_i32(int32_t* p, int n, int32_t v, hipStream_t st);
 // the same contraction with FP64 operands converted on load to FP32 (precision
 // DPG_PREC_FP32) or to hi+lo BF16/FP16 pairs (DPG_PREC_BF16X3/FP16X3, products
 // hi*hi + hi*lo + lo*hi), FP32 MFMA accumulation, FP64 epilogue

@@ -90,6 +90,7 @@ void validate_form(const dpg_form_desc& f) {
   };
   auto check_coef = [&](const dpg_coef& c, bool side_ok) {
     if (c.coef_slot >= f.n_coefs || c.coef_slot < -1) throw std::invalid_argument("coefficient slot out of range");
+    if (c.coef_fn < DPG_FN_VALUE || c.coef_fn > DPG_FN_INV) throw std::invalid_argument("unknown coefficient function");
     if (c.normal_comp >= f.spatial_dim || (!side_ok && c.normal_comp >= 0))
       throw std::invalid_argument("normal component out of range");
   };
@@ -198,6 +199,7 @@ struct BucketPlan {
   int64_t per_cell_bytes = 0;
   double flops = 0.0;
   double gram_flops = 0.0;  // sum over IP terms of QG * n(n+1), n = dofs the term touches
+  int tile[3] = {64, 64, 64};  // FP64 GEMM tile edge per role: Gram, B blocks, Cholesky updates
 };
 
 }  // namespace dpg
@@ -357,11 +359,14 @@ void build_bucket(dpg_plan* plan, const dpg_bucket_desc& desc, BucketPlan& B) {
 
 struct Work {
   double *A, *At, *Tv, *Ts, *Wip, *Wvol, *Wrhs, *Wside, *s, *Ld, *Li, *E, *Pt;
+  int32_t* info;  // per-cell LAPACK INFO of the batch
 };
 
-Work carve(dpg_plan* plan, const BucketPlan& B, int nb) {
+// per-cell regions of a batch of nb cells, carved from `base`
+Work carve(void* base, int32_t* info, const BucketPlan& B, int nb) {
   Work w;
-  double* p = (double*)plan->ws.p;
+  w.info = info;
+  double* p = (double*)base;
   auto take = [&](int64_t per) {
     double* r = p;
     p += per * nb;
@@ -401,10 +406,9 @@ void factor_columns(dpg_plan* plan, const BucketPlan& B, int nb, const Work& w, 
                     hipStream_t st) {
   Profiler& P = plan->prof;
   const int64_t sA = B.nA, lda = B.lda;
-  int32_t* info = (int32_t*)plan->info.p;
+  int32_t* info = w.info;
   for (int k0 = kbeg; k0 < kend; k0 += 64) {
     int kb = std::min(64, kend - k0);
-    int k1 = k0 + kb;
     double rows = B.D - k0;
     hipEvent_t e;
     if (k0 > kbeg) {
@@ -426,7 +430,7 @@ void factor_columns(dpg_plan* plan, const BucketPlan& B, int nb, const Work& w, 
       t.ncells = nb;
       double kk = k0 - kbeg;
       e = P.start(st);
-      launch_gemm(t, kGemmUpdate, st);
+      launch_gemm(t, kGemmUpdate, st, B.tile[2]);
       P.stop(e, st, DPG_STAGE_UPDATE, nb * 2.0 * (rows * kb - kb * (kb - 1) / 2.0) * kk,
              nb * (rows * kk + rows * kb * 2.0) * 8.0);
     }
@@ -454,7 +458,7 @@ void factor_columns(dpg_plan* plan, const BucketPlan& B, int nb, const Work& w, 
     t.beta = beta_trailing;
     t.ncells = nb;
     hipEvent_t e = P.start(st);
-    launch_gemm(t, kGemmUpdate, st);
+    launch_gemm(t, kGemmUpdate, st, B.tile[2]);
     P.stop(e, st, DPG_STAGE_UPDATE, nb * (double)n * (n + 1.0) * t.k,
            nb * ((double)n * t.k + n * (n + 1.0) / 2.0 * (beta_trailing != 0.0 ? 2.0 : 1.0)) * 8.0);
   }
@@ -524,7 +528,7 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
     g.px = g.py = s_ep;
   }
   e = P.start(st);
-  if (prec == DPG_PREC_FP64) launch_gemm(g, kGemmGram, st);
+  if (prec == DPG_PREC_FP64) launch_gemm(g, kGemmGram, st, B.tile[0]);
   else launch_gemm_mixed(g, prec, kGemmGram, st);
   P.stop(e, st, DPG_STAGE_GRAM, nb * B.gram_flops, nb * ((double)KG * N + N * (N + 1.0) / 2.0) * D8);
   if (Gout) {
@@ -555,7 +559,7 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
     b.scol = s_ep;
     b.ss = B.nS;
     if (prec == DPG_PREC_FP64) {
-      launch_gemm(b, kGemmBmat, st);
+      launch_gemm(b, kGemmBmat, st, B.tile[1]);
     } else {
       b.scol = nullptr;
       b.py = s_ep;
@@ -586,7 +590,7 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
       b.scol = s_ep;
       b.ss = B.nS;
       if (prec == DPG_PREC_FP64) {
-        launch_gemm(b, kGemmBmat, st);
+        launch_gemm(b, kGemmBmat, st, B.tile[1]);
       } else {
         b.scol = nullptr;
         b.py = s_ep;
@@ -606,7 +610,7 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
     launch_scaling(w.A, lda, sA, 0, B.D, N, N, w.s, B.nS, nb, 1.0, st);
     P.stop(e, st, DPG_STAGE_SCALE, 0.0, nb * ((double)N * (N + 1) / 2.0 + (double)(B.D - N) * N) * 16.0);
   }
-  HIP_CHECK(hipMemsetAsync(plan->info.p, 0, sizeof(int32_t) * nb, st));
+  launch_fill_i32(w.info, nb, 0, st);
   // the trailing block starts as 0 and is written (beta = 0) by the closing SYRK
   factor_columns(plan, B, nb, w, 0, N, 0.0, st);
 }
@@ -621,6 +625,25 @@ void stage_schur(dpg_plan* plan, const BucketPlan& B, int nb, const Work& w, hip
   launch_scaling(w.A, B.lda, B.nA, N, B.D, F, B.D - N, w.s, B.nS, nb, -1.0, st);
   P.stop(e, st, DPG_STAGE_SCALE, 0.0, nb * (double)(B.D - N) * (B.D - N + 1) / 2.0 * 16.0);
   factor_columns(plan, B, nb, w, N, N + F, 1.0, st);
+}
+
+// dpg_assemble's pipeline for cells [c0, c0+nb) of one batch
+void assemble_batch(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int c0, int nb, const Work& w,
+                    const dpg_outputs& o, hipStream_t st) {
+  const int N = B.L.N, M = B.L.M, F = B.L.F, T = B.L.T;
+  stage_factor(plan, B, bt, c0, nb, w, nullptr, o.G, st);
+  hipEvent_t e = plan->prof.start(st);
+  if (o.K) launch_extract_sym(w.A, B.lda, B.nA, N, M, -1.0, o.K + (size_t)c0 * M * M, nb, st);
+  if (o.rhs) launch_extract_row(w.A, B.lda, B.nA, N + M, N, M, -1.0, o.rhs + (size_t)c0 * M, nb, st);
+  plan->prof.stop(e, st, DPG_STAGE_EXTRACT, 0.0, nb * ((double)M * M * 2.0 + M * 2.0) * 8.0);
+  if (o.schur || o.schur_rhs) {
+    stage_schur(plan, B, nb, w, st);
+    e = plan->prof.start(st);
+    if (o.schur) launch_extract_sym(w.A, B.lda, B.nA, N + F, T, 1.0, o.schur + (size_t)c0 * T * T, nb, st);
+    if (o.schur_rhs)
+      launch_extract_row(w.A, B.lda, B.nA, N + M, N + F, T, 1.0, o.schur_rhs + (size_t)c0 * T, nb, st);
+    plan->prof.stop(e, st, DPG_STAGE_EXTRACT, 0.0, nb * ((double)T * T * 2.0 + T * 2.0) * 8.0);
+  }
 }
 
 void check_batch(const dpg_plan* plan, int bucket, const dpg_batch* batch) {
@@ -643,9 +666,9 @@ void prepare_workspace(dpg_plan* plan, const BucketPlan& B, int nb) {
   plan->info.ensure(sizeof(int32_t) * std::max(nb, 1));
 }
 
-int count_failures(dpg_plan* plan, int nb, int32_t* out_info, hipStream_t st) {
+int count_failures(const Work& w, int nb, int32_t* out_info, hipStream_t st) {
   std::vector<int32_t> h(nb);
-  HIP_CHECK(hipMemcpyAsync(h.data(), plan->info.p, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, st));
+  HIP_CHECK(hipMemcpyAsync(h.data(), w.info, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, st));
   HIP_CHECK(hipStreamSynchronize(st));
   int bad = 0;
   for (int i = 0; i < nb; ++i) bad += h[i] != 0;
@@ -729,33 +752,17 @@ int dpg_assemble(dpg_plan* plan, int bucket, const dpg_batch* batch, const dpg_o
     if (batch->ncells == 0) return 0;
     hipStream_t st = (hipStream_t)stream;
     const dpg::BucketPlan& B = plan->buckets[bucket];
-    const int N = B.L.N, M = B.L.M, F = B.L.F, T = B.L.T;
     int C = batch->ncells;
     int maxb = dpg::batch_cells(plan, B, C);
     dpg::prepare_workspace(plan, B, maxb);
     int bad = 0;
     for (int c0 = 0; c0 < C; c0 += maxb) {
       int nb = std::min(maxb, C - c0);
-      dpg::Work w = dpg::carve(plan, B, nb);
-      dpg::stage_factor(plan, B, *batch, c0, nb, w, nullptr, outputs->G, st);
-      hipEvent_t e = plan->prof.start(st);
-      if (outputs->K) dpg::launch_extract_sym(w.A, B.lda, B.nA, N, M, -1.0, outputs->K + (size_t)c0 * M * M, nb, st);
-      if (outputs->rhs)
-        dpg::launch_extract_row(w.A, B.lda, B.nA, N + M, N, M, -1.0, outputs->rhs + (size_t)c0 * M, nb, st);
-      plan->prof.stop(e, st, DPG_STAGE_EXTRACT, 0.0, nb * ((double)M * M * 2.0 + M * 2.0) * 8.0);
-      if (outputs->schur || outputs->schur_rhs) {
-        dpg::stage_schur(plan, B, nb, w, st);
-        e = plan->prof.start(st);
-        if (outputs->schur)
-          dpg::launch_extract_sym(w.A, B.lda, B.nA, N + F, T, 1.0, outputs->schur + (size_t)c0 * T * T, nb, st);
-        if (outputs->schur_rhs)
-          dpg::launch_extract_row(w.A, B.lda, B.nA, N + M, N + F, T, 1.0, outputs->schur_rhs + (size_t)c0 * T, nb,
-                                  st);
-        plan->prof.stop(e, st, DPG_STAGE_EXTRACT, 0.0, nb * ((double)T * T * 2.0 + T * 2.0) * 8.0);
-      }
+      dpg::Work w = dpg::carve(plan->ws.p, (int32_t*)plan->info.p, B, nb);
+      dpg::assemble_batch(plan, B, *batch, c0, nb, w, *outputs, st);
       plan->prof.collect(st, nb);
       // the next batch reuses the workspace in stream order
-      if (outputs->info) bad += dpg::count_failures(plan, nb, outputs->info + c0, st);
+      if (outputs->info) bad += dpg::count_failures(w, nb, outputs->info + c0, st);
     }
     return bad;
   });
@@ -775,11 +782,11 @@ int dpg_energy_error(dpg_plan* plan, int bucket, const dpg_batch* batch, const d
     int bad = 0;
     for (int c0 = 0; c0 < C; c0 += maxb) {
       int nb = std::min(maxb, C - c0);
-      dpg::Work w = dpg::carve(plan, B, nb);
+      dpg::Work w = dpg::carve(plan->ws.p, (int32_t*)plan->info.p, B, nb);
       dpg::stage_factor(plan, B, *batch, c0, nb, w, u, nullptr, st);
       dpg::launch_extract_row(w.A, B.lda, B.nA, B.D - 1, B.D - 1, 1, -1.0, eta2 + c0, nb, st);
       plan->prof.collect(st, nb);
-      bad += dpg::count_failures(plan, nb, nullptr, st);
+      bad += dpg::count_failures(w, nb, nullptr, st);
     }
     return bad;
   });
@@ -803,7 +810,7 @@ int dpg_adjoint(dpg_plan* plan, int bucket, const dpg_batch* batch, const double
     int bad = 0;
     for (int c0 = 0; c0 < C; c0 += maxb) {
       int nb = std::min(maxb, C - c0);
-      dpg::Work w = dpg::carve(plan, B, nb);
+      dpg::Work w = dpg::carve(plan->ws.p, (int32_t*)plan->info.p, B, nb);
       dpg::stage_factor(plan, B, *batch, c0, nb, w, u, nullptr, st);
       dpg::launch_extract_row(w.A, B.lda, B.nA, B.D - 1, B.D - 1, 1, -1.0, eta2 + c0, nb, st);
       if (grad) {
@@ -821,10 +828,215 @@ int dpg_adjoint(dpg_plan* plan, int bucket, const dpg_batch* batch, const double
         dpg::launch_grad_reduce(w.Pt, B.nP, L.QG + L.Q + S * L.Qs, nc, nb, grad + (size_t)c0 * nc, st);
       }
       plan->prof.collect(st, nb);
-      bad += dpg::count_failures(plan, nb, nullptr, st);
+      bad += dpg::count_failures(w, nb, nullptr, st);
     }
     return bad;
   });
+}
+
+int dpg_plan_get_tiles(const dpg_plan* plan, int bucket, dpg_tiles* out) {
+  return dpg::guarded([&] {
+    if (!plan || !out) throw std::invalid_argument("NULL argument");
+    if (bucket < 0 || bucket >= (int)plan->buckets.size()) throw std::invalid_argument("bucket index out of range");
+    const int* t = plan->buckets[bucket].tile;
+    *out = dpg_tiles{t[0], t[1], t[2], 0};
+  });
+}
+
+int dpg_plan_set_tiles(dpg_plan* plan, int bucket, const dpg_tiles* tiles) {
+  return dpg::guarded([&] {
+    if (!plan || !tiles) throw std::invalid_argument("NULL argument");
+    if (bucket < 0 || bucket >= (int)plan->buckets.size()) throw std::invalid_argument("bucket index out of range");
+    int v[3] = {tiles->gram, tiles->bmat, tiles->update};
+    for (int r = 0; r < 3; ++r)
+      if (v[r] != 64 && v[r] != 32) throw std::invalid_argument("tile edges must be 64 or 32");
+    for (int r = 0; r < 3; ++r) plan->buckets[bucket].tile[r] = v[r];
+  });
+}
+
+int dpg_plan_autotune(dpg_plan* plan, int bucket, const dpg_batch* batch, int sample, void* stream) {
+  return dpg::guarded([&] {
+    dpg::check_batch(plan, bucket, batch);
+    if (sample <= 0) throw std::invalid_argument("sample must be positive");
+    hipStream_t st = (hipStream_t)stream;
+    dpg::BucketPlan& B = plan->buckets[bucket];
+    int ns = std::min(sample, batch->ncells);
+    if (ns == 0) return;
+    ns = std::min(ns, dpg::batch_cells(plan, B, ns));
+    dpg::prepare_workspace(plan, B, ns);
+    dpg::Work w = dpg::carve(plan->ws.p, (int32_t*)plan->info.p, B, ns);
+    // time with the plan's profiler, keeping the caller's accumulated timings
+    bool on = plan->prof.on;
+    dpg_timings saved = plan->prof.acc;
+    plan->prof.on = true;
+    const int stage_of[3] = {DPG_STAGE_GRAM, DPG_STAGE_BMAT, DPG_STAGE_UPDATE};
+    const int cand[2] = {64, 32};
+    double ms[2][3] = {{0}};
+    try {
+      for (int c = 0; c < 2; ++c) {
+        for (int r = 0; r < 3; ++r) B.tile[r] = cand[c];
+        for (int pass = 0; pass < 2; ++pass) {  // pass 0 warms up
+          plan->prof.acc = dpg_timings{};
+          dpg::stage_factor(plan, B, *batch, 0, ns, w, nullptr, nullptr, st);
+          dpg::stage_schur(plan, B, ns, w, st);
+          plan->prof.collect(st, ns);
+        }
+        for (int r = 0; r < 3; ++r) ms[c][r] = plan->prof.acc.ms[stage_of[r]];
+      }
+    } catch (...) {
+      plan->prof.on = on;
+      plan->prof.acc = saved;
+      throw;
+    }
+    for (int r = 0; r < 3; ++r) B.tile[r] = ms[1][r] < ms[0][r] ? cand[1] : cand[0];
+    plan->prof.on = on;
+    plan->prof.acc = saved;
+  });
+}
+
+}  // extern "C"
+
+struct dpg_graph {
+  dpg_plan* plan = nullptr;
+  std::vector<std::unique_ptr<dpg::DeviceBuffer>> bufs;
+  std::vector<hipStream_t> lanes;
+  std::vector<hipEvent_t> events;
+  hipStream_t origin = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  size_t bytes = 0;
+  ~dpg_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto e : events) (void)hipEventDestroy(e);
+    for (auto s : lanes) (void)hipStreamDestroy(s);
+    if (origin) (void)hipStreamDestroy(origin);
+  }
+};
+
+extern "C" {
+
+int dpg_graph_create(dpg_plan* plan, const dpg_work_item* items, int nitems, int lanes, void* stream,
+                     dpg_graph** out) {
+  return dpg::guarded([&] {
+    (void)stream;
+    if (!plan || !out || (nitems > 0 && !items)) throw std::invalid_argument("NULL argument");
+    if (nitems <= 0) throw std::invalid_argument("no work items");
+    if (lanes < 1 || lanes > 16) throw std::invalid_argument("lanes must be in 1..16");
+    *out = nullptr;
+    for (int i = 0; i < nitems; ++i) dpg::check_batch(plan, items[i].bucket, &items[i].batch);
+    std::unique_ptr<dpg_graph> g(new dpg_graph());
+    g->plan = plan;
+    // longest processing time first: items by decreasing flops, each to the
+    // lane with the least work so far
+    std::vector<int> order(nitems);
+    std::vector<double> cost(nitems);
+    for (int i = 0; i < nitems; ++i) {
+      order[i] = i;
+      cost[i] = (double)items[i].batch.ncells * plan->buckets[items[i].bucket].flops;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    lanes = std::min(lanes, nitems);
+    std::vector<std::vector<int>> lane_items(lanes);
+    std::vector<double> load(lanes, 0.0);
+    for (int i : order) {
+      int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      lane_items[l].push_back(i);
+      load[l] += cost[i];
+    }
+    // per-lane workspace: the largest batch any of its items uses
+    double budget = plan->opt.workspace_bytes > 0 ? plan->opt.workspace_bytes : 16.0 * (1ull << 30);
+    std::vector<int> nb_of(nitems, 0);
+    std::vector<dpg::DeviceBuffer*> ws(lanes), inf(lanes);
+    for (int l = 0; l < lanes; ++l) {
+      size_t need = 0;
+      int cells = 1;
+      for (int i : lane_items[l]) {
+        const dpg::BucketPlan& B = plan->buckets[items[i].bucket];
+        int64_t nb = (int64_t)(budget / lanes / (double)B.per_cell_bytes);
+        if (plan->opt.max_batch_cells > 0) nb = std::min<int64_t>(nb, plan->opt.max_batch_cells);
+        nb = std::max<int64_t>(1, std::min<int64_t>({nb, (int64_t)items[i].batch.ncells, 65535}));
+        nb_of[i] = (int)nb;
+        need = std::max(need, (size_t)B.per_cell_bytes * nb);
+        cells = std::max(cells, (int)nb);
+      }
+      g->bufs.emplace_back(new dpg::DeviceBuffer());
+      g->bufs.back()->ensure(std::max<size_t>(need, 256));
+      ws[l] = g->bufs.back().get();
+      g->bufs.emplace_back(new dpg::DeviceBuffer());
+      g->bufs.back()->ensure(sizeof(int32_t) * cells);
+      inf[l] = g->bufs.back().get();
+      g->bytes += ws[l]->bytes + inf[l]->bytes;
+    }
+    HIP_CHECK(hipStreamCreateWithFlags(&g->origin, hipStreamNonBlocking));
+    for (int l = 0; l < lanes; ++l) {
+      hipStream_t s;
+      HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+      g->lanes.push_back(s);
+    }
+    for (int l = 0; l < 2 * lanes + 1; ++l) {
+      hipEvent_t e;
+      HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      g->events.push_back(e);
+    }
+    // capture: fork every lane from the origin stream, join them back
+    bool prof_on = plan->prof.on;
+    plan->prof.on = false;  // no timing events inside the graph
+    bool capturing = false;
+    try {
+      HIP_CHECK(hipStreamBeginCapture(g->origin, hipStreamCaptureModeRelaxed));
+      capturing = true;
+      HIP_CHECK(hipEventRecord(g->events[0], g->origin));
+      for (int l = 0; l < lanes; ++l) {
+        hipStream_t st = g->lanes[l];
+        HIP_CHECK(hipStreamWaitEvent(st, g->events[0], 0));
+        for (int i : lane_items[l]) {
+          const dpg_work_item& it = items[i];
+          const dpg::BucketPlan& B = plan->buckets[it.bucket];
+          int C = it.batch.ncells;
+          for (int c0 = 0; c0 < C; c0 += nb_of[i]) {
+            int nb = std::min(nb_of[i], C - c0);
+            int32_t* info = it.out.info ? it.out.info + c0 : (int32_t*)inf[l]->p;
+            dpg::Work w = dpg::carve(ws[l]->p, info, B, nb);
+            dpg::assemble_batch(plan, B, it.batch, c0, nb, w, it.out, st);
+          }
+        }
+        HIP_CHECK(hipEventRecord(g->events[1 + l], st));
+        HIP_CHECK(hipStreamWaitEvent(g->origin, g->events[1 + l], 0));
+      }
+      capturing = false;
+      HIP_CHECK(hipStreamEndCapture(g->origin, &g->graph));
+    } catch (...) {
+      plan->prof.on = prof_on;
+      if (capturing) {
+        hipGraph_t tmp = nullptr;
+        (void)hipStreamEndCapture(g->origin, &tmp);
+        if (tmp) (void)hipGraphDestroy(tmp);
+      }
+      throw;
+    }
+    plan->prof.on = prof_on;
+    HIP_CHECK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    *out = g.release();
+  });
+}
+
+int dpg_graph_launch(dpg_graph* graph, void* stream) {
+  return dpg::guarded([&] {
+    if (!graph) throw std::invalid_argument("graph is NULL");
+    HIP_CHECK(hipGraphLaunch(graph->exec, (hipStream_t)stream));
+  });
+}
+
+int dpg_graph_workspace(const dpg_graph* graph, size_t* bytes) {
+  return dpg::guarded([&] {
+    if (!graph || !bytes) throw std::invalid_argument("NULL argument");
+    *bytes = graph->bytes;
+  });
+}
+
+int dpg_graph_destroy(dpg_graph* graph) {
+  return dpg::guarded([&] { delete graph; });
 }
 
 }  // extern "C"

@@ -28,6 +28,8 @@ def form(name, dim=2, mu=1.0):
         check(L.dpg_form_cdr_uw(dim, ct.byref(f)))
     elif name == "stokes":
         check(L.dpg_form_stokes_vgp_uw(dim, mu, ct.byref(f)))
+    elif name == "cd":
+        check(L.dpg_form_convection_diffusion_uw(dim, ct.byref(f)))
     else:
         raise ValueError(f"unknown formulation {name}")
     return f
@@ -62,6 +64,17 @@ def mesh_hex(nx, ny, nz, jitter, seed):
     par = np.zeros((C, 6), dtype=np.int8)
     check(lib().dpg_mesh_hex(nx, ny, nz, jitter, seed, nodes.ctypes.data, par.ctypes.data))
     return nodes, par
+
+
+def mesh_quad_graded(n0, levels, jitter, seed):
+    """hp mesh graded toward x = 1: (nodes (C,4,2), parities (C,4), refinement level (C,))."""
+    L = lib()
+    C = check(L.dpg_mesh_quad_graded(n0, levels, jitter, seed, None, None, None))
+    nodes = np.zeros((C, 4, 2))
+    par = np.zeros((C, 4), dtype=np.int8)
+    lvl = np.zeros((C,), dtype=np.int32)
+    check(L.dpg_mesh_quad_graded(n0, levels, jitter, seed, nodes.ctypes.data, par.ctypes.data, lvl.ctypes.data))
+    return nodes, par, lvl
 
 
 def load_sine_modes(dim, ncomp, nmodes, kmax, seed):
@@ -127,6 +140,44 @@ class LocalAssembler:
                 out[name] = {"ms": t.ms[i], "launches": int(t.launches[i]), "flops": t.flops[i], "bytes": t.bytes[i]}
         return out, int(t.cells)
 
+    def tiles(self, bucket):
+        """FP64 GEMM tile edges (gram, bmat, update) used for a bucket."""
+        t = capi.Tiles()
+        check(lib().dpg_plan_get_tiles(self._plan, bucket, ct.byref(t)))
+        return (t.gram, t.bmat, t.update)
+
+    def set_tiles(self, bucket, gram, bmat, update):
+        check(lib().dpg_plan_set_tiles(self._plan, bucket, ct.byref(capi.Tiles(gram, bmat, update, 0))))
+
+    def autotune(self, bucket, nodes, parities, coefs=None, load=None, sample=256, stream=None):
+        """Times both tile edges per GEMM role on `sample` cells and keeps the faster; returns the choice."""
+        t = self._torch
+        b = self._batch(nodes, parities, coefs, load)
+        s = stream.cuda_stream if stream is not None else t.cuda.current_stream().cuda_stream
+        check(lib().dpg_plan_autotune(self._plan, bucket, ct.byref(b), sample, ct.c_void_p(s)))
+        return self.tiles(bucket)
+
+    def _outputs(self, bucket, C, dev, out, want_G, want_schur, want_info):
+        t = self._torch
+        info = self.info[bucket]
+        o = out if out is not None else {}
+        if "K" not in o:
+            o["K"] = t.empty((C, info.M, info.M), dtype=t.float64, device=dev)
+        if "rhs" not in o:
+            o["rhs"] = t.empty((C, info.M), dtype=t.float64, device=dev)
+        if want_G and "G" not in o:
+            o["G"] = t.empty((C, info.N, info.N), dtype=t.float64, device=dev)
+        if want_schur and "schur" not in o:
+            o["schur"] = t.empty((C, info.T, info.T), dtype=t.float64, device=dev)
+        if want_schur and "schur_rhs" not in o:
+            o["schur_rhs"] = t.empty((C, info.T), dtype=t.float64, device=dev)
+        if want_info and "info" not in o:
+            o["info"] = t.zeros((C,), dtype=t.int32, device=dev)
+        outs = capi.Outputs()
+        for key in ("K", "rhs", "G", "schur", "schur_rhs", "info"):
+            setattr(outs, key, o[key].data_ptr() if key in o else None)
+        return o, outs
+
     def _batch(self, nodes, parities, coefs, load):
         t = self._torch
         b = capi.Batch()
@@ -148,25 +199,7 @@ class LocalAssembler:
 
         `nodes`, `parities`, `coefs` are device tensors. Returns (outputs dict, #non-SPD cells)."""
         t = self._torch
-        info = self.info[bucket]
-        C = nodes.shape[0]
-        dev = nodes.device
-        o = out if out is not None else {}
-        if "K" not in o:
-            o["K"] = t.empty((C, info.M, info.M), dtype=t.float64, device=dev)
-        if "rhs" not in o:
-            o["rhs"] = t.empty((C, info.M), dtype=t.float64, device=dev)
-        if want_G and "G" not in o:
-            o["G"] = t.empty((C, info.N, info.N), dtype=t.float64, device=dev)
-        if want_schur and "schur" not in o:
-            o["schur"] = t.empty((C, info.T, info.T), dtype=t.float64, device=dev)
-        if want_schur and "schur_rhs" not in o:
-            o["schur_rhs"] = t.empty((C, info.T), dtype=t.float64, device=dev)
-        if want_info and "info" not in o:
-            o["info"] = t.zeros((C,), dtype=t.int32, device=dev)
-        outs = capi.Outputs()
-        for key in ("K", "rhs", "G", "schur", "schur_rhs", "info"):
-            setattr(outs, key, o[key].data_ptr() if key in o else None)
+        o, outs = self._outputs(bucket, nodes.shape[0], nodes.device, out, want_G, want_schur, want_info)
         b = self._batch(nodes, parities, coefs, load)
         s = stream.cuda_stream if stream is not None else t.cuda.current_stream().cuda_stream
         nbad = check(lib().dpg_assemble(self._plan, bucket, ct.byref(b), ct.byref(outs), ct.c_void_p(s)))
@@ -196,6 +229,57 @@ class LocalAssembler:
         check(lib().dpg_adjoint(self._plan, bucket, ct.byref(b), ct.c_void_p(u.data_ptr()),
                                 ct.c_void_p(eta2.data_ptr()), ct.c_void_p(grad.data_ptr()), ct.c_void_p(s)))
         return eta2, grad
+
+
+class AssemblyGraph:
+    """All element types of an hp mesh assembled by one captured hipGraph launch.
+
+    `items` is a list of dicts with keys bucket, nodes, parities and optionally
+    coefs, load, want_schur, want_G, want_info, out (device tensors, kept alive
+    by this object because the graph holds their addresses)."""
+
+    def __init__(self, assembler, items, lanes=4, stream=None):
+        t = assembler._torch
+        self.assembler = assembler
+        self.items = []
+        arr = (capi.WorkItem * len(items))()
+        for i, it in enumerate(items):
+            nodes = it["nodes"]
+            o, outs = assembler._outputs(it["bucket"], nodes.shape[0], nodes.device, it.get("out"),
+                                         it.get("want_G", False), it.get("want_schur", False),
+                                         it.get("want_info", True))
+            arr[i].bucket = it["bucket"]
+            arr[i].batch = assembler._batch(nodes, it["parities"], it.get("coefs"), it.get("load"))
+            arr[i].out = outs
+            self.items.append(dict(it, out=o))
+        s = stream.cuda_stream if stream is not None else t.cuda.current_stream().cuda_stream
+        g = ct.c_void_p()
+        check(lib().dpg_graph_create(assembler._plan, arr, len(items), lanes, ct.c_void_p(s), ct.byref(g)))
+        self._graph = g
+
+    def launch(self, stream=None):
+        t = self.assembler._torch
+        s = stream.cuda_stream if stream is not None else t.cuda.current_stream().cuda_stream
+        check(lib().dpg_graph_launch(self._graph, ct.c_void_p(s)))
+
+    def outputs(self):
+        return [it["out"] for it in self.items]
+
+    def workspace_bytes(self):
+        n = ct.c_size_t()
+        check(lib().dpg_graph_workspace(self._graph, ct.byref(n)))
+        return n.value
+
+    def close(self):
+        if self._graph:
+            lib().dpg_graph_destroy(self._graph)
+            self._graph = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class EnergyErrorFunction:

@@ -60,14 +60,14 @@ enum {
   DPG_OP_DIV = 7,        /* div tau                           */
   DPG_OP_DOT_NORMAL = 8  /* tau . n  (boundary-only)          */
 };
-enum { DPG_FN_VALUE = 0, DPG_FN_SQRT = 1 };
+enum { DPG_FN_VALUE = 0, DPG_FN_SQRT = 1, DPG_FN_INV = 2 };
 enum { DPG_PREC_FP64 = 0, DPG_PREC_FP32 = 1, DPG_PREC_BF16X3 = 2, DPG_PREC_FP16X3 = 3 };
 
 /* coefficient of a scalar product: scale * [fn(coef[c][slot])] * [n_comp] */
 typedef struct {
   double scale;
   int32_t coef_slot;   /* -1: none, else per-cell coefficient slot        */
-  int32_t coef_fn;     /* DPG_FN_VALUE or DPG_FN_SQRT                      */
+  int32_t coef_fn;     /* DPG_FN_VALUE, DPG_FN_SQRT or DPG_FN_INV (1/a)      */
   int32_t normal_comp; /* -1: none, else outward unit normal component    */
   int32_t pad;
 } dpg_coef;
@@ -223,6 +223,9 @@ int dpg_form_poisson_uw(int spatial_dim, dpg_form_desc* out);
  * coefficients [eps, beta_1..beta_d, alpha]; sqrt(eps) enters as in
  * ConvectionDiffusionReactionFormulation.cpp:71-111 */
 int dpg_form_cdr_uw(int spatial_dim, dpg_form_desc* out);
+/* ConvectionDiffusionFormulation (ultraweak, graph norm), ConvectionDiffusionFormulation.cpp:23-96,
+ * with per-cell coefficients [eps, beta_1..beta_d]; (sigma/eps, tau) uses DPG_FN_INV */
+int dpg_form_convection_diffusion_uw(int spatial_dim, dpg_form_desc* out);
 /* StokesVGPFormulation steady, conforming traces, StokesVGPFormulation.cpp:372-408, rhs :1015-1028 */
 int dpg_form_stokes_vgp_uw(int spatial_dim, double mu, dpg_form_desc* out);
 /* TBF::graphNorm (BF.cpp:1039-1141): replace out->ipterms from the bilinear terms */
@@ -234,6 +237,18 @@ int dpg_form_graph_norm(dpg_form_desc* form);
 int dpg_mesh_quad(int nx, int ny, double jitter, uint64_t seed, double* nodes, int8_t* parities);
 /* nx*ny*nz hexes on [0,1]^3, shards Hexahedron<8> vertex order; nodes (C,8,3); parities (C,6) */
 int dpg_mesh_hex(int nx, int ny, int nz, double jitter, uint64_t seed, double* nodes, int8_t* parities);
+/* hp mesh graded toward the outflow edge x = 1: an n0*n0 grid whose column of
+ * cells touching x = 1 is refined isotropically `levels` times (each level
+ * halves h; 2:1 balanced, hanging nodes on the refinement fronts). Cells are
+ * numbered coarse levels first, then y, then x; regular interior vertices are
+ * jittered by U(-jitter, jitter) times the smallest adjacent h, hanging ones
+ * stay on their edges. Parity rule of GlobalDofAssignment.cpp:165-185 with the
+ * neighbour across the side (the coarse cell, or the first fine cell across a
+ * refined side): +1 on the boundary or toward a higher cell number.
+ * Returns the number of cells n0^2 + 3 n0 (2^levels - 1); with nodes = NULL
+ * only the count. nodes (C,4,2) CCW, parities (C,4), level (C) or NULL. */
+int dpg_mesh_quad_graded(int n0, int levels, double jitter, uint64_t seed, double* nodes, int8_t* parities,
+                         int32_t* level);
 /* sine-mode load: nmodes modes per component, amp ~ N(0,1), k ~ U{1..kmax} */
 int dpg_load_sine_modes(int spatial_dim, int ncomp, int nmodes, int kmax, uint64_t seed, dpg_load_desc* out);
 /* xoshiro256** seeded by splitmix64: n uniforms on [lo,hi) / n normals (Box-Muller) */
@@ -277,6 +292,49 @@ int dpg_energy_error(dpg_plan* plan, int bucket, const dpg_batch* batch, const d
  * values recomputed at the quadrature points. grad is (C, n_coefs) or NULL. */
 int dpg_adjoint(dpg_plan* plan, int bucket, const dpg_batch* batch, const double* u, double* eta2, double* grad,
                 void* stream);
+
+/* ---- hp meshes: tile selection and the captured all-bucket launch --------
+ * No counterpart in the reference, which assembles element types one after the
+ * other on the CPU (Solution.cpp:1162-1298); these serve the same loop. */
+
+/* FP64 GEMM tile edge (64 or 32) per contraction role of one bucket */
+typedef struct {
+  int32_t gram;   /* G = A^T A                        */
+  int32_t bmat;   /* B blocks                         */
+  int32_t update; /* Cholesky panel/trailing updates  */
+  int32_t pad;
+} dpg_tiles;
+int dpg_plan_get_tiles(const dpg_plan* plan, int bucket, dpg_tiles* out);
+int dpg_plan_set_tiles(dpg_plan* plan, int bucket, const dpg_tiles* tiles);
+/* Autotuner: times both tile edges for every role on the first `sample` cells
+ * of `batch` (HIP events, one warm-up pass each) and keeps the faster per role.
+ * The choice never changes results: both tiles accumulate every output in the
+ * same k order. Blocks until done; `stream` is used for the timing runs. */
+int dpg_plan_autotune(dpg_plan* plan, int bucket, const dpg_batch* batch, int sample, void* stream);
+
+/* one element type's share of a mesh: its cells, coefficients and outputs */
+typedef struct {
+  int32_t bucket;
+  int32_t pad;
+  dpg_batch batch;
+  dpg_outputs out; /* info (C) optional: LAPACK INFO per cell, written on device */
+} dpg_work_item;
+
+typedef struct dpg_graph dpg_graph;
+/* Captures the assembly of every item (dpg_assemble's pipeline, FP64 or the
+ * plan's precision) into one hipGraph: items are ordered by decreasing flops
+ * and dealt to `lanes` streams forked inside the graph (longest processing
+ * time first), each lane with its own workspace of at most
+ * options.workspace_bytes / lanes. The items' device pointers are baked into
+ * the graph; replays recompute from their current contents. `stream` is only
+ * used for capture and is not launched on. */
+int dpg_graph_create(dpg_plan* plan, const dpg_work_item* items, int nitems, int lanes, void* stream,
+                     dpg_graph** out);
+/* one hipGraphLaunch of the captured assembly on `stream` (asynchronous) */
+int dpg_graph_launch(dpg_graph* graph, void* stream);
+/* device bytes held by the graph's lane workspaces */
+int dpg_graph_workspace(const dpg_graph* graph, size_t* bytes);
+int dpg_graph_destroy(dpg_graph* graph);
 
 #ifdef __cplusplus
 }

@@ -26,6 +26,7 @@
 //   energy error Solution.cpp:2653-2798 (r = l - B u, Cholesky solve G e = r, r^T e)
 //   graph norm   BF.cpp:1039-1141
 //   formulations PoissonFormulation.cpp:30-88, ConvectionDiffusionReactionFormulation.cpp:71-111,
+//                ConvectionDiffusionFormulation.cpp:23-96,
 //                StokesVGPFormulation.cpp:372-408,1015-1028
 //
 // Loops are written for clarity first; OpenMP spreads cells over threads (the
@@ -446,7 +447,7 @@ double coef_value(const dpg_coef& c, const double* cellCoefs, const double* norm
   double v = c.scale;
   if (c.coef_slot >= 0) {
     double a = cellCoefs[c.coef_slot];
-    v *= (c.coef_fn == DPG_FN_SQRT) ? std::sqrt(a) : a;
+    v *= (c.coef_fn == DPG_FN_SQRT) ? std::sqrt(a) : (c.coef_fn == DPG_FN_INV ? 1.0 / a : a);
   }
   if (c.normal_comp >= 0) v *= normal ? normal[c.normal_comp] : 0.0;
   return v;
@@ -1035,6 +1036,33 @@ int oracle_form_cdr_uw(int dim, dpg_form_desc* f) {
   for (int d = 0; d < dim; ++d) addB(f, 0, 0, 0, DPG_OP_DX + d, C(-1.0, 1 + d));          // -(beta u, grad v)
   addB(f, 3, 0, 0, DPG_OP_VALUE, C(-1.0));                                                // -(sigma_n, v)
   addB(f, 0, 0, 0, DPG_OP_VALUE, C(1.0, dim + 1));                                        // (alpha u, v)
+  f->n_rhs = 1;
+  f->rhs[0].test_var = 0;
+  f->rhs[0].test_op = DPG_OP_VALUE;
+  f->rhs[0].load_comp = 0;
+  f->rhs[0].scale = 1.0;
+  return oracle_graph_norm(f);
+}
+
+// ConvectionDiffusionFormulation (ConvectionDiffusionFormulation.cpp:23-96), per-cell [eps, beta]
+int oracle_form_convection_diffusion_uw(int dim, dpg_form_desc* f) {
+  std::memset(f, 0, sizeof(*f));
+  f->spatial_dim = dim;
+  f->n_coefs = dim + 1;
+  f->n_trial = 4;
+  f->trial_kind[0] = DPG_FIELD;
+  f->trial_kind[1] = DPG_FIELD_VECTOR;
+  f->trial_kind[2] = DPG_TRACE;
+  f->trial_kind[3] = DPG_FLUX;
+  f->n_test = 2;
+  f->test_space[0] = DPG_HGRAD;
+  f->test_space[1] = DPG_HDIV;
+  for (int d = 0; d < dim; ++d) addB(f, 1, d, 1, DPG_OP_X + d, C(1.0, 0, DPG_FN_INV));  // (sigma/eps, tau)
+  addB(f, 0, 0, 1, DPG_OP_DIV, C(1.0));                                                // (u, div tau)
+  addB(f, 2, 0, 1, DPG_OP_DOT_NORMAL, C(-1.0));                                        // -(uhat, tau.n)
+  for (int d = 0; d < dim; ++d) addB(f, 0, 0, 0, DPG_OP_DX + d, C(-1.0, 1 + d));        // (-beta u, grad v)
+  for (int d = 0; d < dim; ++d) addB(f, 1, d, 0, DPG_OP_DX + d, C(1.0));                // (sigma, grad v)
+  addB(f, 3, 0, 0, DPG_OP_VALUE, C(1.0));                                              // (tc, v)
   f->n_rhs = 1;
   f->rhs[0].test_var = 0;
   f->rhs[0].test_op = DPG_OP_VALUE;

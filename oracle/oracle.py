@@ -41,6 +41,7 @@ def lib():
         L.oracle_form_poisson_uw.argtypes = [ct.c_int, P(capi.FormDesc)]
         L.oracle_form_cdr_uw.argtypes = [ct.c_int, P(capi.FormDesc)]
         L.oracle_form_stokes_vgp_uw.argtypes = [ct.c_int, ct.c_double, P(capi.FormDesc)]
+        L.oracle_form_convection_diffusion_uw.argtypes = [ct.c_int, P(capi.FormDesc)]
         _lib = L
     return _lib
 
@@ -129,6 +130,8 @@ def form(name, dim, mu=1.0):
         rc = lib().oracle_form_cdr_uw(dim, ct.byref(f))
     elif name == "stokes":
         rc = lib().oracle_form_stokes_vgp_uw(dim, mu, ct.byref(f))
+    elif name == "cd":
+        rc = lib().oracle_form_convection_diffusion_uw(dim, ct.byref(f))
     else:
         raise ValueError(name)
     assert rc == 0

@@ -57,3 +57,8 @@ def poisson3d(nx, ny, nz, k=4, kt=5, seed=SEED[4], jitter=0.05):
     nodes, par = dpg.mesh_hex(nx, ny, nz, jitter, seed)
     load = dpg.load_sine_modes(3, 1, 16, 6, seed + 100)
     return f, b, nodes, par, None, load
+
+
+
+# the hp workload lives in the package (bench.py uses it too)
+from camellia2_amd.workloads import cd_hp, hp_groups  # noqa: E402,F401

@@ -45,7 +45,7 @@ def _bterms(f):
                    f.bterms[i].coef.normal_comp) for i in range(f.n_bterms))
 
 
-@pytest.mark.parametrize("name,dim", [("poisson", 2), ("poisson", 3), ("cdr", 2), ("cdr", 3), ("stokes", 2)])
+@pytest.mark.parametrize("name,dim", [("poisson", 2), ("poisson", 3), ("cdr", 2), ("cdr", 3), ("stokes", 2), ("cd", 2)])
 def test_formulations_match_oracle_tables(name, dim):
     f = dpg.form(name, dim)
     fo = oracle.form(name, dim)
@@ -127,3 +127,46 @@ def test_plan_without_gpu_reports_error():
     rc = L.dpg_plan_create(ct.byref(f), arr, 1, None, ct.byref(plan))
     assert rc < 0 and not plan.value
     assert len(L.dpg_last_error()) > 0
+
+
+def test_graded_mesh_is_a_consistent_hp_mesh():
+    n0, levels = 4, 5
+    nodes, par, lvl = dpg.mesh_quad_graded(n0, levels, 0.05, 11)
+    C = nodes.shape[0]
+    assert C == n0 * n0 + 3 * n0 * (2 ** levels - 1)
+    assert np.array_equal(np.bincount(lvl)[1:], [n0 * 2 ** l for l in range(1, levels)] + [n0 * 2 ** (levels + 1)])
+    n2, p2, _ = dpg.mesh_quad_graded(n0, levels, 0.05, 11)
+    assert np.array_equal(nodes, n2) and np.array_equal(par, p2)
+    x, y = nodes[..., 0], nodes[..., 1]
+    area = 0.5 * np.sum(x * np.roll(y, -1, 1) - np.roll(x, -1, 1) * y, axis=1)
+    assert area.min() > 0 and abs(area.sum() - 1.0) < 1e-13  # a tiling: no gaps, no overlaps
+    assert np.all(np.diff(lvl) >= 0)  # coarse cells first
+
+    def key(pt):
+        return (round(pt[0], 12), round(pt[1], 12))
+    edges = {}
+    for c in range(C):
+        for s in range(4):
+            a, b = nodes[c, s], nodes[c, (s + 1) % 4]
+            edges.setdefault(frozenset([key(a), key(b)]), []).append((c, s, a, b))
+    on_boundary = lambda a, b: (abs(a[0] - b[0]) < 1e-14 and abs(a[0]) % 1.0 < 1e-14) or \
+        (abs(a[1] - b[1]) < 1e-14 and abs(a[1]) % 1.0 < 1e-14)
+    coarse_mid, fine = {}, []
+    for e, cs in edges.items():
+        if len(cs) == 2:  # conforming interior side: opposite parities
+            assert par[cs[0][0], cs[0][1]] == -par[cs[1][0], cs[1][1]]
+        else:
+            c, s, a, b = cs[0]
+            if on_boundary(a, b):
+                assert par[c, s] == 1
+            else:
+                fine.append(cs[0])
+                coarse_mid[key(0.5 * (a + b))] = cs[0]
+    nhang = 0
+    for c, s, a, b in fine:
+        for pt in (a, b):
+            if key(pt) in coarse_mid and coarse_mid[key(pt)][0] != c:
+                cc, cs_, _, _ = coarse_mid[key(pt)]
+                assert lvl[cc] == lvl[c] - 1 and par[cc, cs_] == 1 and par[c, s] == -1
+                nhang += 1
+    assert nhang == 2 * n0 * (2 ** levels - 1)  # two fine sides per refined coarse side

@@ -164,10 +164,12 @@ def test_graph_norm_poisson_terms():
     assert got == want
 
 
-def _nodal_trial_coefficients_linear(nodes, par, k, a):
-    """Trial coefficients of u = a0 + a1 x + a2 y (sigma = grad u, uhat = u,
-    sigma_n_hat = sigma.n * parity) on 2D Poisson UW, by nodal interpolation
-    at the GLL nodes of the field (Cn(k-1)), trace (Cn(k)) and flux (Cn(k-1)) bases."""
+def _nodal_trial_coefficients_linear(nodes, par, k, a, eps=None, beta=None):
+    """Trial coefficients of u = a0 + a1 x + a2 y by nodal interpolation at the
+    GLL nodes of the field (Cn(k-1)), trace (Cn(k)) and flux (Cn(k-1)) bases.
+    Poisson UW (beta None): sigma = grad u, uhat = u, sigma_n_hat = sigma.n * parity.
+    ConvectionDiffusion (per-cell eps, constant beta): sigma = eps grad u,
+    uhat = u, tc = (beta u - sigma).n * parity."""
     C = nodes.shape[0]
     gf, gt = oracle.gll_nodes(k - 1), oracle.gll_nodes(k)
     nf, nt = len(gf), len(gt)
@@ -186,9 +188,10 @@ def _nodal_trial_coefficients_linear(nodes, par, k, a):
             for i in range(nf):
                 x = phys(c, (gf[i], gf[j]))
                 fld[j * nf + i] = a[0] + a[1] * x[0] + a[2] * x[1]
+        e = 1.0 if beta is None else eps[c]
         u[c, :nf * nf] = fld
-        u[c, nf * nf:2 * nf * nf] = a[1]
-        u[c, 2 * nf * nf:3 * nf * nf] = a[2]
+        u[c, nf * nf:2 * nf * nf] = e * a[1]
+        u[c, 2 * nf * nf:3 * nf * nf] = e * a[2]
         off = 3 * nf * nf
         for s, (p, q) in enumerate(sides):
             for i, t in enumerate(gt):
@@ -199,7 +202,14 @@ def _nodal_trial_coefficients_linear(nodes, par, k, a):
         for s, (p, q) in enumerate(sides):
             T = nodes[c, q] - nodes[c, p]
             n = np.array([T[1], -T[0]]) / np.hypot(*T)
-            u[c, off + s * nf:off + (s + 1) * nf] = (a[1] * n[0] + a[2] * n[1]) * par[c, s]
+            dudn = a[1] * n[0] + a[2] * n[1]
+            for i, t in enumerate(gf):
+                if beta is None:
+                    val = dudn
+                else:
+                    x = phys(c, 0.5 * (ref[p] + ref[q]) + 0.5 * (ref[q] - ref[p]) * t)
+                    val = (beta[0] * n[0] + beta[1] * n[1]) * (a[0] + a[1] * x[0] + a[2] * x[1]) - e * dudn
+                u[c, off + s * nf + i] = val * par[c, s]
     return u
 
 
@@ -220,6 +230,36 @@ def test_poisson_linear_solution_has_zero_residual(k, kt):
     assert np.abs(r).max() < 1e-13 * scale
     eta2 = oracle.energy_error(f, bk, nodes, par, u)
     assert np.abs(eta2).max() < 1e-24 * scale ** 2 * 1e4
+
+
+@pytest.mark.parametrize("k,kt", [(2, 3), (3, 5), (2, 2)])
+def test_convection_diffusion_linear_solution_has_zero_residual(k, kt):
+    """ConvectionDiffusionFormulation with per-cell eps (entering as 1/eps in
+    (sigma/eps, tau)) and constant beta: u linear, sigma = eps grad u and
+    f = div(beta u - sigma) = beta . grad u (a constant load) give l = B u.
+    kt = k is the dp = 1 case (fewer test than trial dofs)."""
+    nodes, par = _quad_mesh(3, 3, 0.1, 5)
+    par[:, 3] = -1
+    C = nodes.shape[0]
+    eps = 10.0 ** np.linspace(-4, -2, C)
+    beta = (np.cos(0.7), np.sin(0.7))
+    coefs = np.stack([eps, np.full(C, beta[0]), np.full(C, beta[1])], axis=1)
+    a = (0.4, -0.9, 1.3)
+    f = oracle.form("cd", 2)
+    bk = capi.BucketDesc(capi.QUAD, k, kt, 0)
+    load = capi.LoadDesc()
+    load.ncomp = 1
+    load.constant[0] = beta[0] * a[1] + beta[1] * a[2]
+    u = _nodal_trial_coefficients_linear(nodes, par, k, a, eps=eps, beta=beta)
+    o = oracle.assemble(f, bk, nodes, par, coefs=coefs, load=load, want=("B", "l"))
+    r = o["l"] - np.einsum("cmn,cm->cn", o["B"], u)
+    for c in range(C):
+        scale = np.abs(o["B"][c]).max() * np.abs(u[c]).max()
+        assert np.abs(r[c]).max() < 1e-12 * scale, c
+    # the same u with eps entering as eps (not 1/eps) leaves a residual
+    wrong = _nodal_trial_coefficients_linear(nodes, par, k, a, eps=1.0 / eps, beta=beta)
+    rw = o["l"] - np.einsum("cmn,cm->cn", o["B"], wrong)
+    assert np.abs(rw).max() > 1e-3
 
 
 def test_poisson_residual_detects_wrong_parity():
