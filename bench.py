@@ -33,7 +33,7 @@ CONFIGS = {
                k=4, kt=5, modes=8, seed=0xC0FFEE + 2),
     "c1": dict(name="poisson2d_quad16_p2_test4", form=("poisson", 2), topo="quad", n=(16, 16), jitter=0.10,
                k=3, kt=4, modes=8, seed=0xC0FFEE + 1),
-    "c3": dict(name="cd2d_hp_graded_quads49360_p1-6_dp1-3", hp=True, n0=16, levels=10),
+    "c3": dict(name="cd2d_hp_graded_49296quads_128tris_p1-6_dp1-3", hp=True, n0=16, levels=10, tri_cols=4),
 }
 
 
@@ -78,7 +78,7 @@ def cpu_baseline(f, b, nodes, par, load, gpu_K, target_s=12.0):
             "sample": f"first {n} cells of the same mesh, oracle K+rhs+Schur, {dt:.1f} s wall, {threads} OpenMP threads"}, err
 
 
-def cpu_baseline_hp(f, buckets, groups, nodes, par, coefs, load, gpu_K, target_s=12.0):
+def cpu_baseline_hp(f, groups, load, gpu_K, target_s=12.0):
     """Oracle on a stratified sample of the hp mesh (the same share of every
     element type), plus the GPU-vs-oracle error of K on that sample."""
     from oracle import oracle
@@ -86,23 +86,23 @@ def cpu_baseline_hp(f, buckets, groups, nodes, par, coefs, load, gpu_K, target_s
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     ff = oracle.form_like(f)
     per = []
-    for b, idx in zip(buckets, groups.values()):
-        sel = idx[:threads]
+    for g in groups:
+        n = min(threads, g["ncells"])
         t0 = time.perf_counter()
-        oracle.assemble(ff, b, nodes[sel], par[sel], coefs[sel], load, want=("K", "rhs", "S", "g"), nthreads=threads)
-        per.append((time.perf_counter() - t0) / len(sel))
-    full = sum(t * len(idx) for t, idx in zip(per, groups.values()))
+        oracle.assemble(ff, g["bucket"], g["nodes"][:n], g["parities"][:n], g["coefs"][:n], load,
+                        want=("K", "rhs", "S", "g"), nthreads=threads)
+        per.append((time.perf_counter() - t0) / n)
+    full = sum(t * g["ncells"] for t, g in zip(per, groups))
     frac = min(1.0, target_s / full)
     n_tot, t_tot, err = 0, 0.0, 0.0
-    for bi, (b, idx) in enumerate(zip(buckets, groups.values())):
-        n = max(threads, int(round(frac * len(idx) / threads)) * threads)
-        sel = idx[:n]
+    for bi, g in enumerate(groups):
+        n = min(g["ncells"], max(1, int(round(frac * g["ncells"]))))
         t0 = time.perf_counter()
-        o = oracle.assemble(ff, b, nodes[sel], par[sel], coefs[sel], load, want=("K", "rhs", "S", "g"),
-                            nthreads=threads)
+        o = oracle.assemble(ff, g["bucket"], g["nodes"][:n], g["parities"][:n], g["coefs"][:n], load,
+                            want=("K", "rhs", "S", "g"), nthreads=threads)
         t_tot += time.perf_counter() - t0
-        n_tot += len(sel)
-        Kg = gpu_K[bi][:len(sel)].cpu().numpy()
+        n_tot += n
+        Kg = gpu_K[bi][:n].cpu().numpy()
         err = max(err, float(np.max(np.linalg.norm((Kg - o["K"]).reshape(len(sel), -1), axis=1) /
                                     np.linalg.norm(o["K"].reshape(len(sel), -1), axis=1))))
     return {"value": n_tot / t_tot, "unit": "cells/s", "cores": threads, "kind": "port",
@@ -115,27 +115,27 @@ def bench_hp(args, cfg):
     against per-bucket plain launches (default and autotuned tiles)."""
     import torch
 
-    from camellia2_amd import dpg
-    from camellia2_amd.workloads import cd_hp, hp_buckets, hp_groups
+    from camellia2_amd import capi, dpg
+    from camellia2_amd.workloads import cd_hp, hp_groups
 
-    f, nodes, par, coefs, load, p, dp, lvl = cd_hp(cfg["n0"], cfg["levels"])
-    groups = hp_groups(p, dp)
-    buckets = hp_buckets(groups)
-    C = nodes.shape[0]
+    f, cells, load = cd_hp(cfg["n0"], cfg["levels"], tri_cols=cfg["tri_cols"])
+    groups = hp_groups(cells)
+    buckets = [g["bucket"] for g in groups]
+    C = sum(g["ncells"] for g in groups)
+    ntri = sum(g["ncells"] for g in groups if g["topology"] == capi.TRI)
     dev = torch.device("cuda:0")
     ws = args.workspace_gib * 2 ** 30
     infos = [dpg.sizes(f, b) for b in buckets]
-    flops = sum(len(idx) * i.flops_per_cell for idx, i in zip(groups.values(), infos))
-    ins = [(torch.from_numpy(np.ascontiguousarray(nodes[idx])).to(dev),
-            torch.from_numpy(np.ascontiguousarray(par[idx])).to(dev),
-            torch.from_numpy(np.ascontiguousarray(coefs[idx])).to(dev)) for idx in groups.values()]
+    flops = sum(g["ncells"] * i.flops_per_cell for g, i in zip(groups, infos))
+    ins = [(torch.from_numpy(g["nodes"]).to(dev), torch.from_numpy(g["parities"]).to(dev),
+            torch.from_numpy(g["coefs"]).to(dev)) for g in groups]
 
     def alloc():
-        return [{"K": torch.empty((len(idx), i.M, i.M), dtype=torch.float64, device=dev),
-                 "rhs": torch.empty((len(idx), i.M), dtype=torch.float64, device=dev),
-                 "schur": torch.empty((len(idx), i.T, i.T), dtype=torch.float64, device=dev),
-                 "schur_rhs": torch.empty((len(idx), i.T), dtype=torch.float64, device=dev)}
-                for idx, i in zip(groups.values(), infos)]
+        return [{"K": torch.empty((g["ncells"], i.M, i.M), dtype=torch.float64, device=dev),
+                 "rhs": torch.empty((g["ncells"], i.M), dtype=torch.float64, device=dev),
+                 "schur": torch.empty((g["ncells"], i.T, i.T), dtype=torch.float64, device=dev),
+                 "schur_rhs": torch.empty((g["ncells"], i.T), dtype=torch.float64, device=dev)}
+                for g, i in zip(groups, infos)]
 
     out_e, out_g = alloc(), alloc()
 
@@ -203,7 +203,8 @@ def bench_hp(args, cfg):
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded graded hp mesh, per-cell p, dp, eps; sine-mode load)",
-        "config": {"workload": cfg["name"], "cells": C, "element_types": len(buckets), "lanes": args.lanes,
+        "config": {"workload": cfg["name"], "cells": C, "quads": C - ntri, "triangles": ntri,
+                   "element_types": len(buckets), "lanes": args.lanes,
                    "outputs": "K,rhs,schur,schur_rhs", "precision": "fp64",
                    "mode": "one hipGraph launch over all element types"},
         "gflops": round(flops / dt_graph / 1e9, 1),
@@ -215,13 +216,14 @@ def bench_hp(args, cfg):
         "graph_equals_plain_bitwise": bool(same),
         "non_spd_cells": nbad,
         "autotune": {"seconds": round(t_tune, 2),
-                     "tiles": {f"k{k}_kt{kt}": list(t) for (k, kt), t in zip(groups, tiles)}},
+                     "tiles": {("tri" if g["topology"] == capi.TRI else "quad") + f"_k{g['k']}_kt{g['kt']}": list(t)
+                               for g, t in zip(groups, tiles)}},
         "graph_workspace_gib": round(gws / 2 ** 30, 2),
         "roofline": roofline,
         "stages": stages,
     }
     if not args.no_cpu:
-        cpu, err = cpu_baseline_hp(f, buckets, groups, nodes, par, coefs, load, [o["K"] for o in out_e])
+        cpu, err = cpu_baseline_hp(f, groups, load, [o["K"] for o in out_e])
         result["cpu_baseline"] = cpu
         result["rel_err_K_vs_fp64_oracle"] = err
     print(json.dumps(result))

@@ -16,7 +16,7 @@ MAX_RHS_TERMS = 8
 MAX_LOAD_MODES = 32
 MAX_COEFS = 8
 
-QUAD, HEX = 0, 1
+QUAD, HEX, TRI = 0, 1, 2
 FIELD, FIELD_VECTOR, TRACE, FLUX = 0, 1, 2, 3
 HGRAD, HDIV = 0, 1
 OP_VALUE, OP_DX, OP_DY, OP_DZ, OP_X, OP_Y, OP_Z, OP_DIV, OP_DOT_NORMAL = range(9)

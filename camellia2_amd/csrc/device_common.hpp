@@ -22,7 +22,7 @@ constexpr int kMaxGroups = 40;
 struct DevProgram {
   int32_t dim, nvar, npr;  // npr = dim + 1 reference prims per test dof
   int32_t n_lt, n_vb, n_sb, n_rhs;
-  int32_t pad;
+  int32_t nverts;  // vertices of the cell topology (geometry map)
   int32_t lt_beg[kMaxGroups + 1];
   int32_t vb_beg[kMaxGroups + 1];
   int32_t sb_beg[kMaxGroups + 1];

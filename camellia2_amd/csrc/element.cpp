@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <memory>
 #include <stdexcept>
 
 namespace dpg {
@@ -176,6 +177,155 @@ void LagrangeLine::eval(double t, double* val, double* der) const {
   }
 }
 
+namespace {
+// Jacobi P_0..P_n^(alpha,0) and derivatives at x (three-term recurrence and its derivative)
+void jacobi_a0(int n, double alpha, double x, double* P, double* dP) {
+  P[0] = 1.0;
+  dP[0] = 0.0;
+  if (n == 0) return;
+  P[1] = 0.5 * ((alpha + 2.0) * x + alpha);
+  dP[1] = 0.5 * (alpha + 2.0);
+  for (int k = 2; k <= n; ++k) {
+    double a = 2.0 * k + alpha;
+    double c0 = 2.0 * k * (k + alpha) * (a - 2.0);
+    double c1 = (a - 1.0) * a * (a - 2.0), c2 = (a - 1.0) * alpha * alpha;
+    double c3 = 2.0 * (k + alpha - 1.0) * (k - 1.0) * a;
+    P[k] = ((c1 * x + c2) * P[k - 1] - c3 * P[k - 2]) / c0;
+    dP[k] = (c1 * P[k - 1] + (c1 * x + c2) * dP[k - 1] - c3 * dP[k - 2]) / c0;
+  }
+}
+}  // namespace
+
+DubinerTri::DubinerTri(int n) : n_(n) {
+  for (int d = 0; d <= n; ++d)
+    for (int j = 0; j <= d; ++j) {
+      int i = d - j;
+      i_.push_back(i);
+      j_.push_back(j);
+      // ||psi_ij||^2 = 1 / (2 (2i+1) (i+j+1)) on the unit triangle
+      scale_.push_back(std::sqrt(2.0 * (2.0 * i + 1.0) * (i + j + 1.0)));
+    }
+}
+
+void DubinerTri::eval(double x, double y, double* val, double* dx, double* dy) const {
+  double r = 2.0 * x - 1.0, s = 2.0 * y - 1.0;
+  double q = 0.5 * (1.0 - s);  // (1-b)/2, b = s
+  double a = (s < 1.0) ? 2.0 * (1.0 + r) / (1.0 - s) - 1.0 : -1.0;
+  double Pa[16], dPa[16], Pb[16], dPb[16], qp[17];
+  jacobi_a0(n_, 0.0, a, Pa, dPa);
+  qp[0] = 1.0;
+  for (int k = 1; k <= n_; ++k) qp[k] = qp[k - 1] * q;
+  int m = 0;
+  for (int d = 0; d <= n_; ++d)
+    for (int j = 0; j <= d; ++j, ++m) {
+      int i = d - j;
+      jacobi_a0(j, 2.0 * i + 1.0, s, Pb, dPb);
+      double fa = Pa[i], dfa = dPa[i], g = Pb[j], dg = dPb[j];
+      val[m] = scale_[m] * fa * qp[i] * g;
+      if (dx || dy) {
+        double dr = 0.0, ds = fa * dg * qp[i];
+        if (i > 0) {
+          dr = dfa * g * qp[i - 1];
+          ds += dfa * 0.5 * (1.0 + a) * g * qp[i - 1] - fa * 0.5 * i * g * qp[i - 1];
+        }
+        if (dx) dx[m] = 2.0 * scale_[m] * dr;
+        if (dy) dy[m] = 2.0 * scale_[m] * ds;
+      }
+    }
+}
+
+namespace {
+// warp of the 1D equispaced nodes onto the GLL nodes, divided by the edge bubble
+double warp_factor(int n, const LagrangeLine& eq, const std::vector<double>& gll, const std::vector<double>& req,
+                   double r) {
+  if (std::fabs(r) >= 1.0 - 1e-10) return 0.0;
+  double l[32];
+  eq.eval(r, l, nullptr);
+  double w = 0.0;
+  for (int i = 0; i <= n; ++i) w += l[i] * (gll[i] - req[i]);
+  return w / (1.0 - r * r);
+}
+}  // namespace
+
+std::vector<double> warp_blend_tri(int n) {
+  std::vector<double> out;
+  if (n == 0) return {1.0 / 3.0, 1.0 / 3.0};
+  static const double alpopt[15] = {0.0000, 0.0000, 1.4152, 0.1001, 0.2751, 0.9800, 1.0999, 1.2832,
+                                    1.3648, 1.4773, 1.4959, 1.5743, 1.5770, 1.6223, 1.6258};
+  double alpha = n < 16 ? alpopt[n - 1] : 5.0 / 3.0;
+  std::vector<double> gll = gll_points(n), req(n + 1);
+  for (int i = 0; i <= n; ++i) req[i] = -1.0 + 2.0 * i / n;
+  LagrangeLine eq(req);
+  const double s3 = std::sqrt(3.0), c2 = std::cos(2.0 * M_PI / 3.0), s2 = std::sin(2.0 * M_PI / 3.0);
+  const double c4 = std::cos(4.0 * M_PI / 3.0), s4 = std::sin(4.0 * M_PI / 3.0);
+  for (int row = 0; row <= n; ++row)
+    for (int col = 0; col + row <= n; ++col) {
+      double L1 = (double)row / n, L3 = (double)col / n, L2 = 1.0 - L1 - L3;
+      // equilateral triangle (-1,-1/sqrt3), (1,-1/sqrt3), (0,2/sqrt3)
+      double x = -L2 + L3, y = (-L2 - L3 + 2.0 * L1) / s3;
+      double w1 = 4.0 * L2 * L3 * warp_factor(n, eq, gll, req, L3 - L2) * (1.0 + (alpha * L1) * (alpha * L1));
+      double w2 = 4.0 * L1 * L3 * warp_factor(n, eq, gll, req, L1 - L3) * (1.0 + (alpha * L2) * (alpha * L2));
+      double w3 = 4.0 * L1 * L2 * warp_factor(n, eq, gll, req, L2 - L1) * (1.0 + (alpha * L3) * (alpha * L3));
+      x += w1 + c2 * w2 + c4 * w3;
+      y += s2 * w2 + s4 * w3;
+      // back to barycentric coordinates, then to the unit triangle
+      double b1 = (s3 * y + 1.0) / 3.0, b2 = (-3.0 * x - s3 * y + 2.0) / 6.0, b3 = (3.0 * x - s3 * y + 2.0) / 6.0;
+      double rr = -b2 + b3 - b1, ss = -b2 - b3 + b1;
+      out.push_back(0.5 * (rr + 1.0));
+      out.push_back(0.5 * (ss + 1.0));
+    }
+  return out;
+}
+
+LagrangeTri::LagrangeTri(int n, const std::vector<double>& nodes) : P_(n) {
+  int m = P_.size();
+  if ((int)nodes.size() != 2 * m) throw std::invalid_argument("triangle node count");
+  // V[i][k] = psi_k(node i); coef = V^-1 (Gauss-Jordan with partial pivoting)
+  std::vector<double> V((size_t)m * m), I((size_t)m * m, 0.0);
+  for (int i = 0; i < m; ++i) {
+    P_.eval(nodes[2 * i], nodes[2 * i + 1], &V[(size_t)i * m], nullptr, nullptr);
+    I[(size_t)i * m + i] = 1.0;
+  }
+  for (int c = 0; c < m; ++c) {
+    int p = c;
+    for (int r = c + 1; r < m; ++r)
+      if (std::fabs(V[(size_t)r * m + c]) > std::fabs(V[(size_t)p * m + c])) p = r;
+    for (int j = 0; j < m; ++j) {
+      std::swap(V[(size_t)c * m + j], V[(size_t)p * m + j]);
+      std::swap(I[(size_t)c * m + j], I[(size_t)p * m + j]);
+    }
+    double d = V[(size_t)c * m + c];
+    for (int j = 0; j < m; ++j) {
+      V[(size_t)c * m + j] /= d;
+      I[(size_t)c * m + j] /= d;
+    }
+    for (int r = 0; r < m; ++r) {
+      if (r == c) continue;
+      double f = V[(size_t)r * m + c];
+      if (f == 0.0) continue;
+      for (int j = 0; j < m; ++j) {
+        V[(size_t)r * m + j] -= f * V[(size_t)c * m + j];
+        I[(size_t)r * m + j] -= f * I[(size_t)c * m + j];
+      }
+    }
+  }
+  // phi_k(x) = sum_j psi_j(x) Vinv[j][k]
+  coef_.assign((size_t)m * m, 0.0);
+  for (int k = 0; k < m; ++k)
+    for (int j = 0; j < m; ++j) coef_[(size_t)k * m + j] = I[(size_t)j * m + k];
+}
+
+void LagrangeTri::eval(double x, double y, double* val) const {
+  int m = P_.size();
+  std::vector<double> psi(m);
+  P_.eval(x, y, psi.data(), nullptr, nullptr);
+  for (int k = 0; k < m; ++k) {
+    double v = 0.0;
+    for (int j = 0; j < m; ++j) v += coef_[(size_t)k * m + j] * psi[j];
+    val[k] = v;
+  }
+}
+
 RefCell make_ref_cell(int topo) {
   RefCell c;
   c.topo = topo;
@@ -193,6 +343,13 @@ RefCell make_ref_cell(int topo) {
     c.verts = {-1, -1, -1, 1, -1, -1, 1, 1, -1, -1, 1, -1, -1, -1, 1, 1, -1, 1, 1, 1, 1, -1, 1, 1};
     c.nodes_per_side = 4;  // shards Hexahedron<8> face node map
     c.side_nodes = {0, 1, 5, 4, 1, 2, 6, 5, 2, 3, 7, 6, 0, 4, 7, 3, 0, 3, 2, 1, 4, 5, 6, 7};
+  } else if (topo == DPG_TRI) {
+    c.dim = 2;
+    c.nverts = 3;
+    c.nsides = 3;
+    c.verts = {0, 0, 1, 0, 0, 1};  // shards Triangle<3>
+    c.nodes_per_side = 2;
+    c.side_nodes = {0, 1, 1, 2, 2, 0};
   } else {
     throw std::invalid_argument("unsupported cell topology");
   }
@@ -234,7 +391,8 @@ ElementLayout make_layout(const dpg_form_desc& f, const dpg_bucket_desc& b) {
   L.trace_deg = L.k;
   L.flux_deg = L.k - 1;
   int sd = L.dim - 1;
-  L.nf = ipow(L.field_deg + 1, L.dim);
+  const bool tri = L.topo == DPG_TRI;
+  L.nf = tri ? (L.field_deg + 1) * (L.field_deg + 2) / 2 : ipow(L.field_deg + 1, L.dim);
   L.ntr = ipow(L.trace_deg + 1, sd);
   L.nfl = ipow(L.flux_deg + 1, sd);
   bool hasField = false, hasTrace = false, hasFlux = false, sawSide = false;
@@ -271,9 +429,12 @@ ElementLayout make_layout(const dpg_form_desc& f, const dpg_bucket_desc& b) {
   off = 0;
   for (int v = 0; v < f.n_test; ++v) {
     int card;
-    if (f.test_space[v] == DPG_HGRAD) card = ipow(L.kt + 1, L.dim);
-    else if (f.test_space[v] == DPG_HDIV) card = L.dim == 2 ? 2 * L.kt * (L.kt + 1) : 3 * L.kt * L.kt * (L.kt + 1);
-    else throw std::invalid_argument("unknown test space");
+    if (f.test_space[v] == DPG_HGRAD)
+      card = tri ? (L.kt + 1) * (L.kt + 2) / 2 : ipow(L.kt + 1, L.dim);
+    else if (f.test_space[v] == DPG_HDIV)
+      card = tri ? L.kt * (L.kt + 2) : (L.dim == 2 ? 2 * L.kt * (L.kt + 1) : 3 * L.kt * L.kt * (L.kt + 1));
+    else
+      throw std::invalid_argument("unknown test space");
     L.test_off.push_back(off);
     L.test_card.push_back(card);
     for (int i = 0; i < card; ++i) L.var_of_test_dof.push_back(v);
@@ -290,6 +451,12 @@ ElementLayout make_layout(const dpg_form_desc& f, const dpg_bucket_desc& b) {
   L.Q1 = (maxTrial + maxTest) / 2 + 1;
   L.QG1 = (2 * maxTest) / 2 + 1;
   L.Qs1 = (sideTrial + maxTest) / 2 + 1;
+  if (tri) {
+    // collapsed (Duffy) Gauss rule: the extra (1-b)/2 Jacobian raises the degree
+    // in b by one, so floor(d/2)+2 points per direction integrate degree d exactly
+    L.Q1 = (maxTrial + maxTest) / 2 + 2;
+    L.QG1 = (2 * maxTest) / 2 + 2;
+  }
   L.Q = ipow(L.Q1, L.dim);
   L.QG = ipow(L.QG1, L.dim);
   L.Qs = ipow(L.Qs1, sd);
@@ -378,6 +545,47 @@ void test_prims(const dpg_form_desc& f, const ElementLayout& L, const LagrangeLi
   }
 }
 
+// triangle test prims: HGRAD = Dubiner P_kt (v, dv/dx, dv/dy); HDIV = RT_kt as
+// (psi, 0), (0, psi) for psi in P_{kt-1}, then (x - xc) psi for the kt Dubiner
+// functions of degree exactly kt-1 (their top-degree parts span the homogeneous
+// polynomials of degree kt-1, so the set is a basis of RT_kt = P_{kt-1}^2 + x P~_{kt-1}).
+// K, rhs, Schur and the energy error do not depend on the choice of test basis.
+void test_prims_tri(const dpg_form_desc& f, const ElementLayout& L, const DubinerTri& Pt, const DubinerTri& Pr,
+                    const double* xi, double* out) {
+  const int N = L.N;
+  std::vector<double> v(Pt.size()), dx(Pt.size()), dy(Pt.size());
+  std::vector<double> rv(Pr.size()), rdx(Pr.size()), rdy(Pr.size());
+  Pt.eval(xi[0], xi[1], v.data(), dx.data(), dy.data());
+  Pr.eval(xi[0], xi[1], rv.data(), rdx.data(), rdy.data());
+  const double xc = xi[0] - 1.0 / 3.0, yc = xi[1] - 1.0 / 3.0;
+  const int nlow = Pr.size(), ntop = Pr.degree() + 1;
+  for (int tv = 0; tv < f.n_test; ++tv) {
+    int off = L.test_off[tv];
+    if (f.test_space[tv] == DPG_HGRAD) {
+      for (int m = 0; m < Pt.size(); ++m) {
+        out[0 * N + off + m] = v[m];
+        out[1 * N + off + m] = dx[m];
+        out[2 * N + off + m] = dy[m];
+      }
+    } else {
+      for (int m = 0; m < nlow; ++m) {
+        out[0 * N + off + m] = rv[m];
+        out[1 * N + off + m] = 0.0;
+        out[2 * N + off + m] = rdx[m];
+        out[0 * N + off + nlow + m] = 0.0;
+        out[1 * N + off + nlow + m] = rv[m];
+        out[2 * N + off + nlow + m] = rdy[m];
+      }
+      for (int t = 0; t < ntop; ++t) {
+        int m = nlow - ntop + t, b = off + 2 * nlow + t;
+        out[0 * N + b] = xc * rv[m];
+        out[1 * N + b] = yc * rv[m];
+        out[2 * N + b] = 2.0 * rv[m] + xc * rdx[m] + yc * rdy[m];
+      }
+    }
+  }
+}
+
 void scalar_values(int dim, const LagrangeLine& B, const double* xi, double* out) {
   double v[3][16];
   for (int a = 0; a < dim; ++a) B.eval(xi[a], v[a], nullptr);
@@ -392,6 +600,14 @@ void scalar_values(int dim, const LagrangeLine& B, const double* xi, double* out
 }
 
 void vertex_basis(const RefCell& rc, const double* xi, double* Nv, double* dN) {
+  if (rc.topo == DPG_TRI) {  // barycentric: 1-x-y, x, y
+    Nv[0] = 1.0 - xi[0] - xi[1];
+    Nv[1] = xi[0];
+    Nv[2] = xi[1];
+    const double g[6] = {-1.0, -1.0, 1.0, 0.0, 0.0, 1.0};
+    for (int k = 0; k < 6; ++k) dN[k] = g[k];
+    return;
+  }
   for (int v = 0; v < rc.nverts; ++v) {
     const double* p = &rc.verts[(size_t)v * rc.dim];
     double fac[3], g[3];
@@ -411,8 +627,13 @@ void vertex_basis(const RefCell& rc, const double* xi, double* Nv, double* dN) {
   }
 }
 
+struct TriBases {
+  DubinerTri Pt, Pr;  // test degree kt and kt-1
+  LagrangeTri Fld;    // field: nodal on warp-and-blend nodes of degree k-1
+};
+
 void fill_pointset(const dpg_form_desc& f, const ElementLayout& L, const LagrangeLine& Cl, const LagrangeLine& Op,
-                   const LagrangeLine& Fld, PointSet& ps) {
+                   const LagrangeLine& Fld, const TriBases* tb, PointSet& ps) {
   int dim = L.dim, V = L.rc.nverts;
   ps.R.assign((size_t)ps.npts * (dim + 1) * L.N, 0.0);
   ps.dN.assign((size_t)ps.npts * V * dim, 0.0);
@@ -420,10 +641,30 @@ void fill_pointset(const dpg_form_desc& f, const ElementLayout& L, const Lagrang
   ps.field.assign((size_t)ps.npts * L.nf, 0.0);
   for (int q = 0; q < ps.npts; ++q) {
     const double* xi = &ps.xi[(size_t)q * dim];
-    test_prims(f, L, Cl, Op, xi, &ps.R[(size_t)q * (dim + 1) * L.N]);
+    if (tb) {
+      test_prims_tri(f, L, tb->Pt, tb->Pr, xi, &ps.R[(size_t)q * (dim + 1) * L.N]);
+      tb->Fld.eval(xi[0], xi[1], &ps.field[(size_t)q * L.nf]);
+    } else {
+      test_prims(f, L, Cl, Op, xi, &ps.R[(size_t)q * (dim + 1) * L.N]);
+      if (L.field_deg >= 1 || L.nf == 1) scalar_values(dim, Fld, xi, &ps.field[(size_t)q * L.nf]);
+    }
     vertex_basis(L.rc, xi, &ps.Nv[(size_t)q * V], &ps.dN[(size_t)q * V * dim]);
-    if (L.field_deg >= 1 || L.nf == 1) scalar_values(dim, Fld, xi, &ps.field[(size_t)q * L.nf]);
   }
+}
+
+// collapsed Gauss rule on the unit triangle, n x n points, b (y) outer
+void tri_points(int n, std::vector<double>& xi, std::vector<double>& w) {
+  Rule1D r = gauss_legendre(n);
+  xi.assign((size_t)n * n * 2, 0.0);
+  w.assign((size_t)n * n, 0.0);
+  for (int jb = 0; jb < n; ++jb)
+    for (int ia = 0; ia < n; ++ia) {
+      double a = r.x[ia], b = r.x[jb];
+      int q = jb * n + ia;
+      xi[2 * q + 0] = 0.25 * (1.0 + a) * (1.0 - b);
+      xi[2 * q + 1] = 0.5 * (1.0 + b);
+      w[q] = r.w[ia] * r.w[jb] * (1.0 - b) * 0.125;  // (1-b)/2 Jacobian, 1/4 to the unit triangle
+    }
 }
 
 void tensor_points(int dim, const Rule1D& r, std::vector<double>& xi, std::vector<double>& w) {
@@ -452,14 +693,23 @@ RefTables make_tables(const dpg_form_desc& f, const ElementLayout& L) {
   LagrangeLine Cl(gll_points(L.kt));
   LagrangeLine Op(gauss_legendre(L.kt).x);
   LagrangeLine Fld(gll_points(std::max(L.field_deg, 1)));
+  std::unique_ptr<TriBases> tb;
+  if (L.topo == DPG_TRI) {
+    tb.reset(new TriBases());
+    tb->Pt = DubinerTri(L.kt);
+    tb->Pr = DubinerTri(L.kt - 1);
+    tb->Fld = LagrangeTri(L.field_deg, warp_blend_tri(L.field_deg));
+  }
   // volume stiffness cache
-  tensor_points(dim, gauss_legendre(L.Q1), T.vol.xi, T.vol.w);
+  if (tb) tri_points(L.Q1, T.vol.xi, T.vol.w);
+  else tensor_points(dim, gauss_legendre(L.Q1), T.vol.xi, T.vol.w);
   T.vol.npts = L.Q;
-  fill_pointset(f, L, Cl, Op, Fld, T.vol);
+  fill_pointset(f, L, Cl, Op, Fld, tb.get(), T.vol);
   // Gram cache
-  tensor_points(dim, gauss_legendre(L.QG1), T.ip.xi, T.ip.w);
+  if (tb) tri_points(L.QG1, T.ip.xi, T.ip.w);
+  else tensor_points(dim, gauss_legendre(L.QG1), T.ip.xi, T.ip.w);
   T.ip.npts = L.QG;
-  fill_pointset(f, L, Cl, Op, Fld, T.ip);
+  fill_pointset(f, L, Cl, Op, Fld, tb.get(), T.ip);
   // sides
   std::vector<double> sxi, sw;
   tensor_points(dim - 1, gauss_legendre(L.Qs1), sxi, sw);
@@ -478,7 +728,7 @@ RefTables make_tables(const dpg_form_desc& f, const ElementLayout& L) {
       T.side.w[(size_t)s * L.Qs + q] = sw[q];
     }
   }
-  fill_pointset(f, L, Cl, Op, Fld, T.side);
+  fill_pointset(f, L, Cl, Op, Fld, tb.get(), T.side);
   // side trial bases on the side reference points
   LagrangeLine Tr(gll_points(L.trace_deg));
   T.trace.assign((size_t)L.Qs * L.ntr, 0.0);

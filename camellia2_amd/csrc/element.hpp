@@ -35,6 +35,44 @@ class LagrangeLine {
   std::vector<double> coef_;  // (n, n)
 };
 
+// Triangle bases (reference triangle (0,0),(1,0),(0,1)) -------------------------
+// Orthonormal Dubiner basis of P_n: psi_ij(x,y) = P_i(a) ((1-b)/2)^i P_j^(2i+1,0)(b)
+// in collapsed coordinates a = 2(1+r)/(1-s)-1, b = s, r = 2x-1, s = 2y-1
+// (Koornwinder; Sherwin-Karniadakis, Hesthaven-Warburton 2008, sec. 6.1),
+// ordered by total degree then j; normalised to unit L2 norm on the triangle.
+class DubinerTri {
+ public:
+  DubinerTri() = default;
+  explicit DubinerTri(int n);
+  int degree() const { return n_; }
+  int size() const { return (n_ + 1) * (n_ + 2) / 2; }
+  // values, d/dx, d/dy (each size()) at (x, y); dx/dy may be NULL
+  void eval(double x, double y, double* val, double* dx, double* dy) const;
+
+ private:
+  int n_ = 0;
+  std::vector<int> i_, j_;
+  std::vector<double> scale_;
+};
+
+// Warp-and-blend interpolation nodes of degree n on the reference triangle
+// (Warburton 2006; Hesthaven-Warburton 2008, Nodes2D), rows bottom to top,
+// x fastest within a row: (n+1)(n+2)/2 points (x, y).
+std::vector<double> warp_blend_tri(int n);
+
+// Nodal Lagrange basis on given triangle nodes, through the Dubiner Vandermonde.
+class LagrangeTri {
+ public:
+  LagrangeTri() = default;
+  LagrangeTri(int n, const std::vector<double>& nodes);
+  int size() const { return P_.size(); }
+  void eval(double x, double y, double* val) const;
+
+ private:
+  DubinerTri P_;
+  std::vector<double> coef_;  // (size, size): phi_k = sum_m coef[k][m] psi_m
+};
+
 // Reference cell data ---------------------------------------------------------
 struct RefCell {
   int topo = 0, dim = 0, nverts = 0, nsides = 0;

@@ -145,7 +145,7 @@ __global__ __launch_bounds__(128) void point_kernel(int kind, const DevProgram* 
   int gid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= ncells * npts) return;
   int c = gid / npts, p = gid % npts;
-  const int V = DIM == 2 ? 4 : 8;
+  const int V = prog->nverts;
   const int npr = DIM + 1;
   const int nvar = prog->nvar;
   const double* X = nodes + (size_t)c * V * DIM;
@@ -887,7 +887,7 @@ __global__ __launch_bounds__(128) void adjoint_point_kernel(int kind, const DevP
   int gid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= ncells * npts) return;
   const int c = gid / npts, p = gid % npts;
-  const int V = DIM == 2 ? 4 : 8;
+  const int V = prog->nverts;
   constexpr int NPR = DIM + 1;
   const int nvar = prog->nvar, ncoef = prog->ncoef;
   const double* X = nodes + (size_t)c * V * DIM;

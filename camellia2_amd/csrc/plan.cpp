@@ -222,6 +222,7 @@ void compile_program(const dpg_form_desc& f, BucketPlan& B) {
   DevProgram& P = B.prog;
   std::memset(&P, 0, sizeof(P));
   P.dim = f.spatial_dim;
+  P.nverts = B.L.rc.nverts;
   P.nvar = f.n_test;
   P.npr = f.spatial_dim + 1;
   int ns = 0;

@@ -40,7 +40,7 @@ extern "C" {
 #define DPG_MAX_COEFS 8
 
 /* cell topologies */
-enum { DPG_QUAD = 0, DPG_HEX = 1 };
+enum { DPG_QUAD = 0, DPG_HEX = 1, DPG_TRI = 2 };
 /* trial variable kinds (VarFactory::fieldVar / traceVar / fluxVar).
  * With H1 order k: FIELD and FIELD_VECTOR use HGRAD Cn(k-1) (HVOL, identity
  * transform), TRACE uses Cn(k) on every side (HGRAD trace), FLUX uses

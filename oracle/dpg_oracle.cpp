@@ -146,7 +146,11 @@ struct RefCell {
 
 RefCell ref_cell(int topo) {
   RefCell c;
-  if (topo == DPG_QUAD) {
+  if (topo == DPG_TRI) {  // shards Triangle<3>
+    c.dim = 2; c.nverts = 3; c.nsides = 3;
+    c.verts = {0, 0, 1, 0, 0, 1};
+    c.side_nodes = {{0, 1}, {1, 2}, {2, 0}};
+  } else if (topo == DPG_QUAD) {
     c.dim = 2; c.nverts = 4; c.nsides = 4;
     c.verts = {-1, -1, 1, -1, 1, 1, -1, 1};
     c.side_nodes = {{0, 1}, {1, 2}, {2, 3}, {3, 0}};
@@ -160,6 +164,11 @@ RefCell ref_cell(int topo) {
 
 // vertex (C1) basis values and reference gradients at xi
 void vertex_basis(const RefCell& rc, const double* xi, double* N, double* dN /*(V,dim)*/) {
+  if (rc.nverts == 3) {  // linear triangle
+    N[0] = 1.0 - xi[0] - xi[1]; N[1] = xi[0]; N[2] = xi[1];
+    dN[0] = -1.0; dN[1] = -1.0; dN[2] = 1.0; dN[3] = 0.0; dN[4] = 0.0; dN[5] = 1.0;
+    return;
+  }
   for (int v = 0; v < rc.nverts; ++v) {
     const double* p = &rc.verts[v * rc.dim];
     double f[3], g[3];
@@ -219,6 +228,7 @@ struct Element {
 
 int scalar_card(int dim, int n) { return ipow(n + 1, dim); }
 int hdiv_card(int dim, int n) { return dim == 2 ? 2 * n * (n + 1) : 3 * n * n * (n + 1); }
+int tri_field_card(int n) { return (n + 1) * (n + 2) / 2; }
 
 bool build_element(const dpg_form_desc* f, const dpg_bucket_desc* b, Element& e) {
   e.topo = b->topology;
@@ -238,8 +248,9 @@ bool build_element(const dpg_form_desc* f, const dpg_bucket_desc* b, Element& e)
     int kind = f->trial_kind[v];
     e.trial_offset.push_back(off);
     int card;
-    if (kind == DPG_FIELD) { card = scalar_card(e.dim, e.field_deg); off += card; hasField = true; nf += card; }
-    else if (kind == DPG_FIELD_VECTOR) { card = scalar_card(e.dim, e.field_deg); off += e.dim * card; hasField = true; nf += e.dim * card; }
+    int fc = e.topo == DPG_TRI ? tri_field_card(e.field_deg) : scalar_card(e.dim, e.field_deg);
+    if (kind == DPG_FIELD) { card = fc; off += card; hasField = true; nf += card; }
+    else if (kind == DPG_FIELD_VECTOR) { card = fc; off += e.dim * card; hasField = true; nf += e.dim * card; }
     else if (kind == DPG_TRACE) { card = scalar_card(sideDim, e.trace_deg); off += e.rc.nsides * card; hasTrace = true; }
     else { card = scalar_card(sideDim, e.flux_deg); off += e.rc.nsides * card; hasFlux = true; }
     e.trial_card.push_back(card);
@@ -249,6 +260,7 @@ bool build_element(const dpg_form_desc* f, const dpg_bucket_desc* b, Element& e)
   off = 0;
   for (int v = 0; v < f->n_test; ++v) {
     int card = f->test_space[v] == DPG_HGRAD ? scalar_card(e.dim, e.kt) : hdiv_card(e.dim, e.kt);
+    if (e.topo == DPG_TRI) card = f->test_space[v] == DPG_HGRAD ? tri_field_card(e.kt) : e.kt * (e.kt + 2);
     e.test_offset.push_back(off);
     e.test_card.push_back(card);
     off += card;
@@ -265,6 +277,10 @@ bool build_element(const dpg_form_desc* f, const dpg_bucket_desc* b, Element& e)
   e.QG1 = (2 * e.maxTestDeg) / 2 + 1;
   int sideTrial = hasTrace ? e.trace_deg : (hasFlux ? e.flux_deg : e.maxTrialDeg);
   e.Qs1 = (sideTrial + e.maxTestDeg) / 2 + 1;
+  if (e.topo == DPG_TRI) {  // collapsed rule: one more point per direction for the (1-b) Jacobian
+    e.Q1 = (e.maxTrialDeg + e.maxTestDeg) / 2 + 2;
+    e.QG1 = (2 * e.maxTestDeg) / 2 + 2;
+  }
   // fields must precede traces for the condensed (Schur) output
   return true;
 }
@@ -340,6 +356,208 @@ void hdiv_tensor(int dim, const Lagrange1D& C, const Lagrange1D& O, const double
   }
 }
 
+// ------------------------------------------------------------ triangles
+// (Intrepid's HGRAD_TRI_Cn / HDIV_TRI_In are not vendored.) Test spaces: any
+// basis of P_n and RT_n gives the same K, rhs, Schur and energy error, so the
+// oracle uses its own: orthogonal Koornwinder polynomials (unnormalised,
+// ordered i outer, j inner) and RT_n = {(p,0),(0,p)} for p in P_{n-1}
+// interleaved, plus x p_top for the degree-(n-1) members, x measured from
+// vertex 0. Field (trial) basis: Lagrange on warp-and-blend nodes of degree p
+// (Warburton 2006), rows bottom to top, x fastest.
+
+// Jacobi P_0..P_n^(a,b)(x), three-term recurrence
+void jacobi(int n, double a, double b, double x, double* P) {
+  P[0] = 1.0;
+  if (n == 0) return;
+  P[1] = 0.5 * (a - b + (a + b + 2.0) * x);
+  for (int k = 1; k < n; ++k) {
+    double c = 2.0 * k + a + b;
+    double a1 = 2.0 * (k + 1) * (k + a + b + 1.0) * c;
+    double a2 = (c + 1.0) * (a * a - b * b);
+    double a3 = c * (c + 1.0) * (c + 2.0);
+    double a4 = 2.0 * (k + a) * (k + b) * (c + 2.0);
+    P[k + 1] = ((a2 + a3 * x) * P[k] - a4 * P[k - 1]) / a1;
+  }
+}
+
+// d/dx P_m^(a,b) = (m+a+b+1)/2 P_{m-1}^(a+1,b+1)
+double jacobi_der(int m, double a, double b, double x) {
+  if (m == 0) return 0.0;
+  double P[24];
+  jacobi(m - 1, a + 1.0, b + 1.0, x, P);
+  return 0.5 * (m + a + b + 1.0) * P[m - 1];
+}
+
+int tri_card(int n) { return (n + 1) * (n + 2) / 2; }
+
+// Koornwinder basis of P_n at (x,y): values and (d/dx, d/dy), order i outer, j inner
+void koornwinder(int n, double x, double y, double* val, double* gx, double* gy) {
+  double r = 2.0 * x - 1.0, s = 2.0 * y - 1.0;
+  double h = 0.5 * (1.0 - s);
+  double a = (h > 0.0) ? (1.0 + r) / h - 1.0 : -1.0;
+  double La[24];
+  jacobi(n, 0.0, 0.0, a, La);
+  int m = 0;
+  for (int i = 0; i <= n; ++i) {
+    double Jb[24];
+    jacobi(n - i, 2.0 * i + 1.0, 0.0, s, Jb);
+    double hi = std::pow(h, i), him1 = i > 0 ? std::pow(h, i - 1) : 0.0;
+    double dLa = jacobi_der(i, 0.0, 0.0, a);
+    for (int j = 0; j + i <= n; ++j, ++m) {
+      double g = Jb[j], dg = jacobi_der(j, 2.0 * i + 1.0, 0.0, s);
+      val[m] = La[i] * hi * g;
+      if (gx) {
+        // d/dr [La(a) h^i] = La'(a) h^(i-1);  d/ds = La'(a) (1+a)/2 h^(i-1) - La(a) i/2 h^(i-1)
+        double dr = dLa * him1 * g;
+        double ds = (dLa * 0.5 * (1.0 + a) - La[i] * 0.5 * i) * him1 * g + La[i] * hi * dg;
+        gx[m] = 2.0 * dr;
+        gy[m] = 2.0 * ds;
+      }
+    }
+  }
+}
+
+// warp-and-blend nodes of degree n (rows bottom to top, x fastest), on the unit triangle
+void warp_blend_nodes(int n, vector<double>& out) {
+  out.clear();
+  if (n == 0) { out = {1.0 / 3.0, 1.0 / 3.0}; return; }
+  static const double alpopt[15] = {0.0, 0.0, 1.4152, 0.1001, 0.2751, 0.9800, 1.0999, 1.2832,
+                                    1.3648, 1.4773, 1.4959, 1.5743, 1.5770, 1.6223, 1.6258};
+  double alpha = n < 16 ? alpopt[n - 1] : 5.0 / 3.0;
+  vector<double> gl;
+  gll_nodes(n, gl);
+  vector<double> eqn(n + 1);
+  for (int i = 0; i <= n; ++i) eqn[i] = -1.0 + 2.0 * i / n;
+  Lagrange1D eq;
+  eq.nodes = eqn;
+  auto warp = [&](double t) {  // (interpolated GLL shift) / (1 - t^2), zero at the ends
+    if (std::fabs(t) >= 1.0 - 1e-10) return 0.0;
+    double l[32];
+    eq.eval(t, l, nullptr);
+    double w = 0.0;
+    for (int i = 0; i <= n; ++i) w += l[i] * (gl[i] - eqn[i]);
+    return w / (1.0 - t * t);
+  };
+  // equilateral reference: vertices v0 (-1,-1/sqrt3), v1 (1,-1/sqrt3), v2 (0, 2/sqrt3);
+  // each edge's warp acts along that edge's direction
+  const double r3 = std::sqrt(3.0);
+  const double V[3][2] = {{-1.0, -1.0 / r3}, {1.0, -1.0 / r3}, {0.0, 2.0 / r3}};
+  for (int row = 0; row <= n; ++row)
+    for (int col = 0; col + row <= n; ++col) {
+      double l2 = (double)row / n, l1 = (double)col / n, l0 = 1.0 - l1 - l2;  // barycentric of v0, v1, v2
+      double bc[3] = {l0, l1, l2};
+      double X = l0 * V[0][0] + l1 * V[1][0] + l2 * V[2][0];
+      double Y = l0 * V[0][1] + l1 * V[1][1] + l2 * V[2][1];
+      // edge (p -> q), opposite vertex o: blend 4 l_p l_q, warp of (l_q - l_p), direction (v_q - v_p)/2
+      const int E[3][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}};
+      for (int k = 0; k < 3; ++k) {
+        int p = E[k][0], q = E[k][1], o = E[k][2];
+        double w = 4.0 * bc[p] * bc[q] * warp(bc[q] - bc[p]) * (1.0 + (alpha * bc[o]) * (alpha * bc[o]));
+        X += w * 0.5 * (V[q][0] - V[p][0]);
+        Y += w * 0.5 * (V[q][1] - V[p][1]);
+      }
+      // back to the unit triangle through the barycentric coordinates of (X, Y)
+      double b2 = (Y - V[0][1]) / (V[2][1] - V[0][1]);
+      double b1 = 0.5 * (X - V[0][0] - b2 * (V[2][0] - V[0][0]));
+      out.push_back(b1);
+      out.push_back(b2);
+    }
+}
+
+// Lagrange basis on the warp-and-blend nodes of degree n: phi = psi^T C,
+// C = V^-1 with V[i][m] = psi_m(node i), solved in long double
+struct LagrangeTri {
+  int n = 0;
+  vector<double> C;  // (card, card), phi_k = sum_m psi_m C[m][k]
+  void build(int deg) {
+    n = deg;
+    int m = tri_card(n);
+    vector<double> nodes;
+    warp_blend_nodes(n, nodes);
+    vector<long double> A((size_t)m * m), X((size_t)m * m, 0.0L);
+    vector<double> psi(m);
+    for (int i = 0; i < m; ++i) {
+      koornwinder(n, nodes[2 * i], nodes[2 * i + 1], psi.data(), nullptr, nullptr);
+      for (int k = 0; k < m; ++k) A[(size_t)i * m + k] = psi[k];
+      X[(size_t)i * m + i] = 1.0L;
+    }
+    for (int c = 0; c < m; ++c) {
+      int p = c;
+      for (int r = c + 1; r < m; ++r)
+        if (std::fabs((double)A[(size_t)r * m + c]) > std::fabs((double)A[(size_t)p * m + c])) p = r;
+      for (int j = 0; j < m; ++j) {
+        std::swap(A[(size_t)c * m + j], A[(size_t)p * m + j]);
+        std::swap(X[(size_t)c * m + j], X[(size_t)p * m + j]);
+      }
+      long double d = A[(size_t)c * m + c];
+      for (int j = 0; j < m; ++j) { A[(size_t)c * m + j] /= d; X[(size_t)c * m + j] /= d; }
+      for (int r = 0; r < m; ++r) {
+        if (r == c) continue;
+        long double f = A[(size_t)r * m + c];
+        for (int j = 0; j < m; ++j) { A[(size_t)r * m + j] -= f * A[(size_t)c * m + j]; X[(size_t)r * m + j] -= f * X[(size_t)c * m + j]; }
+      }
+    }
+    // A V^-1 = I: X holds V^-1 (rows m, cols i); phi_k = sum_m psi_m Vinv[m][k]
+    C.assign((size_t)m * m, 0.0);
+    for (int a = 0; a < m; ++a)
+      for (int k = 0; k < m; ++k) C[(size_t)a * m + k] = (double)X[(size_t)a * m + k];
+  }
+  void eval(double x, double y, double* val) const {
+    int m = tri_card(n);
+    vector<double> psi(m);
+    koornwinder(n, x, y, psi.data(), nullptr, nullptr);
+    for (int k = 0; k < m; ++k) {
+      double v = 0.0;
+      for (int a = 0; a < m; ++a) v += psi[a] * C[(size_t)a * m + k];
+      val[k] = v;
+    }
+  }
+};
+
+// collapsed Gauss rule on the unit triangle: n points in a (x-like) and in b (y), weight (1-b)/8
+void tri_rule(int n, vector<double>& pts, vector<double>& wts) {
+  vector<double> x, w;
+  gauss_rule(n, x, w);
+  pts.assign((size_t)n * n * 2, 0.0);
+  wts.assign((size_t)n * n, 0.0);
+  for (int jb = 0; jb < n; ++jb)
+    for (int ia = 0; ia < n; ++ia) {
+      int q = jb * n + ia;
+      double a = x[ia], b = x[jb];
+      pts[2 * q] = 0.25 * (1.0 + a) * (1.0 - b);
+      pts[2 * q + 1] = 0.5 * (1.0 + b);
+      wts[q] = w[ia] * w[jb] * 0.125 * (1.0 - b);
+    }
+}
+
+// triangle test values in the reference frame: HGRAD (card) + grads (card, 2); HDIV (card, 2) + div (card)
+void tri_hgrad(int n, const double* xi, double* val, double* grad) {
+  int m = tri_card(n);
+  vector<double> gx(m), gy(m);
+  koornwinder(n, xi[0], xi[1], val, gx.data(), gy.data());
+  for (int i = 0; i < m; ++i) { grad[2 * i] = gx[i]; grad[2 * i + 1] = gy[i]; }
+}
+void tri_hdiv(int n, const double* xi, double* val, double* div) {
+  int m = tri_card(n - 1);
+  vector<double> p(m), gx(m), gy(m);
+  koornwinder(n - 1, xi[0], xi[1], p.data(), gx.data(), gy.data());
+  int bf = 0;
+  for (int k = 0; k < m; ++k) {
+    val[2 * bf] = p[k]; val[2 * bf + 1] = 0.0; div[bf] = gx[k]; ++bf;
+    val[2 * bf] = 0.0; val[2 * bf + 1] = p[k]; div[bf] = gy[k]; ++bf;
+  }
+  // members of exact degree n-1: i + j = n-1 (index of (i, j) in i-outer order)
+  int k = 0;
+  for (int i = 0; i <= n - 1; ++i)
+    for (int j = 0; j + i <= n - 1; ++j, ++k) {
+      if (i + j != n - 1) continue;
+      double x = xi[0], y = xi[1];
+      val[2 * bf] = x * p[k]; val[2 * bf + 1] = y * p[k];
+      div[bf] = 2.0 * p[k] + x * gx[k] + y * gy[k];
+      ++bf;
+    }
+}
+
 // ------------------------------------------------------------ point geometry
 struct Geo {
   double x[3];
@@ -396,12 +614,24 @@ struct Tabulator {
   const dpg_form_desc* f;
   const Element* e;
   Lagrange1D testC, testO, fieldL, traceL, fluxL;
+  LagrangeTri fieldT;
   Tabulator(const dpg_form_desc* f_, const Element* e_) : f(f_), e(e_) {
     testC = closed_basis(e->kt);
     testO = open_basis(e->kt);
     fieldL = closed_basis(e->field_deg);
+    if (e->topo == DPG_TRI) fieldT.build(e->field_deg);
     traceL = closed_basis(e->trace_deg);
     if (e->flux_deg >= 1) fluxL = closed_basis(e->flux_deg);
+  }
+  // field basis values at reference point xi
+  void field_values(const double* xi, double* out) const {
+    if (e->topo == DPG_TRI) fieldT.eval(xi[0], xi[1], out);
+    else scalar_tensor(e->dim, fieldL, xi, out, nullptr);
+  }
+  // volume cubature with n points per direction
+  void volume_rule(int n, vector<double>& pts, vector<double>& wts) const {
+    if (e->topo == DPG_TRI) tri_rule(n, pts, wts);
+    else tensor_rule(e->dim, n, pts, wts);
   }
   // test values at reference point xi with geometry g; normal may be null (volume)
   void test_values(const double* xi, const Geo& g, const double* normal, TestVals& tv) const {
@@ -412,7 +642,8 @@ struct Tabulator {
     for (int t = 0; t < f->n_test; ++t) {
       int off = e->test_offset[t], card = e->test_card[t];
       if (f->test_space[t] == DPG_HGRAD) {
-        scalar_tensor(dim, testC, xi, val.data(), grad.data());
+        if (e->topo == DPG_TRI) tri_hgrad(e->kt, xi, val.data(), grad.data());
+        else scalar_tensor(dim, testC, xi, val.data(), grad.data());
         for (int i = 0; i < card; ++i) {
           tv.op(DPG_OP_VALUE)[off + i] = val[i];
           for (int d = 0; d < dim; ++d) {
@@ -422,7 +653,8 @@ struct Tabulator {
           }
         }
       } else {
-        hdiv_tensor(dim, testC, testO, xi, val.data(), div.data());
+        if (e->topo == DPG_TRI) tri_hdiv(e->kt, xi, val.data(), div.data());
+        else hdiv_tensor(dim, testC, testO, xi, val.data(), div.data());
         for (int i = 0; i < card; ++i) {
           double phys[3];
           for (int d = 0; d < dim; ++d) {
@@ -478,11 +710,11 @@ void cell_BGl(const dpg_form_desc* f, const Element& e, const Tabulator& tab, co
   const RefCell& rc = e.rc;
   TestVals tv;
   Geo g;
-  vector<double> fv(64), tr(64);
+  vector<double> fv(128), tr(64);
   // ---- volume terms on the stiffness cache
   if (B || l) {
     vector<double> pts, wts;
-    tensor_rule(dim, e.Q1, pts, wts);
+    tab.volume_rule(e.Q1, pts, wts);
     if (B) std::memset(B, 0, sizeof(double) * M * N);
     if (l) std::memset(l, 0, sizeof(double) * N);
     for (size_t q = 0; q < wts.size(); ++q) {
@@ -491,7 +723,7 @@ void cell_BGl(const dpg_form_desc* f, const Element& e, const Tabulator& tab, co
       double wm = wts[q] * g.detJ;  // computeCellMeasure
       tab.test_values(xi, g, nullptr, tv);
       if (B) {
-        scalar_tensor(dim, tab.fieldL, xi, fv.data(), nullptr);
+        tab.field_values(xi, fv.data());
         for (int ti = 0; ti < f->n_bterms; ++ti) {
           const dpg_bterm& t = f->bterms[ti];
           if (bterm_on_side(f, t)) continue;
@@ -534,7 +766,7 @@ void cell_BGl(const dpg_form_desc* f, const Element& e, const Tabulator& tab, co
           }
           geometry(rc, nodes, xi, g);
           // physical tangents J c_{a+1}; measure and outward unit normal
-          double T[2][3];
+          double T[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
           for (int a = 0; a < dim - 1; ++a)
             for (int d = 0; d < dim; ++d) {
               double sacc = 0.0;
@@ -556,7 +788,7 @@ void cell_BGl(const dpg_form_desc* f, const Element& e, const Tabulator& tab, co
           for (int d = 0; d < dim; ++d) nrm[d] /= len;
           double wm = swts[q] * len;
           tab.test_values(xi, g, nrm, tv);
-          scalar_tensor(dim, tab.fieldL, xi, fv.data(), nullptr);
+          tab.field_values(xi, fv.data());
           for (int ti = 0; ti < f->n_bterms; ++ti) {
             const dpg_bterm& t = f->bterms[ti];
             if (!bterm_on_side(f, t)) continue;
@@ -590,7 +822,7 @@ void cell_BGl(const dpg_form_desc* f, const Element& e, const Tabulator& tab, co
   // ---- Gram matrix on the IP cache
   if (G) {
     vector<double> pts, wts;
-    tensor_rule(dim, e.QG1, pts, wts);
+    tab.volume_rule(e.QG1, pts, wts);
     std::memset(G, 0, sizeof(double) * N * N);
     vector<double> a(N);
     for (size_t q = 0; q < wts.size(); ++q) {
@@ -838,6 +1070,23 @@ int oracle_gauss_rule(int n, double* x, double* w) {
   for (int i = 0; i < n; ++i) { x[i] = xx[i]; w[i] = ww[i]; }
   return 0;
 }
+// warp-and-blend nodes of degree n on the unit triangle: (n+1)(n+2)/2 (x, y) pairs
+int oracle_tri_nodes(int n, double* xy) {
+  vector<double> v;
+  warp_blend_nodes(n, v);
+  for (size_t i = 0; i < v.size(); ++i) xy[i] = v[i];
+  return (int)(v.size() / 2);
+}
+
+// collapsed Gauss rule with n x n points on the unit triangle
+int oracle_tri_rule(int n, double* xy, double* w) {
+  vector<double> p, ww;
+  tri_rule(n, p, ww);
+  for (size_t i = 0; i < p.size(); ++i) xy[i] = p[i];
+  for (size_t i = 0; i < ww.size(); ++i) w[i] = ww[i];
+  return (int)ww.size();
+}
+
 int oracle_gll_nodes(int n, double* x) {
   vector<double> xx;
   gll_nodes(n, xx);

@@ -33,6 +33,8 @@ def lib():
         L.oracle_sizes.argtypes = [P(capi.FormDesc), P(capi.BucketDesc), vp]
         L.oracle_gauss_rule.argtypes = [ct.c_int, vp, vp]
         L.oracle_gll_nodes.argtypes = [ct.c_int, vp]
+        L.oracle_tri_nodes.argtypes = [ct.c_int, vp]
+        L.oracle_tri_rule.argtypes = [ct.c_int, vp, vp]
         L.oracle_assemble.argtypes = [P(capi.FormDesc), P(capi.BucketDesc), ct.c_int, vp, vp, vp,
                                       P(capi.LoadDesc), vp, vp, vp, vp, vp, vp, vp, vp, ct.c_int]
         L.oracle_energy_error.argtypes = [P(capi.FormDesc), P(capi.BucketDesc), ct.c_int, vp, vp, vp,
@@ -79,6 +81,20 @@ def gll_nodes(n):
     x = np.zeros(n + 1)
     lib().oracle_gll_nodes(n, _p(x))
     return x
+
+
+def tri_nodes(n):
+    """warp-and-blend nodes of degree n on the unit triangle, ((n+1)(n+2)/2, 2)"""
+    xy = np.zeros(((n + 1) * (n + 2) // 2, 2))
+    lib().oracle_tri_nodes(n, _p(xy))
+    return xy
+
+
+def tri_rule(n):
+    """collapsed Gauss rule, n*n points (x, y) and weights on the unit triangle"""
+    xy, w = np.zeros((n * n, 2)), np.zeros(n * n)
+    lib().oracle_tri_rule(n, _p(xy), _p(w))
+    return xy, w
 
 
 def assemble(form, bucket, nodes, parities, coefs=None, load=None, want=("K", "rhs"), nthreads=0):

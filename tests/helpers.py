@@ -1,4 +1,4 @@
-"""Shared helpers: seeded workloads and error norms."""
+"""Shared helpers: seeded workloads, error norms, exactly representable solutions."""
 import numpy as np
 
 from camellia2_amd import capi, dpg
@@ -62,3 +62,67 @@ def poisson3d(nx, ny, nz, k=4, kt=5, seed=SEED[4], jitter=0.05):
 
 # the hp workload lives in the package (bench.py uses it too)
 from camellia2_amd.workloads import cd_hp, hp_groups  # noqa: E402,F401
+
+
+REF_VERTS = {4: np.array([[-1.0, -1.0], [1.0, -1.0], [1.0, 1.0], [-1.0, 1.0]]),
+             3: np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]])}
+
+
+def _vertex_basis(ref, xi):
+    if len(ref) == 3:
+        return np.array([1.0 - xi[0] - xi[1], xi[0], xi[1]])
+    return np.array([(1 + ref[v, 0] * xi[0]) * (1 + ref[v, 1] * xi[1]) / 4 for v in range(4)])
+
+
+def linear_trial_coefficients(nodes, par, k, a, eps=None, beta=None):
+    """Trial coefficients of u = a0 + a1 x + a2 y on quads (C,4,2) or triangles
+    (C,3,2) by nodal interpolation: the field at its nodes (GLL tensor nodes on
+    quads, warp-and-blend nodes on triangles, degree k-1), traces (Cn(k)) and
+    fluxes (Cn(k-1)) at the GLL nodes of each side.
+    Poisson UW (beta None): sigma = grad u, uhat = u, sigma_n_hat = sigma.n * parity.
+    ConvectionDiffusion (per-cell eps, constant beta): sigma = eps grad u,
+    uhat = u, tc = (beta u - sigma).n * parity."""
+    from oracle import oracle
+    C, V = nodes.shape[0], nodes.shape[1]
+    ref = REF_VERTS[V]
+    gf, gt = oracle.gll_nodes(k - 1), oracle.gll_nodes(k)
+    if V == 3:
+        fnodes = oracle.tri_nodes(k - 1)
+    else:
+        fnodes = np.array([(gf[i], gf[j]) for j in range(len(gf)) for i in range(len(gf))])
+    nfld, nt, nfl = len(fnodes), len(gt), len(gf)
+    M = 3 * nfld + V * nt + V * nfl
+    u = np.zeros((C, M))
+    sides = [(s, (s + 1) % V) for s in range(V)]
+
+    def phys(c, xi):
+        return _vertex_basis(ref, xi) @ nodes[c]
+
+    def lin(x):
+        return a[0] + a[1] * x[0] + a[2] * x[1]
+
+    for c in range(C):
+        e = 1.0 if beta is None else eps[c]
+        u[c, :nfld] = [lin(phys(c, p)) for p in fnodes]
+        u[c, nfld:2 * nfld] = e * a[1]
+        u[c, 2 * nfld:3 * nfld] = e * a[2]
+        off = 3 * nfld
+        for s, (p, q) in enumerate(sides):
+            for i, t in enumerate(gt):
+                u[c, off + s * nt + i] = lin(phys(c, 0.5 * (ref[p] + ref[q]) + 0.5 * (ref[q] - ref[p]) * t))
+        off += V * nt
+        for s, (p, q) in enumerate(sides):
+            T = nodes[c, q] - nodes[c, p]
+            n = np.array([T[1], -T[0]]) / np.hypot(*T)
+            dudn = a[1] * n[0] + a[2] * n[1]
+            for i, t in enumerate(gf):
+                if beta is None:
+                    val = dudn
+                else:
+                    x = phys(c, 0.5 * (ref[p] + ref[q]) + 0.5 * (ref[q] - ref[p]) * t)
+                    val = (beta[0] * n[0] + beta[1] * n[1]) * lin(x) - e * dudn
+                u[c, off + s * nfl + i] = val * par[c, s]
+    return u
+
+
+from camellia2_amd.workloads import split_quads  # noqa: E402,F401

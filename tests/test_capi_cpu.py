@@ -62,6 +62,9 @@ def test_formulations_match_oracle_tables(name, dim):
     ("stokes", 2, capi.QUAD, 4, 5, 228, 184, 25, 5, 36),     # C2
     ("poisson", 3, capi.HEX, 4, 5, 666, 502, 125, 25, 216),  # C4
     ("cdr", 2, capi.QUAD, 3, 4, 65, 55, 16, 4, 25),          # C5
+    ("cd", 2, capi.QUAD, 7, 9, 280, 207, 81, 9, 100),        # C3 largest quad type
+    ("cd", 2, capi.TRI, 7, 9, 154, 129, 100, 9, 121),        # C3 largest triangle type
+    ("cd", 2, capi.TRI, 2, 2, 14, 24, 16, 3, 16),            # C3 smallest triangle type
 ])
 def test_bucket_sizes(name, dim, topo, k, kt, N, M, Q, Qs, QG):
     f = dpg.form(name, dim)

@@ -105,7 +105,7 @@ def test_energy_error_matches_oracle():
 def test_energy_error_vanishes_for_exact_linear_solution():
     """u = a + b x + c y is in the trial space of a bilinear mesh and solves the
     Poisson UW equations with f = 0, so its energy error is zero."""
-    from test_oracle import _nodal_trial_coefficients_linear
+    from helpers import linear_trial_coefficients as _nodal_trial_coefficients_linear
 
     f, b, nodes, par, _, _ = poisson2d(4, 4, k=2, kt=3)
     u = _nodal_trial_coefficients_linear(nodes, par, 2, (0.2, 1.1, -0.4))

@@ -9,6 +9,8 @@ import pytest
 from camellia2_amd import capi
 from oracle import oracle
 
+from helpers import linear_trial_coefficients, split_quads
+
 
 def load(golden_dir, name):
     with open(os.path.join(golden_dir, name)) as f:
@@ -164,53 +166,7 @@ def test_graph_norm_poisson_terms():
     assert got == want
 
 
-def _nodal_trial_coefficients_linear(nodes, par, k, a, eps=None, beta=None):
-    """Trial coefficients of u = a0 + a1 x + a2 y by nodal interpolation at the
-    GLL nodes of the field (Cn(k-1)), trace (Cn(k)) and flux (Cn(k-1)) bases.
-    Poisson UW (beta None): sigma = grad u, uhat = u, sigma_n_hat = sigma.n * parity.
-    ConvectionDiffusion (per-cell eps, constant beta): sigma = eps grad u,
-    uhat = u, tc = (beta u - sigma).n * parity."""
-    C = nodes.shape[0]
-    gf, gt = oracle.gll_nodes(k - 1), oracle.gll_nodes(k)
-    nf, nt = len(gf), len(gt)
-    M = 3 * nf * nf + 4 * nt + 4 * nf
-    u = np.zeros((C, M))
-    ref = REF_QUAD[0]
-    sides = [(0, 1), (1, 2), (2, 3), (3, 0)]
-
-    def phys(c, xi):
-        N = np.array([(1 + ref[v, 0] * xi[0]) * (1 + ref[v, 1] * xi[1]) / 4 for v in range(4)])
-        return N @ nodes[c]
-
-    for c in range(C):
-        fld = np.zeros(nf * nf)
-        for j in range(nf):
-            for i in range(nf):
-                x = phys(c, (gf[i], gf[j]))
-                fld[j * nf + i] = a[0] + a[1] * x[0] + a[2] * x[1]
-        e = 1.0 if beta is None else eps[c]
-        u[c, :nf * nf] = fld
-        u[c, nf * nf:2 * nf * nf] = e * a[1]
-        u[c, 2 * nf * nf:3 * nf * nf] = e * a[2]
-        off = 3 * nf * nf
-        for s, (p, q) in enumerate(sides):
-            for i, t in enumerate(gt):
-                xi = 0.5 * (ref[p] + ref[q]) + 0.5 * (ref[q] - ref[p]) * t
-                x = phys(c, xi)
-                u[c, off + s * nt + i] = a[0] + a[1] * x[0] + a[2] * x[1]
-        off += 4 * nt
-        for s, (p, q) in enumerate(sides):
-            T = nodes[c, q] - nodes[c, p]
-            n = np.array([T[1], -T[0]]) / np.hypot(*T)
-            dudn = a[1] * n[0] + a[2] * n[1]
-            for i, t in enumerate(gf):
-                if beta is None:
-                    val = dudn
-                else:
-                    x = phys(c, 0.5 * (ref[p] + ref[q]) + 0.5 * (ref[q] - ref[p]) * t)
-                    val = (beta[0] * n[0] + beta[1] * n[1]) * (a[0] + a[1] * x[0] + a[2] * x[1]) - e * dudn
-                u[c, off + s * nf + i] = val * par[c, s]
-    return u
+_nodal_trial_coefficients_linear = linear_trial_coefficients
 
 
 @pytest.mark.parametrize("k,kt", [(2, 3), (3, 4)])
@@ -302,3 +258,106 @@ def test_indefinite_gram_reports_info():
     G[1, 1] = -1.0
     K, rhs, info = oracle.factored_cholesky_solve(G.ravel(order="F"), np.eye(3).ravel(), np.ones(3), 3, 3)
     assert info == 2
+
+
+# ------------------------------------------------------------------ triangles
+@pytest.mark.parametrize("n", [2, 3, 5, 8, 12])
+def test_triangle_rule_is_exact(n):
+    """n x n collapsed Gauss points integrate x^a y^b exactly for a+b <= 2n-2:
+    the integral over the unit triangle is a! b! / (a+b+2)!."""
+    from math import factorial
+    xy, w = oracle.tri_rule(n)
+    assert np.all(w > 0) and np.all(xy[:, 0] + xy[:, 1] < 1) and np.all(xy > 0)
+    for a in range(2 * n - 1):
+        for b in range(2 * n - 1 - a):
+            exact = factorial(a) * factorial(b) / factorial(a + b + 2)
+            got = np.sum(w * xy[:, 0] ** a * xy[:, 1] ** b)
+            assert abs(got - exact) <= 1e-14 * exact + 1e-17, (a, b)
+
+
+@pytest.mark.parametrize("n", range(1, 10))
+def test_triangle_warp_blend_nodes(n):
+    """Lattice for n <= 2; edge nodes at the 1D GLL points on all three edges;
+    the node set is invariant under the symmetries of the triangle; the
+    interior node of n = 3 is the centroid."""
+    xy = oracle.tri_nodes(n)
+    assert xy.shape == ((n + 1) * (n + 2) // 2, 2)
+    if n <= 2:
+        lat = np.array([(i / n, j / n) for j in range(n + 1) for i in range(n + 1 - j)])
+        assert np.abs(xy - lat).max() < 1e-15
+    g = 0.5 * (oracle.gll_nodes(n) + 1.0)
+    bary = np.stack([1 - xy[:, 0] - xy[:, 1], xy[:, 0], xy[:, 1]], axis=1)
+    for k in range(3):  # edge opposite vertex k: bary[k] == 0
+        on = np.abs(bary[:, k]) < 1e-14
+        other = bary[on][:, (k + 1) % 3]
+        assert np.abs(np.sort(other) - g).max() < 1e-14
+    key = lambda b: sorted(tuple(np.round(r, 12)) for r in b)
+    for perm in ([1, 0, 2], [0, 2, 1], [2, 1, 0], [1, 2, 0]):
+        assert key(bary[:, perm]) == key(bary)
+    if n == 3:
+        assert np.abs(xy[5] - 1.0 / 3.0).max() < 1e-15
+    # lattice order: row j (n+1-j nodes) from the left edge to the hypotenuse, x increasing
+    i0 = 0
+    for j in range(n + 1):
+        row = xy[i0:i0 + n + 1 - j]
+        assert abs(row[0, 0]) < 1e-14 and abs(row[-1].sum() - 1.0) < 1e-14
+        assert np.all(np.diff(row[:, 0]) > 0)
+        i0 += n + 1 - j
+
+
+def test_triangle_gram_of_l2_product_is_diagonal():
+    """IP = (v, v) on P_n of an affine triangle: the oracle's orthogonal test
+    basis gives a diagonal G with G_ii = 2 |T| / (2 (2i+1)(i+j+1))."""
+    f = capi.FormDesc()
+    f.spatial_dim, f.n_trial, f.n_test = 2, 1, 1
+    f.trial_kind[0] = capi.FIELD
+    f.test_space[0] = capi.HGRAD
+    add_ip(f, [(0, capi.OP_VALUE, coef())])
+    nodes = np.array([[[0.1, 0.2], [0.9, 0.35], [0.3, 1.1]]])
+    e1, e2 = nodes[0, 1] - nodes[0, 0], nodes[0, 2] - nodes[0, 0]
+    area = 0.5 * abs(e1[0] * e2[1] - e1[1] * e2[0])
+    n = 6
+    G = oracle.assemble(f, capi.BucketDesc(capi.TRI, 2, n, 0), nodes, np.ones((1, 3), np.int8), want=("G",))["G"][0]
+    want = [2 * area / (2 * (2 * i + 1) * (i + j + 1)) for i in range(n + 1) for j in range(n + 1 - i)]
+    assert np.abs(G - np.diag(want)).max() < 1e-14 * max(want)
+
+
+def _tri_mesh(nx, ny, jitter, seed):
+    nodes, par = _quad_mesh(nx, ny, jitter, seed)
+    return split_quads(nodes, par)
+
+
+@pytest.mark.parametrize("k,kt", [(2, 3), (3, 5), (4, 4), (7, 9)])
+def test_triangle_poisson_linear_solution_has_zero_residual(k, kt):
+    """The linear-solution invariant on triangles: normals, side measures and
+    parities of the three sides, the RT test space and the nodal field basis."""
+    nodes, par = _tri_mesh(2, 2, 0.1, 3)
+    f = oracle.form("poisson", 2)
+    bk = capi.BucketDesc(capi.TRI, k, kt, 0)
+    u = linear_trial_coefficients(nodes, par, k, (0.3, -1.2, 0.7))
+    o = oracle.assemble(f, bk, nodes, par, want=("B", "l"))
+    r = o["l"] - np.einsum("cmn,cm->cn", o["B"], u)
+    scale = np.abs(o["B"]).max() * np.abs(u).max()
+    assert np.abs(r).max() < 1e-12 * scale
+    wrong = linear_trial_coefficients(nodes, np.ones_like(par), k, (0.3, -1.2, 0.7))
+    assert np.abs(o["l"] - np.einsum("cmn,cm->cn", o["B"], wrong)).max() > 1e-3
+
+
+@pytest.mark.parametrize("k,kt", [(2, 3), (5, 7), (3, 3)])
+def test_triangle_convection_diffusion_linear_solution_has_zero_residual(k, kt):
+    nodes, par = _tri_mesh(2, 2, 0.1, 9)
+    C = nodes.shape[0]
+    eps = 10.0 ** np.linspace(-4, -2, C)
+    beta = (np.cos(2.1), np.sin(2.1))
+    coefs = np.stack([eps, np.full(C, beta[0]), np.full(C, beta[1])], axis=1)
+    a = (0.4, -0.9, 1.3)
+    load = capi.LoadDesc()
+    load.ncomp = 1
+    load.constant[0] = beta[0] * a[1] + beta[1] * a[2]
+    f = oracle.form("cd", 2)
+    u = linear_trial_coefficients(nodes, par, k, a, eps=eps, beta=beta)
+    o = oracle.assemble(f, capi.BucketDesc(capi.TRI, k, kt, 0), nodes, par, coefs=coefs, load=load,
+                        want=("B", "l"))
+    r = o["l"] - np.einsum("cmn,cm->cn", o["B"], u)
+    for c in range(C):
+        assert np.abs(r[c]).max() < 1e-12 * np.abs(o["B"][c]).max() * np.abs(u[c]).max(), c
