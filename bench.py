@@ -242,6 +242,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-schur", action="store_true")
     ap.add_argument("--lanes", type=int, default=4, help="hp config: concurrent streams inside the graph")
+    ap.add_argument("--sumfact", action="store_true",
+                    help="opt-in sum-factorised G (fewer flops, K accurate only to ~1e-12..1e-1; not the default)")
     ap.add_argument("--calibrate-copy-gib", type=float, default=0.0,
                     help="also run a plain device copy of this size (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -266,6 +268,8 @@ def main():
     d_par = torch.from_numpy(par).to(dev)
     ws = args.workspace_gib * 2 ** 30
     A = dpg.LocalAssembler(f, [b], workspace_bytes=ws, precision=PRECISIONS[args.precision])
+    if args.sumfact:
+        A.set_tiles(0, 64, 64, 64, 1)
     out = {
         "K": torch.empty((C, info.M, info.M), dtype=torch.float64, device=dev),
         "rhs": torch.empty((C, info.M), dtype=torch.float64, device=dev),
@@ -294,6 +298,8 @@ def main():
     # one more step on a plan that times every launch with HIP events on the
     # launch stream (kept out of the timed steps above)
     Ap = dpg.LocalAssembler(f, [b], workspace_bytes=ws, profile=True, precision=PRECISIONS[args.precision])
+    if args.sumfact:
+        Ap.set_tiles(0, 64, 64, 64, 1)
     step(Ap)
     torch.cuda.synchronize()
     tm, _ = Ap.timings(reset=True)
@@ -330,7 +336,7 @@ def main():
         "data": "synthetic (seeded jittered mesh and sine-mode load)",
         "config": {"workload": cfg["name"], "cells": C, "N": info.N, "M": info.M, "T": info.T,
                    "outputs": "K,rhs" + (",schur,schur_rhs" if want_schur else ""),
-                   "precision": args.precision},
+                   "precision": args.precision, "gram": "sum-factorised (opt-in)" if args.sumfact else "dense"},
         "gflops": round(info.flops_per_cell * cells_per_s / 1e9, 1),
         "algorithmic_flop_per_cell": info.flops_per_cell,
         "roofline": roofline,

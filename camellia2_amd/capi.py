@@ -98,7 +98,7 @@ class Outputs(ct.Structure):
 
 
 class Tiles(ct.Structure):
-    _fields_ = [("gram", ct.c_int32), ("bmat", ct.c_int32), ("update", ct.c_int32), ("pad", ct.c_int32)]
+    _fields_ = [("gram", ct.c_int32), ("bmat", ct.c_int32), ("update", ct.c_int32), ("sumfact", ct.c_int32)]
 
 
 class WorkItem(ct.Structure):

@@ -38,6 +38,27 @@ struct DevProgram {
   DSummand s[kMaxSummands];
 };
 
+// Sum-factorised Gram matrix (tensor-product cells): the test functions of a
+// block (HGRAD var, or one component of an HDIV var) are products of 1D
+// families per direction: L (closed Lagrange), L' (its derivative), O (open
+// Lagrange). A pair couples prim a of block A with prim b of block B:
+//   G_AB += sum_q M_ab(q) R_a(i, q) R_b(j, q),  M_ab(q) = sum_lt W[q][lt][a] W[q][lt][b].
+constexpr int kSfFamL = 0, kSfFamD = 1, kSfFamO = 2;
+constexpr int kSfMaxCls = 9;
+struct SfPair {
+  int16_t a, b;       // (var * npr + prim) of the two sides
+  int8_t fa[3], fb[3];  // 1D family per direction
+  int8_t cls, pad;    // class of the direction-1 family pair
+};
+struct SfBlockPair {
+  int32_t offA, offB;  // first test dof of each block (offA >= offB)
+  int32_t nA[3], nB[3];  // 1D sizes per direction
+  int32_t diag;        // A == B: lower triangle only
+  int32_t npairs, pair0, ncls;
+  int8_t clsfa[kSfMaxCls], clsfb[kSfMaxCls];  // direction-1 families of each class
+  int16_t pad;
+};
+
 // per-bucket device tables (reference data shared by all cells)
 struct DevTables {
   const double* R_vol;    // (Q, npr, N)

@@ -280,29 +280,33 @@ std::vector<double> warp_blend_tri(int n) {
 LagrangeTri::LagrangeTri(int n, const std::vector<double>& nodes) : P_(n) {
   int m = P_.size();
   if ((int)nodes.size() != 2 * m) throw std::invalid_argument("triangle node count");
-  // V[i][k] = psi_k(node i); coef = V^-1 (Gauss-Jordan with partial pivoting)
-  std::vector<double> V((size_t)m * m), I((size_t)m * m, 0.0);
+  // V[i][k] = psi_k(node i); coef = V^-1, by Gauss-Jordan with partial pivoting in
+  // 80-bit arithmetic: the nodal (trial) values must be accurate to the last bit
+  // because they enter B directly
+  std::vector<long double> V((size_t)m * m), I((size_t)m * m, 0.0L);
+  std::vector<double> row(m);
   for (int i = 0; i < m; ++i) {
-    P_.eval(nodes[2 * i], nodes[2 * i + 1], &V[(size_t)i * m], nullptr, nullptr);
-    I[(size_t)i * m + i] = 1.0;
+    P_.eval(nodes[2 * i], nodes[2 * i + 1], row.data(), nullptr, nullptr);
+    for (int k = 0; k < m; ++k) V[(size_t)i * m + k] = row[k];
+    I[(size_t)i * m + i] = 1.0L;
   }
   for (int c = 0; c < m; ++c) {
     int p = c;
     for (int r = c + 1; r < m; ++r)
-      if (std::fabs(V[(size_t)r * m + c]) > std::fabs(V[(size_t)p * m + c])) p = r;
+      if (fabsl(V[(size_t)r * m + c]) > fabsl(V[(size_t)p * m + c])) p = r;
     for (int j = 0; j < m; ++j) {
       std::swap(V[(size_t)c * m + j], V[(size_t)p * m + j]);
       std::swap(I[(size_t)c * m + j], I[(size_t)p * m + j]);
     }
-    double d = V[(size_t)c * m + c];
+    long double d = V[(size_t)c * m + c];
     for (int j = 0; j < m; ++j) {
       V[(size_t)c * m + j] /= d;
       I[(size_t)c * m + j] /= d;
     }
     for (int r = 0; r < m; ++r) {
       if (r == c) continue;
-      double f = V[(size_t)r * m + c];
-      if (f == 0.0) continue;
+      long double f = V[(size_t)r * m + c];
+      if (f == 0.0L) continue;
       for (int j = 0; j < m; ++j) {
         V[(size_t)r * m + j] -= f * V[(size_t)c * m + j];
         I[(size_t)r * m + j] -= f * I[(size_t)c * m + j];
@@ -310,7 +314,7 @@ LagrangeTri::LagrangeTri(int n, const std::vector<double>& nodes) : P_(n) {
     }
   }
   // phi_k(x) = sum_j psi_j(x) Vinv[j][k]
-  coef_.assign((size_t)m * m, 0.0);
+  coef_.assign((size_t)m * m, 0.0L);
   for (int k = 0; k < m; ++k)
     for (int j = 0; j < m; ++j) coef_[(size_t)k * m + j] = I[(size_t)j * m + k];
 }
@@ -320,9 +324,9 @@ void LagrangeTri::eval(double x, double y, double* val) const {
   std::vector<double> psi(m);
   P_.eval(x, y, psi.data(), nullptr, nullptr);
   for (int k = 0; k < m; ++k) {
-    double v = 0.0;
-    for (int j = 0; j < m; ++j) v += coef_[(size_t)k * m + j] * psi[j];
-    val[k] = v;
+    long double v = 0.0L;  // the expansion cancels: sum in 80-bit, round once
+    for (int j = 0; j < m; ++j) v += coef_[(size_t)k * m + j] * (long double)psi[j];
+    val[k] = (double)v;
   }
 }
 

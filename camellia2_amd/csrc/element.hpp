@@ -70,7 +70,7 @@ class LagrangeTri {
 
  private:
   DubinerTri P_;
-  std::vector<double> coef_;  // (size, size): phi_k = sum_m coef[k][m] psi_m
+  std::vector<long double> coef_;  // (size, size): phi_k = sum_m coef[k][m] psi_m
 };
 
 // Reference cell data ---------------------------------------------------------

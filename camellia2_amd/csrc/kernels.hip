@@ -430,6 +430,129 @@ __global__ __launch_bounds__(64) void gemm_gram_f64_t32(GemmArgs g) { gemm_tn_f6
 __global__ __launch_bounds__(64) void gemm_bmat_f64_t32(GemmArgs g) { gemm_tn_f64_body<32>(g); }
 __global__ __launch_bounds__(64) void gemm_update_f64_t32(GemmArgs g) { gemm_tn_f64_body<32>(g); }
 
+// ------------------------------------------------- sum-factorised Gram
+// One workgroup per (block pair, cell). G_AB[i, j] with i = (i1, i2, i3),
+// j = (j1, j2, j3), x fastest, is
+//   sum_pairs sum_q M_p(q) F_fa1(i1,q1) F_fb1(j1,q1) F_fa2(i2,q2) F_fb2(j2,q2) F_fa3(i3,q3) F_fb3(j3,q3)
+// evaluated direction by direction for each slab (i3, j3) (3D) or (i2, j2) (2D):
+//   T1[p][q2][q1]      = sum_q3 F_fa3(i3,q3) F_fb3(j3,q3) M_p(q1,q2,q3)
+//   T2[cls][i2,j2][q1] = sum_{p in cls} sum_q2 F_fa2(i2,q2) F_fb2(j2,q2) T1[p][q2][q1]
+//   G[(i1,i2),(j1,j2)] = sum_cls sum_q1 F_cfa(i1,q1) F_cfb(j1,q1) T2[cls][i2,j2][q1]
+// where cls groups the pairs by their direction-1 families. M_p is formed
+// from the point weights W in LDS first. Writes are unscaled (the DPOEQU
+// factors are taken from G's diagonal afterwards).
+template <int DIM>
+__global__ __launch_bounds__(256) void gram_sf_kernel(const SfBlockPair* __restrict__ bps,
+                                                      const SfPair* __restrict__ pairs,
+                                                      const double* __restrict__ F, int nF, int q,
+                                                      const double* __restrict__ Wip, int n_lt, int na, int QG,
+                                                      double* __restrict__ A, int64_t lda, int64_t sA) {
+  extern __shared__ double sm[];
+  const SfBlockPair bp = bps[blockIdx.x];
+  const int c = blockIdx.y, t = threadIdx.x, NT = blockDim.x;
+  const int np = bp.npairs, ncls = bp.ncls;
+  const SfPair* pr = pairs + bp.pair0;
+  double* Ms = sm;                 // np * QG
+  double* Fs = Ms + np * QG;       // 3 * nF * q
+  double* T1 = Fs + 3 * nF * q;    // 3D: np*q*q   2D: ncls*q
+  double* T2 = T1 + (DIM == 3 ? np * q * q : ncls * q);  // 3D: ncls * nA2*nB2 * q
+  const double* W = Wip + (size_t)c * QG * n_lt * na;
+  for (int e = t; e < np * QG; e += NT) {
+    int p = e / QG, qq = e - p * QG;
+    const double* w = W + (size_t)qq * n_lt * na;
+    int a = pr[p].a, b = pr[p].b;
+    double acc = 0.0;
+    for (int lt = 0; lt < n_lt; ++lt) acc = fma(w[lt * na + a], w[lt * na + b], acc);
+    Ms[e] = acc;
+  }
+  for (int e = t; e < 3 * nF * q; e += NT) Fs[e] = F[e];
+  __syncthreads();
+  double* Ac = A + (size_t)c * sA;
+  auto fam = [&](int f, int i) { return Fs + (f * nF + i) * q; };
+  if constexpr (DIM == 3) {
+    const int q2n = q * q;
+    const int n2 = bp.nA[1] * bp.nB[1];
+    const int nIA = bp.nA[0] * bp.nA[1], nJB = bp.nB[0] * bp.nB[1];
+    for (int i3 = 0; i3 < bp.nA[2]; ++i3)
+      for (int j3 = 0; j3 < bp.nB[2]; ++j3) {
+        if (bp.diag && j3 > i3) continue;
+        for (int e = t; e < np * q2n; e += NT) {
+          int p = e / q2n, r = e - p * q2n;
+          const double* fa = fam(pr[p].fa[2], i3);
+          const double* fb = fam(pr[p].fb[2], j3);
+          const double* m = Ms + p * QG + r;
+          double acc = 0.0;
+          for (int q3 = 0; q3 < q; ++q3) acc = fma(fa[q3] * fb[q3], m[q3 * q2n], acc);
+          T1[e] = acc;
+        }
+        __syncthreads();
+        for (int e = t; e < ncls * n2 * q; e += NT) {
+          int cl = e / (n2 * q), r = e - cl * n2 * q;
+          int ij = r / q, q1 = r - ij * q;
+          int i2 = ij % bp.nA[1], j2 = ij / bp.nA[1];
+          double acc = 0.0;
+          for (int p = 0; p < np; ++p) {
+            if (pr[p].cls != cl) continue;
+            const double* fa = fam(pr[p].fa[1], i2);
+            const double* fb = fam(pr[p].fb[1], j2);
+            const double* tp = T1 + p * q2n + q1;
+            for (int q2 = 0; q2 < q; ++q2) acc = fma(fa[q2] * fb[q2], tp[q2 * q], acc);
+          }
+          T2[e] = acc;
+        }
+        __syncthreads();
+        for (int e = t; e < nIA * nJB; e += NT) {
+          int jl = e / nIA, il = e - jl * nIA;
+          int row = bp.offA + il + nIA * i3, col = bp.offB + jl + nJB * j3;
+          if (bp.diag && row < col) continue;
+          int i1 = il % bp.nA[0], i2 = il / bp.nA[0];
+          int j1 = jl % bp.nB[0], j2 = jl / bp.nB[0];
+          double acc = 0.0;
+          for (int cl = 0; cl < ncls; ++cl) {
+            const double* fa = fam(bp.clsfa[cl], i1);
+            const double* fb = fam(bp.clsfb[cl], j1);
+            const double* tp = T2 + (cl * n2 + i2 + bp.nA[1] * j2) * q;
+            for (int q1 = 0; q1 < q; ++q1) acc = fma(fa[q1] * fb[q1], tp[q1], acc);
+          }
+          Ac[(size_t)row + (size_t)col * lda] = acc;
+        }
+        __syncthreads();
+      }
+  } else {
+    for (int i2 = 0; i2 < bp.nA[1]; ++i2)
+      for (int j2 = 0; j2 < bp.nB[1]; ++j2) {
+        if (bp.diag && j2 > i2) continue;
+        for (int e = t; e < ncls * q; e += NT) {
+          int cl = e / q, q1 = e - cl * q;
+          double acc = 0.0;
+          for (int p = 0; p < np; ++p) {
+            if (pr[p].cls != cl) continue;
+            const double* fa = fam(pr[p].fa[1], i2);
+            const double* fb = fam(pr[p].fb[1], j2);
+            const double* m = Ms + p * QG + q1;
+            for (int q2 = 0; q2 < q; ++q2) acc = fma(fa[q2] * fb[q2], m[q2 * q], acc);
+          }
+          T1[e] = acc;
+        }
+        __syncthreads();
+        for (int e = t; e < bp.nA[0] * bp.nB[0]; e += NT) {
+          int j1 = e / bp.nA[0], i1 = e - j1 * bp.nA[0];
+          int row = bp.offA + i1 + bp.nA[0] * i2, col = bp.offB + j1 + bp.nB[0] * j2;
+          if (bp.diag && row < col) continue;
+          double acc = 0.0;
+          for (int cl = 0; cl < ncls; ++cl) {
+            const double* fa = fam(bp.clsfa[cl], i1);
+            const double* fb = fam(bp.clsfb[cl], j1);
+            const double* tp = T1 + cl * q;
+            for (int q1 = 0; q1 < q; ++q1) acc = fma(fa[q1] * fb[q1], tp[q1], acc);
+          }
+          Ac[(size_t)row + (size_t)col * lda] = acc;
+        }
+        __syncthreads();
+      }
+  }
+}
+
 // --------------------------------------------------- mixed-precision GEMM
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -1122,6 +1245,19 @@ void launch_panel(double* A, int64_t lda, int64_t sA, int D, int k0, int kb, dou
 __global__ void fill_i32(int32_t* __restrict__ p, int n, int32_t v) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
+}
+
+void launch_gram_sf(int dim, const SfBlockPair* bps, int nbp, const SfPair* pairs, const double* F, int nF, int q,
+                    const double* Wip, int n_lt, int na, int QG, double* A, int64_t lda, int64_t sA, int ncells,
+                    size_t lds_bytes, hipStream_t st) {
+  if (ncells <= 0 || nbp <= 0) return;
+  if (lds_bytes > 64 * 1024) throw DeviceError("gram_sf: LDS request above 64 KiB");
+  dim3 grid(nbp, ncells);
+  if (dim == 3)
+    gram_sf_kernel<3><<<grid, 256, lds_bytes, st>>>(bps, pairs, F, nF, q, Wip, n_lt, na, QG, A, lda, sA);
+  else
+    gram_sf_kernel<2><<<grid, 256, lds_bytes, st>>>(bps, pairs, F, nF, q, Wip, n_lt, na, QG, A, lda, sA);
+  check_launch("gram_sf_kernel");
 }
 
 // a kernel rather than hipMemsetAsync, so that captured graphs replay it reliably

@@ -52,6 +52,10 @@ void launch_ip_operand(int dim, const double* Wip, int n_lt, int nvar, const dou
 enum { kGemmGram = 0, kGemmBmat = 1, kGemmUpdate = 2 };
 void launch_gemm(GemmArgs g, int role, hipStream_t st, int tile = 64);
 void launch_fill_i32(int32_t* p, int n, int32_t v, hipStream_t st);
+// sum-factorised Gram matrix, unscaled, lower triangle of A's first N rows/columns
+void launch_gram_sf(int dim, const SfBlockPair* bps, int nbp, const SfPair* pairs, const double* F, int nF, int q,
+                    const double* Wip, int n_lt, int na, int QG, double* A, int64_t lda, int64_t sA, int ncells,
+                    size_t lds_bytes, hipStream_t st);
 // the same contraction with FP64 operands converted on load to FP32 (precision
 // DPG_PREC_FP32) or to hi+lo BF16/FP16 pairs (DPG_PREC_BF16X3/FP16X3, products
 // hi*hi + hi*lo + lo*hi), FP32 MFMA accumulation, FP64 epilogue

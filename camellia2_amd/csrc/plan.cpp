@@ -200,6 +200,14 @@ struct BucketPlan {
   double flops = 0.0;
   double gram_flops = 0.0;  // sum over IP terms of QG * n(n+1), n = dofs the term touches
   int tile[3] = {64, 64, 64};  // FP64 GEMM tile edge per role: Gram, B blocks, Cholesky updates
+  // sum-factorised Gram (tensor-product cells, FP64): block pairs, pairs, 1D tables
+  bool sf_ok = false;   // available for this bucket
+  bool sf_on = false;   // used (opt-in)
+  int sf_nbp = 0, sf_nF = 0, sf_q = 0;
+  size_t sf_lds = 0;
+  const SfBlockPair* sf_bps = nullptr;
+  const SfPair* sf_pairs = nullptr;
+  const double* sf_F = nullptr;
 };
 
 }  // namespace dpg
@@ -292,6 +300,145 @@ void compile_program(const dpg_form_desc& f, BucketPlan& B) {
   P.rhs_beg[1] = ns;
 }
 
+// Tensor-product structure of the test functions for the sum-factorised Gram
+// matrix: blocks (HGRAD var; HDIV var component), their prims' 1D families,
+// and the (block, block) pairs that some IP term couples.
+void build_sum_factorisation(dpg_plan* plan, BucketPlan& B) {
+  const dpg_form_desc& f = plan->form;
+  const ElementLayout& L = B.L;
+  if (L.topo != DPG_QUAD && L.topo != DPG_HEX) return;
+  const int dim = L.dim, npr = dim + 1, kt = L.kt;
+  struct Blk {
+    int var, off, n[3], nprim, prim[4], fam[4][3];
+  };
+  std::vector<Blk> blocks;
+  for (int v = 0; v < f.n_test; ++v) {
+    if (f.test_space[v] == DPG_HGRAD) {
+      Blk b{};
+      b.var = v;
+      b.off = L.test_off[v];
+      for (int d = 0; d < 3; ++d) b.n[d] = d < dim ? kt + 1 : 1;
+      b.nprim = npr;
+      for (int r = 0; r < npr; ++r) {
+        b.prim[r] = r;
+        for (int d = 0; d < 3; ++d) b.fam[r][d] = (r >= 1 && d == r - 1) ? kSfFamD : kSfFamL;
+      }
+      blocks.push_back(b);
+    } else {
+      int off = L.test_off[v];
+      for (int cpt = 0; cpt < dim; ++cpt) {
+        Blk b{};
+        b.var = v;
+        b.off = off;
+        int card = 1;
+        for (int d = 0; d < 3; ++d) {
+          b.n[d] = d < dim ? (d == cpt ? kt + 1 : kt) : 1;
+          card *= b.n[d];
+        }
+        b.nprim = 2;
+        b.prim[0] = cpt;  // reference component cpt
+        b.prim[1] = dim;  // reference divergence
+        for (int d = 0; d < 3; ++d) {
+          b.fam[0][d] = d == cpt ? kSfFamL : kSfFamO;
+          b.fam[1][d] = d == cpt ? kSfFamD : kSfFamO;
+        }
+        blocks.push_back(b);
+        off += card;
+      }
+    }
+  }
+  // prims each IP term reaches (op_to_prims: grad -> all derivative prims, HDIV value -> all components)
+  const int na = f.n_test * npr;
+  std::vector<std::vector<char>> touch(f.n_ipterms, std::vector<char>(na, 0));
+  for (int i = 0; i < f.n_ipterms; ++i)
+    for (int k = 0; k < f.ipterms[i].n; ++k) {
+      int v = f.ipterms[i].s[k].test_var, op = f.ipterms[i].s[k].test_op;
+      if (op == DPG_OP_VALUE) touch[i][v * npr] = 1;
+      else if (op >= DPG_OP_DX && op < DPG_OP_DX + dim)
+        for (int e = 1; e <= dim; ++e) touch[i][v * npr + e] = 1;
+      else if (op >= DPG_OP_X && op < DPG_OP_X + dim)
+        for (int e = 0; e < dim; ++e) touch[i][v * npr + e] = 1;
+      else if (op == DPG_OP_DIV) touch[i][v * npr + dim] = 1;
+    }
+  auto coupled = [&](int a, int b) {
+    for (int i = 0; i < f.n_ipterms; ++i)
+      if (touch[i][a] && touch[i][b]) return true;
+    return false;
+  };
+  std::vector<SfBlockPair> bps;
+  std::vector<SfPair> pairs;
+  for (size_t ia = 0; ia < blocks.size(); ++ia)
+    for (size_t ib = 0; ib <= ia; ++ib) {
+      const Blk &A = blocks[ia], &Bb = blocks[ib];
+      SfBlockPair bp{};
+      bp.offA = A.off;
+      bp.offB = Bb.off;
+      for (int d = 0; d < 3; ++d) {
+        bp.nA[d] = A.n[d];
+        bp.nB[d] = Bb.n[d];
+      }
+      bp.diag = ia == ib;
+      bp.pair0 = (int)pairs.size();
+      for (int ra = 0; ra < A.nprim; ++ra)
+        for (int rb = 0; rb < Bb.nprim; ++rb) {
+          int a = A.var * npr + A.prim[ra], b = Bb.var * npr + Bb.prim[rb];
+          if (!coupled(a, b)) continue;
+          SfPair p{};
+          p.a = (int16_t)a;
+          p.b = (int16_t)b;
+          for (int d = 0; d < 3; ++d) {
+            p.fa[d] = (int8_t)A.fam[ra][d];
+            p.fb[d] = (int8_t)Bb.fam[rb][d];
+          }
+          int cl = -1;
+          for (int k = 0; k < bp.ncls; ++k)
+            if (bp.clsfa[k] == p.fa[0] && bp.clsfb[k] == p.fb[0]) cl = k;
+          if (cl < 0) {
+            if (bp.ncls >= kSfMaxCls) throw std::logic_error("sum factorisation: too many classes");
+            cl = bp.ncls++;
+            bp.clsfa[cl] = p.fa[0];
+            bp.clsfb[cl] = p.fb[0];
+          }
+          p.cls = (int8_t)cl;
+          pairs.push_back(p);
+        }
+      bp.npairs = (int)pairs.size() - bp.pair0;
+      if (bp.npairs > 0) bps.push_back(bp);
+    }
+  // 1D tables on the Gram points: L, L', O (O padded with a zero row)
+  const int q = L.QG1, nF = kt + 1;
+  Rule1D r = gauss_legendre(q);
+  LagrangeLine Cl(gll_points(kt)), Op(gauss_legendre(kt).x);
+  std::vector<double> F((size_t)3 * nF * q, 0.0);
+  std::vector<double> v(nF), dv(nF), o(nF);
+  for (int k = 0; k < q; ++k) {
+    Cl.eval(r.x[k], v.data(), dv.data());
+    Op.eval(r.x[k], o.data(), nullptr);
+    for (int i = 0; i < nF; ++i) {
+      F[((size_t)kSfFamL * nF + i) * q + k] = v[i];
+      F[((size_t)kSfFamD * nF + i) * q + k] = dv[i];
+      F[((size_t)kSfFamO * nF + i) * q + k] = i < kt ? o[i] : 0.0;
+    }
+  }
+  size_t lds = 0;
+  for (const SfBlockPair& bp : bps) {
+    size_t t1 = dim == 3 ? (size_t)bp.npairs * q * q : (size_t)bp.ncls * q;
+    size_t t2 = dim == 3 ? (size_t)bp.ncls * bp.nA[1] * bp.nB[1] * q : 0;
+    lds = std::max(lds, sizeof(double) * ((size_t)bp.npairs * L.QG + 3 * (size_t)nF * q + t1 + t2));
+  }
+  if (bps.empty() || lds > 64 * 1024) return;  // the dense contraction stays in use
+  auto& own = plan->tables;
+  B.sf_bps = upload(own, bps);
+  B.sf_pairs = upload(own, pairs);
+  B.sf_F = upload(own, F);
+  B.sf_nbp = (int)bps.size();
+  B.sf_nF = nF;
+  B.sf_q = q;
+  B.sf_lds = lds;
+  B.sf_ok = true;
+  B.sf_on = false;  // opt-in only (dpg_plan_set_tiles): see the accuracy note on dpg_tiles.sumfact
+}
+
 void build_bucket(dpg_plan* plan, const dpg_bucket_desc& desc, BucketPlan& B) {
   const dpg_form_desc& f = plan->form;
   B.desc = desc;
@@ -309,6 +456,7 @@ void build_bucket(dpg_plan* plan, const dpg_bucket_desc& desc, BucketPlan& B) {
     B.gram_flops += (double)L.QG * n * (n + 1.0);
   }
   compile_program(f, B);
+  build_sum_factorisation(plan, B);
   RefTables T = make_tables(f, L);
   auto& own = plan->tables;
   B.tab.R_vol = upload(own, T.vol.R);
@@ -487,21 +635,40 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
     launch_point(dim, 2, B.prog_d, B.tab, S * L.Qs, L.Qs, nb, nodes, par, coefs, plan->form.n_coefs, S, bt.load,
                  w.Wside, nullptr, st);
   P.stop(e, st, DPG_STAGE_POINT, 0.0, nb * D8 * (B.nWip + B.nWvol + B.nWrhs + B.nWside + V * dim));
+  const int prec = plan->opt.precision;
+  const bool sf = B.sf_on && prec == DPG_PREC_FP64;
   // operands: weighted IP-term values (with the DPOEQU factors), B operands
   e = P.start(st);
   const int KG = B.prog.n_lt * L.QG;
-  launch_ip_operand(dim, w.Wip, B.prog.n_lt, nvar, B.tab.R_ip, B.tab.var_of_dof, N, L.QG, nb, w.At, B.nAt, B.Npad,
-                    w.s, B.nS, st);
+  if (!sf)
+    launch_ip_operand(dim, w.Wip, B.prog.n_lt, nvar, B.tab.R_ip, B.tab.var_of_dof, N, L.QG, nb, w.At, B.nAt, B.Npad,
+                      w.s, B.nS, st);
   launch_operand(dim, w.Wvol, B.prog.n_vb, nvar, B.tab.R_vol, B.tab.var_of_dof, N, L.Q, nb, w.Tv, B.nTv,
                  (int64_t)L.Q * B.Npad, B.Npad, st);
   if (B.prog.n_sb > 0)
     launch_operand(dim, w.Wside, B.prog.n_sb, nvar, B.tab.R_side, B.tab.var_of_dof, N, S * L.Qs, nb, w.Ts, B.nTs,
                    (int64_t)S * L.Qs * B.Npad, B.Npad, st);
-  double opnd = (double)KG + (double)B.prog.n_vb * L.Q + (double)B.prog.n_sb * S * L.Qs;
+  double opnd = (sf ? 0.0 : (double)KG) + (double)B.prog.n_vb * L.Q + (double)B.prog.n_sb * S * L.Qs;
   P.stop(e, st, DPG_STAGE_OPERAND, nb * opnd * N * 2.0 * B.npr, nb * opnd * N * D8);
+  if (sf) {
+    // sum-factorised G (unscaled), optional copy-out, then the DPOEQU factors
+    // from its diagonal and S G S in place; B and l are scaled in their epilogues
+    e = P.start(st);
+    launch_gram_sf(dim, B.sf_bps, B.sf_nbp, B.sf_pairs, B.sf_F, B.sf_nF, B.sf_q, w.Wip, B.prog.n_lt,
+                   nvar * B.npr, L.QG, w.A, lda, sA, nb, B.sf_lds, st);
+    P.stop(e, st, DPG_STAGE_GRAM, nb * B.gram_flops, nb * ((double)B.nWip + N * (N + 1.0) / 2.0) * D8);
+    if (Gout) {
+      e = P.start(st);
+      launch_extract_sym(w.A, lda, sA, 0, N, 1.0, Gout + (size_t)c0 * N * N, nb, st);
+      P.stop(e, st, DPG_STAGE_EXTRACT, 0.0, nb * (double)N * N * 2.0 * D8);
+    }
+    e = P.start(st);
+    launch_scaling(w.A, lda, sA, 0, N, N, N, w.s, B.nS, nb, 1.0, st);
+    P.stop(e, st, DPG_STAGE_SCALE, 0.0, nb * (double)N * (N + 1) / 2.0 * 16.0);
+  }
   // DPOEQU scaling is applied in the GEMM epilogues, except when G itself is an
-  // output: then G is written unscaled, copied out, and scaled in place
-  const double* s_ep = Gout ? nullptr : w.s;
+  // output of the dense path: then G is written unscaled, copied out, and scaled in place
+  const double* s_ep = (Gout && !sf) ? nullptr : w.s;
   // G = At^T At (lower), scaled by s_i s_j in the epilogue
   GemmArgs g{};
   g.X = w.At;
@@ -522,17 +689,18 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
   g.srow = s_ep;
   g.scol = s_ep;
   g.ss = B.nS;
-  const int prec = plan->opt.precision;
   if (prec != DPG_PREC_FP64) {
     // low-precision operands are pre-scaled (keeps FP16 in range): X(k,i) s_i, Y(k,j) s_j
     g.srow = g.scol = nullptr;
     g.px = g.py = s_ep;
   }
-  e = P.start(st);
-  if (prec == DPG_PREC_FP64) launch_gemm(g, kGemmGram, st, B.tile[0]);
-  else launch_gemm_mixed(g, prec, kGemmGram, st);
-  P.stop(e, st, DPG_STAGE_GRAM, nb * B.gram_flops, nb * ((double)KG * N + N * (N + 1.0) / 2.0) * D8);
-  if (Gout) {
+  if (!sf) {
+    e = P.start(st);
+    if (prec == DPG_PREC_FP64) launch_gemm(g, kGemmGram, st, B.tile[0]);
+    else launch_gemm_mixed(g, prec, kGemmGram, st);
+    P.stop(e, st, DPG_STAGE_GRAM, nb * B.gram_flops, nb * ((double)KG * N + N * (N + 1.0) / 2.0) * D8);
+  }
+  if (Gout && !sf) {
     e = P.start(st);
     launch_extract_sym(w.A, lda, sA, 0, N, 1.0, Gout + (size_t)c0 * N * N, nb, st);
     P.stop(e, st, DPG_STAGE_EXTRACT, 0.0, nb * (double)N * N * 2.0 * D8);
@@ -606,7 +774,7 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
   launch_residual_row(w.A, lda, sA, N, M, N, N + M, N + M + 1, u ? u + (size_t)c0 * M : nullptr, nb, st);
   bfl += 2.0 * L.Q * N * B.npr + (u ? 2.0 * M * N : 0.0);
   P.stop(e, st, DPG_STAGE_BMAT, nb * bfl, nb * bby);
-  if (Gout) {
+  if (Gout && !sf) {
     e = P.start(st);
     launch_scaling(w.A, lda, sA, 0, B.D, N, N, w.s, B.nS, nb, 1.0, st);
     P.stop(e, st, DPG_STAGE_SCALE, 0.0, nb * ((double)N * (N + 1) / 2.0 + (double)(B.D - N) * N) * 16.0);
@@ -840,7 +1008,7 @@ int dpg_plan_get_tiles(const dpg_plan* plan, int bucket, dpg_tiles* out) {
     if (!plan || !out) throw std::invalid_argument("NULL argument");
     if (bucket < 0 || bucket >= (int)plan->buckets.size()) throw std::invalid_argument("bucket index out of range");
     const int* t = plan->buckets[bucket].tile;
-    *out = dpg_tiles{t[0], t[1], t[2], 0};
+    *out = dpg_tiles{t[0], t[1], t[2], plan->buckets[bucket].sf_on ? 1 : 0};
   });
 }
 
@@ -851,7 +1019,11 @@ int dpg_plan_set_tiles(dpg_plan* plan, int bucket, const dpg_tiles* tiles) {
     int v[3] = {tiles->gram, tiles->bmat, tiles->update};
     for (int r = 0; r < 3; ++r)
       if (v[r] != 64 && v[r] != 32) throw std::invalid_argument("tile edges must be 64 or 32");
+    if (tiles->sumfact != 0 && tiles->sumfact != 1) throw std::invalid_argument("sumfact must be 0 or 1");
+    if (tiles->sumfact && !plan->buckets[bucket].sf_ok)
+      throw std::invalid_argument("the sum-factorised G is not available for this bucket");
     for (int r = 0; r < 3; ++r) plan->buckets[bucket].tile[r] = v[r];
+    plan->buckets[bucket].sf_on = tiles->sumfact != 0;
   });
 }
 
@@ -872,6 +1044,7 @@ int dpg_plan_autotune(dpg_plan* plan, int bucket, const dpg_batch* batch, int sa
     plan->prof.on = true;
     const int stage_of[3] = {DPG_STAGE_GRAM, DPG_STAGE_BMAT, DPG_STAGE_UPDATE};
     const int cand[2] = {64, 32};
+    // the G path (dense or the opt-in sum-factorised one) is kept as set
     double ms[2][3] = {{0}};
     try {
       for (int c = 0; c < 2; ++c) {

@@ -141,13 +141,16 @@ class LocalAssembler:
         return out, int(t.cells)
 
     def tiles(self, bucket):
-        """FP64 GEMM tile edges (gram, bmat, update) used for a bucket."""
+        """FP64 contraction choices of a bucket: (gram, bmat, update) tile edges and sumfact (0/1)."""
         t = capi.Tiles()
         check(lib().dpg_plan_get_tiles(self._plan, bucket, ct.byref(t)))
-        return (t.gram, t.bmat, t.update)
+        return (t.gram, t.bmat, t.update, t.sumfact)
 
-    def set_tiles(self, bucket, gram, bmat, update):
-        check(lib().dpg_plan_set_tiles(self._plan, bucket, ct.byref(capi.Tiles(gram, bmat, update, 0))))
+    def set_tiles(self, bucket, gram, bmat, update, sumfact=None):
+        """sumfact None keeps the current G path; 1 asks for the sum-factorised G (quads, hexes)."""
+        if sumfact is None:
+            sumfact = self.tiles(bucket)[3]
+        check(lib().dpg_plan_set_tiles(self._plan, bucket, ct.byref(capi.Tiles(gram, bmat, update, sumfact))))
 
     def autotune(self, bucket, nodes, parities, coefs=None, load=None, sample=256, stream=None):
         """Times both tile edges per GEMM role on `sample` cells and keeps the faster; returns the choice."""

@@ -297,19 +297,25 @@ int dpg_adjoint(dpg_plan* plan, int bucket, const dpg_batch* batch, const double
  * No counterpart in the reference, which assembles element types one after the
  * other on the CPU (Solution.cpp:1162-1298); these serve the same loop. */
 
-/* FP64 GEMM tile edge (64 or 32) per contraction role of one bucket */
+/* FP64 contraction choices of one bucket: GEMM tile edge (64 or 32) per role,
+ * and whether G comes from the sum-factorised kernel (quads and hexes only).
+ * sumfact = 1 is opt-in: it contracts G = sum_q M(q) R R direction by direction
+ * (about 60x fewer flops than A^T A on the 32^3 hex workload) but, no longer a
+ * Gram product of computed vectors, its rounding is amplified by G's
+ * conditioning: K moves by 2e-12 (32^3-hex orders) up to 1e-1 (Stokes at h =
+ * 1/128) from the dense result. The autotuner never switches it on. */
 typedef struct {
-  int32_t gram;   /* G = A^T A                        */
-  int32_t bmat;   /* B blocks                         */
-  int32_t update; /* Cholesky panel/trailing updates  */
-  int32_t pad;
+  int32_t gram;    /* G = A^T A (dense path)                        */
+  int32_t bmat;    /* B blocks                                      */
+  int32_t update;  /* Cholesky panel/trailing updates               */
+  int32_t sumfact; /* 1: sum-factorised G (opt-in, see above)       */
 } dpg_tiles;
 int dpg_plan_get_tiles(const dpg_plan* plan, int bucket, dpg_tiles* out);
 int dpg_plan_set_tiles(dpg_plan* plan, int bucket, const dpg_tiles* tiles);
 /* Autotuner: times both tile edges for every role on the first `sample` cells
  * of `batch` (HIP events, one warm-up pass each) and keeps the faster per role.
- * The choice never changes results: both tiles accumulate every output in the
- * same k order. Blocks until done; `stream` is used for the timing runs. */
+ * Tile edges never change results (every output accumulates in the same k
+ * order). The G path is left as set. Blocks until done; `stream` runs the timings. */
 int dpg_plan_autotune(dpg_plan* plan, int bucket, const dpg_batch* batch, int sample, void* stream);
 
 /* one element type's share of a mesh: its cells, coefficients and outputs */

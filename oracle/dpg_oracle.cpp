@@ -468,7 +468,7 @@ void warp_blend_nodes(int n, vector<double>& out) {
 // C = V^-1 with V[i][m] = psi_m(node i), solved in long double
 struct LagrangeTri {
   int n = 0;
-  vector<double> C;  // (card, card), phi_k = sum_m psi_m C[m][k]
+  vector<long double> C;  // (card, card), phi_k = sum_m psi_m C[m][k]
   void build(int deg) {
     n = deg;
     int m = tri_card(n);
@@ -498,18 +498,16 @@ struct LagrangeTri {
       }
     }
     // A V^-1 = I: X holds V^-1 (rows m, cols i); phi_k = sum_m psi_m Vinv[m][k]
-    C.assign((size_t)m * m, 0.0);
-    for (int a = 0; a < m; ++a)
-      for (int k = 0; k < m; ++k) C[(size_t)a * m + k] = (double)X[(size_t)a * m + k];
+    C.assign(X.begin(), X.end());
   }
   void eval(double x, double y, double* val) const {
     int m = tri_card(n);
     vector<double> psi(m);
     koornwinder(n, x, y, psi.data(), nullptr, nullptr);
     for (int k = 0; k < m; ++k) {
-      double v = 0.0;
-      for (int a = 0; a < m; ++a) v += psi[a] * C[(size_t)a * m + k];
-      val[k] = v;
+      long double v = 0.0L;  // 80-bit sum: the expansion cancels
+      for (int a = 0; a < m; ++a) v += (long double)psi[a] * C[(size_t)a * m + k];
+      val[k] = (double)v;
     }
   }
 };
@@ -530,32 +528,72 @@ void tri_rule(int n, vector<double>& pts, vector<double>& wts) {
     }
 }
 
+// Triangle test bases. Variant 0 (default): the Koornwinder order and scaling
+// above, RT interleaved, top-degree complement x p measured from vertex 0.
+// Variant 1: the same polynomials orthonormalised (norm^2 = 1/(2(2i+1)(i+j+1))),
+// ordered by total degree then j, RT as all (p,0), all (p,0)... then (x - c) p_top
+// with c the centroid. Both span P_n and RT_n, so K does not depend on the variant;
+// the two differ only in how G and B round.
+static int g_tri_basis = 0;
+
+// Koornwinder values/gradients in the order and scaling of the active variant
+void tri_poly(int n, double x, double y, double* val, double* gx, double* gy) {
+  int m = tri_card(n);
+  if (g_tri_basis == 0) { koornwinder(n, x, y, val, gx, gy); return; }
+  vector<double> v(m), a(m), b(m);
+  koornwinder(n, x, y, v.data(), a.data(), b.data());
+  // i-outer index of (i, j): sum_{i' < i} (n - i' + 1) + j
+  auto io = [&](int i, int j) { return i * (n + 1) - i * (i - 1) / 2 + j; };
+  int k = 0;
+  for (int d = 0; d <= n; ++d)
+    for (int j = 0; j <= d; ++j, ++k) {
+      int i = d - j, src = io(i, j);
+      double sc = std::sqrt(2.0 * (2.0 * i + 1.0) * (i + j + 1.0));
+      val[k] = sc * v[src];
+      if (gx) { gx[k] = sc * a[src]; gy[k] = sc * b[src]; }
+    }
+}
+
 // triangle test values in the reference frame: HGRAD (card) + grads (card, 2); HDIV (card, 2) + div (card)
 void tri_hgrad(int n, const double* xi, double* val, double* grad) {
   int m = tri_card(n);
   vector<double> gx(m), gy(m);
-  koornwinder(n, xi[0], xi[1], val, gx.data(), gy.data());
+  tri_poly(n, xi[0], xi[1], val, gx.data(), gy.data());
   for (int i = 0; i < m; ++i) { grad[2 * i] = gx[i]; grad[2 * i + 1] = gy[i]; }
 }
 void tri_hdiv(int n, const double* xi, double* val, double* div) {
   int m = tri_card(n - 1);
   vector<double> p(m), gx(m), gy(m);
-  koornwinder(n - 1, xi[0], xi[1], p.data(), gx.data(), gy.data());
+  tri_poly(n - 1, xi[0], xi[1], p.data(), gx.data(), gy.data());
   int bf = 0;
-  for (int k = 0; k < m; ++k) {
-    val[2 * bf] = p[k]; val[2 * bf + 1] = 0.0; div[bf] = gx[k]; ++bf;
-    val[2 * bf] = 0.0; val[2 * bf + 1] = p[k]; div[bf] = gy[k]; ++bf;
+  if (g_tri_basis == 0) {
+    for (int k = 0; k < m; ++k) {
+      val[2 * bf] = p[k]; val[2 * bf + 1] = 0.0; div[bf] = gx[k]; ++bf;
+      val[2 * bf] = 0.0; val[2 * bf + 1] = p[k]; div[bf] = gy[k]; ++bf;
+    }
+  } else {
+    for (int k = 0; k < m; ++k) { val[2 * bf] = p[k]; val[2 * bf + 1] = 0.0; div[bf] = gx[k]; ++bf; }
+    for (int k = 0; k < m; ++k) { val[2 * bf] = 0.0; val[2 * bf + 1] = p[k]; div[bf] = gy[k]; ++bf; }
   }
-  // members of exact degree n-1: i + j = n-1 (index of (i, j) in i-outer order)
+  // members of exact degree n-1, times the position vector
+  double cx = g_tri_basis == 0 ? 0.0 : 1.0 / 3.0;
+  double x = xi[0] - cx, y = xi[1] - cx;
   int k = 0;
-  for (int i = 0; i <= n - 1; ++i)
-    for (int j = 0; j + i <= n - 1; ++j, ++k) {
-      if (i + j != n - 1) continue;
-      double x = xi[0], y = xi[1];
+  if (g_tri_basis == 0) {
+    for (int i = 0; i <= n - 1; ++i)
+      for (int j = 0; j + i <= n - 1; ++j, ++k) {
+        if (i + j != n - 1) continue;
+        val[2 * bf] = x * p[k]; val[2 * bf + 1] = y * p[k];
+        div[bf] = 2.0 * p[k] + x * gx[k] + y * gy[k];
+        ++bf;
+      }
+  } else {
+    for (k = m - n; k < m; ++k) {  // the last n in degree order have degree n-1
       val[2 * bf] = x * p[k]; val[2 * bf + 1] = y * p[k];
       div[bf] = 2.0 * p[k] + x * gx[k] + y * gy[k];
       ++bf;
     }
+  }
 }
 
 // ------------------------------------------------------------ point geometry
@@ -703,9 +741,9 @@ double load_value(const dpg_load_desc* L, int comp, const double* x, int dim) {
 
 // ----------------------------------------------------- per-cell B, G, l
 // B is (M, N) row-major = stiffnessEnriched(c, trial, test); G (N, N); l (N)
-void cell_BGl(const dpg_form_desc* f, const Element& e, const Tabulator& tab, const double* nodes,
-              const int8_t* par, const double* coefs, const dpg_load_desc* load, double* B, double* G,
-              double* l) {
+template <typename Acc>
+void cell_BGl_t(const dpg_form_desc* f, const Element& e, const Tabulator& tab, const double* nodes,
+                const int8_t* par, const double* coefs, const dpg_load_desc* load, Acc* B, Acc* G, Acc* l) {
   int dim = e.dim, N = e.N, M = e.M;
   const RefCell& rc = e.rc;
   TestVals tv;
@@ -715,8 +753,8 @@ void cell_BGl(const dpg_form_desc* f, const Element& e, const Tabulator& tab, co
   if (B || l) {
     vector<double> pts, wts;
     tab.volume_rule(e.Q1, pts, wts);
-    if (B) std::memset(B, 0, sizeof(double) * M * N);
-    if (l) std::memset(l, 0, sizeof(double) * N);
+    if (B) std::fill(B, B + (size_t)M * N, Acc(0));
+    if (l) std::fill(l, l + N, Acc(0));
     for (size_t q = 0; q < wts.size(); ++q) {
       const double* xi = &pts[q * dim];
       geometry(rc, nodes, xi, g);
@@ -734,9 +772,9 @@ void cell_BGl(const dpg_form_desc* f, const Element& e, const Tabulator& tab, co
           int voff = e.test_offset[t.test_var], vcard = e.test_card[t.test_var];
           const double* tvals = tv.op(t.test_op);
           for (int j = 0; j < card; ++j) {
-            double uj = cf * fv[j];
-            double* Brow = B + (size_t)(toff + j) * N;
-            for (int i = voff; i < voff + vcard; ++i) Brow[i] += uj * tvals[i];
+            Acc uj = (Acc)cf * (Acc)fv[j];
+            Acc* Brow = B + (size_t)(toff + j) * N;
+            for (int i = voff; i < voff + vcard; ++i) Brow[i] += uj * (Acc)tvals[i];
           }
         }
       }
@@ -746,7 +784,7 @@ void cell_BGl(const dpg_form_desc* f, const Element& e, const Tabulator& tab, co
           double fx = load_value(load, r.load_comp, g.x, dim) * r.scale * wm;
           int voff = e.test_offset[r.test_var], vcard = e.test_card[r.test_var];
           const double* tvals = tv.op(r.test_op);
-          for (int i = voff; i < voff + vcard; ++i) l[i] += fx * tvals[i];
+          for (int i = voff; i < voff + vcard; ++i) l[i] += (Acc)fx * (Acc)tvals[i];
         }
       }
     }
@@ -810,9 +848,9 @@ void cell_BGl(const dpg_form_desc* f, const Element& e, const Tabulator& tab, co
             int voff = e.test_offset[t.test_var], vcard = e.test_card[t.test_var];
             const double* tvals = tv.op(t.test_op);
             for (int j = 0; j < card; ++j) {
-              double uj = cf * uv[j];
-              double* Brow = B + (size_t)(toff + j) * N;
-              for (int i = voff; i < voff + vcard; ++i) Brow[i] += uj * tvals[i];
+              Acc uj = (Acc)cf * (Acc)uv[j];
+              Acc* Brow = B + (size_t)(toff + j) * N;
+              for (int i = voff; i < voff + vcard; ++i) Brow[i] += uj * (Acc)tvals[i];
             }
           }
         }
@@ -823,7 +861,7 @@ void cell_BGl(const dpg_form_desc* f, const Element& e, const Tabulator& tab, co
   if (G) {
     vector<double> pts, wts;
     tab.volume_rule(e.QG1, pts, wts);
-    std::memset(G, 0, sizeof(double) * N * N);
+    std::fill(G, G + (size_t)N * N, Acc(0));
     vector<double> a(N);
     for (size_t q = 0; q < wts.size(); ++q) {
       const double* xi = &pts[q * dim];
@@ -844,14 +882,34 @@ void cell_BGl(const dpg_form_desc* f, const Element& e, const Tabulator& tab, co
           hi = std::max(hi, voff + vcard);
         }
         for (int i = lo; i < hi; ++i) {
-          double ai = wm * a[i];
-          if (ai == 0.0) continue;
-          double* Grow = G + (size_t)i * N;
-          for (int k = lo; k < hi; ++k) Grow[k] += ai * a[k];
+          Acc ai = (Acc)wm * (Acc)a[i];
+          if (ai == Acc(0)) continue;
+          Acc* Grow = G + (size_t)i * N;
+          for (int k = lo; k < hi; ++k) Grow[k] += ai * (Acc)a[k];
         }
       }
     }
   }
+}
+
+// Accumulation of B, G, l: FP64 (the reference's arithmetic) or, for the
+// extended-precision checker, 80-bit sums of the same FP64 point values,
+// rounded once at the end.
+static int g_accum_extended = 0;
+
+void cell_BGl(const dpg_form_desc* f, const Element& e, const Tabulator& tab, const double* nodes,
+              const int8_t* par, const double* coefs, const dpg_load_desc* load, double* B, double* G,
+              double* l) {
+  if (!g_accum_extended) {
+    cell_BGl_t<double>(f, e, tab, nodes, par, coefs, load, B, G, l);
+    return;
+  }
+  vector<long double> Bx(B ? (size_t)e.M * e.N : 0), Gx(G ? (size_t)e.N * e.N : 0), lx(l ? e.N : 0);
+  cell_BGl_t<long double>(f, e, tab, nodes, par, coefs, load, B ? Bx.data() : nullptr, G ? Gx.data() : nullptr,
+                          l ? lx.data() : nullptr);
+  for (size_t i = 0; i < Bx.size(); ++i) B[i] = (double)Bx[i];
+  for (size_t i = 0; i < Gx.size(); ++i) G[i] = (double)Gx[i];
+  for (size_t i = 0; i < lx.size(); ++i) l[i] = (double)lx[i];
 }
 
 // ------------------------------------------------------------ dense kernels
@@ -1085,6 +1143,19 @@ int oracle_tri_rule(int n, double* xy, double* w) {
   for (size_t i = 0; i < p.size(); ++i) xy[i] = p[i];
   for (size_t i = 0; i < ww.size(); ++i) w[i] = ww[i];
   return (int)ww.size();
+}
+
+// 1: accumulate B, G, l in 80-bit (extended-precision checker); 0: FP64
+int oracle_set_extended_accumulation(int on) {
+  g_accum_extended = on ? 1 : 0;
+  return 0;
+}
+
+// selects the triangle test-basis variant (0 default, 1 orthonormal, centroid complement)
+int oracle_set_tri_test_basis(int variant) {
+  if (variant != 0 && variant != 1) return -1;
+  g_tri_basis = variant;
+  return 0;
 }
 
 int oracle_gll_nodes(int n, double* x) {

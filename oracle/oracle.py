@@ -34,6 +34,8 @@ def lib():
         L.oracle_gauss_rule.argtypes = [ct.c_int, vp, vp]
         L.oracle_gll_nodes.argtypes = [ct.c_int, vp]
         L.oracle_tri_nodes.argtypes = [ct.c_int, vp]
+        L.oracle_set_tri_test_basis.argtypes = [ct.c_int]
+        L.oracle_set_extended_accumulation.argtypes = [ct.c_int]
         L.oracle_tri_rule.argtypes = [ct.c_int, vp, vp]
         L.oracle_assemble.argtypes = [P(capi.FormDesc), P(capi.BucketDesc), ct.c_int, vp, vp, vp,
                                       P(capi.LoadDesc), vp, vp, vp, vp, vp, vp, vp, vp, ct.c_int]
@@ -81,6 +83,21 @@ def gll_nodes(n):
     x = np.zeros(n + 1)
     lib().oracle_gll_nodes(n, _p(x))
     return x
+
+
+class tri_test_basis:
+    """context manager: the oracle's triangle test-basis variant (0 default,
+    1 orthonormal by degree with a centroid RT complement)"""
+
+    def __init__(self, variant):
+        self.variant = variant
+
+    def __enter__(self):
+        if lib().oracle_set_tri_test_basis(self.variant) != 0:
+            raise ValueError(self.variant)
+
+    def __exit__(self, *a):
+        lib().oracle_set_tri_test_basis(0)
 
 
 def tri_nodes(n):
@@ -192,15 +209,23 @@ def extended_schur(K, rhs, F):
 
 
 def with_extended(form, bucket, nodes, parities, coefs=None, load=None, schur=False):
-    """Oracle outputs plus extended-precision K, rhs (and S, g) computed from the
-    oracle's own FP64 G, B, l. The distance of the FP64 outputs from these is
-    the accuracy the reference algorithm reaches in FP64 on these inputs."""
+    """Oracle outputs plus extended-precision K, rhs (and S, g): G, B, l summed
+    in 80-bit from the same FP64 point values (basis values, weights, Jacobians)
+    and then factored, solved and condensed in 80-bit. The distance of the FP64
+    outputs from these is the accuracy the reference algorithm reaches in FP64
+    on these inputs, end to end."""
     want = ("B", "G", "l", "K", "rhs") + (("S", "g") if schur else ())
     o = assemble(form, bucket, nodes, parities, coefs, load, want=want)
+    lib().oracle_set_extended_accumulation(1)
+    try:
+        ox = assemble(form, bucket, nodes, parities, coefs, load, want=("B", "G", "l"))
+    finally:
+        lib().oracle_set_extended_accumulation(0)
     F = sizes(form, bucket)["F"]
     x = {k: np.zeros_like(o[k]) for k in ("K", "rhs") + (("S", "g") if schur else ())}
     for c in range(nodes.shape[0]):
-        x["K"][c], x["rhs"][c] = extended_K(o["G"][c], o["B"][c], o["l"][c])
+        x["K"][c], x["rhs"][c] = extended_K(ox["G"][c], ox["B"][c], ox["l"][c])
         if schur:
             x["S"][c], x["g"][c] = extended_schur(x["K"][c], x["rhs"][c], F)
     return o, x
+

@@ -25,7 +25,8 @@ def c3_case(n0, levels, tri_cols=1):
 
 def check_types_against_oracle(f, groups, load, want_schur=True):
     """each element type vs the oracle: K, rhs (and the Schur complement) within
-    max(1e-12, 4 x the oracle's own distance from an 80-bit evaluation)"""
+    max(1e-12, 4 x floor) of the 80-bit evaluation, floor = the oracle's own
+    FP64 distance from it"""
     A = dpg.LocalAssembler(f, [g["bucket"] for g in groups])
     worst = 0.0
     for bi, g in enumerate(groups):
@@ -35,13 +36,24 @@ def check_types_against_oracle(f, groups, load, want_schur=True):
         torch.cuda.synchronize()
         res = {key: v.cpu().numpy() for key, v in out.items()}
         assert nbad == 0 and np.all(res["info"] == 0)
-        o, x = oracle.with_extended(oracle.form_like(f), g["bucket"], g["nodes"], g["parities"], g["coefs"], load,
-                                    schur=want_schur)
+        tri = g["topology"] == capi.TRI
+        args = (oracle.form_like(f), g["bucket"], g["nodes"], g["parities"], g["coefs"], load)
+        # triangles: the oracle's test basis of the library's family (orthonormal,
+        # centroid RT complement), so that both round G and B alike; its default
+        # basis spans the same spaces, which the K agreement below checks
+        with oracle.tri_test_basis(1 if tri else 0):
+            o, x = oracle.with_extended(*args, schur=want_schur)
+        if tri:
+            o0 = oracle.assemble(*args, want=("K",))
+            eb = rel_fro(o0["K"], o["K"])
+            print(f"  oracle K, two triangle test bases: {eb:.2e}")
+            assert eb < 1e-8
         pairs = [("K", "K"), ("rhs", "rhs")] + ([("schur", "S"), ("schur_rhs", "g")] if want_schur else [])
         for gk, ok in pairs:
             floor = rel_fro(o[ok], x[ok])
             err = rel_fro(res[gk], x[ok])
-            print(f"  {gk:10s} gpu-vs-80bit {err:.2e} oracle-vs-80bit {floor:.2e}")
+            print(f"  {gk:10s} gpu-vs-80bit {err:.2e} oracle-vs-80bit {floor:.2e} gpu-vs-oracle "
+                  f"{rel_fro(res[gk], o[ok]):.2e}")
             assert err <= max(1e-12, 4.0 * floor), (g["topology"], g["k"], g["kt"], gk)
             worst = max(worst, err)
     A.close()
@@ -49,20 +61,21 @@ def check_types_against_oracle(f, groups, load, want_schur=True):
 
 
 def test_c3_every_element_type_matches_oracle():
-    """All (p, dp) quad types and the triangle types of a small graded mesh
-    (eps down to 1e-4) against the oracle, Schur complement included."""
-    f, groups, load = c3_case(2, 3)
+    """All 18 (p, dp) quad types and the triangle types of a small graded mesh
+    (75 cells, eps down to 1e-4) against the oracle, Schur complement included."""
+    f, groups, load = c3_case(3, 3)
     assert sum(g["topology"] == capi.QUAD for g in groups) == 18
     check_types_against_oracle(f, groups, load)
 
 
 @pytest.mark.parametrize("form", ["cd", "poisson"])
 def test_triangle_element_types_match_oracle(form):
-    """Triangles: the product's test bases (orthonormal Dubiner, RT from it with
-    a centroid complement, degree-major order) differ from the oracle's
-    (unnormalised Koornwinder, interleaved RT, vertex complement), so agreement
-    of K also checks that both span P_kt and RT_kt; the field basis is nodal on
-    warp-and-blend nodes in both."""
+    """Triangles against the oracle. The oracle's default triangle test basis
+    (unnormalised Koornwinder, interleaved RT, vertex complement) differs from
+    the library's (orthonormal Dubiner by degree, centroid complement): its K
+    agreeing to 1e-8 with the oracle's K in the library's basis family shows both
+    span P_kt and RT_kt (K of other spaces differs at O(1)); the tight check is
+    against the latter. The field basis is nodal on warp-and-blend nodes in both."""
     nodes, par = dpg.mesh_quad(3, 3, 0.1, 21)
     tn, tp = split_quads(nodes, par)
     C = tn.shape[0]
@@ -78,25 +91,41 @@ def test_triangle_element_types_match_oracle(form):
 
 
 def test_tile_edges_give_identical_results_and_autotune_picks_one():
+    """Dense-G tile edges 64 and 32 give bit-identical results; the opt-in
+    sum-factorised G agrees with the dense G to rounding (its K does not: see
+    the accuracy note on dpg_tiles.sumfact)."""
     f, groups, load = c3_case(2, 3)
     sel = [groups[0], groups[-1]]  # largest and smallest element type
     A = dpg.LocalAssembler(f, [g["bucket"] for g in sel])
     for j, g in enumerate(sel):
         args = (dev(g["nodes"]), dev(g["parities"]), dev(g["coefs"]), load)
-        assert A.tiles(j) == (64, 64, 64)
-        a, _ = A.local_stiffness_and_rhs(j, *args, want_schur=True)
+        assert A.tiles(j) == (64, 64, 64, 0)
+        A.set_tiles(j, 64, 64, 64, 1)
+        sfr, _ = A.local_stiffness_and_rhs(j, *args, want_schur=True, want_G=True)
+        sfr = {key: v.cpu().numpy() for key, v in sfr.items()}
+        A.set_tiles(j, 64, 64, 64, 0)
+        a, _ = A.local_stiffness_and_rhs(j, *args, want_schur=True, want_G=True)
         a = {key: v.cpu().numpy() for key, v in a.items()}
-        A.set_tiles(j, 32, 32, 32)
-        b, _ = A.local_stiffness_and_rhs(j, *args, want_schur=True)
+        A.set_tiles(j, 32, 32, 32, 0)
+        b, _ = A.local_stiffness_and_rhs(j, *args, want_schur=True, want_G=True)
         b = {key: v.cpu().numpy() for key, v in b.items()}
-        for key in ("K", "rhs", "schur", "schur_rhs"):
+        for key in ("K", "rhs", "schur", "schur_rhs", "G"):
             assert np.array_equal(a[key], b[key]), key
+        eg = rel_fro(sfr["G"], a["G"])
+        ek = rel_fro(sfr["K"], a["K"])
+        print(f"type {g['topology'], g['k'], g['kt']}: sum-factorised vs dense G {eg:.2e}, K {ek:.2e}")
+        assert eg < 1e-14
         t = A.autotune(j, *args, sample=g["ncells"])
-        print(f"type {g['topology'], g['k'], g['kt']}: autotuned tiles {t}")
-        assert all(v in (32, 64) for v in t)
+        print(f"  autotuned {t}")
+        assert all(v in (32, 64) for v in t[:3]) and t[3] == 0
     with pytest.raises(capi.DPGError):
         A.set_tiles(0, 48, 64, 64)
     A.close()
+    At = dpg.LocalAssembler(f, [dpg.bucket(capi.TRI, 3, 4)])
+    assert At.tiles(0) == (64, 64, 64, 0)
+    with pytest.raises(capi.DPGError):
+        At.set_tiles(0, 64, 64, 64, 1)
+    At.close()
 
 
 def test_graph_replays_match_per_bucket_launches():

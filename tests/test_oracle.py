@@ -361,3 +361,27 @@ def test_triangle_convection_diffusion_linear_solution_has_zero_residual(k, kt):
     r = o["l"] - np.einsum("cmn,cm->cn", o["B"], u)
     for c in range(C):
         assert np.abs(r[c]).max() < 1e-12 * np.abs(o["B"][c]).max() * np.abs(u[c]).max(), c
+
+
+def test_extended_accumulation_changes_only_rounding():
+    """The 80-bit checker sums the same FP64 point values: G, B, l agree with
+    the FP64 sums to rounding, and its K is within the FP64 floor of the oracle's."""
+    nodes, par = _quad_mesh(2, 2, 0.1, 13)
+    f = oracle.form("poisson", 2)
+    bk = capi.BucketDesc(capi.QUAD, 3, 4, 0)
+    load = capi.LoadDesc()
+    load.ncomp, load.nmodes[0], load.amp[0][0] = 1, 1, 1.3
+    load.k[0][0][0], load.k[0][0][1] = 2, 3
+    o, x = oracle.with_extended(f, bk, nodes, par, load=load, schur=True)
+    lib = oracle.lib()
+    lib.oracle_set_extended_accumulation(1)
+    try:
+        ox = oracle.assemble(f, bk, nodes, par, load=load, want=("B", "G", "l"))
+    finally:
+        lib.oracle_set_extended_accumulation(0)
+    for k in ("B", "G", "l"):
+        d = np.abs(ox[k] - o[k]).max() / np.abs(o[k]).max()
+        assert 0 < d < 1e-14, k  # different rounding, same sums
+    for k in ("K", "rhs", "S", "g"):
+        rel = np.linalg.norm(o[k] - x[k]) / np.linalg.norm(x[k])
+        assert rel < 1e-11, k
