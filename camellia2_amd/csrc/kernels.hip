@@ -815,6 +815,95 @@ __global__ __launch_bounds__(256) void potrf_block(const double* __restrict__ A,
   }
 }
 
+// lane `k`'s value of a double, in every lane (k wave-uniform), through the
+// LDS crossbar (ds_bpermute: no LDS allocation, result in vector registers)
+__device__ inline double wave_bcast(double v, int k) {
+  long long b = __builtin_bit_cast(long long, v);
+  int lo = __builtin_amdgcn_ds_bpermute(k << 2, (int)(b & 0xffffffffLL));
+  int hi = __builtin_amdgcn_ds_bpermute(k << 2, (int)(b >> 32));
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Diagonal 64x64 block of one cell per wavefront (4 cells per workgroup), in
+// registers: lane i holds row i of the block. Right-looking Cholesky with the
+// pivot and the scaled column broadcast by readlane (no LDS, no barriers), then
+// the inverse row by row (row p of L^-1 is final once rows < p have been folded
+// in): writes the factor to Ld[(k0+r)*64 + c] and Li[m][j] = Linv[j][m].
+// A non-positive pivot sets info to its 1-based column (first failure only) and
+// turns the rest of the cell's factor into NaN, as LAPACK leaves it unusable.
+__global__ __launch_bounds__(256) void potrf_wave(const double* __restrict__ A, int64_t lda, int64_t sA, int k0,
+                                                  int kb, double* __restrict__ Ld, int64_t sLd,
+                                                  double* __restrict__ Li, int32_t* __restrict__ info,
+                                                  int info_base, int ncells) {
+  const int lane = threadIdx.x & 63;
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (c >= ncells) return;  // whole wavefronts only: no barrier below
+  const double* Ar = A + (size_t)c * sA + (size_t)(k0 + lane) + (size_t)k0 * lda;  // row `lane`, column k0
+  double a[PNB];
+  int ll, lf, li, ls;  // see below
+  asm volatile("v_mov_b32 %0, %1" : "=v"(ll) : "v"(lane));
+#pragma unroll
+  for (int cc = 0; cc < PNB; ++cc) {
+    double v = (ll == cc) ? 1.0 : 0.0;
+    if (ll < kb && cc < kb && cc <= ll) v = *Ar;
+    a[cc] = v;
+    Ar += lda;
+  }
+  // per-phase opaque copies of the lane index (ll, lf, li, ls): keep the compiler
+  // from sharing the 64 per-column lane masks across phases, which would hold
+  // them all live in scalar registers
+  asm volatile("v_mov_b32 %0, %1" : "=v"(lf) : "v"(lane));
+  int fail = -1;
+#pragma unroll
+  for (int j = 0; j < PNB; ++j) {
+    __builtin_amdgcn_sched_barrier(0);
+    double d = wave_bcast(a[j], j);
+    const bool bad = !(d > 0.0);
+    fail = (fail < 0 && bad) ? j : fail;
+    d = bad ? __builtin_nan("") : d;
+    const double sd = sqrt(d), rs = 1.0 / sd;
+    const double lij = lf > j ? a[j] * rs : 0.0;
+    a[j] = lf == j ? sd : (lf > j ? lij : a[j]);
+#pragma unroll
+    for (int k = j + 1; k < PNB; ++k) {
+      a[k] = fma(-lij, wave_bcast(a[j], k), a[k]);
+      if ((k & 3) == 3) __builtin_amdgcn_sched_barrier(0);  // bounds live broadcast values (SGPRs)
+    }
+  }
+  double* Lc = Ld + (size_t)c * sLd;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(ls) : "v"(lane));
+  if (ls < kb) {
+#pragma unroll
+    for (int cc = 0; cc < PNB; ++cc) Lc[(size_t)(k0 + ls) * PNB + cc] = (cc <= ls && cc < kb) ? a[cc] : 0.0;
+  }
+  if (lane == 0 && fail >= 0 && fail < kb && info) atomicCAS(&info[c], 0, info_base + k0 + fail + 1);
+  // inverse in place, row by row: before step p, a[q] (q < p) holds lane's
+  // accumulator sum_{m<p} L[lane][m] Linv[m][q] (or its final Linv row if lane < p)
+  // and a[q] (q >= p) still holds L[lane][q]. Step p finalises row p in lane p,
+  // Linv[p][q] = -acc[q] / L_pp, Linv[p][p] = 1 / L_pp, and folds it into the
+  // lanes below: acc[q] += L[lane][p] Linv[p][q].
+  asm volatile("v_mov_b32 %0, %1" : "=v"(li) : "v"(lane));
+#pragma unroll
+  for (int p = 0; p < PNB; ++p) {
+    __builtin_amdgcn_sched_barrier(0);
+    const double ip = 1.0 / wave_bcast(a[p], p);
+    const double lp = li > p ? a[p] : 0.0;
+    const bool own = li == p;
+#pragma unroll
+    for (int q = 0; q < p; ++q) a[q] = own ? -a[q] * ip : a[q];
+    a[p] = own ? ip : 0.0;
+#pragma unroll
+    for (int q = 0; q <= p; ++q) {
+      a[q] = fma(lp, wave_bcast(a[q], p), a[q]);
+      if ((q & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  double* Lic = Li + (size_t)c * PNB * PNB + lane;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(ls) : "v"(lane));
+#pragma unroll
+  for (int m = 0; m < PNB; ++m) Lic[(size_t)m * PNB] = (m <= ls) ? a[m] : 0.0;
+}
+
 __global__ __launch_bounds__(256) void panel_trsm(double* __restrict__ A, int64_t lda, int64_t sA, int D, int k0,
                                                   int kb, const double* __restrict__ Li) {
   __shared__ double Ys[PNB][PNB + 4];
@@ -1233,8 +1322,8 @@ void launch_gemm_mixed(GemmArgs g, int precision, int role, hipStream_t st) {
 
 void launch_panel(double* A, int64_t lda, int64_t sA, int D, int k0, int kb, double* Ld, int64_t sLd, double* Li,
                   int ncells, int32_t* info, int info_base, hipStream_t st) {
-  potrf_block<<<dim3(ncells), 256, 0, st>>>(A, lda, sA, k0, kb, Ld, sLd, Li, info, info_base);
-  check_launch("potrf_block");
+  potrf_wave<<<dim3((ncells + 3) / 4), 256, 0, st>>>(A, lda, sA, k0, kb, Ld, sLd, Li, info, info_base, ncells);
+  check_launch("potrf_wave");
   int rows = D - k0 - kb;
   if (rows <= 0) return;
   int chunks = (rows + PCH - 1) / PCH;
