@@ -282,7 +282,7 @@ __global__ __launch_bounds__(256) void rhs_kernel(const double* __restrict__ Wrh
 template <int NPR>
 __global__ __launch_bounds__(256) void ip_operand_kernel(const double* __restrict__ Wip, int n_lt, int nvar,
                                                          const double* __restrict__ R,
-                                                         const int32_t* __restrict__ vod, int N, int QG,
+                                                         const int32_t* __restrict__ vod, int N, int QG, int QGp,
                                                          double* __restrict__ out, int64_t sc, int64_t ld,
                                                          double* __restrict__ s, int64_t ss) {
   extern __shared__ double Wl[];
@@ -307,11 +307,14 @@ __global__ __launch_bounds__(256) void ip_operand_kernel(const double* __restric
       double acc = 0.0;
 #pragma unroll
       for (int k = 0; k < NPR; ++k) acc = fma(w[k], r[k], acc);
-      oc[((size_t)q * n_lt + lt) * ld] = acc;
+      oc[((size_t)lt * QGp + q) * ld] = acc;  // term-major rows: term lt, point q
       diag = fma(acc, acc, diag);
     }
   }
-  if (active) s[(size_t)c * ss + j] = 1.0 / sqrt(diag);
+  if (!active) return;
+  for (int lt = 0; lt < n_lt; ++lt)
+    for (int q = QG; q < QGp; ++q) oc[((size_t)lt * QGp + q) * ld] = 0.0;
+  s[(size_t)c * ss + j] = 1.0 / sqrt(diag);
 }
 
 // ------------------------------------------------------------------- GEMM
@@ -364,6 +367,13 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
 #pragma unroll
     for (int bb = 0; bb < 2; ++bb) acc[a][bb] = d4{0.0, 0.0, 0.0, 0.0};
   const int kbeg = 0, kend = g.k;
+  // k-chunks of segments that are zero for this tile are skipped
+  const uint64_t km = g.kmask ? g.kmask[tile] : ~0ull;
+  auto next_chunk = [&](int k0) {
+    if (g.kmask)
+      while (k0 < kend && !((km >> (k0 / g.kseg)) & 1ull)) k0 += GKC;
+    return k0;
+  };
   // each thread stages PER elements of X and of Y per k-chunk; the next chunk
   // is fetched into registers while the current one is multiplied from LDS
   double xr[PER], yr[PER];
@@ -375,8 +385,9 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
       yr[r] = (kg < kend && j0 + ii < g.n) ? Y[(size_t)kg * g.ldy + j0 + ii] : 0.0;
     }
   };
-  fetch(kbeg);
-  for (int k0 = kbeg; k0 < kend; k0 += GKC) {
+  int k0 = next_chunk(kbeg);
+  if (k0 < kend) fetch(k0);
+  while (k0 < kend) {
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < PER; ++r) {
@@ -385,7 +396,8 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
       Ys[e / TT][e % TT] = yr[r];
     }
     __syncthreads();
-    if (k0 + GKC < kend) fetch(k0 + GKC);
+    const int kn = next_chunk(k0 + GKC);
+    if (kn < kend) fetch(kn);
 #pragma unroll
     for (int k4 = 0; k4 < GKC; k4 += 4) {
       const int kr = k4 + (lane >> 4), cc = lane & 15;
@@ -400,6 +412,7 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
         for (int bi = 0; bi < 2; ++bi)
           acc[bj][bi] = __builtin_amdgcn_mfma_f64_16x16x4f64(yb[bj], xa[bi], acc[bj][bi], 0, 0, 0);
     }
+    k0 = kn;
   }
   // v_mfma_f64_16x16x4 result layout: D[a][b] in lane b + 16*(a%4), element a/4,
   // with a the A-operand (Y, C column j) and b the B-operand (X, C row i) index
@@ -1264,14 +1277,14 @@ void launch_rhs(int dim, const double* Wrhs, int nvar, const double* R, const in
 }
 
 void launch_ip_operand(int dim, const double* Wip, int n_lt, int nvar, const double* R, const int32_t* vod, int N,
-                       int QG, int ncells, double* out, int64_t sc, int64_t ld, double* s, int64_t ss,
+                       int QG, int QGp, int ncells, double* out, int64_t sc, int64_t ld, double* s, int64_t ss,
                        hipStream_t st) {
   size_t shm = sizeof(double) * (size_t)n_lt * nvar * (dim + 1);
   dim3 grid((N + 255) / 256, ncells);
   if (dim == 2)
-    ip_operand_kernel<3><<<grid, 256, shm, st>>>(Wip, n_lt, nvar, R, vod, N, QG, out, sc, ld, s, ss);
+    ip_operand_kernel<3><<<grid, 256, shm, st>>>(Wip, n_lt, nvar, R, vod, N, QG, QGp, out, sc, ld, s, ss);
   else
-    ip_operand_kernel<4><<<grid, 256, shm, st>>>(Wip, n_lt, nvar, R, vod, N, QG, out, sc, ld, s, ss);
+    ip_operand_kernel<4><<<grid, 256, shm, st>>>(Wip, n_lt, nvar, R, vod, N, QG, QGp, out, sc, ld, s, ss);
   check_launch("ip_operand_kernel");
 }
 

@@ -35,6 +35,11 @@ struct GemmArgs {
   // before the conversion to the low-precision operand format
   const double* px;
   const double* py;
+  // optional (FP64 GEMMs): k runs over segments of kseg rows (kseg a multiple of
+  // the 16-row k-chunk); bit s of kmask[tile] clear means segment s is zero for
+  // this tile's rows or columns and is skipped (adds exact zeros otherwise)
+  const uint64_t* kmask;
+  int64_t kseg;
 };
 
 void launch_point(int dim, int kind, const DevProgram* prog, const DevTables& tab, int npts, int Qs, int ncells,
@@ -47,7 +52,7 @@ void launch_rhs(int dim, const double* Wrhs, int nvar, const double* R, const in
 // Gram operand: out[c][q*n_lt + lt][j] = weighted value of IP term lt at point q
 // for test dof j (G = out^T out); s[c][j] = 1/sqrt(G_jj)
 void launch_ip_operand(int dim, const double* Wip, int n_lt, int nvar, const double* R, const int32_t* vod, int N,
-                       int QG, int ncells, double* out, int64_t sc, int64_t ld, double* s, int64_t ss,
+                       int QG, int QGp, int ncells, double* out, int64_t sc, int64_t ld, double* s, int64_t ss,
                        hipStream_t st);
 enum { kGemmGram = 0, kGemmBmat = 1, kGemmUpdate = 2 };
 void launch_gemm(GemmArgs g, int role, hipStream_t st, int tile = 64);

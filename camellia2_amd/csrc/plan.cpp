@@ -200,6 +200,10 @@ struct BucketPlan {
   double flops = 0.0;
   double gram_flops = 0.0;  // sum over IP terms of QG * n(n+1), n = dofs the term touches
   int tile[3] = {64, 64, 64};  // FP64 GEMM tile edge per role: Gram, B blocks, Cholesky updates
+  int QGp = 0;  // Gram points per IP term in the operand, padded to the 16-row k-chunk
+  // per lower tile of G (64- and 32-wide tiles): bit lt set when IP term lt
+  // reaches both the tile's rows and its columns
+  const uint64_t* gmask[2] = {nullptr, nullptr};
   // sum-factorised Gram (tensor-product cells, FP64): block pairs, pairs, 1D tables
   bool sf_ok = false;   // available for this bucket
   bool sf_on = false;   // used (opt-in)
@@ -489,7 +493,22 @@ void build_bucket(dpg_plan* plan, const dpg_bucket_desc& desc, BucketPlan& B) {
   B.Npad = round_up(L.N, 8);
   int S = L.rc.nsides, nvar = f.n_test;
   B.nA = B.lda * B.D;
-  B.nAt = (int64_t)B.prog.n_lt * L.QG * B.Npad;  // weighted IP-term values
+  B.QGp = (int)round_up(L.QG, 16);
+  B.nAt = (int64_t)B.prog.n_lt * B.QGp * B.Npad;  // weighted IP-term values, term-major rows
+  {
+    // test variables each IP term reaches
+    std::vector<uint64_t> tv(f.n_test, 0);
+    for (int i = 0; i < f.n_ipterms; ++i)
+      for (int k = 0; k < f.ipterms[i].n; ++k) tv[f.ipterms[i].s[k].test_var] |= 1ull << i;
+    for (int ts = 0; ts < 2; ++ts) {
+      const int T = ts == 0 ? 64 : 32, nt = (L.N + T - 1) / T;
+      std::vector<uint64_t> reach(nt, 0), mask;  // terms reaching some dof of a T-block
+      for (int j = 0; j < L.N; ++j) reach[j / T] |= tv[L.var_of_test_dof[j]];
+      for (int ti = 0; ti < nt; ++ti)
+        for (int tj = 0; tj <= ti; ++tj) mask.push_back(reach[ti] & reach[tj]);
+      B.gmask[ts] = upload(plan->tables, mask);
+    }
+  }
   B.nTv = (int64_t)B.prog.n_vb * L.Q * B.Npad;
   B.nTs = (int64_t)B.prog.n_sb * S * L.Qs * B.Npad;
   B.nWip = (int64_t)L.QG * B.prog.n_lt * nvar * B.npr;
@@ -639,10 +658,10 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
   const bool sf = B.sf_on && prec == DPG_PREC_FP64;
   // operands: weighted IP-term values (with the DPOEQU factors), B operands
   e = P.start(st);
-  const int KG = B.prog.n_lt * L.QG;
+  const int KG = B.prog.n_lt * B.QGp;
   if (!sf)
-    launch_ip_operand(dim, w.Wip, B.prog.n_lt, nvar, B.tab.R_ip, B.tab.var_of_dof, N, L.QG, nb, w.At, B.nAt, B.Npad,
-                      w.s, B.nS, st);
+    launch_ip_operand(dim, w.Wip, B.prog.n_lt, nvar, B.tab.R_ip, B.tab.var_of_dof, N, L.QG, B.QGp, nb, w.At, B.nAt,
+                      B.Npad, w.s, B.nS, st);
   launch_operand(dim, w.Wvol, B.prog.n_vb, nvar, B.tab.R_vol, B.tab.var_of_dof, N, L.Q, nb, w.Tv, B.nTv,
                  (int64_t)L.Q * B.Npad, B.Npad, st);
   if (B.prog.n_sb > 0)
@@ -689,10 +708,13 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
   g.srow = s_ep;
   g.scol = s_ep;
   g.ss = B.nS;
+  g.kmask = B.gmask[B.tile[0] == 64 ? 0 : 1];
+  g.kseg = B.QGp;
   if (prec != DPG_PREC_FP64) {
     // low-precision operands are pre-scaled (keeps FP16 in range): X(k,i) s_i, Y(k,j) s_j
     g.srow = g.scol = nullptr;
     g.px = g.py = s_ep;
+    g.kmask = nullptr;
   }
   if (!sf) {
     e = P.start(st);
