@@ -103,8 +103,8 @@ def cpu_baseline_hp(f, groups, load, gpu_K, target_s=12.0):
         t_tot += time.perf_counter() - t0
         n_tot += n
         Kg = gpu_K[bi][:n].cpu().numpy()
-        err = max(err, float(np.max(np.linalg.norm((Kg - o["K"]).reshape(len(sel), -1), axis=1) /
-                                    np.linalg.norm(o["K"].reshape(len(sel), -1), axis=1))))
+        err = max(err, float(np.max(np.linalg.norm((Kg - o["K"]).reshape(n, -1), axis=1) /
+                                    np.linalg.norm(o["K"].reshape(n, -1), axis=1))))
     return {"value": n_tot / t_tot, "unit": "cells/s", "cores": threads, "kind": "port",
             "sample": f"{n_tot} cells, the first {frac:.1%} of every element type, oracle K+rhs+Schur, "
                       f"{t_tot:.1f} s wall, {threads} OpenMP threads"}, err

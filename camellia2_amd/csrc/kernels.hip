@@ -844,7 +844,7 @@ __device__ inline double wave_bcast(double v, int k) {
 // in): writes the factor to Ld[(k0+r)*64 + c] and Li[m][j] = Linv[j][m].
 // A non-positive pivot sets info to its 1-based column (first failure only) and
 // turns the rest of the cell's factor into NaN, as LAPACK leaves it unusable.
-__global__ __launch_bounds__(256) void potrf_wave(const double* __restrict__ A, int64_t lda, int64_t sA, int k0,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void potrf_wave(const double* __restrict__ A, int64_t lda, int64_t sA, int k0,
                                                   int kb, double* __restrict__ Ld, int64_t sLd,
                                                   double* __restrict__ Li, int32_t* __restrict__ info,
                                                   int info_base, int ncells) {
@@ -880,7 +880,9 @@ __global__ __launch_bounds__(256) void potrf_wave(const double* __restrict__ A, 
 #pragma unroll
     for (int k = j + 1; k < PNB; ++k) {
       a[k] = fma(-lij, wave_bcast(a[j], k), a[k]);
-      if ((k & 3) == 3) __builtin_amdgcn_sched_barrier(0);  // bounds live broadcast values (SGPRs)
+      // materialise the update now: left to itself the compiler sinks it to a[k]'s
+      // next use and keeps every broadcast value live in scalar registers meanwhile
+      asm volatile("" : "+v"(a[k]));
     }
   }
   double* Lc = Ld + (size_t)c * sLd;
@@ -908,7 +910,7 @@ __global__ __launch_bounds__(256) void potrf_wave(const double* __restrict__ A, 
 #pragma unroll
     for (int q = 0; q <= p; ++q) {
       a[q] = fma(lp, wave_bcast(a[q], p), a[q]);
-      if ((q & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+      asm volatile("" : "+v"(a[q]));
     }
   }
   double* Lic = Li + (size_t)c * PNB * PNB + lane;
