@@ -80,7 +80,9 @@ def cpu_baseline(f, b, nodes, par, load, gpu_K, target_s=12.0):
 
 def cpu_baseline_hp(f, groups, load, gpu_K, target_s=12.0):
     """Oracle on a stratified sample of the hp mesh (the same share of every
-    element type), plus the GPU-vs-oracle error of K on that sample."""
+    element type), plus the GPU-vs-oracle error of K on that sample and, for
+    each type's worst cell, the GPU's and the oracle's distance from K summed
+    and solved in 80-bit (the oracle's is what FP64 reaches on that cell)."""
     from oracle import oracle
 
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -95,6 +97,7 @@ def cpu_baseline_hp(f, groups, load, gpu_K, target_s=12.0):
     full = sum(t * g["ncells"] for t, g in zip(per, groups))
     frac = min(1.0, target_s / full)
     n_tot, t_tot, err = 0, 0.0, 0.0
+    worst = {"gpu_vs_80bit": 0.0, "oracle_vs_80bit": 0.0, "ratio_to_floor": 0.0}
     for bi, g in enumerate(groups):
         n = min(g["ncells"], max(1, int(round(frac * g["ncells"]))))
         t0 = time.perf_counter()
@@ -103,11 +106,22 @@ def cpu_baseline_hp(f, groups, load, gpu_K, target_s=12.0):
         t_tot += time.perf_counter() - t0
         n_tot += n
         Kg = gpu_K[bi][:n].cpu().numpy()
-        err = max(err, float(np.max(np.linalg.norm((Kg - o["K"]).reshape(n, -1), axis=1) /
-                                    np.linalg.norm(o["K"].reshape(n, -1), axis=1))))
+        e = np.linalg.norm((Kg - o["K"]).reshape(n, -1), axis=1) / np.linalg.norm(o["K"].reshape(n, -1), axis=1)
+        err = max(err, float(e.max()))
+        # the type's worst cell against K from G, B, l summed and solved in 80-bit:
+        # the oracle's own distance from it is what FP64 can reach on that cell
+        c = int(np.argmax(e))
+        o1, x1 = oracle.with_extended(ff, g["bucket"], g["nodes"][c:c + 1], g["parities"][c:c + 1],
+                                      g["coefs"][c:c + 1], load)
+        nx = np.linalg.norm(x1["K"][0])
+        eg = float(np.linalg.norm(Kg[c] - x1["K"][0]) / nx)
+        eo = float(np.linalg.norm(o1["K"][0] - x1["K"][0]) / nx)
+        worst["gpu_vs_80bit"] = max(worst["gpu_vs_80bit"], eg)
+        worst["oracle_vs_80bit"] = max(worst["oracle_vs_80bit"], eo)
+        worst["ratio_to_floor"] = max(worst["ratio_to_floor"], eg / max(eo, 1e-16))
     return {"value": n_tot / t_tot, "unit": "cells/s", "cores": threads, "kind": "port",
             "sample": f"{n_tot} cells, the first {frac:.1%} of every element type, oracle K+rhs+Schur, "
-                      f"{t_tot:.1f} s wall, {threads} OpenMP threads"}, err
+                      f"{t_tot:.1f} s wall, {threads} OpenMP threads"}, err, worst
 
 
 def bench_hp(args, cfg):
@@ -223,9 +237,10 @@ def bench_hp(args, cfg):
         "stages": stages,
     }
     if not args.no_cpu:
-        cpu, err = cpu_baseline_hp(f, groups, load, [o["K"] for o in out_e])
+        cpu, err, worst = cpu_baseline_hp(f, groups, load, [o["K"] for o in out_e])
         result["cpu_baseline"] = cpu
         result["rel_err_K_vs_fp64_oracle"] = err
+        result["worst_cell_per_type_vs_80bit"] = worst
     print(json.dumps(result))
 
 

@@ -19,6 +19,8 @@
 // -[B l r]^T G^-1 [B l r] in the trailing block, i.e. -K, -rhs and -eta^2.
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include <cmath>
 #include <cstdint>
 
@@ -733,100 +735,16 @@ __global__ __launch_bounds__(256) void gemm_bmat_f16x3(GemmArgs g) { gemm_mixed_
 
 // ------------------------------------------------------------ panel kernels
 // Columns [k0, k0+kb) (kb <= 64) of each cell's lower-stored matrix:
-//   potrf_block  one workgroup per cell: POTRF of the 64x64 diagonal block in
-//                LDS, the factor to Ld, its in-place triangular inverse
-//                (LAPACK DTRTI2 order) to Li as Li[m][j] = Linv[j][m]
-//   panel_trsm   workgroup (chunk, cell): X <- X L^-T = X Li for rows
-//                [k0+kb + chunk*PCH, +PCH) in 64-row tiles on MFMA.
+//   potrf_wave   one wavefront per cell: POTRF of the diagonal block in
+//                registers; the factor to Ld (row-major, for the back solve)
+//                and to Lt (column-major, with the reciprocals of its diagonal)
+//   panel_trsm   one thread per row below the block: X <- X L^-T by
+//                substitution, column by column, L read as wave-uniform scalars.
 // The diagonal block of A itself is left untouched (no later stage reads it).
-constexpr int PNB = 64, PLD = PNB + 1, PCH = 256;
-
-__global__ __launch_bounds__(256) void potrf_block(const double* __restrict__ A, int64_t lda, int64_t sA, int k0,
-                                                   int kb, double* __restrict__ Ld, int64_t sLd,
-                                                   double* __restrict__ Li, int32_t* __restrict__ info,
-                                                   int info_base) {
-  __shared__ double L[PNB][PLD];
-  __shared__ double piv[PNB];
-  const int c = blockIdx.x, t = threadIdx.x;
-  const double* Ac = A + (size_t)c * sA;
-  for (int e = t; e < PNB * PNB; e += 256) {
-    int r = e % PNB, cc = e / PNB;
-    double v;
-    if (r < kb && cc < kb) v = (r >= cc) ? Ac[(size_t)(k0 + r) + (size_t)(k0 + cc) * lda] : 0.0;
-    else v = (r == cc) ? 1.0 : 0.0;
-    L[r][cc] = v;
-  }
-  // packed lower-triangle elements owned by this thread (2080 = 64*65/2 of them)
-  int own_r[9], own_c[9];
-#pragma unroll
-  for (int m = 0; m < 9; ++m) {
-    int e = t + 256 * m;
-    if (e < PNB * (PNB + 1) / 2) {
-      int r = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
-      while ((r + 1) * (r + 2) / 2 <= e) ++r;
-      while (r * (r + 1) / 2 > e) --r;
-      own_r[m] = r;
-      own_c[m] = e - r * (r + 1) / 2;
-    } else {
-      own_r[m] = -1;
-      own_c[m] = -1;
-    }
-  }
-  // right-looking Cholesky, one barrier per column: the update of step j uses
-  // the unscaled column j (l_rj l_cj = L_rj L_cj / d_j); column j is scaled by
-  // 1/sqrt(d_j) one step later, when no update reads it any more
-  int fail = -1;
-  for (int j = 0; j <= PNB; ++j) {
-    __syncthreads();
-    double d = 0.0;
-    if (j < PNB) {
-      d = L[j][j];
-      if (!(d > 0.0)) {
-        if (fail < 0) fail = j;
-        d = __builtin_nan("");
-      }
-      if (t == 0) piv[j] = d;
-    }
-    const double inv = 1.0 / d;
-#pragma unroll
-    for (int m = 0; m < 9; ++m) {
-      const int r = own_r[m], cc = own_c[m];
-      if (r < 0) continue;
-      if (j < PNB && cc > j) L[r][cc] = fma(-L[r][j] * inv, L[cc][j], L[r][cc]);
-      if (j > 0 && cc == j - 1) {
-        const double sd = sqrt(piv[j - 1]);
-        L[r][cc] = (r == cc) ? sd : L[r][cc] / sd;
-      }
-    }
-  }
-  __syncthreads();
-  double* Lc = Ld + (size_t)c * sLd;
-  for (int e = t; e < kb * PNB; e += 256) {
-    int r = e / PNB, cc = e % PNB;
-    Lc[(size_t)(k0 + r) * PNB + cc] = (cc <= r && cc < kb) ? L[r][cc] : 0.0;
-  }
-  if (t == 0 && fail >= 0 && fail < kb && info) atomicCAS(&info[c], 0, info_base + k0 + fail + 1);
-  // in-place inverse, columns from the last: L[j][j] <- 1/L[j][j];
-  // L[j+1:, j] <- -L[j][j]^-1 * Linv[j+1:, j+1:] L[j+1:, j]
-  for (int j = PNB - 1; j >= 0; --j) {
-    __syncthreads();
-    double x = 0.0;
-    const int i = j + 1 + t;
-    if (t < PNB - 1 - j) {
-      for (int m = j + 1; m <= i; ++m) x = fma(L[i][m], L[m][j], x);
-    }
-    const double ajj = L[j][j];
-    __syncthreads();
-    if (t < PNB - 1 - j) L[i][j] = -x / ajj;
-    if (t == 0) L[j][j] = 1.0 / ajj;
-  }
-  __syncthreads();
-  double* Lic = Li + (size_t)c * PNB * PNB;
-  for (int e = t; e < PNB * PNB; e += 256) {
-    int m = e / PNB, jj = e % PNB;  // Li[m][jj] = Linv[jj][m]
-    Lic[e] = (m <= jj) ? L[jj][m] : 0.0;
-  }
-}
+// The solve is a substitution, not a product with an explicit inverse of L: on
+// cells whose equilibrated Gram matrix has condition 1e12 the explicit inverse
+// makes K up to 20x less accurate than the substitution does.
+constexpr int PNB = 64;
 
 // lane `k`'s value of a double, in every lane (k wave-uniform), through the
 // LDS crossbar (ds_bpermute: no LDS allocation, result in vector registers)
@@ -837,38 +755,40 @@ __device__ inline double wave_bcast(double v, int k) {
   return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Diagonal 64x64 block of one cell per wavefront (4 cells per workgroup), in
-// registers: lane i holds row i of the block. Right-looking Cholesky with the
-// pivot and the scaled column broadcast by readlane (no LDS, no barriers), then
-// the inverse row by row (row p of L^-1 is final once rows < p have been folded
-// in): writes the factor to Ld[(k0+r)*64 + c] and Li[m][j] = Linv[j][m].
+// Diagonal block (kb <= NB <= 64 columns of a 64-column panel) of one cell per
+// wavefront (4 cells per workgroup), in registers: lane i holds row i of the
+// block. Right-looking Cholesky with the pivot and the scaled column broadcast
+// across lanes (no LDS, no barriers). Writes the factor to Ld[(k0+r)*64 + c]
+// and, for the panel solve, Lt[j][m] = L[m][j] (m > j), Lt[j][j] = 1 / L[j][j],
+// zero outside the block. NB is the smallest of 8, 16, 32, 64 that holds kb,
+// so short last panels cost little.
 // A non-positive pivot sets info to its 1-based column (first failure only) and
 // turns the rest of the cell's factor into NaN, as LAPACK leaves it unusable.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void potrf_wave(const double* __restrict__ A, int64_t lda, int64_t sA, int k0,
-                                                  int kb, double* __restrict__ Ld, int64_t sLd,
-                                                  double* __restrict__ Li, int32_t* __restrict__ info,
-                                                  int info_base, int ncells) {
+template <int NB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void potrf_wave(
+    const double* __restrict__ A, int64_t lda, int64_t sA, int k0, int kb, double* __restrict__ Ld, int64_t sLd,
+    double* __restrict__ Lt, int32_t* __restrict__ info, int info_base, int ncells) {
   const int lane = threadIdx.x & 63;
   const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (c >= ncells) return;  // whole wavefronts only: no barrier below
   const double* Ar = A + (size_t)c * sA + (size_t)(k0 + lane) + (size_t)k0 * lda;  // row `lane`, column k0
-  double a[PNB];
-  int ll, lf, li, ls;  // see below
+  double a[NB];
+  int ll, lf, ls;  // see below
   asm volatile("v_mov_b32 %0, %1" : "=v"(ll) : "v"(lane));
 #pragma unroll
-  for (int cc = 0; cc < PNB; ++cc) {
+  for (int cc = 0; cc < NB; ++cc) {
     double v = (ll == cc) ? 1.0 : 0.0;
     if (ll < kb && cc < kb && cc <= ll) v = *Ar;
     a[cc] = v;
     Ar += lda;
   }
-  // per-phase opaque copies of the lane index (ll, lf, li, ls): keep the compiler
-  // from sharing the 64 per-column lane masks across phases, which would hold
-  // them all live in scalar registers
+  // per-phase opaque copies of the lane index (ll, lf, ls): keep the compiler
+  // from sharing the per-column lane masks across phases, which would hold them
+  // all live in scalar registers
   asm volatile("v_mov_b32 %0, %1" : "=v"(lf) : "v"(lane));
   int fail = -1;
 #pragma unroll
-  for (int j = 0; j < PNB; ++j) {
+  for (int j = 0; j < NB; ++j) {
     __builtin_amdgcn_sched_barrier(0);
     double d = wave_bcast(a[j], j);
     const bool bad = !(d > 0.0);
@@ -878,7 +798,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void p
     const double lij = lf > j ? a[j] * rs : 0.0;
     a[j] = lf == j ? sd : (lf > j ? lij : a[j]);
 #pragma unroll
-    for (int k = j + 1; k < PNB; ++k) {
+    for (int k = j + 1; k < NB; ++k) {
       a[k] = fma(-lij, wave_bcast(a[j], k), a[k]);
       // materialise the update now: left to itself the compiler sinks it to a[k]'s
       // next use and keeps every broadcast value live in scalar registers meanwhile
@@ -889,82 +809,79 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void p
   asm volatile("v_mov_b32 %0, %1" : "=v"(ls) : "v"(lane));
   if (ls < kb) {
 #pragma unroll
-    for (int cc = 0; cc < PNB; ++cc) Lc[(size_t)(k0 + ls) * PNB + cc] = (cc <= ls && cc < kb) ? a[cc] : 0.0;
+    for (int cc = 0; cc < PNB; ++cc) {  // cc is a constant after unrolling: a[] stays in registers
+      if constexpr (NB == PNB) Lc[(size_t)(k0 + ls) * PNB + cc] = (cc <= ls && cc < kb) ? a[cc < NB ? cc : 0] : 0.0;
+      else Lc[(size_t)(k0 + ls) * PNB + cc] = (cc < NB && cc <= ls && cc < kb) ? a[cc < NB ? cc : 0] : 0.0;
+    }
   }
   if (lane == 0 && fail >= 0 && fail < kb && info) atomicCAS(&info[c], 0, info_base + k0 + fail + 1);
-  // inverse in place, row by row: before step p, a[q] (q < p) holds lane's
-  // accumulator sum_{m<p} L[lane][m] Linv[m][q] (or its final Linv row if lane < p)
-  // and a[q] (q >= p) still holds L[lane][q]. Step p finalises row p in lane p,
-  // Linv[p][q] = -acc[q] / L_pp, Linv[p][p] = 1 / L_pp, and folds it into the
-  // lanes below: acc[q] += L[lane][p] Linv[p][q].
-  asm volatile("v_mov_b32 %0, %1" : "=v"(li) : "v"(lane));
+  // Lt, column j of L contiguous (lane = row): coalesced stores; the diagonal
+  // entry, 1 / L[lane][lane], is written last
+  double* Ltc = Lt + (size_t)c * PNB * PNB + lane;
+  int lt;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(lt) : "v"(lane));
+  const bool row_in = lt < kb;
+  double dl = 1.0;
 #pragma unroll
-  for (int p = 0; p < PNB; ++p) {
+  for (int jj = 0; jj < PNB; ++jj) {
     __builtin_amdgcn_sched_barrier(0);
-    const double ip = 1.0 / wave_bcast(a[p], p);
-    const double lp = li > p ? a[p] : 0.0;
-    const bool own = li == p;
-#pragma unroll
-    for (int q = 0; q < p; ++q) a[q] = own ? -a[q] * ip : a[q];
-    a[p] = own ? ip : 0.0;
-#pragma unroll
-    for (int q = 0; q <= p; ++q) {
-      a[q] = fma(lp, wave_bcast(a[q], p), a[q]);
-      asm volatile("" : "+v"(a[q]));
+    double v = 0.0;
+    if constexpr (NB == PNB) {
+      v = (row_in && jj < kb && lt > jj) ? a[jj] : 0.0;
+      dl = lt == jj ? a[jj] : dl;
+    } else if (jj < NB) {
+      v = (row_in && jj < kb && lt > jj) ? a[jj < NB ? jj : 0] : 0.0;
+      dl = lt == jj ? a[jj < NB ? jj : 0] : dl;
     }
+    Ltc[(size_t)jj * PNB] = v;
   }
-  double* Lic = Li + (size_t)c * PNB * PNB + lane;
-  asm volatile("v_mov_b32 %0, %1" : "=v"(ls) : "v"(lane));
-#pragma unroll
-  for (int m = 0; m < PNB; ++m) Lic[(size_t)m * PNB] = (m <= ls) ? a[m] : 0.0;
+  if (row_in) Ltc[(size_t)lt * PNB] = 1.0 / dl;
 }
 
-__global__ __launch_bounds__(256) void panel_trsm(double* __restrict__ A, int64_t lda, int64_t sA, int D, int k0,
-                                                  int kb, const double* __restrict__ Li) {
-  __shared__ double Ys[PNB][PNB + 4];
-  const int chunk = blockIdx.x, c = blockIdx.y, t = threadIdx.x;
-  double* Ac = A + (size_t)c * sA;
-  const double* Lic = Li + (size_t)c * PNB * PNB;
-  for (int e = t; e < PNB * PNB; e += 256) Ys[e / PNB][e % PNB] = Lic[e];
-  __syncthreads();
-  const int lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
-  const int rbeg = k0 + kb + chunk * PCH;
-  const int rend = min(D, rbeg + PCH);
-  for (int r0 = rbeg; r0 < rend; r0 += PNB) {
-    d4 acc[2][2];
+// one column of the substitution: y_J *= 1/L_JJ, y_m -= L_mJ y_J (m > J)
+template <int NB, int J>
+__device__ __forceinline__ void trsm_column(double (&y)[NB], const double* __restrict__ L) {
+  y[J] *= L[J * PNB + J];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int bb = 0; bb < 2; ++bb) acc[a][bb] = d4{0.0, 0.0, 0.0, 0.0};
-    const int ra = r0 + wr * 32 + (lane & 15), rb = ra + 16;
-#pragma unroll 4
-    for (int k4 = 0; k4 < PNB; k4 += 4) {
-      const int m = k4 + (lane >> 4);
-      // X(r, m) straight from HBM: 16 consecutive rows per 16 lanes
-      double xa[2], yb[2];
-      xa[0] = (m < kb && ra < rend) ? Ac[(size_t)ra + (size_t)(k0 + m) * lda] : 0.0;
-      xa[1] = (m < kb && rb < rend) ? Ac[(size_t)rb + (size_t)(k0 + m) * lda] : 0.0;
-      yb[0] = Ys[m][wc * 32 + (lane & 15)];
-      yb[1] = Ys[m][wc * 32 + 16 + (lane & 15)];
-#pragma unroll
-      for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-        for (int bi = 0; bi < 2; ++bi)
-          acc[bj][bi] = __builtin_amdgcn_mfma_f64_16x16x4f64(yb[bj], xa[bi], acc[bj][bi], 0, 0, 0);
+  for (int m = J + 1; m < NB; ++m) y[m] = fma(-L[J * PNB + m], y[J], y[m]);
+}
+// columns J, J+1, ... while below kb
+template <int NB, int J>
+__device__ __forceinline__ void trsm_columns(double (&y)[NB], const double* __restrict__ L, int kb) {
+  if constexpr (J < NB) {
+    if (J < kb) {
+      __builtin_amdgcn_sched_barrier(0);
+      trsm_column<NB, J>(y, L);
+      trsm_columns<NB, J + 1>(y, L, kb);
     }
-    // all waves read their rows of this tile before any wave overwrites them
-    __syncthreads();
-#pragma unroll
-    for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-      for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          int r = r0 + wr * 32 + bi * 16 + (lane & 15);
-          int j = wc * 32 + bj * 16 + (lane >> 4) + 4 * v;
-          if (r < rend && j < kb) Ac[(size_t)r + (size_t)(k0 + j) * lda] = acc[bj][bi][v];
-        }
   }
+}
+
+// X <- X L^-T for rows [k0+kb, D) of columns [k0, k0+kb), one thread per row:
+// for j = 0..kb-1, y_j *= 1/L_jj, then y_m -= L_mj y_j (m > j) -- the column
+// order of the reference BLAS DTRSM (side R, lower, transposed). L is the same
+// for every thread of a workgroup (one cell), so its values are scalar operands.
+template <int NB>
+__global__ __launch_bounds__(256) void panel_trsm(double* __restrict__ A, int64_t lda, int64_t sA, int D, int k0,
+                                                  int kb, const double* __restrict__ Lt) {
+  const int c = blockIdx.y;
+  const int r = k0 + kb + blockIdx.x * 256 + threadIdx.x;
+  if (r >= D) return;  // no barrier below
+  const double* __restrict__ L = Lt + (size_t)c * PNB * PNB;
+  double* Ar = A + (size_t)c * sA + (size_t)r + (size_t)k0 * lda;
+  // per-phase opaque copies of kb: otherwise the compiler keeps the 64 masks
+  // j < kb of the loads live in scalar registers until the stores
+  int kl, ks, kw;
+  asm volatile("s_mov_b32 %0, %1" : "=s"(kl) : "s"(kb));
+  double y[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) y[j] = j < kl ? Ar[(size_t)j * lda] : 0.0;
+  asm volatile("s_mov_b32 %0, %1" : "=s"(ks) : "s"(kb));
+  trsm_columns<NB, 0>(y, L, ks);
+  asm volatile("s_mov_b32 %0, %1" : "=s"(kw) : "s"(kb));
+#pragma unroll
+  for (int j = 0; j < NB; ++j)
+    if (j < kw) Ar[(size_t)j * lda] = y[j];
 }
 
 // ---------------------------------------------------------- small kernels
@@ -1335,14 +1252,21 @@ void launch_gemm_mixed(GemmArgs g, int precision, int role, hipStream_t st) {
   check_launch("gemm_mixed");
 }
 
-void launch_panel(double* A, int64_t lda, int64_t sA, int D, int k0, int kb, double* Ld, int64_t sLd, double* Li,
+void launch_panel(double* A, int64_t lda, int64_t sA, int D, int k0, int kb, double* Ld, int64_t sLd, double* Lt,
                   int ncells, int32_t* info, int info_base, hipStream_t st) {
-  potrf_wave<<<dim3((ncells + 3) / 4), 256, 0, st>>>(A, lda, sA, k0, kb, Ld, sLd, Li, info, info_base, ncells);
+  const dim3 grid((ncells + 3) / 4);
+  if (kb <= 8) potrf_wave<8><<<grid, 256, 0, st>>>(A, lda, sA, k0, kb, Ld, sLd, Lt, info, info_base, ncells);
+  else if (kb <= 16) potrf_wave<16><<<grid, 256, 0, st>>>(A, lda, sA, k0, kb, Ld, sLd, Lt, info, info_base, ncells);
+  else if (kb <= 32) potrf_wave<32><<<grid, 256, 0, st>>>(A, lda, sA, k0, kb, Ld, sLd, Lt, info, info_base, ncells);
+  else potrf_wave<64><<<grid, 256, 0, st>>>(A, lda, sA, k0, kb, Ld, sLd, Lt, info, info_base, ncells);
   check_launch("potrf_wave");
   int rows = D - k0 - kb;
   if (rows <= 0) return;
-  int chunks = (rows + PCH - 1) / PCH;
-  panel_trsm<<<dim3(chunks, ncells), 256, 0, st>>>(A, lda, sA, D, k0, kb, Li);
+  const dim3 tg((rows + 255) / 256, ncells);
+  if (kb <= 8) panel_trsm<8><<<tg, 256, 0, st>>>(A, lda, sA, D, k0, kb, Lt);
+  else if (kb <= 16) panel_trsm<16><<<tg, 256, 0, st>>>(A, lda, sA, D, k0, kb, Lt);
+  else if (kb <= 32) panel_trsm<32><<<tg, 256, 0, st>>>(A, lda, sA, D, k0, kb, Lt);
+  else panel_trsm<64><<<tg, 256, 0, st>>>(A, lda, sA, D, k0, kb, Lt);
   check_launch("panel_trsm");
 }
 

@@ -66,8 +66,9 @@ void launch_gram_sf(int dim, const SfBlockPair* bps, int nbp, const SfPair* pair
 // hi*hi + hi*lo + lo*hi), FP32 MFMA accumulation, FP64 epilogue
 void launch_gemm_mixed(GemmArgs g, int precision, int role, hipStream_t st);
 // Ld: per-cell (rows, 64) store of the factored diagonal blocks (row k0 + r, column r');
-// Li: per-cell 64x64 scratch for the inverse of the current diagonal block
-void launch_panel(double* A, int64_t lda, int64_t sA, int D, int k0, int kb, double* Ld, int64_t sLd, double* Li,
+// Lt: per-cell 64x64 scratch for the current diagonal block, column-major, with
+// the reciprocals of its diagonal
+void launch_panel(double* A, int64_t lda, int64_t sA, int D, int k0, int kb, double* Ld, int64_t sLd, double* Lt,
                   int ncells, int32_t* info, int info_base, hipStream_t st);
 void launch_zero_trailing(double* A, int64_t lda, int64_t sA, int r0, int D, int ncells, hipStream_t st);
 void launch_scaling(double* A, int64_t lda, int64_t sA, int r0, int D, int n, int ncol, double* s, int64_t ss,

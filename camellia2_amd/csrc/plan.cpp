@@ -517,7 +517,7 @@ void build_bucket(dpg_plan* plan, const dpg_bucket_desc& desc, BucketPlan& B) {
   B.nWside = (int64_t)S * L.Qs * B.prog.n_sb * nvar * B.npr;
   B.nS = round_up(std::max(L.N, L.M), 8);
   B.nLd = (int64_t)round_up(L.N + L.F, 64) * 64;  // factored diagonal blocks of both stages
-  B.nLi = 64 * 64;                                 // inverse of the current diagonal block
+  B.nLi = 64 * 64;                                 // current diagonal block, column-major
   B.nE = round_up(L.N, 8);                          // adjoint: e = G^-1 r
   B.nP = (int64_t)(L.QG + L.Q + S * L.Qs) * std::max(1, f.n_coefs);  // adjoint: per-point partials
   int64_t tot = B.nA + B.nAt + B.nTv + B.nTs + B.nWip + B.nWvol + B.nWrhs + B.nWside + B.nS + B.nLd + B.nLi +

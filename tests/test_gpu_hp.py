@@ -23,10 +23,22 @@ def c3_case(n0, levels, tri_cols=1):
     return f, hp_groups(cells), load
 
 
-def check_types_against_oracle(f, groups, load, want_schur=True):
+def rounding_copies(nodes, n, seed):
+    """n copies of the cells with every vertex moved by up to 2^-51 times the
+    cell diameter: inputs that differ at the rounding level of the geometry"""
+    diam = np.max(np.linalg.norm(nodes - nodes[:, :1], axis=2), axis=1)
+    rng = np.random.default_rng(seed)
+    return [nodes + 2.0 ** -51 * diam[:, None, None] * rng.uniform(-1.0, 1.0, nodes.shape) for _ in range(n)]
+
+
+def check_types_against_oracle(f, groups, load, want_schur=True, copies=3):
     """each element type vs the oracle: K, rhs (and the Schur complement) within
-    max(1e-12, 4 x floor) of the 80-bit evaluation, floor = the oracle's own
-    FP64 distance from it"""
+    max(1e-12, 4 x floor) of the 80-bit evaluation (G, B, l summed and solved in
+    80-bit from the oracle's FP64 point values). floor = the largest distance
+    from it of the oracle's FP64 outputs on the cells and on `copies` copies
+    moved at the rounding level of their geometry: the GPU rounds its point
+    values (Jacobians, weights) independently of the oracle, and on ill
+    conditioned cells that difference alone moves K by the same order."""
     A = dpg.LocalAssembler(f, [g["bucket"] for g in groups])
     worst = 0.0
     for bi, g in enumerate(groups):
@@ -41,8 +53,11 @@ def check_types_against_oracle(f, groups, load, want_schur=True):
         # triangles: the oracle's test basis of the library's family (orthonormal,
         # centroid RT complement), so that both round G and B alike; its default
         # basis spans the same spaces, which the K agreement below checks
+        want = ("K", "rhs") + (("S", "g") if want_schur else ())
         with oracle.tri_test_basis(1 if tri else 0):
             o, x = oracle.with_extended(*args, schur=want_schur)
+            moved = [oracle.assemble(args[0], args[1], nd, *args[3:], want=want)
+                     for nd in rounding_copies(g["nodes"], copies, 7 + bi)]
         if tri:
             o0 = oracle.assemble(*args, want=("K",))
             eb = rel_fro(o0["K"], o["K"])
@@ -50,7 +65,7 @@ def check_types_against_oracle(f, groups, load, want_schur=True):
             assert eb < 1e-8
         pairs = [("K", "K"), ("rhs", "rhs")] + ([("schur", "S"), ("schur_rhs", "g")] if want_schur else [])
         for gk, ok in pairs:
-            floor = rel_fro(o[ok], x[ok])
+            floor = max([rel_fro(o[ok], x[ok])] + [rel_fro(m[ok], x[ok]) for m in moved])
             err = rel_fro(res[gk], x[ok])
             print(f"  {gk:10s} gpu-vs-80bit {err:.2e} oracle-vs-80bit {floor:.2e} gpu-vs-oracle "
                   f"{rel_fro(res[gk], o[ok]):.2e}")
@@ -66,6 +81,20 @@ def test_c3_every_element_type_matches_oracle():
     f, groups, load = c3_case(3, 3)
     assert sum(g["topology"] == capi.QUAD for g in groups) == 18
     check_types_against_oracle(f, groups, load)
+
+
+def test_c3_finest_cells_of_every_type_match_oracle():
+    """The full 49,424-cell graded mesh (10 levels): the two smallest cells of
+    every element type (h down to 5.7e-5 with eps down to 1e-4, where the
+    oracle's own FP64 distance from the 80-bit evaluation of K reaches 1e-4)."""
+    f, groups, load = c3_case(16, 10, tri_cols=4)
+    sel = []
+    for g in groups:
+        nd = g["nodes"]
+        e1, e2 = nd[:, 1] - nd[:, 0], nd[:, -1] - nd[:, 0]
+        idx = np.sort(np.argsort(np.abs(e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0]))[:2])
+        sel.append(dict(g, nodes=nd[idx], parities=g["parities"][idx], coefs=g["coefs"][idx], ncells=len(idx)))
+    check_types_against_oracle(f, sel, load)
 
 
 @pytest.mark.parametrize("form", ["cd", "poisson"])
