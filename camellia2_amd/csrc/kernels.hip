@@ -346,10 +346,15 @@ __device__ inline void tile_coords(int tile, int lower, int tiles_m, int& ti, in
 // TT x TT output tile per workgroup of (TT/32)^2 waves; each wave owns a 32x32
 // sub-tile (2x2 MFMA blocks). TT = 64 for large operands, TT = 32 for buckets
 // whose dimensions would leave most of a 64-tile empty.
+// k runs in chunks of GKC rows staged through LDS; each thread stages one fixed
+// column of X and of Y (rows t/TT + r*NT/TT of a chunk), from a clamped,
+// always valid address, zeroed outside the matrix (no branches per element).
+// Global loads run two chunks ahead of the MFMAs (two register sets).
 template <int TT>
 __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
   constexpr int NT = (TT / 32) * (TT / 32) * 64;  // threads
   constexpr int PER = GKC * TT / NT;              // staged elements per thread and operand
+  constexpr int RS = NT / TT;                     // k-row step between a thread's elements
   constexpr int LD = TT + 4;
   int b = xcd_remap(blockIdx.x, gridDim.x);
   int cell = b / g.ntiles, tile = b % g.ntiles;
@@ -357,8 +362,6 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
   int ti, tj;
   tile_coords(tile, g.lower, g.tiles_m, ti, tj);
   const int i0 = ti * TT, j0 = tj * TT;
-  const double* __restrict__ X = g.X + (size_t)cell * g.sx;
-  const double* __restrict__ Y = g.Y + (size_t)cell * g.sy;
   double* __restrict__ C = g.C + (size_t)cell * g.sc;
   __shared__ double Xs[GKC][LD];
   __shared__ double Ys[GKC][LD];
@@ -368,38 +371,64 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int bb = 0; bb < 2; ++bb) acc[a][bb] = d4{0.0, 0.0, 0.0, 0.0};
-  const int kbeg = 0, kend = g.k;
-  // k-chunks of segments that are zero for this tile are skipped
+  const int kend = g.k;
+  const int kk0 = t / TT, ii = t % TT;
+  const bool vx = i0 + ii < g.m, vy = j0 + ii < g.n;
+  const double* __restrict__ xp = g.X + (size_t)cell * g.sx + (vx ? i0 + ii : 0) + (size_t)kk0 * g.ldx;
+  const double* __restrict__ yp = g.Y + (size_t)cell * g.sy + (vy ? j0 + ii : 0) + (size_t)kk0 * g.ldy;
+  // k-chunks of segments that are zero for this tile are skipped: walk the
+  // segments [s*kseg, (s+1)*kseg) with their mask bits (no division)
   const uint64_t km = g.kmask ? g.kmask[tile] : ~0ull;
-  auto next_chunk = [&](int k0) {
-    if (g.kmask)
-      while (k0 < kend && !((km >> (k0 / g.kseg)) & 1ull)) k0 += GKC;
+  const int kseg = g.kmask ? (int)g.kseg : kend;
+  int seg = 0, segend = kseg;
+  auto skip_zero_segments = [&](int k0) {
+    while (k0 < kend && !((km >> seg) & 1ull)) {
+      k0 = segend;
+      ++seg;
+      segend += kseg;
+    }
     return k0;
   };
-  // each thread stages PER elements of X and of Y per k-chunk; the next chunk
-  // is fetched into registers while the current one is multiplied from LDS
-  double xr[PER], yr[PER];
-  auto fetch = [&](int k0) {
+  auto next_chunk = [&](int k0) {
+    if (k0 >= kend) return k0;
+    k0 += GKC;
+    if (k0 >= segend) {
+      ++seg;
+      segend += kseg;
+      k0 = skip_zero_segments(k0);
+    }
+    return k0;
+  };
+  // raw loads only: the zeroing happens when the chunk is staged, so that no
+  // wait for the loads is needed before then
+  const int ldx = (int)g.ldx, ldy = (int)g.ldy;  // per-cell offsets fit in 32 bits
+  auto fetch = [&](double (&xr)[PER], double (&yr)[PER], int k0) {
+    if (k0 + GKC <= kend) {
 #pragma unroll
-    for (int r = 0; r < PER; ++r) {
-      int e = t + NT * r, kk = e / TT, ii = e % TT, kg = k0 + kk;
-      xr[r] = (kg < kend && i0 + ii < g.m) ? X[(size_t)kg * g.ldx + i0 + ii] : 0.0;
-      yr[r] = (kg < kend && j0 + ii < g.n) ? Y[(size_t)kg * g.ldy + j0 + ii] : 0.0;
+      for (int r = 0; r < PER; ++r) {
+        xr[r] = xp[(k0 + r * RS) * ldx];
+        yr[r] = yp[(k0 + r * RS) * ldy];
+      }
+    } else {  // last, partial chunk (k not a multiple of GKC): clamp the row
+#pragma unroll
+      for (int r = 0; r < PER; ++r) {
+        const int kr = min(k0 + kk0 + r * RS, kend - 1) - kk0;
+        xr[r] = xp[kr * ldx];
+        yr[r] = yp[kr * ldy];
+      }
     }
   };
-  int k0 = next_chunk(kbeg);
-  if (k0 < kend) fetch(k0);
-  while (k0 < kend) {
-    __syncthreads();
+  auto stage = [&](const double (&xr)[PER], const double (&yr)[PER], int k0) {
+    __syncthreads();  // every wave is done reading the previous chunk
 #pragma unroll
     for (int r = 0; r < PER; ++r) {
-      int e = t + NT * r;
-      Xs[e / TT][e % TT] = xr[r];
-      Ys[e / TT][e % TT] = yr[r];
+      const bool vk = k0 + kk0 + r * RS < kend;
+      Xs[kk0 + r * RS][ii] = (vx && vk) ? xr[r] : 0.0;
+      Ys[kk0 + r * RS][ii] = (vy && vk) ? yr[r] : 0.0;
     }
     __syncthreads();
-    const int kn = next_chunk(k0 + GKC);
-    if (kn < kend) fetch(kn);
+  };
+  auto multiply = [&]() {
 #pragma unroll
     for (int k4 = 0; k4 < GKC; k4 += 4) {
       const int kr = k4 + (lane >> 4), cc = lane & 15;
@@ -414,7 +443,28 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
         for (int bi = 0; bi < 2; ++bi)
           acc[bj][bi] = __builtin_amdgcn_mfma_f64_16x16x4f64(yb[bj], xa[bi], acc[bj][bi], 0, 0, 0);
     }
-    k0 = kn;
+  };
+  double xa_[PER], ya_[PER], xb_[PER], yb_[PER];
+  int kc = skip_zero_segments(0);  // chunk to multiply next
+  int kn = next_chunk(kc);         // the one after
+  if (kc < kend) fetch(xa_, ya_, kc);
+  if (kn < kend) fetch(xb_, yb_, kn);
+  // two-chunk software pipeline, unrolled by two so that the register sets
+  // alternate without copies
+  while (kc < kend) {
+    stage(xa_, ya_, kc);
+    int kf = next_chunk(kn);
+    if (kf < kend) fetch(xa_, ya_, kf);
+    multiply();
+    kc = kn;
+    kn = kf;
+    if (kc >= kend) break;
+    stage(xb_, yb_, kc);
+    kf = next_chunk(kn);
+    if (kf < kend) fetch(xb_, yb_, kf);
+    multiply();
+    kc = kn;
+    kn = kf;
   }
   // v_mfma_f64_16x16x4 result layout: D[a][b] in lane b + 16*(a%4), element a/4,
   // with a the A-operand (Y, C column j) and b the B-operand (X, C row i) index
