@@ -15,6 +15,7 @@ sample gives the error of the GPU result against the FP64 oracle.
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -53,63 +54,109 @@ def build_inputs(cfg):
     return f, b, nodes, par, load
 
 
-def cpu_baseline(f, b, nodes, par, load, gpu_K, target_s=12.0):
-    """Oracle on a bounded sample of the workload's cells (all host threads),
-    plus the GPU-vs-oracle error on that sample."""
+ORACLE_BUILD = "g++ -O3 -march=x86-64-v3 -fopenmp, textbook loops (no BLAS/LAPACK on the host)"
+
+
+def host_info():
+    """nproc and the lscpu topology of the host the CPU baseline ran on."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+    except (OSError, subprocess.SubprocessError):
+        return info
+    keys = (("Model name", "model"), ("Socket(s)", "sockets"), ("Core(s) per socket", "cores_per_socket"),
+            ("Thread(s) per core", "threads_per_core"))
+    for line in out.splitlines():
+        for key, name in keys:
+            if line.startswith(key + ":"):
+                info[name] = line.split(":", 1)[1].strip()
+    return info
+
+
+def cpu_baseline(f, b, nodes, par, load, gpu_K, target_s=2.5, reps=5):
+    """Oracle on a bounded sample of the workload's cells: the median of `reps`
+    runs on all host threads (about target_s each) and one run on one thread
+    (about target_s), plus the GPU-vs-oracle error on the sample."""
     from oracle import oracle
 
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     ff = oracle.form_like(f)
+    want = ("K", "rhs", "S", "g")
     # calibrate on one cell per thread, then size the sample to ~target_s
     n0 = max(1, threads)
     t0 = time.perf_counter()
-    oracle.assemble(ff, b, nodes[:n0], par[:n0], None, load, want=("K", "rhs", "S", "g"), nthreads=threads)
-    dt0 = time.perf_counter() - t0
-    per_cell = dt0 / n0
+    oracle.assemble(ff, b, nodes[:n0], par[:n0], None, load, want=want, nthreads=threads)
+    per_cell = (time.perf_counter() - t0) / n0
     n = int(min(nodes.shape[0], max(n0, target_s / max(per_cell, 1e-9))))
     n = max(n0, (n // threads) * threads)
+    times, o = [], None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        o = oracle.assemble(ff, b, nodes[:n], par[:n], None, load, want=want, nthreads=threads)
+        times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
+    n1 = int(max(1, min(n, target_s / max(per_cell * threads, 1e-9))))
     t0 = time.perf_counter()
-    o = oracle.assemble(ff, b, nodes[:n], par[:n], None, load, want=("K", "rhs", "S", "g"), nthreads=threads)
-    dt = time.perf_counter() - t0
+    oracle.assemble(ff, b, nodes[:n1], par[:n1], None, load, want=want, nthreads=1)
+    dt1 = time.perf_counter() - t0
     Kg = gpu_K[:n].cpu().numpy()
     err = float(np.max(np.linalg.norm((Kg - o["K"]).reshape(n, -1), axis=1) /
                        np.linalg.norm(o["K"].reshape(n, -1), axis=1)))
     return {"value": n / dt, "unit": "cells/s", "cores": threads, "kind": "port",
-            "sample": f"first {n} cells of the same mesh, oracle K+rhs+Schur, {dt:.1f} s wall, {threads} OpenMP threads"}, err
+            "sample": f"first {n} cells of the same mesh, oracle K+rhs+Schur, median of {reps} runs "
+                      f"({dt:.2f} s wall each), {threads} OpenMP threads",
+            "one_core_value": n1 / dt1, "one_core_sample": f"first {n1} cells, 1 thread, {dt1:.2f} s",
+            "build": ORACLE_BUILD, "host": host_info()}, err
 
 
-def cpu_baseline_hp(f, groups, load, gpu_K, target_s=12.0):
+def cpu_baseline_hp(f, groups, load, gpu_K, target_s=2.5, reps=5):
     """Oracle on a stratified sample of the hp mesh (the same share of every
-    element type), plus the GPU-vs-oracle error of K on that sample and, for
-    each type's worst cell, the GPU's and the oracle's distance from K summed
-    and solved in 80-bit (the oracle's is what FP64 reaches on that cell)."""
+    element type): the median of `reps` runs on all host threads (about
+    target_s each) and one run on one thread; plus the GPU-vs-oracle error of K
+    on that sample and, for each type's worst cell, the GPU's and the oracle's
+    distance from K summed and solved in 80-bit (the oracle's is what FP64
+    reaches on that cell)."""
     from oracle import oracle
 
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     ff = oracle.form_like(f)
+    want = ("K", "rhs", "S", "g")
     per = []
     for g in groups:
         n = min(threads, g["ncells"])
         t0 = time.perf_counter()
         oracle.assemble(ff, g["bucket"], g["nodes"][:n], g["parities"][:n], g["coefs"][:n], load,
-                        want=("K", "rhs", "S", "g"), nthreads=threads)
+                        want=want, nthreads=threads)
         per.append((time.perf_counter() - t0) / n)
     full = sum(t * g["ncells"] for t, g in zip(per, groups))
-    frac = min(1.0, target_s / full)
-    n_tot, t_tot, err = 0, 0.0, 0.0
-    worst = {"gpu_vs_80bit": 0.0, "oracle_vs_80bit": 0.0, "ratio_to_floor": 0.0}
-    for bi, g in enumerate(groups):
-        n = min(g["ncells"], max(1, int(round(frac * g["ncells"]))))
+
+    def sample(frac):
+        return [min(g["ncells"], max(1, int(round(frac * g["ncells"])))) for g in groups]
+
+    def run(ns, nthreads):
+        outs = []
         t0 = time.perf_counter()
-        o = oracle.assemble(ff, g["bucket"], g["nodes"][:n], g["parities"][:n], g["coefs"][:n], load,
-                            want=("K", "rhs", "S", "g"), nthreads=threads)
-        t_tot += time.perf_counter() - t0
-        n_tot += n
+        for g, n in zip(groups, ns):
+            outs.append(oracle.assemble(ff, g["bucket"], g["nodes"][:n], g["parities"][:n], g["coefs"][:n], load,
+                                        want=want, nthreads=nthreads))
+        return time.perf_counter() - t0, outs
+
+    frac = min(1.0, target_s / full)
+    ns = sample(frac)
+    times, outs = [], None
+    for _ in range(reps):
+        dt_, outs = run(ns, threads)
+        times.append(dt_)
+    dt = float(np.median(times))
+    frac1 = min(1.0, target_s / (full * threads))
+    ns1 = sample(frac1)
+    dt1, _ = run(ns1, 1)
+    err = 0.0
+    worst = {"gpu_vs_80bit": 0.0, "oracle_vs_80bit": 0.0, "ratio_to_floor": 0.0}
+    for bi, (g, n, o) in enumerate(zip(groups, ns, outs)):
         Kg = gpu_K[bi][:n].cpu().numpy()
         e = np.linalg.norm((Kg - o["K"]).reshape(n, -1), axis=1) / np.linalg.norm(o["K"].reshape(n, -1), axis=1)
         err = max(err, float(e.max()))
-        # the type's worst cell against K from G, B, l summed and solved in 80-bit:
-        # the oracle's own distance from it is what FP64 can reach on that cell
         c = int(np.argmax(e))
         o1, x1 = oracle.with_extended(ff, g["bucket"], g["nodes"][c:c + 1], g["parities"][c:c + 1],
                                       g["coefs"][c:c + 1], load)
@@ -119,9 +166,13 @@ def cpu_baseline_hp(f, groups, load, gpu_K, target_s=12.0):
         worst["gpu_vs_80bit"] = max(worst["gpu_vs_80bit"], eg)
         worst["oracle_vs_80bit"] = max(worst["oracle_vs_80bit"], eo)
         worst["ratio_to_floor"] = max(worst["ratio_to_floor"], eg / max(eo, 1e-16))
-    return {"value": n_tot / t_tot, "unit": "cells/s", "cores": threads, "kind": "port",
+    n_tot, n1_tot = sum(ns), sum(ns1)
+    return {"value": n_tot / dt, "unit": "cells/s", "cores": threads, "kind": "port",
             "sample": f"{n_tot} cells, the first {frac:.1%} of every element type, oracle K+rhs+Schur, "
-                      f"{t_tot:.1f} s wall, {threads} OpenMP threads"}, err, worst
+                      f"median of {reps} runs ({dt:.2f} s wall each), {threads} OpenMP threads",
+            "one_core_value": n1_tot / dt1,
+            "one_core_sample": f"{n1_tot} cells, the first {frac1:.2%} of every element type, 1 thread, {dt1:.2f} s",
+            "build": ORACLE_BUILD, "host": host_info()}, err, worst
 
 
 def bench_hp(args, cfg):
@@ -241,7 +292,33 @@ def bench_hp(args, cfg):
         result["cpu_baseline"] = cpu
         result["rel_err_K_vs_fp64_oracle"] = err
         result["worst_cell_per_type_vs_80bit"] = worst
+    attach_peaks(result, args.peaks)
     print(json.dumps(result))
+
+
+def measured_peaks():
+    """Sustained FP64 MFMA / FP64 FMA / HBM rates of this GPU from
+    camellia2_amd/lib/dpg_peaks (a child process), or None if it is not built."""
+    exe = os.path.join(ROOT, "camellia2_amd", "lib", "dpg_peaks")
+    if not os.path.exists(exe):
+        return None
+    try:
+        r = subprocess.run([exe], capture_output=True, text=True, timeout=180)
+        if r.returncode != 0:
+            return {"error": r.stderr.strip()[-300:]}
+        return json.loads(r.stdout.strip().splitlines()[-1])
+    except (OSError, subprocess.SubprocessError, ValueError, IndexError) as e:
+        return {"error": str(e)}
+
+
+def attach_peaks(result, peaks):
+    """measured peaks, and the dominant kernel's fraction of the measured FP64 MFMA rate"""
+    if peaks is None:
+        return
+    result["measured_peaks"] = peaks
+    if "fp64_mfma_tflops" in peaks and peaks["fp64_mfma_tflops"] > 0:
+        r = result["roofline"]
+        r["frac_of_measured_peak"] = round(r["achieved"] / peaks["fp64_mfma_tflops"], 4)
 
 
 def main():
@@ -259,6 +336,7 @@ def main():
     ap.add_argument("--lanes", type=int, default=4, help="hp config: concurrent streams inside the graph")
     ap.add_argument("--sumfact", action="store_true",
                     help="opt-in sum-factorised G (fewer flops, K accurate only to ~1e-12..1e-1; not the default)")
+    ap.add_argument("--no-peaks", action="store_true", help="skip the peak-rate microbenchmark")
     ap.add_argument("--calibrate-copy-gib", type=float, default=0.0,
                     help="also run a plain device copy of this size (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -267,6 +345,8 @@ def main():
               "cells with no exchange between GPUs; a multi-GPU run is one independent process per GPU)",
               file=sys.stderr)
         sys.exit(2)
+    # the microbenchmark runs as a child process before this one touches the GPU
+    args.peaks = None if args.no_peaks else measured_peaks()
 
     import torch
 
@@ -381,8 +461,11 @@ def main():
                     a, r = out["K"][c0:c0 + 1024], ref_K[c0:c0 + 1024]
                     rel.append(torch.linalg.matrix_norm(a - r) / torch.linalg.matrix_norm(r))
                 rel = torch.cat(rel)
-                entry["rel_err_K_vs_fp64_gpu_max"] = float(rel.max())
-                entry["rel_err_K_vs_fp64_gpu_median"] = float(rel.median())
+                fin = torch.isfinite(rel)
+                entry["nonfinite_cells"] = int((~fin).sum())  # low-precision G not SPD / overflow
+                rel = rel[fin]
+                entry["rel_err_K_vs_fp64_gpu_max"] = float(rel.max()) if rel.numel() else None
+                entry["rel_err_K_vs_fp64_gpu_median"] = float(rel.median()) if rel.numel() else None
             mixed[p] = entry
         if ref_K is not None:
             out["K"].copy_(ref_K)
@@ -401,6 +484,7 @@ def main():
         torch.cuda.synchronize()
         result["calibration_copy_bytes"] = n * 8
         del src, dst
+    attach_peaks(result, args.peaks)
     print(json.dumps(result))
 
 

@@ -113,7 +113,8 @@ EXPORTS = [
     "dpg_version", "dpg_last_error", "dpg_form_poisson_uw", "dpg_form_cdr_uw",
     "dpg_form_stokes_vgp_uw", "dpg_form_convection_diffusion_uw", "dpg_form_graph_norm", "dpg_mesh_quad", "dpg_mesh_hex",
     "dpg_load_sine_modes", "dpg_rng_uniform", "dpg_rng_normal", "dpg_bucket_sizes",
-    "dpg_plan_create", "dpg_plan_destroy", "dpg_plan_workspace", "dpg_plan_timings", "dpg_assemble",
+    "dpg_plan_create", "dpg_plan_destroy", "dpg_plan_clone_workspace", "dpg_plan_workspace", "dpg_plan_timings",
+    "dpg_assemble",
     "dpg_energy_error", "dpg_adjoint", "dpg_mesh_quad_graded", "dpg_plan_get_tiles", "dpg_plan_set_tiles",
     "dpg_plan_autotune", "dpg_graph_create", "dpg_graph_launch", "dpg_graph_workspace", "dpg_graph_destroy",
 ]
@@ -146,6 +147,7 @@ def lib():
         L.dpg_bucket_sizes.argtypes = [P(FormDesc), P(BucketDesc), P(BucketInfo)]
         L.dpg_plan_create.argtypes = [P(FormDesc), P(BucketDesc), ct.c_int, P(Options), P(ct.c_void_p)]
         L.dpg_plan_destroy.argtypes = [ct.c_void_p]
+        L.dpg_plan_clone_workspace.argtypes = [ct.c_void_p, ct.c_double, P(ct.c_void_p)]
         L.dpg_plan_workspace.argtypes = [ct.c_void_p, P(ct.c_size_t)]
         L.dpg_plan_timings.argtypes = [ct.c_void_p, P(Timings), ct.c_int]
         L.dpg_assemble.argtypes = [ct.c_void_p, ct.c_int, P(Batch), P(Outputs), ct.c_void_p]

@@ -467,24 +467,55 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
     kn = kf;
   }
   // v_mfma_f64_16x16x4 result layout: D[a][b] in lane b + 16*(a%4), element a/4,
-  // with a the A-operand (Y, C column j) and b the B-operand (X, C row i) index
+  // with a the A-operand (Y, C column j) and b the B-operand (X, C row i) index.
+  // This thread's rows are i0 + wr*32 + bi*16 + (lane & 15) and its columns
+  // j0 + wc*32 + bj*16 + (lane >> 4) + 4v. The epilogue runs in two halves (bj):
+  // each issues all its loads (scales, C when beta != 0) from clamped addresses
+  // before using any, one memory round trip per half instead of one per element.
+  int ir[2];
+  bool iv[2];
+  double sr[2] = {1.0, 1.0};
 #pragma unroll
-  for (int bj = 0; bj < 2; ++bj)
+  for (int bi = 0; bi < 2; ++bi) {
+    ir[bi] = i0 + wr * 32 + bi * 16 + (lane & 15);
+    iv[bi] = ir[bi] < g.m;
+    if (g.srow) sr[bi] = g.srow[(size_t)cell * g.ss + (iv[bi] ? ir[bi] : 0)];
+  }
+#pragma unroll
+  for (int bj = 0; bj < 2; ++bj) {
+    int jc[4];
+    bool jv[4];
+    double sc[4] = {1.0, 1.0, 1.0, 1.0}, cold[2][4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      jc[v] = j0 + wc * 32 + bj * 16 + (lane >> 4) + 4 * v;
+      jv[v] = jc[v] < g.n;
+      if (g.scol) sc[v] = g.scol[(size_t)cell * g.ss + (jv[v] ? jc[v] : 0)];
+    }
+    if (g.beta != 0.0) {
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          cold[bi][v] = C[(size_t)(iv[bi] ? ir[bi] : 0) + (size_t)(jv[v] ? jc[v] : 0) * g.ldc];
+    }
+    // one wait for all of them (vmcnt 0, expcnt and lgkmcnt untouched): without it
+    // the compiler re-waits in every conditional store below, stores included
+    __builtin_amdgcn_s_waitcnt(0x0F70);
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        int i = i0 + wr * 32 + bi * 16 + (lane & 15);
-        int j = j0 + wc * 32 + bj * 16 + (lane >> 4) + 4 * v;
-        if (i < g.m && j < g.n && (!g.lower || i >= j)) {
-          double* cp = C + (size_t)i + (size_t)j * g.ldc;
-          double val = g.alpha * acc[bj][bi][v];
-          if (g.srow) val *= g.srow[(size_t)cell * g.ss + i];
-          if (g.scol) val *= g.scol[(size_t)cell * g.ss + j];
-          if (g.beta != 0.0) val = fma(g.beta, *cp, val);
-          *cp = val;
+        const int i = ir[bi], j = jc[v];
+        if (iv[bi] && jv[v] && (!g.lower || i >= j)) {
+          double val = g.alpha * acc[bj][bi][v];  // (alpha acc) s_i s_j: multiplying by 1 is exact
+          val *= sr[bi];
+          val *= sc[v];
+          if (g.beta != 0.0) val = fma(g.beta, cold[bi][v], val);
+          C[(size_t)i + (size_t)j * g.ldc] = val;
         }
       }
+  }
 }
 
 // one kernel symbol per role and tile so that profiles separate them

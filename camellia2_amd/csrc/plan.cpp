@@ -221,7 +221,10 @@ struct dpg_plan {
   dpg_options opt{};
   int device = 0;
   std::vector<dpg::BucketPlan> buckets;
-  std::vector<std::unique_ptr<dpg::DeviceBuffer>> tables;
+  // device tables (bases, cubature, term programs, masks): read-only once built,
+  // shared by the handles dpg_plan_clone_workspace makes
+  std::shared_ptr<std::vector<std::unique_ptr<dpg::DeviceBuffer>>> tables =
+      std::make_shared<std::vector<std::unique_ptr<dpg::DeviceBuffer>>>();
   dpg::DeviceBuffer ws;
   dpg::DeviceBuffer info;
   dpg::Profiler prof;
@@ -431,7 +434,7 @@ void build_sum_factorisation(dpg_plan* plan, BucketPlan& B) {
     lds = std::max(lds, sizeof(double) * ((size_t)bp.npairs * L.QG + 3 * (size_t)nF * q + t1 + t2));
   }
   if (bps.empty() || lds > 64 * 1024) return;  // the dense contraction stays in use
-  auto& own = plan->tables;
+  auto& own = *plan->tables;
   B.sf_bps = upload(own, bps);
   B.sf_pairs = upload(own, pairs);
   B.sf_F = upload(own, F);
@@ -462,7 +465,7 @@ void build_bucket(dpg_plan* plan, const dpg_bucket_desc& desc, BucketPlan& B) {
   compile_program(f, B);
   build_sum_factorisation(plan, B);
   RefTables T = make_tables(f, L);
-  auto& own = plan->tables;
+  auto& own = *plan->tables;
   B.tab.R_vol = upload(own, T.vol.R);
   B.tab.R_ip = upload(own, T.ip.R);
   B.tab.R_side = upload(own, T.side.R);
@@ -506,7 +509,7 @@ void build_bucket(dpg_plan* plan, const dpg_bucket_desc& desc, BucketPlan& B) {
       for (int j = 0; j < L.N; ++j) reach[j / T] |= tv[L.var_of_test_dof[j]];
       for (int ti = 0; ti < nt; ++ti)
         for (int tj = 0; tj <= ti; ++tj) mask.push_back(reach[ti] & reach[tj]);
-      B.gmask[ts] = upload(plan->tables, mask);
+      B.gmask[ts] = upload(*plan->tables, mask);
     }
   }
   B.nTv = (int64_t)B.prog.n_vb * L.Q * B.Npad;
@@ -919,6 +922,26 @@ int dpg_plan_create(const dpg_form_desc* form, const dpg_bucket_desc* buckets, i
 
 int dpg_plan_destroy(dpg_plan* plan) {
   return dpg::guarded([&] { delete plan; });
+}
+
+int dpg_plan_clone_workspace(const dpg_plan* plan, double workspace_bytes, dpg_plan** out) {
+  return dpg::guarded([&] {
+    if (!plan || !out) throw std::invalid_argument("NULL argument");
+    *out = nullptr;
+    if (!(workspace_bytes >= 0.0)) throw std::invalid_argument("workspace_bytes must be >= 0");
+    int dev = -1;
+    HIP_CHECK(hipGetDevice(&dev));
+    if (dev != plan->device) throw std::invalid_argument("the current device is not the plan's device");
+    std::unique_ptr<dpg_plan> c(new dpg_plan());
+    c->form = plan->form;
+    c->opt = plan->opt;
+    if (workspace_bytes > 0.0) c->opt.workspace_bytes = workspace_bytes;
+    c->device = plan->device;
+    c->buckets = plan->buckets;  // device pointers into the shared tables, tile choices as they are now
+    c->tables = plan->tables;
+    c->prof.on = plan->prof.on;
+    *out = c.release();
+  });
 }
 
 int dpg_plan_workspace(const dpg_plan* plan, size_t* bytes) {

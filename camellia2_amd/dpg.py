@@ -114,6 +114,19 @@ class LocalAssembler:
         self._plan = plan
         self.info = [sizes(f, b) for b in self.buckets]
 
+    def clone(self, workspace_bytes=0.0):
+        """A second assembler on the same device tables with its own workspace
+        (dpg_plan_clone_workspace): the two may run concurrently on different streams."""
+        c = object.__new__(LocalAssembler)
+        c._torch = self._torch
+        c.form = self.form
+        c.buckets = list(self.buckets)
+        c.info = list(self.info)
+        plan = ct.c_void_p()
+        check(lib().dpg_plan_clone_workspace(self._plan, float(workspace_bytes), ct.byref(plan)))
+        c._plan = plan
+        return c
+
     def close(self):
         if self._plan:
             lib().dpg_plan_destroy(self._plan)

@@ -264,6 +264,16 @@ int dpg_bucket_sizes(const dpg_form_desc* form, const dpg_bucket_desc* bucket, d
 int dpg_plan_create(const dpg_form_desc* form, const dpg_bucket_desc* buckets, int nbuckets,
                     const dpg_options* options, dpg_plan** out);
 int dpg_plan_destroy(dpg_plan* plan);
+/* A second handle on a plan for concurrent calls: the device tables (bases,
+ * cubature, term programs, tile choices as they are at the call) are shared
+ * and read-only; the clone has its own workspace, INFO buffer and profiler, so
+ * calls on the two handles may run at the same time on different streams
+ * (calls on one handle are stream-ordered and must not overlap).
+ * workspace_bytes > 0 replaces the original's workspace budget. The tables
+ * are freed with the last handle (dpg_plan_destroy on each). The reference has
+ * no counterpart: TBF::localStiffnessMatrixAndRHS (BF.h:122-125) is single
+ * threaded and its BasisCaches memoise mutable state, so one call at a time. */
+int dpg_plan_clone_workspace(const dpg_plan* plan, double workspace_bytes, dpg_plan** out);
 /* bytes of device workspace the plan holds */
 int dpg_plan_workspace(const dpg_plan* plan, size_t* bytes);
 /* timings accumulated since the last reset (needs options.profile = 1) */
