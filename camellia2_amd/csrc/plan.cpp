@@ -986,8 +986,8 @@ int dpg_energy_error(dpg_plan* plan, int bucket, const dpg_batch* batch, const d
                      void* stream) {
   return dpg::guarded_status([&] {
     dpg::check_batch(plan, bucket, batch);
+    if (batch->ncells == 0) return 0;  // an empty batch needs no buffers
     if (!u || !eta2) throw std::invalid_argument("u and eta2 are required");
-    if (batch->ncells == 0) return 0;
     hipStream_t st = (hipStream_t)stream;
     const dpg::BucketPlan& B = plan->buckets[bucket];
     int C = batch->ncells;
@@ -1010,10 +1010,10 @@ int dpg_adjoint(dpg_plan* plan, int bucket, const dpg_batch* batch, const double
                 void* stream) {
   return dpg::guarded_status([&] {
     dpg::check_batch(plan, bucket, batch);
-    if (!u || !eta2) throw std::invalid_argument("u and eta2 are required");
     if (grad && plan->form.n_coefs <= 0) throw std::invalid_argument("the form has no per-cell coefficients");
     if (plan->opt.precision != DPG_PREC_FP64) throw std::invalid_argument("the adjoint runs in FP64 only");
-    if (batch->ncells == 0) return 0;
+    if (batch->ncells == 0) return 0;  // an empty batch needs no buffers
+    if (!u || !eta2) throw std::invalid_argument("u and eta2 are required");
     hipStream_t st = (hipStream_t)stream;
     const dpg::BucketPlan& B = plan->buckets[bucket];
     const dpg::ElementLayout& L = B.L;
