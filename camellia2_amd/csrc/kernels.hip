@@ -820,7 +820,7 @@ __global__ __launch_bounds__(256) void gemm_bmat_f16x3(GemmArgs g) { gemm_mixed_
 //                registers; the factor to Ld (row-major, for the back solve)
 //                and to Lt (column-major, with the reciprocals of its diagonal)
 //   panel_trsm   one thread per row below the block: X <- X L^-T by
-//                substitution, column by column, L read as wave-uniform scalars.
+//                substitution, column by column, L staged in LDS per cell.
 // The diagonal block of A itself is left untouched (no later stage reads it).
 // The solve is a substitution, not a product with an explicit inverse of L: on
 // cells whose equilibrated Gram matrix has condition 1e12 the explicit inverse
@@ -921,17 +921,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void p
 
 // one column of the substitution: y_J *= 1/L_JJ, y_m -= L_mJ y_J (m > J)
 template <int NB, int J>
-__device__ __forceinline__ void trsm_column(double (&y)[NB], const double* __restrict__ L) {
+__device__ __forceinline__ void trsm_column(double (&y)[NB], const double* L) {
   y[J] *= L[J * PNB + J];
 #pragma unroll
   for (int m = J + 1; m < NB; ++m) y[m] = fma(-L[J * PNB + m], y[J], y[m]);
 }
 // columns J, J+1, ... while below kb
 template <int NB, int J>
-__device__ __forceinline__ void trsm_columns(double (&y)[NB], const double* __restrict__ L, int kb) {
+__device__ __forceinline__ void trsm_columns(double (&y)[NB], const double* L, int kb) {
   if constexpr (J < NB) {
     if (J < kb) {
-      __builtin_amdgcn_sched_barrier(0);
       trsm_column<NB, J>(y, L);
       trsm_columns<NB, J + 1>(y, L, kb);
     }
@@ -940,15 +939,20 @@ __device__ __forceinline__ void trsm_columns(double (&y)[NB], const double* __re
 
 // X <- X L^-T for rows [k0+kb, D) of columns [k0, k0+kb), one thread per row:
 // for j = 0..kb-1, y_j *= 1/L_jj, then y_m -= L_mj y_j (m > j) -- the column
-// order of the reference BLAS DTRSM (side R, lower, transposed). L is the same
-// for every thread of a workgroup (one cell), so its values are scalar operands.
+// order of the reference BLAS DTRSM (side R, lower, transposed). The workgroup
+// (one cell) stages the cell's L columns in LDS once; every thread then reads
+// the same L_mj (LDS broadcast). Read through the scalar cache instead, the
+// L blocks of the cells resident on a CU do not fit and every wave waits on L2.
 template <int NB>
 __global__ __launch_bounds__(256) void panel_trsm(double* __restrict__ A, int64_t lda, int64_t sA, int D, int k0,
                                                   int kb, const double* __restrict__ Lt) {
+  __shared__ double Ls[NB * PNB];
   const int c = blockIdx.y;
+  const double* __restrict__ Lc = Lt + (size_t)c * PNB * PNB;
+  for (int e = threadIdx.x; e < NB * PNB; e += blockDim.x) Ls[e] = Lc[e];
+  __syncthreads();
   const int r = k0 + kb + blockIdx.x * 256 + threadIdx.x;
   if (r >= D) return;  // no barrier below
-  const double* __restrict__ L = Lt + (size_t)c * PNB * PNB;
   double* Ar = A + (size_t)c * sA + (size_t)r + (size_t)k0 * lda;
   // per-phase opaque copies of kb: otherwise the compiler keeps the 64 masks
   // j < kb of the loads live in scalar registers until the stores
@@ -958,7 +962,7 @@ __global__ __launch_bounds__(256) void panel_trsm(double* __restrict__ A, int64_
 #pragma unroll
   for (int j = 0; j < NB; ++j) y[j] = j < kl ? Ar[(size_t)j * lda] : 0.0;
   asm volatile("s_mov_b32 %0, %1" : "=s"(ks) : "s"(kb));
-  trsm_columns<NB, 0>(y, L, ks);
+  trsm_columns<NB, 0>(y, Ls, ks);
   asm volatile("s_mov_b32 %0, %1" : "=s"(kw) : "s"(kb));
 #pragma unroll
   for (int j = 0; j < NB; ++j)
