@@ -451,18 +451,23 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
   if (kn < kend) fetch(xb_, yb_, kn);
   // two-chunk software pipeline, unrolled by two so that the register sets
   // alternate without copies
+  // a wave whose 32x32 sub-tile holds no stored element (all rows >= m, all
+  // columns >= n, or strictly above the diagonal of a lower tile) stages its
+  // share but issues no MFMAs, leaving the SIMD's matrix pipe to other waves
+  const int si0 = i0 + wr * 32, sj0 = j0 + wc * 32;
+  const bool busy = si0 < g.m && sj0 < g.n && (!g.lower || si0 + 31 >= sj0);
   while (kc < kend) {
     stage(xa_, ya_, kc);
     int kf = next_chunk(kn);
     if (kf < kend) fetch(xa_, ya_, kf);
-    multiply();
+    if (busy) multiply();
     kc = kn;
     kn = kf;
     if (kc >= kend) break;
     stage(xb_, yb_, kc);
     kf = next_chunk(kn);
     if (kf < kend) fetch(xb_, yb_, kf);
-    multiply();
+    if (busy) multiply();
     kc = kn;
     kn = kf;
   }
