@@ -1,0 +1,435 @@
+// skeleton.hip -- assembly of the condensed (trace and flux) system of a
+// conforming mesh.
+//
+// After static condensation every cell contributes its Schur complement S_c
+// (T x T over the cell's trace and flux dofs, in its dof order) and its
+// condensed load g_c. The reference maps them to global dofs with the minimum
+// rule (GDAMinimumRule::interpretLocalData, GDAMinimumRule.cpp:714-780, via
+// LocalDofMapper) and inserts them into the global matrix (Solution.cpp:
+// 1263-1293; CondensedDofInterpreter.cpp:1084,1194-1224). On a conforming mesh
+// (no hanging sides, one polynomial order) that rule reduces to:
+//   - an H^1 trace's vertex, edge and face nodes are shared by every cell that
+//     touches the entity (so within one cell the same global dof appears on
+//     several sides, and its contributions add up);
+//   - an L^2 flux's nodes belong to one side, shared by its two cells (the
+//     side parity is already inside the local matrices);
+//   - every cell sees a shared entity in one canonical orientation: edges run
+//     from the smaller to the larger vertex id; a face starts at its smallest
+//     vertex, first axis toward the smaller of that vertex's two face
+//     neighbours.
+//
+// Host: dpg_skeleton_numbering gives the global index of every trailing dof
+// of every cell; dpg_skeleton_nodes the reference coordinates of those dofs'
+// nodes. Device: dpg_skeleton_create sorts the (row, col) keys of all cell
+// contributions (stable radix sort) into the CSR pattern and the order in
+// which each nonzero sums its contributions; dpg_skeleton_assemble sums them,
+// one thread per nonzero, in increasing (cell, local row, local column) order,
+// so every run gives the same bits.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <rocprim/rocprim.hpp>
+#include <string>
+#include <vector>
+
+#include "../../include/dpg_capi.h"
+#include "element.hpp"
+#include "errors.hpp"
+
+#define SK_CHECK(expr)                                                                            \
+  do {                                                                                            \
+    hipError_t e_ = (expr);                                                                       \
+    if (e_ != hipSuccess) throw std::runtime_error(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+namespace dpg {
+namespace {
+
+// the trailing (trace / flux) blocks of a layout, in dof order
+std::vector<TrialBlock> side_blocks(const ElementLayout& L) {
+  std::vector<TrialBlock> out;
+  for (const TrialBlock& b : L.blocks)
+    if (b.side >= 0) out.push_back(b);
+  return out;
+}
+
+// nodes per direction of a side block's nodal basis, minus one
+int side_degree(const ElementLayout& L, const TrialBlock& b) {
+  return b.kind == DPG_TRACE ? L.trace_deg : L.flux_deg;
+}
+
+struct Numbering {
+  int nglobal = 0;
+  std::vector<int32_t> gdof;  // (C, T)
+};
+
+Numbering number_skeleton(const ElementLayout& L, int C, const int32_t* cv) {
+  const int V = L.rc.nverts, S = L.rc.nsides, dim = L.dim, T = L.T, F = L.F;
+  const int nps = L.rc.nodes_per_side;
+  if (L.topo == DPG_HEX && nps != 4) throw std::invalid_argument("hex faces must have 4 nodes");
+  // compact the vertex ids: used ids in increasing order
+  std::vector<int32_t> ids(cv, cv + (size_t)C * V);
+  for (int32_t v : ids)
+    if (v < 0) throw std::invalid_argument("negative vertex id");
+  std::vector<int32_t> uniq = ids;
+  std::sort(uniq.begin(), uniq.end());
+  uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+  auto vid = [&](int c, int lv) {
+    return (int)(std::lower_bound(uniq.begin(), uniq.end(), cv[(size_t)c * V + lv]) - uniq.begin());
+  };
+  for (int c = 0; c < C; ++c)
+    for (int a = 0; a < V; ++a)
+      for (int b = a + 1; b < V; ++b)
+        if (cv[(size_t)c * V + a] == cv[(size_t)c * V + b]) throw std::invalid_argument("a cell repeats a vertex");
+  const int nv = (int)uniq.size();
+  // edges (2D: the sides; 3D: the edges of the faces) and faces, by sorted vertex ids
+  std::map<std::pair<int, int>, int> edges;
+  std::map<std::array<int, 4>, int> faces;
+  auto edge_key = [](int a, int b) { return std::make_pair(std::min(a, b), std::max(a, b)); };
+  for (int c = 0; c < C; ++c)
+    for (int s = 0; s < S; ++s) {
+      const int* sn = &L.rc.side_nodes[(size_t)s * nps];
+      if (dim == 2) {
+        edges.emplace(edge_key(vid(c, sn[0]), vid(c, sn[1])), 0);
+      } else {
+        std::array<int, 4> g{vid(c, sn[0]), vid(c, sn[1]), vid(c, sn[2]), vid(c, sn[3])};
+        for (int k = 0; k < 4; ++k) edges.emplace(edge_key(g[k], g[(k + 1) % 4]), 0);
+        std::sort(g.begin(), g.end());
+        faces.emplace(g, 0);
+      }
+    }
+  int ne = 0, nf = 0;
+  for (auto& e : edges) e.second = ne++;
+  for (auto& f : faces) f.second = nf++;
+  const int nside_global = dim == 2 ? ne : nf;
+  // global blocks per side variable, in trial order
+  std::vector<TrialBlock> sb = side_blocks(L);
+  std::map<int, int> var_base;  // var -> first global dof
+  std::map<int, int> var_kind;
+  int next = 0;
+  for (const TrialBlock& b : sb) {
+    if (var_base.count(b.var)) continue;
+    var_base[b.var] = next;
+    var_kind[b.var] = b.kind;
+    const int n = side_degree(L, b);
+    if (b.kind == DPG_TRACE) {
+      next += nv + ne * (n - 1) + (dim == 3 ? nf * (n - 1) * (n - 1) : 0);
+    } else {
+      next += nside_global * (dim == 2 ? n + 1 : (n + 1) * (n + 1));
+    }
+  }
+  Numbering out;
+  out.nglobal = next;
+  out.gdof.assign((size_t)C * T, -1);
+  for (int c = 0; c < C; ++c)
+    for (const TrialBlock& b : sb) {
+      const int n = side_degree(L, b), base = var_base[b.var];
+      const int* sn = &L.rc.side_nodes[(size_t)b.side * nps];
+      int32_t* g = &out.gdof[(size_t)c * T + (b.offset - F)];
+      if (dim == 2) {
+        const int A = vid(c, sn[0]), B = vid(c, sn[1]);
+        const int e = edges.at(edge_key(A, B));
+        for (int i = 0; i <= n; ++i) {
+          const int ci = A < B ? i : n - i;  // position from the smaller vertex id
+          if (b.kind == DPG_TRACE) {
+            if (i == 0) g[i] = base + A;
+            else if (i == n) g[i] = base + B;
+            else g[i] = base + nv + e * (n - 1) + (ci - 1);
+          } else {
+            g[i] = base + e * (n + 1) + ci;
+          }
+        }
+      } else {
+        const int G[4] = {vid(c, sn[0]), vid(c, sn[1]), vid(c, sn[2]), vid(c, sn[3])};
+        std::array<int, 4> key{G[0], G[1], G[2], G[3]};
+        std::sort(key.begin(), key.end());
+        const int fi = faces.at(key);
+        // corner grid positions in the face parametrisation (units of n)
+        const int pos[4][2] = {{0, 0}, {n, 0}, {n, n}, {0, n}};
+        int m = 0;
+        for (int k = 1; k < 4; ++k)
+          if (G[k] < G[m]) m = k;
+        int un = (m + 1) % 4, vn = (m + 3) % 4;
+        if (G[vn] < G[un]) std::swap(un, vn);
+        const int ux = pos[un][0] - pos[m][0], uy = pos[un][1] - pos[m][1];
+        const int vx = pos[vn][0] - pos[m][0], vy = pos[vn][1] - pos[m][1];
+        for (int bb = 0; bb <= n; ++bb)
+          for (int aa = 0; aa <= n; ++aa) {
+            const int dx = aa - pos[m][0], dy = bb - pos[m][1];
+            const int u = n > 0 ? (dx * ux + dy * uy) / n : 0, v = n > 0 ? (dx * vx + dy * vy) / n : 0;
+            int32_t gg;
+            if (b.kind == DPG_FLUX) {
+              gg = base + fi * (n + 1) * (n + 1) + u + v * (n + 1);
+            } else {
+              int corner = -1;
+              for (int k = 0; k < 4; ++k)
+                if (pos[k][0] == aa && pos[k][1] == bb) corner = k;
+              if (corner >= 0) {
+                gg = base + G[corner];
+              } else if (aa == 0 || aa == n || bb == 0 || bb == n) {
+                int k0, k1, t;  // face edge (k0, k1), distance t from corner k0
+                if (bb == 0) {
+                  k0 = 0, k1 = 1, t = aa;
+                } else if (aa == n) {
+                  k0 = 1, k1 = 2, t = bb;
+                } else if (bb == n) {
+                  k0 = 2, k1 = 3, t = n - aa;
+                } else {
+                  k0 = 3, k1 = 0, t = n - bb;
+                }
+                const int e = edges.at(edge_key(G[k0], G[k1]));
+                const int cd = G[k0] < G[k1] ? t : n - t;
+                gg = base + nv + e * (n - 1) + (cd - 1);
+              } else {
+                gg = base + nv + ne * (n - 1) + fi * (n - 1) * (n - 1) + (u - 1) + (v - 1) * (n - 1);
+              }
+            }
+            g[aa + bb * (n + 1)] = gg;
+          }
+      }
+    }
+  for (int32_t v : out.gdof)
+    if (v < 0 || v >= out.nglobal) throw std::logic_error("skeleton numbering left a dof unassigned");
+  return out;
+}
+
+// ------------------------------------------------------------------ device
+__global__ void expand_keys(const int32_t* __restrict__ gdof, int T, int64_t nglobal, uint64_t E,
+                            uint64_t* __restrict__ keys, uint32_t* __restrict__ idx) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= E) return;
+  const uint64_t TT = (uint64_t)T * T;
+  const uint64_t c = p / TT, rem = p - c * TT, i = rem / T, j = rem - i * T;
+  keys[p] = (uint64_t)gdof[c * T + i] * (uint64_t)nglobal + (uint64_t)gdof[c * T + j];
+  idx[p] = (uint32_t)p;
+}
+
+__global__ void expand_rows(const int32_t* __restrict__ gdof, uint64_t E, uint64_t* __restrict__ keys,
+                            uint32_t* __restrict__ idx) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= E) return;
+  keys[p] = (uint64_t)gdof[p];
+  idx[p] = (uint32_t)p;
+}
+
+// CSR columns and row pointers from the sorted unique keys row*nglobal + col
+__global__ void csr_from_keys(const uint64_t* __restrict__ uk, uint32_t nnz, int64_t nglobal,
+                              int32_t* __restrict__ col, int64_t* __restrict__ row_ptr) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e > nnz) return;
+  const int64_t r = e < nnz ? (int64_t)(uk[e] / (uint64_t)nglobal) : nglobal;
+  const int64_t rp = e > 0 ? (int64_t)(uk[e - 1] / (uint64_t)nglobal) : -1;
+  if (e < nnz) col[e] = (int32_t)(uk[e] - (uint64_t)r * (uint64_t)nglobal);
+  for (int64_t q = rp + 1; q <= r; ++q) row_ptr[q] = (int64_t)e;
+}
+
+// out[e] = sum of src[perm[q]] for q in [seg[e], seg[e+1]), in order
+__global__ void ordered_segment_sum(const double* __restrict__ src, const uint32_t* __restrict__ perm,
+                                    const uint32_t* __restrict__ seg, uint32_t n, double* __restrict__ out) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double acc = 0.0;
+  for (uint32_t q = seg[e]; q < seg[e + 1]; ++q) acc = acc + src[perm[q]];
+  out[e] = acc;
+}
+
+template <class T>
+struct DevArray {
+  T* p = nullptr;
+  size_t n = 0;
+  explicit DevArray(size_t count = 0) { resize(count); }
+  DevArray(const DevArray&) = delete;
+  DevArray& operator=(const DevArray&) = delete;
+  ~DevArray() {
+    if (p) (void)hipFree(p);
+  }
+  void resize(size_t count) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = count;
+    if (count) SK_CHECK(hipMalloc(&p, std::max<size_t>(1, count) * sizeof(T)));
+  }
+};
+
+unsigned key_bits(uint64_t maxkey) {
+  unsigned b = 1;
+  while (b < 64 && (maxkey >> b) != 0) ++b;
+  return b;
+}
+
+// stable sort of (key, index) pairs, then the segments of equal keys:
+// perm = indices in key order, uk = unique keys, seg = segment starts (runs + 1)
+void sort_and_segment(DevArray<uint64_t>& keys, DevArray<uint32_t>& idx, uint64_t E, uint64_t maxkey,
+                      DevArray<uint32_t>& perm, DevArray<uint64_t>& uk, DevArray<uint32_t>& seg, uint32_t& runs,
+                      hipStream_t st) {
+  DevArray<uint64_t> sk(E);
+  perm.resize(E);
+  const unsigned bits = key_bits(maxkey);
+  size_t tb = 0;
+  SK_CHECK(rocprim::radix_sort_pairs(nullptr, tb, keys.p, sk.p, idx.p, perm.p, (size_t)E, 0, bits, st));
+  DevArray<unsigned char> tmp(tb);
+  SK_CHECK(rocprim::radix_sort_pairs(tmp.p, tb, keys.p, sk.p, idx.p, perm.p, (size_t)E, 0, bits, st));
+  keys.resize(0);
+  idx.resize(0);
+  uk.resize(E);
+  DevArray<uint32_t> counts(E + 1), nruns(1);  // the scan below reads one past the runs
+  tb = 0;
+  SK_CHECK(rocprim::run_length_encode(nullptr, tb, sk.p, (unsigned int)E, uk.p, counts.p, nruns.p, st));
+  DevArray<unsigned char> tmp2(tb);
+  SK_CHECK(rocprim::run_length_encode(tmp2.p, tb, sk.p, (unsigned int)E, uk.p, counts.p, nruns.p, st));
+  SK_CHECK(hipMemcpyAsync(&runs, nruns.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  SK_CHECK(hipStreamSynchronize(st));
+  seg.resize((size_t)runs + 1);
+  tb = 0;
+  SK_CHECK(rocprim::exclusive_scan(nullptr, tb, counts.p, seg.p, (uint32_t)0, (size_t)runs + 1,
+                                   rocprim::plus<uint32_t>(), st));
+  DevArray<unsigned char> tmp3(tb);
+  // counts[runs] is past the runs: make it 0 so that seg[runs] = E
+  SK_CHECK(hipMemsetAsync(counts.p + runs, 0, sizeof(uint32_t), st));
+  SK_CHECK(rocprim::exclusive_scan(tmp3.p, tb, counts.p, seg.p, (uint32_t)0, (size_t)runs + 1,
+                                   rocprim::plus<uint32_t>(), st));
+  SK_CHECK(hipStreamSynchronize(st));
+}
+
+}  // namespace
+}  // namespace dpg
+
+struct dpg_skeleton {
+  int ncells = 0, T = 0;
+  int64_t nglobal = 0;
+  uint32_t nnz = 0;
+  dpg::DevArray<uint32_t> perm, seg;  // matrix: contributions in summation order, segment starts
+  dpg::DevArray<uint32_t> rperm, rseg;  // load vector
+  dpg::DevArray<int64_t> row_ptr;
+  dpg::DevArray<int32_t> col;
+};
+
+extern "C" {
+
+int dpg_skeleton_numbering(const dpg_form_desc* form, const dpg_bucket_desc* bucket, int ncells,
+                           const int32_t* cell_vertices, int32_t* gdof) {
+  return dpg::guarded_status([&] {
+    if (!form || !bucket || ncells < 0 || (ncells > 0 && (!cell_vertices || !gdof)))
+      throw std::invalid_argument("NULL argument or negative cell count");
+    dpg::ElementLayout L = dpg::make_layout(*form, *bucket);
+    if (L.T <= 0) throw std::invalid_argument("the form has no trace or flux variables");
+    if (ncells == 0) return 0;
+    dpg::Numbering nb = dpg::number_skeleton(L, ncells, cell_vertices);
+    std::memcpy(gdof, nb.gdof.data(), nb.gdof.size() * sizeof(int32_t));
+    return nb.nglobal;
+  });
+}
+
+int dpg_skeleton_nodes(const dpg_form_desc* form, const dpg_bucket_desc* bucket, double* xi) {
+  return dpg::guarded([&] {
+    if (!form || !bucket || !xi) throw std::invalid_argument("NULL argument");
+    dpg::ElementLayout L = dpg::make_layout(*form, *bucket);
+    const int dim = L.dim;
+    for (const dpg::TrialBlock& b : dpg::side_blocks(L)) {
+      const int n = dpg::side_degree(L, b);
+      const std::vector<double> t = dpg::gll_points(n);
+      const double* p = &L.rc.side_param[(size_t)b.side * 3 * dim];
+      for (int k = 0; k < b.card; ++k) {
+        const int a = k % (n + 1), bb = k / (n + 1);
+        double* x = xi + (size_t)(b.offset - L.F + k) * dim;
+        for (int d = 0; d < dim; ++d)
+          x[d] = p[d] + p[dim + d] * t[a] + (dim == 3 ? p[2 * dim + d] * t[bb] : 0.0);
+      }
+    }
+  });
+}
+
+int dpg_skeleton_create(int ncells, int T, const int32_t* gdof, int nglobal, void* stream, dpg_skeleton** out) {
+  return dpg::guarded([&] {
+    if (!out || ncells <= 0 || T <= 0 || nglobal <= 0 || !gdof) throw std::invalid_argument("bad arguments");
+    *out = nullptr;
+    const uint64_t E = (uint64_t)ncells * T * T;
+    if (E > 0xFFFFFFFFull) throw std::invalid_argument("ncells * T * T must be below 2^32");
+    for (size_t q = 0; q < (size_t)ncells * T; ++q)
+      if (gdof[q] < 0 || gdof[q] >= nglobal) throw std::invalid_argument("gdof entry out of range");
+    hipStream_t st = (hipStream_t)stream;
+    std::unique_ptr<dpg_skeleton> sk(new dpg_skeleton());
+    sk->ncells = ncells;
+    sk->T = T;
+    sk->nglobal = nglobal;
+    dpg::DevArray<int32_t> dg((size_t)ncells * T);
+    SK_CHECK(hipMemcpyAsync(dg.p, gdof, sizeof(int32_t) * dg.n, hipMemcpyHostToDevice, st));
+    {  // matrix pattern and summation order
+      dpg::DevArray<uint64_t> keys(E), uk;
+      dpg::DevArray<uint32_t> idx(E);
+      dpg::expand_keys<<<(unsigned)((E + 255) / 256), 256, 0, st>>>(dg.p, T, nglobal, E, keys.p, idx.p);
+      SK_CHECK(hipGetLastError());
+      const uint64_t maxkey = (uint64_t)nglobal * (uint64_t)nglobal - 1;
+      dpg::sort_and_segment(keys, idx, E, maxkey, sk->perm, uk, sk->seg, sk->nnz, st);
+      sk->col.resize(sk->nnz);
+      sk->row_ptr.resize((size_t)nglobal + 1);
+      dpg::csr_from_keys<<<(unsigned)(((uint64_t)sk->nnz + 1 + 255) / 256), 256, 0, st>>>(uk.p, sk->nnz, nglobal,
+                                                                                         sk->col.p, sk->row_ptr.p);
+      SK_CHECK(hipGetLastError());
+    }
+    {  // load vector: every global dof has at least one contribution
+      const uint64_t E2 = (uint64_t)ncells * T;
+      dpg::DevArray<uint64_t> keys(E2), uk;
+      dpg::DevArray<uint32_t> idx(E2);
+      dpg::expand_rows<<<(unsigned)((E2 + 255) / 256), 256, 0, st>>>(dg.p, E2, keys.p, idx.p);
+      SK_CHECK(hipGetLastError());
+      uint32_t runs = 0;
+      dpg::sort_and_segment(keys, idx, E2, (uint64_t)nglobal - 1, sk->rperm, uk, sk->rseg, runs, st);
+      if ((int64_t)runs != nglobal) throw std::invalid_argument("some global dofs belong to no cell");
+    }
+    SK_CHECK(hipStreamSynchronize(st));
+    *out = sk.release();
+  });
+}
+
+int dpg_skeleton_info(const dpg_skeleton* sk, int64_t* nnz, int64_t* nglobal, const int64_t** row_ptr,
+                      const int32_t** col) {
+  return dpg::guarded([&] {
+    if (!sk) throw std::invalid_argument("NULL skeleton");
+    if (nnz) *nnz = sk->nnz;
+    if (nglobal) *nglobal = sk->nglobal;
+    if (row_ptr) *row_ptr = sk->row_ptr.p;
+    if (col) *col = sk->col.p;
+  });
+}
+
+int dpg_skeleton_pattern(const dpg_skeleton* sk, int64_t* row_ptr, int32_t* col, void* stream) {
+  return dpg::guarded([&] {
+    if (!sk || !row_ptr || !col) throw std::invalid_argument("NULL argument");
+    hipStream_t st = (hipStream_t)stream;
+    SK_CHECK(hipMemcpyAsync(row_ptr, sk->row_ptr.p, sizeof(int64_t) * ((size_t)sk->nglobal + 1),
+                            hipMemcpyDeviceToDevice, st));
+    SK_CHECK(hipMemcpyAsync(col, sk->col.p, sizeof(int32_t) * (size_t)sk->nnz, hipMemcpyDeviceToDevice, st));
+  });
+}
+
+int dpg_skeleton_assemble(const dpg_skeleton* sk, const double* schur, const double* schur_rhs, double* values,
+                          double* rhs, void* stream) {
+  return dpg::guarded([&] {
+    if (!sk) throw std::invalid_argument("NULL skeleton");
+    hipStream_t st = (hipStream_t)stream;
+    if (values) {
+      if (!schur) throw std::invalid_argument("values needs schur");
+      dpg::ordered_segment_sum<<<(unsigned)(((uint64_t)sk->nnz + 255) / 256), 256, 0, st>>>(
+          schur, sk->perm.p, sk->seg.p, sk->nnz, values);
+      SK_CHECK(hipGetLastError());
+    }
+    if (rhs) {
+      if (!schur_rhs) throw std::invalid_argument("rhs needs schur_rhs");
+      dpg::ordered_segment_sum<<<(unsigned)((sk->nglobal + 255) / 256), 256, 0, st>>>(
+          schur_rhs, sk->rperm.p, sk->rseg.p, (uint32_t)sk->nglobal, rhs);
+      SK_CHECK(hipGetLastError());
+    }
+  });
+}
+
+int dpg_skeleton_destroy(dpg_skeleton* sk) {
+  return dpg::guarded([&] { delete sk; });
+}
+
+}  // extern "C"

@@ -321,3 +321,95 @@ class EnergyErrorFunction:
                 return g[:, None] * grad
 
         return _F.apply(coefs)
+
+
+# ---------------------------------------------------------------- condensed global system
+def vertex_ids(nodes):
+    """(C, V) global vertex ids of a mesh given by per-cell vertex coordinates
+    (C, V, d) in which shared vertices carry identical coordinates (as the seeded
+    generators produce): ids in lexicographic order of the distinct points."""
+    C, V, d = nodes.shape
+    _, inv = np.unique(nodes.reshape(C * V, d), axis=0, return_inverse=True)
+    return inv.reshape(C, V).astype(np.int32)
+
+
+def skeleton_numbering(f, bucket, cell_vertices):
+    """Global index (C, T) of every trace and flux dof of every cell of a
+    conforming mesh, and the number of global dofs (dpg_skeleton_numbering)."""
+    cv = np.ascontiguousarray(cell_vertices, dtype=np.int32)
+    T = sizes(f, bucket).T
+    gdof = np.zeros((cv.shape[0], T), dtype=np.int32)
+    n = check(lib().dpg_skeleton_numbering(ct.byref(f), ct.byref(bucket), cv.shape[0], _p32(cv), _p32(gdof)))
+    return gdof, n
+
+
+def skeleton_nodes(f, bucket):
+    """Reference coordinates (T, dim) of the nodes of a cell's trace and flux dofs."""
+    s = sizes(f, bucket)
+    xi = np.zeros((s.T, f.spatial_dim))
+    check(lib().dpg_skeleton_nodes(ct.byref(f), ct.byref(bucket), xi.ctypes.data_as(ct.POINTER(ct.c_double))))
+    return xi
+
+
+def _p32(a):
+    return a.ctypes.data_as(ct.POINTER(ct.c_int32))
+
+
+class CondensedSystem:
+    """The global condensed (trace and flux) system of a conforming mesh on the
+    device: CSR pattern from the cells' global dof indices, and assembly of the
+    cells' Schur complements into it (dpg_skeleton_*)."""
+
+    def __init__(self, gdof, nglobal, stream=None):
+        import torch
+
+        self._torch = torch
+        g = np.ascontiguousarray(gdof, dtype=np.int32)
+        self.ncells, self.T = g.shape
+        self.nglobal = int(nglobal)
+        s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        h = ct.c_void_p()
+        check(lib().dpg_skeleton_create(self.ncells, self.T, _p32(g), self.nglobal, ct.c_void_p(s), ct.byref(h)))
+        self._h = h
+        nnz = ct.c_int64()
+        check(lib().dpg_skeleton_info(h, ct.byref(nnz), None, None, None))
+        self.nnz = nnz.value
+
+    def pattern(self, stream=None):
+        """(row_ptr int64 (nglobal+1), col int32 (nnz)) as device tensors (copies)."""
+        t = self._torch
+        rp = t.empty(self.nglobal + 1, dtype=t.int64, device="cuda")
+        col = t.empty(self.nnz, dtype=t.int32, device="cuda")
+        s = stream.cuda_stream if stream is not None else t.cuda.current_stream().cuda_stream
+        check(lib().dpg_skeleton_pattern(self._h, ct.c_void_p(rp.data_ptr()), ct.c_void_p(col.data_ptr()),
+                                         ct.c_void_p(s)))
+        return rp, col
+
+    def assemble(self, schur, schur_rhs=None, stream=None):
+        """CSR values (nnz) and global load (nglobal) as device tensors."""
+        t = self._torch
+        assert schur.is_cuda and schur.dtype == t.float64 and schur.is_contiguous()
+        assert schur.shape == (self.ncells, self.T, self.T)
+        vals = t.empty(self.nnz, dtype=t.float64, device=schur.device)
+        rhs = None
+        if schur_rhs is not None:
+            assert schur_rhs.is_cuda and schur_rhs.is_contiguous() and schur_rhs.shape == (self.ncells, self.T)
+            rhs = t.empty(self.nglobal, dtype=t.float64, device=schur.device)
+        s = stream.cuda_stream if stream is not None else t.cuda.current_stream().cuda_stream
+        check(lib().dpg_skeleton_assemble(self._h, ct.c_void_p(schur.data_ptr()),
+                                          ct.c_void_p(schur_rhs.data_ptr() if rhs is not None else 0),
+                                          ct.c_void_p(vals.data_ptr()),
+                                          ct.c_void_p(rhs.data_ptr() if rhs is not None else 0), ct.c_void_p(s)))
+        return vals, rhs
+
+    def close(self):
+        if self._h:
+            lib().dpg_skeleton_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

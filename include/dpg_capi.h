@@ -352,6 +352,49 @@ int dpg_graph_launch(dpg_graph* graph, void* stream);
 int dpg_graph_workspace(const dpg_graph* graph, size_t* bytes);
 int dpg_graph_destroy(dpg_graph* graph);
 
+/* ------------------------------------------------------------------------
+ * Condensed (trace and flux) system of a conforming mesh: local-to-global
+ * interpretation of the cells' Schur complements and their sparse assembly.
+ * Replaces GDAMinimumRule::interpretLocalData (GDAMinimumRule.cpp:714-780,
+ * through LocalDofMapper) followed by the global insertion of
+ * TSolution::populateStiffnessAndLoad (Solution.cpp:1263-1293) for the
+ * condensed solve (CondensedDofInterpreter.cpp:1084,1194-1224), on meshes
+ * without hanging sides and with one polynomial order (one bucket): H^1 traces
+ * share vertex, edge and face nodes between cells, L^2 fluxes belong to one
+ * side, shared entities are seen in one canonical orientation.
+ * ---------------------------------------------------------------------- */
+typedef struct dpg_skeleton dpg_skeleton;
+
+/* Host. cell_vertices (C, V): global vertex ids, in the vertex order of the
+ * cell's nodes in dpg_batch. gdof (C, T) receives the global index of each of
+ * the cell's trailing trial dofs (trace and flux blocks in the cell's dof
+ * order: the rows of dpg_outputs.schur). Returns the number of global dofs
+ * (>= 0) or < 0 on error. */
+int dpg_skeleton_numbering(const dpg_form_desc* form, const dpg_bucket_desc* bucket, int ncells,
+                           const int32_t* cell_vertices, int32_t* gdof);
+/* Host. Reference coordinates (T, dim) of the nodes of those T dofs (their
+ * nodal side basis functions are 1 there). */
+int dpg_skeleton_nodes(const dpg_form_desc* form, const dpg_bucket_desc* bucket, double* xi);
+/* Device symbolic phase from a host gdof (C, T): the CSR pattern (rows of
+ * int64 pointers, int32 columns, both triangles) of sum_c P_c^T S_c P_c and
+ * the order in which each nonzero sums its contributions. Needs
+ * C * T * T < 2^32. Synchronises `stream`. */
+int dpg_skeleton_create(int ncells, int T, const int32_t* gdof, int nglobal, void* stream, dpg_skeleton** out);
+/* nnz, global size and the device CSR pattern (owned by the skeleton; any
+ * argument may be NULL) */
+int dpg_skeleton_info(const dpg_skeleton* sk, int64_t* nnz, int64_t* nglobal, const int64_t** row_ptr,
+                      const int32_t** col);
+/* copies the CSR pattern into caller-owned device buffers row_ptr (nglobal + 1)
+ * and col (nnz), stream-ordered */
+int dpg_skeleton_pattern(const dpg_skeleton* sk, int64_t* row_ptr, int32_t* col, void* stream);
+/* Device numeric phase, stream-ordered: values (nnz) from schur (C, T, T) and
+ * rhs (nglobal) from schur_rhs (C, T), every sum in increasing (cell, local
+ * row, local column) order, so the same bits on every call. values or rhs may
+ * be NULL to skip that part. */
+int dpg_skeleton_assemble(const dpg_skeleton* sk, const double* schur, const double* schur_rhs, double* values,
+                          double* rhs, void* stream);
+int dpg_skeleton_destroy(dpg_skeleton* sk);
+
 #ifdef __cplusplus
 }
 #endif
