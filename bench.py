@@ -298,6 +298,74 @@ def bench_hp(args, cfg):
     print(json.dumps(result))
 
 
+def bench_global_assembly(args, f, b, nodes, schur, schur_rhs, cpu_target_s=2.0):
+    """The condensed global system of the same mesh: host numbering, device
+    symbolic phase (CSR pattern + summation order), and the numeric assembly of
+    every cell's Schur complement and load (timed, HBM roofline), with scipy's
+    COO->CSR sum on a bounded sample of cells as the CPU stand-in."""
+    import torch
+
+    from camellia2_amd import dpg
+
+    t0 = time.perf_counter()
+    cv = dpg.vertex_ids(nodes)
+    gdof, n = dpg.skeleton_numbering(f, b, cv)
+    t_num = time.perf_counter() - t0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sysm = dpg.CondensedSystem(gdof, n)
+    torch.cuda.synchronize()
+    t_sym = time.perf_counter() - t0
+    C, T = gdof.shape
+    vals, rhs = sysm.assemble(schur, schur_rhs)  # warm-up
+    torch.cuda.synchronize()
+    reps = max(1, args.steps)
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        sysm.assemble(schur, schur_rhs)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    E = C * T * T
+    # algorithmic bytes: Schur blocks and loads read once, their summation order
+    # (4 B per contribution, 4 B per segment start), CSR values and load written
+    nbytes = 8.0 * E + 4.0 * E + 4.0 * (sysm.nnz + 1) + 8.0 * sysm.nnz + 12.0 * C * T + 4.0 * (n + 1) + 8.0 * n
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    res = {"global_dofs": n, "nnz": sysm.nnz, "contributions": E, "numbering_host_s": round(t_num, 3),
+           "symbolic_device_s": round(t_sym, 3), "ms": round(ms, 3), "cells_per_s": round(C / (ms * 1e-3), 1),
+           "roofline": {"bound": "hbm", "kernel": "ordered_segment_sum", "achieved": round(gbs, 1),
+                        "peak": PEAK["hbm"] / 1e9, "unit": "GB/s", "frac": round(gbs * 1e9 / PEAK["hbm"], 4),
+                        "bytes_per_launch": nbytes}}
+    if args.peaks and "hbm_read_gbs" in args.peaks:
+        res["roofline"]["frac_of_measured_copy"] = round(gbs / args.peaks["hbm_copy_gbs"], 4)
+    sysm.close()
+    del vals, rhs
+    if not args.no_cpu:
+        import scipy.sparse as sp
+
+        Sh = schur[:min(C, 64)].cpu().numpy()
+        gd = gdof[:Sh.shape[0]]
+        rows = np.repeat(gd, T, axis=1).ravel()
+        cols = np.tile(gd, (1, T)).ravel()
+        t0 = time.perf_counter()
+        sp.coo_matrix((Sh.ravel(), (rows, cols)), shape=(n, n)).tocsr()
+        per = (time.perf_counter() - t0) / Sh.shape[0]
+        nc = int(min(C, max(Sh.shape[0], cpu_target_s / max(per, 1e-9))))
+        Sh = schur[:nc].cpu().numpy()
+        gd = gdof[:nc]
+        rows = np.repeat(gd, T, axis=1).ravel()
+        cols = np.tile(gd, (1, T)).ravel()
+        t0 = time.perf_counter()
+        sp.coo_matrix((Sh.ravel(), (rows, cols)), shape=(n, n)).tocsr()
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": nc / dt, "unit": "cells/s", "cores": 1, "kind": "port",
+                               "sample": f"scipy coo_matrix(...).tocsr() of the first {nc} cells' Schur blocks, "
+                                         f"{dt:.2f} s"}
+    return res
+
+
 def measured_peaks():
     """Sustained FP64 MFMA / FP64 FMA / HBM rates of this GPU from
     camellia2_amd/lib/dpg_peaks (a child process), or None if it is not built."""
@@ -339,6 +407,8 @@ def main():
     ap.add_argument("--sumfact", action="store_true",
                     help="opt-in sum-factorised G (fewer flops, K accurate only to ~1e-12..1e-1; not the default)")
     ap.add_argument("--no-peaks", action="store_true", help="skip the peak-rate microbenchmark")
+    ap.add_argument("--no-assembly", action="store_true",
+                    help="skip the condensed global system (numbering, CSR pattern, assembly)")
     ap.add_argument("--calibrate-copy-gib", type=float, default=0.0,
                     help="also run a plain device copy of this size (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -439,6 +509,8 @@ def main():
         "roofline": roofline,
         "stages": stages,
     }
+    if want_schur and not args.no_assembly and args.precision == "fp64":
+        result["global_assembly"] = bench_global_assembly(args, f, b, nodes, out["schur"], out["schur_rhs"])
     # mixed-precision forms of the same step, timed the same way, with their
     # distance from the FP64 GPU result over all cells
     comps = [p for p in args.compare.split(",") if p and p != args.precision]
