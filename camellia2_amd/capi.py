@@ -118,7 +118,8 @@ EXPORTS = [
     "dpg_energy_error", "dpg_adjoint", "dpg_mesh_quad_graded", "dpg_plan_get_tiles", "dpg_plan_set_tiles",
     "dpg_plan_autotune", "dpg_graph_create", "dpg_graph_launch", "dpg_graph_workspace", "dpg_graph_destroy",
     "dpg_skeleton_numbering", "dpg_skeleton_nodes", "dpg_skeleton_create", "dpg_skeleton_info",
-    "dpg_skeleton_pattern", "dpg_skeleton_assemble", "dpg_skeleton_destroy",
+    "dpg_skeleton_pattern", "dpg_skeleton_assemble", "dpg_skeleton_solve", "dpg_mark_cells", "dpg_skeleton_destroy",
+    "dpg_recover_fields",
 ]
 
 _lib = None
@@ -175,6 +176,11 @@ def lib():
         L.dpg_skeleton_assemble.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p,
                                             ct.c_void_p]
         L.dpg_skeleton_destroy.argtypes = [ct.c_void_p]
+        L.dpg_skeleton_solve.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_double,
+                                         ct.c_int, P(ct.c_int), P(ct.c_double), ct.c_void_p]
+        L.dpg_mark_cells.argtypes = [ct.c_void_p, ct.c_int, ct.c_double, ct.c_void_p, ct.c_void_p]
+        L.dpg_recover_fields.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                         ct.c_void_p]
         _lib = L
     return _lib
 

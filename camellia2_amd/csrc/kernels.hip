@@ -1094,6 +1094,33 @@ __global__ __launch_bounds__(256) void backsolve_kernel(const double* __restrict
   for (int i = t; i < N; i += blockDim.x) e[(size_t)c * se + i] = s[(size_t)c * ss + i] * z[i];
 }
 
+// Field recovery after a condensed solve, one workgroup per cell: from K
+// (row-major M x M, symmetric) and rhs (M) of the local assembly and the
+// trace/flux part u_T of u (entries F..M-1), the lower triangle of S K_FF S
+// (s_j = 1/sqrt(K_jj)) into A and below it the row A[F][j] = s_j b_j with
+// b = rhs_F - K_FT u_T. K's symmetry lets every read run along rows of K.
+__global__ __launch_bounds__(256) void recover_setup(const double* __restrict__ K, const double* __restrict__ rhs,
+                                                     const double* __restrict__ u, int M, int F,
+                                                     double* __restrict__ A, int64_t lda, int64_t sA,
+                                                     double* __restrict__ s, int64_t ss) {
+  const int c = blockIdx.x, t = threadIdx.x;
+  const double* Kc = K + (size_t)c * M * M;
+  const double* uc = u + (size_t)c * M;
+  double* Ac = A + (size_t)c * sA;
+  double* sc = s + (size_t)c * ss;
+  for (int j = t; j < F; j += blockDim.x) sc[j] = 1.0 / sqrt(Kc[(size_t)j * M + j]);
+  __syncthreads();
+  for (int e = t; e < F * F; e += blockDim.x) {
+    const int i = e % F, j = e / F;  // A[i][j], i >= j, read as K[j][i]
+    if (i >= j) Ac[(size_t)i + (size_t)j * lda] = sc[i] * sc[j] * Kc[(size_t)j * M + i];
+  }
+  for (int j = t; j < F; j += blockDim.x) {
+    double b = rhs[(size_t)c * M + j];
+    for (int q = F; q < M; ++q) b = fma(-Kc[(size_t)q * M + j], uc[q], b);  // K[j][q] = K[q][j]
+    Ac[(size_t)F + (size_t)j * lda] = sc[j] * b;
+  }
+}
+
 __device__ inline double dcoef_value(const DSummand& sm, int k, const double* cc, const double* n) {
   if (sm.slot != k) return 0.0;
   double v = sm.scale;
@@ -1411,6 +1438,12 @@ void launch_extract_sym(const double* A, int64_t lda, int64_t sA, int r0, int n,
   int nt = (n + 31) / 32;
   extract_sym<<<dim3(nt, nt, ncells), 256, 0, st>>>(A, lda, sA, r0, n, sign, out);
   check_launch("extract_sym");
+}
+
+void launch_recover_setup(const double* K, const double* rhs, const double* u, int M, int F, double* A, int64_t lda,
+                          int64_t sA, double* s, int64_t ss, int ncells, hipStream_t st) {
+  recover_setup<<<dim3(ncells), 256, 0, st>>>(K, rhs, u, M, F, A, lda, sA, s, ss);
+  check_launch("recover_setup");
 }
 
 void launch_backsolve(const double* A, int64_t lda, int64_t sA, int N, int row_r, const double* Ld, int64_t sLd,

@@ -78,6 +78,9 @@ void launch_residual_row(double* A, int64_t lda, int64_t sA, int N, int M, int r
 void launch_extract_sym(const double* A, int64_t lda, int64_t sA, int r0, int n, double sign, double* out, int ncells,
                         hipStream_t st);
 // adjoint: e = S Lt^-T y (y = residual row of A), per-point gradient partials, reduction
+// field recovery after a condensed solve: S K_FF S and the row s (rhs_F - K_FT u_T) into A
+void launch_recover_setup(const double* K, const double* rhs, const double* u, int M, int F, double* A, int64_t lda,
+                          int64_t sA, double* s, int64_t ss, int ncells, hipStream_t st);
 void launch_backsolve(const double* A, int64_t lda, int64_t sA, int N, int row_r, const double* Ld, int64_t sLd,
                       const double* s, int64_t ss, double* e, int64_t se, int ncells, hipStream_t st);
 void launch_adjoint_points(int dim, int kind, const DevProgram* prog, const DevTables& tab, int npts, int Qs,

@@ -1006,6 +1006,39 @@ int dpg_energy_error(dpg_plan* plan, int bucket, const dpg_batch* batch, const d
   });
 }
 
+int dpg_recover_fields(dpg_plan* plan, int bucket, int ncells, const double* K, const double* rhs, double* u,
+                       void* stream) {
+  return dpg::guarded_status([&] {
+    if (!plan) throw std::invalid_argument("plan is NULL");
+    if (bucket < 0 || bucket >= (int)plan->buckets.size()) throw std::invalid_argument("bucket index out of range");
+    if (ncells < 0) throw std::invalid_argument("ncells < 0");
+    if (ncells == 0) return 0;  // an empty batch needs no buffers
+    if (!K || !rhs || !u) throw std::invalid_argument("K, rhs and u are required");
+    hipStream_t st = (hipStream_t)stream;
+    const dpg::BucketPlan& B = plan->buckets[bucket];
+    const int F = B.L.F, M = B.L.M;
+    if (F == 0) return 0;
+    // the field block and one augmented row, in the bucket's workspace layout
+    dpg::BucketPlan R = B;
+    R.D = F + 1;
+    int maxb = dpg::batch_cells(plan, B, ncells);
+    dpg::prepare_workspace(plan, B, maxb);
+    int bad = 0;
+    for (int c0 = 0; c0 < ncells; c0 += maxb) {
+      int nb = std::min(maxb, ncells - c0);
+      dpg::Work w = dpg::carve(plan->ws.p, (int32_t*)plan->info.p, B, nb);
+      const size_t o = (size_t)c0 * M;
+      dpg::launch_recover_setup(K + o * M, rhs + o, u + o, M, F, w.A, B.lda, B.nA, w.s, B.nS, nb, st);
+      dpg::launch_fill_i32(w.info, nb, 0, st);
+      dpg::factor_columns(plan, R, nb, w, 0, F, 0.0, st);
+      dpg::launch_backsolve(w.A, B.lda, B.nA, F, F, w.Ld, B.nLd, w.s, B.nS, u + o, M, nb, st);
+      plan->prof.collect(st, nb);
+      bad += dpg::count_failures(w, nb, nullptr, st);
+    }
+    return bad;
+  });
+}
+
 int dpg_adjoint(dpg_plan* plan, int bucket, const dpg_batch* batch, const double* u, double* eta2, double* grad,
                 void* stream) {
   return dpg::guarded_status([&] {

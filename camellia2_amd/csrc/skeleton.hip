@@ -29,6 +29,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <map>
@@ -238,6 +239,117 @@ __global__ void ordered_segment_sum(const double* __restrict__ src, const uint32
   out[e] = acc;
 }
 
+// ----------------------------------------------------------- global solve
+// y[row] = (A x)[row] for free rows, 0 for fixed rows; one wavefront per row,
+// reduced across the wavefront in a fixed order
+__global__ __launch_bounds__(256) void spmv_free_rows(const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                      const double* __restrict__ val, const double* __restrict__ x,
+                                                      const int32_t* __restrict__ fixed, int64_t n,
+                                                      double* __restrict__ y) {
+  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;  // whole wavefronts: n rows per 64-lane groups
+  double acc = 0.0;
+  if (!fixed[row])
+    for (int64_t q = rp[row] + lane; q < rp[row + 1]; q += 64) acc = fma(val[q], x[col[q]], acc);
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+  if (lane == 0) y[row] = acc;
+}
+
+__global__ void csr_diagonal(const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
+                             const double* __restrict__ val, int64_t n, double* __restrict__ d) {
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  int64_t lo = rp[row], hi = rp[row + 1];
+  while (lo < hi) {  // columns are sorted within a row
+    const int64_t mid = (lo + hi) / 2;
+    if (col[mid] < row) lo = mid + 1;
+    else hi = mid;
+  }
+  d[row] = (lo < rp[row + 1] && col[lo] == row) ? val[lo] : 0.0;
+}
+
+constexpr int kRedBlocks = 1024;  // fixed grid: the same summation order on every call
+
+// part[b] = sum over the grid-stride slice of block b of a[i] * b[i] (op 0),
+// or of max |a[i]| (op 1); tree-reduced in shared memory
+__global__ __launch_bounds__(256) void reduce_partial(const double* __restrict__ a, const double* __restrict__ b,
+                                                      int64_t n, int op, double* __restrict__ part) {
+  __shared__ double sh[256];
+  const int t = threadIdx.x;
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + t; i < n; i += (int64_t)gridDim.x * 256)
+    acc = op == 0 ? fma(a[i], b[i], acc) : fmax(acc, fabs(a[i]));
+  sh[t] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) sh[t] = op == 0 ? sh[t] + sh[t + w] : fmax(sh[t], sh[t + w]);
+    __syncthreads();
+  }
+  if (t == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(256) void reduce_final(const double* __restrict__ part, int np, int op,
+                                                    double* __restrict__ out) {
+  __shared__ double sh[256];
+  const int t = threadIdx.x;
+  double acc = 0.0;
+  for (int i = t; i < np; i += 256) acc = op == 0 ? acc + part[i] : fmax(acc, part[i]);
+  sh[t] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) sh[t] = op == 0 ? sh[t] + sh[t + w] : fmax(sh[t], sh[t + w]);
+    __syncthreads();
+  }
+  if (t == 0) out[0] = sh[0];
+}
+
+// r = b - q on free rows (0 on fixed), z = r / d, p = z
+__global__ void cg_start(const double* __restrict__ b, const double* __restrict__ q, const double* __restrict__ d,
+                         const int32_t* __restrict__ fixed, int64_t n, double* __restrict__ r, double* __restrict__ z,
+                         double* __restrict__ p) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double ri = fixed[i] ? 0.0 : b[i] - q[i];
+  const double zi = fixed[i] ? 0.0 : ri / d[i];
+  r[i] = ri;
+  z[i] = zi;
+  p[i] = zi;
+}
+
+// x += alpha p, r -= alpha q (p and q are 0 on fixed rows)
+__global__ void cg_step(double alpha, const double* __restrict__ p, const double* __restrict__ q, int64_t n,
+                        double* __restrict__ x, double* __restrict__ r) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  x[i] = fma(alpha, p[i], x[i]);
+  r[i] = fma(-alpha, q[i], r[i]);
+}
+
+// z = r / d on free rows, 0 on fixed ones
+__global__ void cg_precondition(const double* __restrict__ r, const double* __restrict__ d,
+                                const int32_t* __restrict__ fixed, int64_t n, double* __restrict__ z) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  z[i] = fixed[i] ? 0.0 : r[i] / d[i];
+}
+
+// p = z + beta p
+__global__ void cg_direction(double beta, const double* __restrict__ z, int64_t n, double* __restrict__ p) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  p[i] = fma(beta, p[i], z[i]);
+}
+
+__global__ void mark_above(const double* __restrict__ err, int n, const double* __restrict__ emax, double rel,
+                           int32_t* __restrict__ flags, double* __restrict__ fcount) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int f = err[i] > rel * emax[0] ? 1 : 0;
+  flags[i] = f;
+  fcount[i] = f;
+}
+
 template <class T>
 struct DevArray {
   T* p = nullptr;
@@ -295,6 +407,20 @@ void sort_and_segment(DevArray<uint64_t>& keys, DevArray<uint32_t>& idx, uint64_
                                    rocprim::plus<uint32_t>(), st));
   SK_CHECK(hipStreamSynchronize(st));
 }
+
+// reduction of n values on `st` (op 0: dot of a and b; op 1: max |a|), read back
+double reduce(const double* a, const double* b, int64_t n, int op, DevArray<double>& part, DevArray<double>& one,
+              hipStream_t st) {
+  reduce_partial<<<kRedBlocks, 256, 0, st>>>(a, b, n, op, part.p);
+  reduce_final<<<1, 256, 0, st>>>(part.p, kRedBlocks, op, one.p);
+  SK_CHECK(hipGetLastError());
+  double v = 0.0;
+  SK_CHECK(hipMemcpyAsync(&v, one.p, sizeof(double), hipMemcpyDeviceToHost, st));
+  SK_CHECK(hipStreamSynchronize(st));
+  return v;
+}
+
+unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)((n + per - 1) / per); }
 
 }  // namespace
 }  // namespace dpg
@@ -425,6 +551,58 @@ int dpg_skeleton_assemble(const dpg_skeleton* sk, const double* schur, const dou
           schur_rhs, sk->rperm.p, sk->rseg.p, (uint32_t)sk->nglobal, rhs);
       SK_CHECK(hipGetLastError());
     }
+  });
+}
+
+int dpg_skeleton_solve(const dpg_skeleton* sk, const double* values, const double* rhs, const int32_t* fixed,
+                       double* x, double tol, int maxit, int* iters, double* relres, void* stream) {
+  return dpg::guarded([&] {
+    if (!sk || !values || !rhs || !fixed || !x) throw std::invalid_argument("NULL argument");
+    if (!(tol >= 0.0) || maxit < 0) throw std::invalid_argument("tol must be >= 0 and maxit >= 0");
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t n = sk->nglobal;
+    dpg::DevArray<double> r(n), z(n), p(n), q(n), d(n), part(dpg::kRedBlocks), one(1);
+    const unsigned bn = dpg::blocks_for(n), bw = dpg::blocks_for(n * 64);
+    dpg::csr_diagonal<<<bn, 256, 0, st>>>(sk->row_ptr.p, sk->col.p, values, n, d.p);
+    dpg::spmv_free_rows<<<bw, 256, 0, st>>>(sk->row_ptr.p, sk->col.p, values, x, fixed, n, q.p);
+    dpg::cg_start<<<bn, 256, 0, st>>>(rhs, q.p, d.p, fixed, n, r.p, z.p, p.p);
+    SK_CHECK(hipGetLastError());
+    const double rr0 = dpg::reduce(r.p, r.p, n, 0, part, one, st);
+    double rz = dpg::reduce(r.p, z.p, n, 0, part, one, st);
+    double rr = rr0;
+    int it = 0;
+    while (it < maxit && rr > tol * tol * rr0 && rr0 > 0.0) {
+      dpg::spmv_free_rows<<<bw, 256, 0, st>>>(sk->row_ptr.p, sk->col.p, values, p.p, fixed, n, q.p);
+      const double pq = dpg::reduce(p.p, q.p, n, 0, part, one, st);
+      if (!(pq > 0.0)) throw std::runtime_error("the free block of the system is not positive definite");
+      const double alpha = rz / pq;
+      dpg::cg_step<<<bn, 256, 0, st>>>(alpha, p.p, q.p, n, x, r.p);
+      rr = dpg::reduce(r.p, r.p, n, 0, part, one, st);
+      ++it;
+      if (rr <= tol * tol * rr0) break;
+      dpg::cg_precondition<<<bn, 256, 0, st>>>(r.p, d.p, fixed, n, z.p);
+      const double rz_new = dpg::reduce(r.p, z.p, n, 0, part, one, st);
+      dpg::cg_direction<<<bn, 256, 0, st>>>(rz_new / rz, z.p, n, p.p);
+      SK_CHECK(hipGetLastError());
+      rz = rz_new;
+    }
+    if (iters) *iters = it;
+    if (relres) *relres = rr0 > 0.0 ? std::sqrt(rr / rr0) : 0.0;
+  });
+}
+
+int dpg_mark_cells(const double* err, int ncells, double rel_threshold, int32_t* flags, void* stream) {
+  return dpg::guarded_status([&] {
+    if (ncells < 0) throw std::invalid_argument("ncells < 0");
+    if (ncells == 0) return 0;
+    if (!err || !flags) throw std::invalid_argument("NULL argument");
+    hipStream_t st = (hipStream_t)stream;
+    dpg::DevArray<double> part(dpg::kRedBlocks), emax(1), fc(ncells), one(1);
+    dpg::reduce_partial<<<dpg::kRedBlocks, 256, 0, st>>>(err, err, ncells, 1, part.p);
+    dpg::reduce_final<<<1, 256, 0, st>>>(part.p, dpg::kRedBlocks, 1, emax.p);
+    dpg::mark_above<<<dpg::blocks_for(ncells), 256, 0, st>>>(err, ncells, emax.p, rel_threshold, flags, fc.p);
+    SK_CHECK(hipGetLastError());
+    return (int)dpg::reduce(fc.p, fc.p, ncells, 0, part, one, st);  // flags are 0 or 1: the count
   });
 }
 

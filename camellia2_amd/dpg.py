@@ -221,6 +221,19 @@ class LocalAssembler:
         nbad = check(lib().dpg_assemble(self._plan, bucket, ct.byref(b), ct.byref(outs), ct.c_void_p(s)))
         return o, nbad
 
+    def recover_fields(self, bucket, K, rhs, u, stream=None):
+        """In place: the field entries of u (C, M) from its trace/flux entries,
+        u_F = K_FF^-1 (rhs_F - K_FT u_T) per cell (dpg_recover_fields); K (C,M,M)
+        and rhs (C,M) from local_stiffness_and_rhs. Returns the non-SPD cell count."""
+        t = self._torch
+        C = u.shape[0]
+        for a in (K, rhs, u):
+            assert a.is_cuda and a.dtype == t.float64 and a.is_contiguous() and a.shape[0] == C
+        s = stream.cuda_stream if stream is not None else t.cuda.current_stream().cuda_stream
+        return check(lib().dpg_recover_fields(self._plan, bucket, C, ct.c_void_p(K.data_ptr()),
+                                              ct.c_void_p(rhs.data_ptr()), ct.c_void_p(u.data_ptr()),
+                                              ct.c_void_p(s)))
+
     def energy_error(self, bucket, nodes, parities, u, coefs=None, load=None, stream=None):
         """eta2[c] = r^T G^-1 r with r = l - B u (TSolution::rankLocalEnergyError reports sqrt)."""
         t = self._torch
@@ -402,6 +415,20 @@ class CondensedSystem:
                                           ct.c_void_p(rhs.data_ptr() if rhs is not None else 0), ct.c_void_p(s)))
         return vals, rhs
 
+    def solve(self, values, rhs, fixed, x, tol=1e-12, maxit=10000, stream=None):
+        """In place on x: the free dofs of values (CSR) x = rhs with x[fixed] kept
+        (fixed: int32 0/1 per global dof); Jacobi-preconditioned CG on the device.
+        Returns (iterations, final relative residual)."""
+        t = self._torch
+        for a, dt in ((values, t.float64), (rhs, t.float64), (fixed, t.int32), (x, t.float64)):
+            assert a.is_cuda and a.dtype == dt and a.is_contiguous()
+        it, rel = ct.c_int(), ct.c_double()
+        s = stream.cuda_stream if stream is not None else t.cuda.current_stream().cuda_stream
+        check(lib().dpg_skeleton_solve(self._h, ct.c_void_p(values.data_ptr()), ct.c_void_p(rhs.data_ptr()),
+                                       ct.c_void_p(fixed.data_ptr()), ct.c_void_p(x.data_ptr()), float(tol),
+                                       int(maxit), ct.byref(it), ct.byref(rel), ct.c_void_p(s)))
+        return it.value, rel.value
+
     def close(self):
         if self._h:
             lib().dpg_skeleton_destroy(self._h)
@@ -413,3 +440,16 @@ class CondensedSystem:
         except Exception:
             pass
 
+
+
+def mark_cells(err, rel_threshold, stream=None):
+    """flags (int32 device tensor) of the cells with err > rel_threshold * max err,
+    and their count (dpg_mark_cells)."""
+    import torch
+
+    assert err.is_cuda and err.dtype == torch.float64 and err.is_contiguous()
+    flags = torch.empty(err.shape[0], dtype=torch.int32, device=err.device)
+    s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    n = check(lib().dpg_mark_cells(ct.c_void_p(err.data_ptr()), err.shape[0], float(rel_threshold),
+                                   ct.c_void_p(flags.data_ptr()), ct.c_void_p(s)))
+    return flags, n

@@ -352,6 +352,19 @@ int dpg_graph_launch(dpg_graph* graph, void* stream);
 int dpg_graph_workspace(const dpg_graph* graph, size_t* bytes);
 int dpg_graph_destroy(dpg_graph* graph);
 
+/* Field dofs back from a condensed solve: for every cell of the batch,
+ * u_F = K_FF^-1 (rhs_F - K_FT u_T), with K (C, M, M) and rhs (C, M) the
+ * dpg_assemble outputs of these cells and u (C, M) the cells' trial
+ * coefficients, whose trailing T entries (traces and fluxes, e.g. gathered
+ * from a global solution) are read and whose leading F entries are written.
+ * Replaces the field back-substitution of
+ * CondensedDofInterpreter::interpretGlobalCoefficients
+ * (CondensedDofInterpreter.cpp:1229-1330); K_FF is equilibrated and factored
+ * by the same partial Cholesky as the local assembly (the reference uses an
+ * LU). Returns the number of cells whose K_FF was not SPD. */
+int dpg_recover_fields(dpg_plan* plan, int bucket, int ncells, const double* K, const double* rhs, double* u,
+                       void* stream);
+
 /* ------------------------------------------------------------------------
  * Condensed (trace and flux) system of a conforming mesh: local-to-global
  * interpretation of the cells' Schur complements and their sparse assembly.
@@ -393,6 +406,21 @@ int dpg_skeleton_pattern(const dpg_skeleton* sk, int64_t* row_ptr, int32_t* col,
  * be NULL to skip that part. */
 int dpg_skeleton_assemble(const dpg_skeleton* sk, const double* schur, const double* schur_rhs, double* values,
                           double* rhs, void* stream);
+/* Solves the assembled system for its free dofs with the others prescribed:
+ * rows of free dofs of A x = rhs, with x[d] given where fixed[d] = 1 (device
+ * arrays of nglobal; x in/out, its free entries the initial guess). Jacobi-
+ * preconditioned conjugate gradients (the free block is symmetric positive
+ * definite) until ||r|| <= tol ||r_0|| or maxit iterations, every reduction
+ * in a fixed order. Stands in for the global solve of TSolution::solve
+ * (Solution.cpp:3480, Solver.h:111-118: Amesos direct or an iterative solver).
+ * iters and relres (final ||r|| / ||r_0||) may be NULL. */
+int dpg_skeleton_solve(const dpg_skeleton* sk, const double* values, const double* rhs, const int32_t* fixed,
+                       double* x, double tol, int maxit, int* iters, double* relres, void* stream);
+/* flags[c] = err[c] > rel_threshold * max err (device arrays of ncells): the
+ * marking of TRefinementStrategy::getCellsAboveErrorThreshhold
+ * (RefinementStrategy.cpp:249-257; ErrorIndicator.cpp:24-33), e.g. with
+ * err = sqrt(eta2) from dpg_energy_error. Returns the number of marked cells. */
+int dpg_mark_cells(const double* err, int ncells, double rel_threshold, int32_t* flags, void* stream);
 int dpg_skeleton_destroy(dpg_skeleton* sk);
 
 #ifdef __cplusplus

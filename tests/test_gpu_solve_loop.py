@@ -1,0 +1,116 @@
+"""The solve loop on the device after local assembly: global condensed system,
+CG solve with the trace prescribed on the boundary, field recovery per cell,
+energy error per cell and marking -- checked end to end: a linear exact
+solution comes back in every trial variable with zero energy error, the CG
+solution matches a direct solve of the same system, and for a sine load the
+energy error falls under refinement."""
+import numpy as np
+import pytest
+
+from camellia2_amd import capi, dpg
+
+from helpers import linear_trial_coefficients
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+sp = pytest.importorskip("scipy.sparse")
+spla = pytest.importorskip("scipy.sparse.linalg")
+
+
+def dev(a):
+    return None if a is None else torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
+
+
+def boundary_trace_mask(f, b, gdof, cv, n):
+    """1 on the global trace dofs of boundary sides (sides of one cell only)"""
+    V = cv.shape[1]
+    sides = [(i, (i + 1) % V) for i in range(V)]
+    nt = b.h1_order + 1
+    count = {}
+    for c in range(cv.shape[0]):
+        for s in sides:
+            key = tuple(sorted(cv[c, list(s)]))
+            count[key] = count.get(key, 0) + 1
+    mask = np.zeros(n, dtype=np.int32)
+    for c in range(cv.shape[0]):
+        for si, s in enumerate(sides):
+            if count[tuple(sorted(cv[c, list(s)]))] == 1:
+                mask[gdof[c, si * nt:(si + 1) * nt]] = 1
+    return mask
+
+
+def solve_loop(f, b, nodes, par, load, fixed_values=None, tol=1e-13):
+    """local assembly -> condensed global system -> CG -> field recovery ->
+    energy error; returns (u (C,M) device, eta2 (C) device, iterations, system pieces)"""
+    A = dpg.LocalAssembler(f, [b])
+    dn, dp = dev(nodes), dev(par)
+    out, nbad = A.local_stiffness_and_rhs(0, dn, dp, None, load, want_schur=True)
+    assert nbad == 0
+    cv = dpg.vertex_ids(nodes)
+    gdof, n = dpg.skeleton_numbering(f, b, cv)
+    sysm = dpg.CondensedSystem(gdof, n)
+    vals, rhs = sysm.assemble(out["schur"], out["schur_rhs"])
+    fixed = boundary_trace_mask(f, b, gdof, cv, n)
+    x = np.zeros(n) if fixed_values is None else np.where(fixed == 1, fixed_values, 0.0)
+    dx, dfixed = dev(x), dev(fixed)
+    iters, rel = sysm.solve(vals, rhs, dfixed, dx, tol=tol, maxit=20000)
+    s = dpg.sizes(f, b)
+    u = torch.zeros((nodes.shape[0], s.M), dtype=torch.float64, device="cuda:0")
+    u[:, s.F:] = dx[torch.from_numpy(gdof.astype(np.int64)).to("cuda:0")]
+    assert A.recover_fields(0, out["K"], out["rhs"], u) == 0
+    eta2 = A.energy_error(0, dn, dp, u, None, load)
+    torch.cuda.synchronize()
+    A.close()
+    pieces = dict(sysm=sysm, vals=vals, rhs=rhs, fixed=fixed, x=dx, gdof=gdof, n=n)
+    return u, eta2, iters, rel, pieces
+
+
+def test_linear_solution_through_the_whole_loop():
+    f = dpg.form("poisson", 2)
+    b = dpg.bucket(capi.QUAD, 3, 5)
+    nodes, par = dpg.mesh_quad(7, 6, 0.15, 101)
+    a = np.array([0.3, -1.2, 0.7])
+    exact = linear_trial_coefficients(nodes, par, 3, a)
+    s = dpg.sizes(f, b)
+    cv = dpg.vertex_ids(nodes)
+    gdof, n = dpg.skeleton_numbering(f, b, cv)
+    xg = np.zeros(n)
+    xg[gdof.ravel()] = exact[:, s.F:].ravel()
+    u, eta2, iters, rel, pc = solve_loop(f, b, nodes, par, None, fixed_values=xg)
+    print(f"{n} global dofs, CG {iters} iterations, relres {rel:.1e}")
+    assert rel <= 1e-13
+    x = pc["x"].cpu().numpy()
+    assert np.abs(x - xg).max() <= 1e-9 * np.abs(xg).max()
+    un = u.cpu().numpy()
+    assert np.abs(un - exact).max() <= 1e-9 * np.abs(exact).max()  # fields recovered too
+    assert eta2.max().item() <= 1e-18
+    # the CG solution against a direct solve of the same system
+    rp, col = pc["sysm"].pattern()
+    Ah = sp.csr_matrix((pc["vals"].cpu().numpy(), col.cpu().numpy(), rp.cpu().numpy()), shape=(n, n))
+    free = np.flatnonzero(pc["fixed"] == 0)
+    fx = np.flatnonzero(pc["fixed"] == 1)
+    xd = xg.copy()
+    xd[free] = spla.spsolve(Ah[free][:, free].tocsc(), pc["rhs"].cpu().numpy()[free] - Ah[free][:, fx] @ xg[fx])
+    assert np.abs(x - xd).max() <= 1e-10 * np.abs(xd).max()
+    pc["sysm"].close()
+
+
+def test_energy_error_falls_under_refinement_and_marking():
+    f = dpg.form("poisson", 2)
+    b = dpg.bucket(capi.QUAD, 3, 5)
+    load = dpg.load_sine_modes(2, 1, 4, 3, 0xC0FFEE + 111)  # modes vanish on the boundary: u = 0 there
+    totals = []
+    for nx in (4, 8):
+        nodes, par = dpg.mesh_quad(nx, nx, 0.1, 121)
+        u, eta2, iters, rel, pc = solve_loop(f, b, nodes, par, load)
+        assert rel <= 1e-12
+        pc["sysm"].close()
+        err = torch.sqrt(eta2).contiguous()
+        totals.append(float(torch.sqrt(eta2.sum())))
+        flags, cnt = dpg.mark_cells(err, 0.5)
+        e = err.cpu().numpy()
+        want = e > 0.5 * e.max()
+        assert cnt == int(want.sum()) and np.array_equal(flags.cpu().numpy() == 1, want)
+        print(f"{nx}x{nx}: CG {iters} iterations, energy error {totals[-1]:.3e}, {cnt} cells marked")
+    assert totals[1] < totals[0] / 4.0
