@@ -364,6 +364,48 @@ def skeleton_nodes(f, bucket):
     return xi
 
 
+# cell-local vertices of each side (the reference cells' side node maps)
+SIDE_VERTICES = {
+    capi.QUAD: [(0, 1), (1, 2), (2, 3), (3, 0)],
+    capi.TRI: [(0, 1), (1, 2), (2, 0)],
+    capi.HEX: [(0, 1, 5, 4), (1, 2, 6, 5), (2, 3, 7, 6), (0, 4, 7, 3), (0, 3, 2, 1), (4, 5, 6, 7)],
+}
+
+
+def side_blocks(f, bucket):
+    """(variable, kind, side, offset within the trailing T dofs, cardinality) of
+    every trace and flux block, in the cell's dof order"""
+    dim = f.spatial_dim
+    nsides = len(SIDE_VERTICES[bucket.topology])
+    out, off = [], 0
+    for v in range(f.n_trial):
+        kind = f.trial_kind[v]
+        if kind not in (capi.TRACE, capi.FLUX):
+            continue
+        card = (bucket.h1_order + 1) ** (dim - 1) if kind == capi.TRACE else bucket.h1_order ** (dim - 1)
+        for s in range(nsides):
+            out.append((v, kind, s, off, card))
+            off += card
+    return out
+
+
+def boundary_trace_mask(f, bucket, gdof, cell_vertices, nglobal):
+    """int32 (nglobal): 1 on the global trace dofs of boundary sides (sides that
+    belong to one cell only), the dofs a Dirichlet condition on the trace fixes"""
+    cv = np.asarray(cell_vertices)
+    sides = SIDE_VERTICES[bucket.topology]
+    keys = np.sort(cv[:, np.array(sides)], axis=2)  # (C, S, nodes per side)
+    flat = keys.reshape(-1, keys.shape[2])
+    _, inv, cnt = np.unique(flat, axis=0, return_inverse=True, return_counts=True)
+    on_bd = (cnt[inv.ravel()] == 1).reshape(keys.shape[:2])  # (C, S)
+    mask = np.zeros(nglobal, dtype=np.int32)
+    for v, kind, s, off, card in side_blocks(f, bucket):
+        if kind == capi.TRACE:
+            cells = np.flatnonzero(on_bd[:, s])
+            mask[gdof[cells, off:off + card].ravel()] = 1
+    return mask
+
+
 def _p32(a):
     return a.ctypes.data_as(ct.POINTER(ct.c_int32))
 
