@@ -495,3 +495,53 @@ def mark_cells(err, rel_threshold, stream=None):
     n = check(lib().dpg_mark_cells(ct.c_void_p(err.data_ptr()), err.shape[0], float(rel_threshold),
                                    ct.c_void_p(flags.data_ptr()), ct.c_void_p(s)))
     return flags, n
+
+
+# ----------------------------------------------------------- input preparation on the device
+def side_parities(topology, cell_vertices, stream=None):
+    """(C, S) int8 side parities of a conforming mesh from device cell -> vertex ids (dpg_side_parities)."""
+    import torch
+
+    cv = cell_vertices
+    assert cv.is_cuda and cv.dtype == torch.int32 and cv.is_contiguous()
+    S = len(SIDE_VERTICES[topology])
+    par = torch.empty((cv.shape[0], S), dtype=torch.int8, device=cv.device)
+    s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    check(lib().dpg_side_parities(topology, cv.shape[0], ct.c_void_p(cv.data_ptr()), ct.c_void_p(par.data_ptr()),
+                                  ct.c_void_p(s)))
+    return par
+
+
+def sort_cells(type_key, stream=None):
+    """Element-type buckets of device int32 type keys: (perm, offsets (nb+1), keys (nb)) (dpg_sort_cells)."""
+    import torch
+
+    assert type_key.is_cuda and type_key.dtype == torch.int32 and type_key.is_contiguous()
+    C = type_key.shape[0]
+    perm = torch.empty(C, dtype=torch.int32, device=type_key.device)
+    offsets = torch.empty(C + 1, dtype=torch.int32, device=type_key.device)
+    keys = torch.empty(max(C, 1), dtype=torch.int32, device=type_key.device)
+    s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    nb = check(lib().dpg_sort_cells(C, ct.c_void_p(type_key.data_ptr()), ct.c_void_p(perm.data_ptr()),
+                                    ct.c_void_p(offsets.data_ptr()), ct.c_void_p(keys.data_ptr()), ct.c_void_p(s)))
+    return perm, offsets[:nb + 1], keys[:nb]
+
+
+def gather_cells(perm, nodes, parities, coefs=None, stream=None):
+    """nodes, parities (and coefs) of the cells in perm order, on the device (dpg_gather_cells)."""
+    import torch
+
+    C = perm.shape[0]
+    out_n = torch.empty_like(nodes)
+    out_p = torch.empty_like(parities)
+    out_c = torch.empty_like(coefs) if coefs is not None else None
+    nd = nodes[0].numel() if C else 0
+    S = parities.shape[1]
+    nc = coefs.shape[1] if coefs is not None else 0
+    s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    check(lib().dpg_gather_cells(C, ct.c_void_p(perm.data_ptr()), nd, ct.c_void_p(nodes.data_ptr()),
+                                 ct.c_void_p(out_n.data_ptr()), S, ct.c_void_p(parities.data_ptr()),
+                                 ct.c_void_p(out_p.data_ptr()), nc,
+                                 ct.c_void_p(coefs.data_ptr() if coefs is not None else 0),
+                                 ct.c_void_p(out_c.data_ptr() if out_c is not None else 0), ct.c_void_p(s)))
+    return out_n, out_p, out_c

@@ -366,6 +366,31 @@ int dpg_recover_fields(dpg_plan* plan, int bucket, int ncells, const double* K, 
                        void* stream);
 
 /* ------------------------------------------------------------------------
+ * Preparation of the inputs on the device (device arrays, stream-ordered
+ * unless stated).
+ * ---------------------------------------------------------------------- */
+/* Side parities (C, S) of a conforming mesh of one topology from its cell ->
+ * vertex ids (C, V): +1 on the boundary, +1 for the lower cell ID of two peer
+ * cells, -1 for the other -- the rule of the reference's cell side parities
+ * (GlobalDofAssignment.cpp:160-185) for meshes without hanging sides. Sides
+ * are matched by their vertex ids. Synchronises the stream; an error when a
+ * side has more than two cells. */
+int dpg_side_parities(int topology, int ncells, const int32_t* cell_vertices, int8_t* parities, void* stream);
+/* Element-type bucketing (Solution.cpp:1162-1215 groups cells by ElementType):
+ * perm (C) lists the cells sorted by type_key (>= 0), stably; bucket b holds
+ * perm[offsets[b] .. offsets[b+1]) and has key bucket_keys[b]. offsets needs
+ * room for C + 1 entries, bucket_keys for C. Synchronises the stream; returns
+ * the number of buckets. */
+int dpg_sort_cells(int ncells, const int32_t* type_key, int32_t* perm, int32_t* offsets, int32_t* bucket_keys,
+                   void* stream);
+/* Gathers cells into bucket order: out[i] = in[perm[i]] for the nodes (C,
+ * nodes_per_cell doubles), parities (C, nsides) and coefficients (C, ncoefs);
+ * any size may be 0 (its arrays unused). */
+int dpg_gather_cells(int ncells, const int32_t* perm, int nodes_per_cell, const double* nodes, double* nodes_out,
+                     int nsides, const int8_t* parities, int8_t* parities_out, int ncoefs, const double* coefs,
+                     double* coefs_out, void* stream);
+
+/* ------------------------------------------------------------------------
  * Condensed (trace and flux) system of a conforming mesh: local-to-global
  * interpretation of the cells' Schur complements and their sparse assembly.
  * Replaces GDAMinimumRule::interpretLocalData (GDAMinimumRule.cpp:714-780,
