@@ -119,7 +119,7 @@ EXPORTS = [
     "dpg_plan_autotune", "dpg_graph_create", "dpg_graph_launch", "dpg_graph_workspace", "dpg_graph_destroy",
     "dpg_skeleton_numbering", "dpg_skeleton_nodes", "dpg_skeleton_create", "dpg_skeleton_info",
     "dpg_skeleton_pattern", "dpg_skeleton_assemble", "dpg_skeleton_solve", "dpg_mark_cells", "dpg_skeleton_destroy",
-    "dpg_recover_fields", "dpg_side_parities", "dpg_sort_cells", "dpg_gather_cells",
+    "dpg_recover_fields", "dpg_riesz_representation", "dpg_side_parities", "dpg_sort_cells", "dpg_gather_cells",
 ]
 
 _lib = None
@@ -183,6 +183,8 @@ def lib():
         L.dpg_gather_cells.argtypes = [ct.c_int, ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_int,
                                        ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p]
         L.dpg_mark_cells.argtypes = [ct.c_void_p, ct.c_int, ct.c_double, ct.c_void_p, ct.c_void_p]
+        L.dpg_riesz_representation.argtypes = [ct.c_void_p, ct.c_int, P(Batch), ct.c_void_p, ct.c_void_p,
+                                               ct.c_void_p]
         L.dpg_recover_fields.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p,
                                          ct.c_void_p]
         _lib = L

@@ -1081,6 +1081,32 @@ int dpg_adjoint(dpg_plan* plan, int bucket, const dpg_batch* batch, const double
   });
 }
 
+int dpg_riesz_representation(dpg_plan* plan, int bucket, const dpg_batch* batch, const double* u, double* e,
+                             void* stream) {
+  return dpg::guarded_status([&] {
+    dpg::check_batch(plan, bucket, batch);
+    if (plan->opt.precision != DPG_PREC_FP64) throw std::invalid_argument("runs in FP64 only");
+    if (batch->ncells == 0) return 0;  // an empty batch needs no buffers
+    if (!u || !e) throw std::invalid_argument("u and e are required");
+    hipStream_t st = (hipStream_t)stream;
+    const dpg::BucketPlan& B = plan->buckets[bucket];
+    const int N = B.L.N;
+    int C = batch->ncells;
+    int maxb = dpg::batch_cells(plan, B, C);
+    dpg::prepare_workspace(plan, B, maxb);
+    int bad = 0;
+    for (int c0 = 0; c0 < C; c0 += maxb) {
+      int nb = std::min(maxb, C - c0);
+      dpg::Work w = dpg::carve(plan->ws.p, (int32_t*)plan->info.p, B, nb);
+      dpg::stage_factor(plan, B, *batch, c0, nb, w, u, nullptr, st);
+      dpg::launch_backsolve(w.A, B.lda, B.nA, N, B.D - 1, w.Ld, B.nLd, w.s, B.nS, e + (size_t)c0 * N, N, nb, st);
+      plan->prof.collect(st, nb);
+      bad += dpg::count_failures(w, nb, nullptr, st);
+    }
+    return bad;
+  });
+}
+
 int dpg_plan_get_tiles(const dpg_plan* plan, int bucket, dpg_tiles* out) {
   return dpg::guarded([&] {
     if (!plan || !out) throw std::invalid_argument("NULL argument");

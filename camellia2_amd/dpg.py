@@ -246,6 +246,18 @@ class LocalAssembler:
                                      ct.c_void_p(eta2.data_ptr()), ct.c_void_p(s)))
         return eta2
 
+    def riesz_representation(self, bucket, nodes, parities, u, coefs=None, load=None, stream=None):
+        """e (C, N): the error representation G^-1 (l - B u) of every cell (dpg_riesz_representation)."""
+        t = self._torch
+        C = nodes.shape[0]
+        e = t.empty((C, self.info[bucket].N), dtype=t.float64, device=nodes.device)
+        assert u.is_cuda and u.dtype == t.float64 and u.is_contiguous()
+        b = self._batch(nodes, parities, coefs, load)
+        s = stream.cuda_stream if stream is not None else t.cuda.current_stream().cuda_stream
+        check(lib().dpg_riesz_representation(self._plan, bucket, ct.byref(b), ct.c_void_p(u.data_ptr()),
+                                             ct.c_void_p(e.data_ptr()), ct.c_void_p(s)))
+        return e
+
     def energy_error_and_gradient(self, bucket, nodes, parities, u, coefs, load=None, stream=None):
         """(eta2 (C,), d eta2 / d coefs (C, n_coefs)) from the adjoint kernels (dpg_adjoint)."""
         t = self._torch

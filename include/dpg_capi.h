@@ -352,6 +352,14 @@ int dpg_graph_launch(dpg_graph* graph, void* stream);
 int dpg_graph_workspace(const dpg_graph* graph, size_t* bytes);
 int dpg_graph_destroy(dpg_graph* graph);
 
+/* The error representation function of each cell (RieszRep, RieszRep.cpp:86,
+ * 244; the e of TSolution::computeErrorRepresentation, Solution.cpp:2688-2722):
+ * e (C, N) solves G e = l - B u in the cell's test basis (test dof order),
+ * from the same factorisation dpg_energy_error uses; eta2 = r^T e. Returns
+ * the number of non-SPD cells. */
+int dpg_riesz_representation(dpg_plan* plan, int bucket, const dpg_batch* batch, const double* u, double* e,
+                             void* stream);
+
 /* Field dofs back from a condensed solve: for every cell of the batch,
  * u_F = K_FF^-1 (rhs_F - K_FT u_T), with K (C, M, M) and rhs (C, M) the
  * dpg_assemble outputs of these cells and u (C, M) the cells' trial

@@ -114,3 +114,27 @@ def test_energy_error_falls_under_refinement_and_marking():
         assert cnt == int(want.sum()) and np.array_equal(flags.cpu().numpy() == 1, want)
         print(f"{nx}x{nx}: CG {iters} iterations, energy error {totals[-1]:.3e}, {cnt} cells marked")
     assert totals[1] < totals[0] / 4.0
+
+
+def test_riesz_representation_solves_the_gram_system():
+    """e = G^-1 (l - B u) per cell against a direct solve with the oracle's G, B,
+    l on the same cells, and r^T e against the energy error."""
+    from oracle import oracle
+
+    from helpers import cdr2d, rel_fro
+
+    f, b, nodes, par, coefs, load = cdr2d(4, 4)
+    s = dpg.sizes(f, b)
+    u = np.asarray(dpg.rng_normal(17, nodes.shape[0] * s.M)).reshape(nodes.shape[0], s.M)
+    A = dpg.LocalAssembler(f, [b])
+    args = (dev(nodes), dev(par), dev(u))
+    e = A.riesz_representation(0, args[0], args[1], args[2], dev(coefs), load).cpu().numpy()
+    eta2 = A.energy_error(0, args[0], args[1], args[2], dev(coefs), load).cpu().numpy()
+    A.close()
+    o = oracle.assemble(oracle.form_like(f), b, nodes, par, coefs, load, want=("B", "G", "l"))
+    r = o["l"] - np.einsum("cmn,cm->cn", o["B"], u)
+    ex = np.linalg.solve(o["G"], r[..., None])[..., 0]
+    err = rel_fro(e, ex)
+    print(f"riesz representation vs direct solve: {err:.2e}")
+    assert err < 1e-9
+    assert np.allclose(np.einsum("cn,cn->c", r, e), eta2, rtol=1e-9, atol=0)
