@@ -25,7 +25,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2.5e15, "hbm": 8.0e12}
-PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2, "fp16x3": 3}
+PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2, "fp16x3": 3, "int8split": 4}
 
 CONFIGS = {
     "c4": dict(name="poisson3d_hex32_p3_test5", form=("poisson", 3), topo="hex", n=(32, 32, 32), jitter=0.05,
@@ -73,10 +73,11 @@ def host_info():
     return info
 
 
-def cpu_baseline(f, b, nodes, par, load, gpu_K, target_s=2.5, reps=5):
+def cpu_baseline(f, b, nodes, par, load, gpu_K, target_s=2.5, reps=5, extra=None):
     """Oracle on a bounded sample of the workload's cells: the median of `reps`
     runs on all host threads (about target_s each) and one run on one thread
-    (about target_s), plus the GPU-vs-oracle error on the sample."""
+    (about target_s), plus the GPU-vs-oracle error on the sample (also for the
+    leading cells' K of each entry of `extra`, name -> device tensor)."""
     from oracle import oracle
 
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -99,14 +100,19 @@ def cpu_baseline(f, b, nodes, par, load, gpu_K, target_s=2.5, reps=5):
     t0 = time.perf_counter()
     oracle.assemble(ff, b, nodes[:n1], par[:n1], None, load, want=want, nthreads=1)
     dt1 = time.perf_counter() - t0
-    Kg = gpu_K[:n].cpu().numpy()
-    err = float(np.max(np.linalg.norm((Kg - o["K"]).reshape(n, -1), axis=1) /
-                       np.linalg.norm(o["K"].reshape(n, -1), axis=1)))
+    def max_rel(Kdev):
+        m = min(n, Kdev.shape[0])
+        Kg = Kdev[:m].cpu().numpy()
+        return float(np.max(np.linalg.norm((Kg - o["K"][:m]).reshape(m, -1), axis=1) /
+                            np.linalg.norm(o["K"][:m].reshape(m, -1), axis=1))), m
+
+    err, _ = max_rel(gpu_K)
+    extra_err = {k: max_rel(v) for k, v in (extra or {}).items()}
     return {"value": n / dt, "unit": "cells/s", "cores": threads, "kind": "port",
             "sample": f"first {n} cells of the same mesh, oracle K+rhs+Schur, median of {reps} runs "
                       f"({dt:.2f} s wall each), {threads} OpenMP threads",
             "one_core_value": n1 / dt1, "one_core_sample": f"first {n1} cells, 1 thread, {dt1:.2f} s",
-            "build": ORACLE_BUILD, "host": host_info()}, err
+            "build": ORACLE_BUILD, "host": host_info()}, err, extra_err
 
 
 def cpu_baseline_hp(f, groups, load, gpu_K, target_s=2.5, reps=5):
@@ -444,7 +450,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c4", choices=sorted(CONFIGS))
     ap.add_argument("--precision", default="fp64", choices=sorted(PRECISIONS))
-    ap.add_argument("--compare", default="fp32,bf16x3,fp16x3",
+    ap.add_argument("--compare", default="int8split,fp32,bf16x3,fp16x3",
                     help="mixed precisions timed against the FP64 form (empty: none)")
     ap.add_argument("--workspace-gib", type=float, default=64.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -565,6 +571,7 @@ def main():
     if comps:
         ref_K = out["K"].clone() if args.precision == "fp64" else None
         mixed = {}
+        heads = {}
         for p in comps:
             Am = dpg.LocalAssembler(f, [b], workspace_bytes=ws, precision=PRECISIONS[p])
             step(Am)
@@ -589,14 +596,18 @@ def main():
                 entry["rel_err_K_vs_fp64_gpu_max"] = float(rel.max()) if rel.numel() else None
                 entry["rel_err_K_vs_fp64_gpu_median"] = float(rel.median()) if rel.numel() else None
             mixed[p] = entry
+            heads[p] = out["K"][:min(C, 2048)].clone()  # for the distance from the oracle
         if ref_K is not None:
             out["K"].copy_(ref_K)
             del ref_K
         result["mixed_precision"] = mixed
     if not args.no_cpu:
-        cpu, err = cpu_baseline(f, b, nodes, par, load, out["K"])
+        cpu, err, extra_err = cpu_baseline(f, b, nodes, par, load, out["K"], extra=heads if comps else None)
         result["cpu_baseline"] = cpu
         result["rel_err_K_vs_fp64_oracle"] = err
+        for p, (e, m) in extra_err.items():
+            result["mixed_precision"][p]["rel_err_K_vs_fp64_oracle_max"] = e
+            result["mixed_precision"][p]["oracle_cells_compared"] = m
     if args.calibrate_copy_gib > 0:
         n = int(args.calibrate_copy_gib * 2 ** 30 // 8)
         src = torch.ones(n, dtype=torch.float64, device=dev)

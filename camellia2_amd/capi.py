@@ -21,7 +21,7 @@ FIELD, FIELD_VECTOR, TRACE, FLUX = 0, 1, 2, 3
 HGRAD, HDIV = 0, 1
 OP_VALUE, OP_DX, OP_DY, OP_DZ, OP_X, OP_Y, OP_Z, OP_DIV, OP_DOT_NORMAL = range(9)
 FN_VALUE, FN_SQRT, FN_INV = 0, 1, 2
-PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_FP16X3 = 0, 1, 2, 3
+PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_FP16X3, PREC_INT8SPLIT = 0, 1, 2, 3, 4
 
 
 class Coef(ct.Structure):

@@ -819,6 +819,202 @@ __global__ __launch_bounds__(256) void gemm_bmat_bf16x3(GemmArgs g) { gemm_mixed
 __global__ __launch_bounds__(256) void gemm_gram_f16x3(GemmArgs g) { gemm_mixed_body<DPG_PREC_FP16X3>(g); }
 __global__ __launch_bounds__(256) void gemm_bmat_f16x3(GemmArgs g) { gemm_mixed_body<DPG_PREC_FP16X3>(g); }
 
+// ------------------------------------------- int8 digit split of the Gram
+// Producer: one thread per test dof j (rows N..NR-1 write zero digits). Pass 1
+// sums the squares of the operand column (the DPOEQU factor s_j, the same sum
+// as ip_operand_kernel's); pass 2 recomputes the values for 16 points at a
+// time, equilibrates them (|x| <= 1), splits each into kI8Digits signed
+// digits by exact steps (64x, then rint and 128 * remainder) and stores 16
+// bytes per plane: the 16 points are half of a 32-point k-chunk of one term.
+template <int NPR>
+__global__ __launch_bounds__(256) void ip_digits_kernel(const double* __restrict__ Wip, int n_lt, int nvar,
+                                                        const double* __restrict__ R,
+                                                        const int32_t* __restrict__ vod, int N, int QG, int QGp,
+                                                        int NR, int8_t* __restrict__ dig, int64_t sd,
+                                                        double* __restrict__ s, int64_t ss) {
+  constexpr int S = kI8Digits, QB = 16;
+  extern __shared__ double Wl[];  // QB points x (n_lt, nvar, NPR)
+  const int wslice = n_lt * nvar * NPR;
+  const int c = blockIdx.y, t = threadIdx.x;
+  const int j = blockIdx.x * blockDim.x + t;
+  const bool active = j < N;
+  const int var = active ? vod[j] : 0;
+  double diag = 0.0;
+  for (int q0 = 0; q0 < QG; q0 += QB) {
+    const int nq = min(QB, QG - q0);
+    __syncthreads();
+    const double* Wp = Wip + ((size_t)c * QG + q0) * wslice;
+    for (int e = t; e < nq * wslice; e += blockDim.x) Wl[e] = Wp[e];
+    __syncthreads();
+    if (!active) continue;
+    for (int e = 0; e < nq; ++e) {
+      double r[NPR];
+#pragma unroll
+      for (int k = 0; k < NPR; ++k) r[k] = R[((size_t)(q0 + e) * NPR + k) * N + j];
+      for (int lt = 0; lt < n_lt; ++lt) {
+        const double* w = Wl + ((e * n_lt + lt) * nvar + var) * NPR;
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < NPR; ++k) acc = fma(w[k], r[k], acc);
+        diag = fma(acc, acc, diag);
+      }
+    }
+  }
+  const double sj = 1.0 / sqrt(diag);
+  if (active) s[(size_t)c * ss + j] = sj;
+  const double sx = (active && diag > 0.0) ? sj : 0.0;  // an all-zero column stays zero
+  const int nch = n_lt * QGp / 32;
+  int8_t* dc = dig + (size_t)c * sd + (size_t)j * 32;
+  for (int q0 = 0; q0 < QGp; q0 += QB) {
+    const int nq = max(0, min(QB, QG - q0));
+    __syncthreads();
+    const double* Wp = Wip + ((size_t)c * QG + q0) * wslice;
+    for (int e = t; e < nq * wslice; e += blockDim.x) Wl[e] = Wp[e];
+    __syncthreads();
+    if (j >= NR) continue;
+    double r[QB][NPR];
+#pragma unroll
+    for (int e = 0; e < QB; ++e)
+#pragma unroll
+      for (int k = 0; k < NPR; ++k) r[e][k] = (active && e < nq) ? R[((size_t)(q0 + e) * NPR + k) * N + j] : 0.0;
+    const int h = (q0 / QB) & 1;
+    for (int lt = 0; lt < n_lt; ++lt) {
+      uint32_t pk[S][4];
+#pragma unroll
+      for (int p = 0; p < S; ++p)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) pk[p][u] = 0u;
+#pragma unroll
+      for (int e = 0; e < QB; ++e) {
+        double v = 0.0;
+        if (e < nq) {  // points past QG are padding (and their LDS slots are not loaded)
+          const double* w = Wl + ((e * n_lt + lt) * nvar + var) * NPR;
+#pragma unroll
+          for (int k = 0; k < NPR; ++k) v = fma(w[k], r[e][k], v);
+        }
+        double rr = v * sx * 64.0;
+#pragma unroll
+        for (int p = 0; p < S; ++p) {
+          const double d = rint(rr);
+          rr = (rr - d) * 128.0;
+          pk[p][e >> 2] |= ((uint32_t)(int)d & 0xFFu) << (8 * (e & 3));
+        }
+      }
+      const int chunk = (lt * QGp + q0) >> 5;
+#pragma unroll
+      for (int p = 0; p < S; ++p)
+        *reinterpret_cast<uint4*>(dc + (size_t)(p * nch + chunk) * NR * 32 + 16 * h) =
+            make_uint4(pk[p][0], pk[p][1], pk[p][2], pk[p][3]);
+    }
+  }
+}
+
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x16 __attribute__((ext_vector_type(16)));
+
+// 64x64 lower tiles, 4 waves of 32x32 sub-tiles. Per 32-k chunk the digit
+// planes of the tile's rows (and, off the diagonal, of its columns) are staged
+// through LDS ([plane][row][32 bytes], rows padded to 48 bytes); lane l reads
+// row l%32, bytes 16*(l/32) .. +16 of a plane as its MFMA operand.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void gemm_gram_i8(I8GramArgs a) {
+  constexpr int S = kI8Digits, RL = 48, PER = S * 128 / 256;  // 16-byte pieces per thread and operand
+  int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int cell = b / a.ntiles, tile = b % a.ntiles;
+  if (cell >= a.ncells) return;
+  int ti, tj;
+  tile_coords(tile, 1, a.tiles_m, ti, tj);
+  const int i0 = ti * 64, j0 = tj * 64;
+  const bool dtile = ti == tj;
+  __shared__ __attribute__((aligned(16))) int8_t Xs[S][64][RL];
+  __shared__ __attribute__((aligned(16))) int8_t Ys[S][64][RL];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
+  const int8_t* __restrict__ D = a.dig + (size_t)cell * a.sd;
+  const size_t plane = (size_t)a.nch * a.NR * 32;
+  i32x16 acc[S];
+#pragma unroll
+  for (int l = 0; l < S; ++l)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[l][v] = 0;
+  // piece e = t + 256 r: plane e/128, row (e/2)%64, half e%2
+  i32x4 xr[PER], yr[PER];
+  auto fetch = [&](int c) {
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+      const int e = t + 256 * r, p = e >> 7, row = (e >> 1) & 63, h = e & 1;
+      const int8_t* src = D + p * plane + ((size_t)c * a.NR + row) * 32 + 16 * h;
+      xr[r] = *reinterpret_cast<const i32x4*>(src + (size_t)i0 * 32);
+      yr[r] = *reinterpret_cast<const i32x4*>(src + (size_t)(dtile ? i0 : j0) * 32);  // a diagonal tile reads its rows twice (L1 hits)
+    }
+  };
+  auto stage = [&]() {
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+      const int e = t + 256 * r, p = e >> 7, row = (e >> 1) & 63, h = e & 1;
+      *reinterpret_cast<i32x4*>(&Xs[p][row][16 * h]) = xr[r];
+      *reinterpret_cast<i32x4*>(&Ys[p][row][16 * h]) = yr[r];
+    }
+    __syncthreads();
+  };
+  const int fr = lane & 31, fk = 16 * (lane >> 5);
+  auto multiply = [&]() {
+    i32x4 bf[S];
+#pragma unroll
+    for (int q = 0; q < S; ++q) bf[q] = *reinterpret_cast<const i32x4*>(&Ys[q][wc * 32 + fr][fk]);
+#pragma unroll
+    for (int p = 0; p < S; ++p) {
+      const i32x4 af = *reinterpret_cast<const i32x4*>(&Xs[p][wr * 32 + fr][fk]);
+#pragma unroll
+      for (int q = 0; q < S - p; ++q) acc[p + q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af, bf[q], acc[p + q], 0, 0, 0);
+    }
+  };
+  // k-chunks of segments that are zero for this tile are skipped (kseg/32 chunks per segment)
+  const uint64_t km = a.kmask ? a.kmask[tile] : ~0ull;
+  const int cps = a.kmask ? (int)(a.kseg / 32) : a.nch;
+  int seg = 0, segend = cps;
+  auto skip_zero_segments = [&](int c) {
+    while (c < a.nch && !((km >> seg) & 1ull)) {
+      c = segend;
+      ++seg;
+      segend += cps;
+    }
+    return c;
+  };
+  auto next_chunk = [&](int c) {
+    if (c >= a.nch) return c;
+    ++c;
+    if (c >= segend) {
+      ++seg;
+      segend += cps;
+      c = skip_zero_segments(c);
+    }
+    return c;
+  };
+  const int si0 = i0 + wr * 32, sj0 = j0 + wc * 32;
+  const bool busy = si0 < a.N && sj0 < a.N && si0 + 31 >= sj0;
+  int kc = skip_zero_segments(0);
+  if (kc < a.nch) fetch(kc);
+  while (kc < a.nch) {
+    stage();
+    const int kn = next_chunk(kc);
+    if (kn < a.nch) fetch(kn);
+    if (busy) multiply();
+    kc = kn;
+  }
+  // 32x32 i32 result layout: D[a][b] (a the first operand's row, b the second's)
+  // in lane b + 32*((a/4)%2), element 4*(a/8) + a%4
+  double* __restrict__ C = a.C + (size_t)cell * a.sc;
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    const int i = si0 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+    const int j = sj0 + (lane & 31);
+    double val = 0.0;
+#pragma unroll
+    for (int l = S - 1; l >= 0; --l) val += ldexp((double)acc[l][v], 2 - 7 * (l + 2));  // exact scaling
+    if (i < a.N && j < a.N && i >= j) C[(size_t)i + (size_t)j * a.ldc] = val;
+  }
+}
+
 // ------------------------------------------------------------ panel kernels
 // Columns [k0, k0+kb) (kb <= 64) of each cell's lower-stored matrix:
 //   potrf_wave   one wavefront per cell: POTRF of the diagonal block in
@@ -1343,6 +1539,33 @@ void launch_gemm(GemmArgs g, int role, hipStream_t st, int tile) {
     else gemm_update_f64_t32<<<grid, 64, 0, st>>>(g);
   }
   check_launch("gemm_tn_f64");
+}
+
+void launch_ip_digits(int dim, const double* Wip, int n_lt, int nvar, const double* R, const int32_t* vod, int N,
+                      int QG, int QGp, int NR, int ncells, int8_t* dig, int64_t sd, double* s, int64_t ss,
+                      hipStream_t st) {
+  if (QGp % 32 != 0 || NR % 64 != 0 || NR < N) throw DeviceError("ip_digits: QGp must be a multiple of 32, NR of 64");
+  size_t shm = sizeof(double) * 16 * (size_t)n_lt * nvar * (dim + 1);
+  dim3 grid((NR + 255) / 256, ncells);
+  if (dim == 2)
+    ip_digits_kernel<3><<<grid, 256, shm, st>>>(Wip, n_lt, nvar, R, vod, N, QG, QGp, NR, dig, sd, s, ss);
+  else
+    ip_digits_kernel<4><<<grid, 256, shm, st>>>(Wip, n_lt, nvar, R, vod, N, QG, QGp, NR, dig, sd, s, ss);
+  check_launch("ip_digits_kernel");
+}
+
+void launch_gram_i8(I8GramArgs a, hipStream_t st) {
+  if (a.N <= 0 || a.ncells <= 0) return;
+  if (a.NR % 64 != 0 || a.NR < a.N) throw DeviceError("gram_i8: NR must be a multiple of 64 and >= N");
+  if (a.kmask && (a.kseg % 32 != 0 || a.kseg <= 0)) throw DeviceError("gram_i8: kseg must be a multiple of 32");
+  // exact int32 sums: a level holds at most kI8Digits products of |d| <= 64 per k
+  if ((int64_t)a.nch * 32 * kI8Digits * 64 * 64 >= (int64_t)1 << 31) throw DeviceError("gram_i8: k too large");
+  a.tiles_m = (a.N + 63) / 64;
+  a.ntiles = a.tiles_m * (a.tiles_m + 1) / 2;
+  int64_t nblocks = (int64_t)a.ntiles * a.ncells;
+  if (nblocks > 0x7fffffff) throw DeviceError("gemm grid too large");
+  gemm_gram_i8<<<dim3((unsigned)nblocks), 256, 0, st>>>(a);
+  check_launch("gemm_gram_i8");
 }
 
 void launch_gemm_mixed(GemmArgs g, int precision, int role, hipStream_t st) {

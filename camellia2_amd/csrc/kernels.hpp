@@ -65,6 +65,33 @@ void launch_gram_sf(int dim, const SfBlockPair* bps, int nbp, const SfPair* pair
 // DPG_PREC_FP32) or to hi+lo BF16/FP16 pairs (DPG_PREC_BF16X3/FP16X3, products
 // hi*hi + hi*lo + lo*hi), FP32 MFMA accumulation, FP64 epilogue
 void launch_gemm_mixed(GemmArgs g, int precision, int role, hipStream_t st);
+// Int8 digit split of the Gram product (precision DPG_PREC_INT8SPLIT).
+// The equilibrated operand x(k,j) = s_j out(k,j), with s_j = 1/||out(:,j)|| so
+// that |x| <= 1, is written as kI8Digits signed digits of 7 bits:
+//   x = sum_p d_p 2^(1-7p) + r,   d_p in [-64, 64],   |r| <= 2^(-7 kI8Digits).
+// Digit planes, bytes per cell: kI8Digits * (KG/32) * NR * 32, plane p, k-chunk c
+// (32 consecutive k), row j at ((p*(KG/32) + c)*NR + j)*32 + k%32; NR = N rounded
+// up to 64, rows N..NR-1 zero. s[c][j] as launch_ip_operand.
+constexpr int kI8Digits = 8;
+void launch_ip_digits(int dim, const double* Wip, int n_lt, int nvar, const double* R, const int32_t* vod, int N,
+                      int QG, int QGp, int NR, int ncells, int8_t* dig, int64_t sd, double* s, int64_t ss,
+                      hipStream_t st);
+// G(i,j) (lower, i >= j, rows and columns < N) = sum_k x(k,i) x(k,j) from the
+// digit planes: the digit products d_p(i) d_q(j) with p + q <= kI8Digits + 1
+// on v_mfma_i32_32x32x32_i8, exact int32 sums per level p + q, the levels
+// scaled by 2^(2-7(p+q)) and added in FP64 from the smallest. kmask/kseg as
+// GemmArgs, kseg in k (a multiple of 32).
+struct I8GramArgs {
+  const int8_t* dig;
+  int64_t sd;  // bytes per cell
+  int32_t N, NR, nch, ncells;
+  double* C;
+  int64_t ldc, sc;
+  const uint64_t* kmask;
+  int64_t kseg;
+  int32_t tiles_m, ntiles;
+};
+void launch_gram_i8(I8GramArgs a, hipStream_t st);
 // Ld: per-cell (rows, 64) store of the factored diagonal blocks (row k0 + r, column r');
 // Lt: per-cell 64x64 scratch for the current diagonal block, column-major, with
 // the reciprocals of its diagonal

@@ -200,7 +200,9 @@ struct BucketPlan {
   double flops = 0.0;
   double gram_flops = 0.0;  // sum over IP terms of QG * n(n+1), n = dofs the term touches
   int tile[3] = {64, 64, 64};  // FP64 GEMM tile edge per role: Gram, B blocks, Cholesky updates
-  int QGp = 0;  // Gram points per IP term in the operand, padded to the 16-row k-chunk
+  int QGp = 0;  // Gram points per IP term in the operand, padded to the k-chunk (16 rows; 32 for int8 digits)
+  int NR = 0;         // int8 digit planes: rows (N rounded up to 64)
+  int64_t nDig = 0;   // int8 digit planes: bytes per cell
   // per lower tile of G (64- and 32-wide tiles): bit lt set when IP term lt
   // reaches both the tile's rows and its columns
   const uint64_t* gmask[2] = {nullptr, nullptr};
@@ -496,8 +498,15 @@ void build_bucket(dpg_plan* plan, const dpg_bucket_desc& desc, BucketPlan& B) {
   B.Npad = round_up(L.N, 8);
   int S = L.rc.nsides, nvar = f.n_test;
   B.nA = B.lda * B.D;
-  B.QGp = (int)round_up(L.QG, 16);
+  const bool i8 = plan->opt.precision == DPG_PREC_INT8SPLIT;
+  B.QGp = (int)round_up(L.QG, i8 ? 32 : 16);
   B.nAt = (int64_t)B.prog.n_lt * B.QGp * B.Npad;  // weighted IP-term values, term-major rows
+  if (i8) {
+    // the same region holds the operand's int8 digit planes instead
+    B.NR = (int)round_up(L.N, 64);
+    B.nDig = (int64_t)kI8Digits * B.prog.n_lt * B.QGp * B.NR;
+    B.nAt = std::max(B.nAt, (B.nDig + 7) / 8);
+  }
   {
     // test variables each IP term reaches
     std::vector<uint64_t> tv(f.n_test, 0);
@@ -662,7 +671,12 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
   // operands: weighted IP-term values (with the DPOEQU factors), B operands
   e = P.start(st);
   const int KG = B.prog.n_lt * B.QGp;
-  if (!sf)
+  const bool i8 = prec == DPG_PREC_INT8SPLIT;
+  if (i8 && Gout) throw std::invalid_argument("G is an output of the FP64 and FP32-class precisions only");
+  if (i8)
+    launch_ip_digits(dim, w.Wip, B.prog.n_lt, nvar, B.tab.R_ip, B.tab.var_of_dof, N, L.QG, B.QGp, B.NR, nb,
+                     (int8_t*)w.At, B.nAt * (int64_t)sizeof(double), w.s, B.nS, st);
+  else if (!sf)
     launch_ip_operand(dim, w.Wip, B.prog.n_lt, nvar, B.tab.R_ip, B.tab.var_of_dof, N, L.QG, B.QGp, nb, w.At, B.nAt,
                       B.Npad, w.s, B.nS, st);
   launch_operand(dim, w.Wvol, B.prog.n_vb, nvar, B.tab.R_vol, B.tab.var_of_dof, N, L.Q, nb, w.Tv, B.nTv,
@@ -713,13 +727,30 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
   g.ss = B.nS;
   g.kmask = B.gmask[B.tile[0] == 64 ? 0 : 1];
   g.kseg = B.QGp;
-  if (prec != DPG_PREC_FP64) {
+  if (prec != DPG_PREC_FP64 && !i8) {
     // low-precision operands are pre-scaled (keeps FP16 in range): X(k,i) s_i, Y(k,j) s_j
     g.srow = g.scol = nullptr;
     g.px = g.py = s_ep;
     g.kmask = nullptr;
   }
-  if (!sf) {
+  if (i8) {
+    // G = x^T x from the digit planes (x already carries the DPOEQU factors)
+    I8GramArgs a{};
+    a.dig = (const int8_t*)w.At;
+    a.sd = B.nAt * (int64_t)sizeof(double);
+    a.N = N;
+    a.NR = B.NR;
+    a.nch = KG / 32;
+    a.ncells = nb;
+    a.C = w.A;
+    a.ldc = lda;
+    a.sc = sA;
+    a.kmask = B.gmask[0];
+    a.kseg = B.QGp;
+    e = P.start(st);
+    launch_gram_i8(a, st);
+    P.stop(e, st, DPG_STAGE_GRAM, nb * B.gram_flops, nb * ((double)KG * B.NR * kI8Digits + N * (N + 1.0) / 2.0 * D8));
+  } else if (!sf) {
     e = P.start(st);
     if (prec == DPG_PREC_FP64) launch_gemm(g, kGemmGram, st, B.tile[0]);
     else launch_gemm_mixed(g, prec, kGemmGram, st);
@@ -752,7 +783,7 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
     b.ncells = nb;
     b.scol = s_ep;
     b.ss = B.nS;
-    if (prec == DPG_PREC_FP64) {
+    if (prec == DPG_PREC_FP64 || i8) {
       launch_gemm(b, kGemmBmat, st, B.tile[1]);
     } else {
       b.scol = nullptr;
@@ -783,7 +814,7 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
       b.ncells = nb;
       b.scol = s_ep;
       b.ss = B.nS;
-      if (prec == DPG_PREC_FP64) {
+      if (prec == DPG_PREC_FP64 || i8) {
         launch_gemm(b, kGemmBmat, st, B.tile[1]);
       } else {
         b.scol = nullptr;
@@ -907,7 +938,7 @@ int dpg_plan_create(const dpg_form_desc* form, const dpg_bucket_desc* buckets, i
     std::unique_ptr<dpg_plan> plan(new dpg_plan());
     plan->form = *form;
     if (options) plan->opt = *options;
-    if (plan->opt.precision < DPG_PREC_FP64 || plan->opt.precision > DPG_PREC_FP16X3)
+    if (plan->opt.precision < DPG_PREC_FP64 || plan->opt.precision > DPG_PREC_INT8SPLIT)
       throw std::invalid_argument("unknown precision");
     if (plan->opt.refine != 0)
       throw std::invalid_argument("refine: a refinement of G^-1 B cannot correct an error that sits in G itself "

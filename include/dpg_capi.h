@@ -61,7 +61,11 @@ enum {
   DPG_OP_DOT_NORMAL = 8  /* tau . n  (boundary-only)          */
 };
 enum { DPG_FN_VALUE = 0, DPG_FN_SQRT = 1, DPG_FN_INV = 2 };
-enum { DPG_PREC_FP64 = 0, DPG_PREC_FP32 = 1, DPG_PREC_BF16X3 = 2, DPG_PREC_FP16X3 = 3 };
+/* DPG_PREC_INT8SPLIT: the Gram product G = A^T A on int8 MFMA with the
+   equilibrated operand split into 8 signed 7-bit digits (error-free splitting;
+   digit products exact in int32, levels summed in FP64); B, l and everything
+   after them FP64. G is not available as an output in this mode. */
+enum { DPG_PREC_FP64 = 0, DPG_PREC_FP32 = 1, DPG_PREC_BF16X3 = 2, DPG_PREC_FP16X3 = 3, DPG_PREC_INT8SPLIT = 4 };
 
 /* coefficient of a scalar product: scale * [fn(coef[c][slot])] * [n_comp] */
 typedef struct {
