@@ -118,7 +118,8 @@ EXPORTS = [
     "dpg_energy_error", "dpg_adjoint", "dpg_mesh_quad_graded", "dpg_plan_get_tiles", "dpg_plan_set_tiles",
     "dpg_plan_autotune", "dpg_graph_create", "dpg_graph_launch", "dpg_graph_workspace", "dpg_graph_destroy",
     "dpg_skeleton_numbering", "dpg_skeleton_nodes", "dpg_skeleton_create", "dpg_skeleton_info",
-    "dpg_skeleton_pattern", "dpg_skeleton_assemble", "dpg_skeleton_solve", "dpg_mark_cells", "dpg_skeleton_destroy",
+    "dpg_skeleton_pattern", "dpg_skeleton_assemble", "dpg_skeleton_solve", "dpg_skeleton_entities",
+    "dpg_skeleton_set_blocks", "dpg_mark_cells", "dpg_skeleton_destroy",
     "dpg_recover_fields", "dpg_riesz_representation", "dpg_side_parities", "dpg_sort_cells", "dpg_gather_cells",
 ]
 
@@ -167,6 +168,8 @@ def lib():
         L.dpg_graph_workspace.argtypes = [ct.c_void_p, P(ct.c_size_t)]
         L.dpg_graph_destroy.argtypes = [ct.c_void_p]
         L.dpg_skeleton_numbering.argtypes = [P(FormDesc), P(BucketDesc), ct.c_int, P(ct.c_int32), P(ct.c_int32)]
+        L.dpg_skeleton_entities.argtypes = [P(FormDesc), P(BucketDesc), ct.c_int, P(ct.c_int32), P(ct.c_int32)]
+        L.dpg_skeleton_set_blocks.argtypes = [ct.c_void_p, P(ct.c_int32), ct.c_int, ct.c_void_p]
         L.dpg_skeleton_nodes.argtypes = [P(FormDesc), P(BucketDesc), P(ct.c_double)]
         L.dpg_skeleton_create.argtypes = [ct.c_int, ct.c_int, P(ct.c_int32), ct.c_int, ct.c_void_p,
                                           P(ct.c_void_p)]

@@ -41,6 +41,7 @@
 #include "../../include/dpg_capi.h"
 #include "element.hpp"
 #include "errors.hpp"
+#include "kernels.hpp"
 
 #define SK_CHECK(expr)                                                                            \
   do {                                                                                            \
@@ -67,6 +68,11 @@ int side_degree(const ElementLayout& L, const TrialBlock& b) {
 struct Numbering {
   int nglobal = 0;
   std::vector<int32_t> gdof;  // (C, T)
+  // mesh entity of every global dof: vertices [0, nv), edges [nv, nv+ne), faces
+  // (3D) [nv+ne, nv+ne+nf); a side's flux dofs share its entity with the trace
+  // dofs interior to it
+  std::vector<int32_t> entity;
+  int nentities = 0;
 };
 
 Numbering number_skeleton(const ElementLayout& L, int C, const int32_t* cv) {
@@ -127,6 +133,8 @@ Numbering number_skeleton(const ElementLayout& L, int C, const int32_t* cv) {
   Numbering out;
   out.nglobal = next;
   out.gdof.assign((size_t)C * T, -1);
+  out.entity.assign((size_t)next, -1);
+  out.nentities = nv + ne + (dim == 3 ? nf : 0);
   for (int c = 0; c < C; ++c)
     for (const TrialBlock& b : sb) {
       const int n = side_degree(L, b), base = var_base[b.var];
@@ -138,11 +146,12 @@ Numbering number_skeleton(const ElementLayout& L, int C, const int32_t* cv) {
         for (int i = 0; i <= n; ++i) {
           const int ci = A < B ? i : n - i;  // position from the smaller vertex id
           if (b.kind == DPG_TRACE) {
-            if (i == 0) g[i] = base + A;
-            else if (i == n) g[i] = base + B;
-            else g[i] = base + nv + e * (n - 1) + (ci - 1);
+            if (i == 0) g[i] = base + A, out.entity[g[i]] = A;
+            else if (i == n) g[i] = base + B, out.entity[g[i]] = B;
+            else g[i] = base + nv + e * (n - 1) + (ci - 1), out.entity[g[i]] = nv + e;
           } else {
             g[i] = base + e * (n + 1) + ci;
+            out.entity[g[i]] = nv + e;
           }
         }
       } else {
@@ -163,15 +172,17 @@ Numbering number_skeleton(const ElementLayout& L, int C, const int32_t* cv) {
           for (int aa = 0; aa <= n; ++aa) {
             const int dx = aa - pos[m][0], dy = bb - pos[m][1];
             const int u = n > 0 ? (dx * ux + dy * uy) / n : 0, v = n > 0 ? (dx * vx + dy * vy) / n : 0;
-            int32_t gg;
+            int32_t gg, ent;
             if (b.kind == DPG_FLUX) {
               gg = base + fi * (n + 1) * (n + 1) + u + v * (n + 1);
+              ent = nv + ne + fi;
             } else {
               int corner = -1;
               for (int k = 0; k < 4; ++k)
                 if (pos[k][0] == aa && pos[k][1] == bb) corner = k;
               if (corner >= 0) {
                 gg = base + G[corner];
+                ent = G[corner];
               } else if (aa == 0 || aa == n || bb == 0 || bb == n) {
                 int k0, k1, t;  // face edge (k0, k1), distance t from corner k0
                 if (bb == 0) {
@@ -186,16 +197,21 @@ Numbering number_skeleton(const ElementLayout& L, int C, const int32_t* cv) {
                 const int e = edges.at(edge_key(G[k0], G[k1]));
                 const int cd = G[k0] < G[k1] ? t : n - t;
                 gg = base + nv + e * (n - 1) + (cd - 1);
+                ent = nv + e;
               } else {
                 gg = base + nv + ne * (n - 1) + fi * (n - 1) * (n - 1) + (u - 1) + (v - 1) * (n - 1);
+                ent = nv + ne + fi;
               }
             }
             g[aa + bb * (n + 1)] = gg;
+            out.entity[gg] = ent;
           }
       }
     }
   for (int32_t v : out.gdof)
     if (v < 0 || v >= out.nglobal) throw std::logic_error("skeleton numbering left a dof unassigned");
+  for (int32_t v : out.entity)
+    if (v < 0 || v >= out.nentities) throw std::logic_error("skeleton numbering left a dof without entity");
   return out;
 }
 
@@ -341,6 +357,102 @@ __global__ void cg_direction(double beta, const double* __restrict__ z, int64_t 
   p[i] = fma(beta, p[i], z[i]);
 }
 
+// Block-Jacobi over mesh entities. Block b holds the global dofs
+// dofs[ptr[b] .. ptr[b+1]) (nb <= 64); its dense diagonal block of the matrix,
+// row-major at off2[b], rows and columns of fixed dofs replaced by the identity.
+// One workgroup of 64 threads per block; entries by binary search in the
+// sorted columns of the row.
+__global__ __launch_bounds__(64) void block_extract(const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                    const double* __restrict__ val, const int32_t* __restrict__ fixed,
+                                                    const int32_t* __restrict__ ptr, const int32_t* __restrict__ dofs,
+                                                    const int64_t* __restrict__ off2, double* __restrict__ blk) {
+  const int b = blockIdx.x, p0 = ptr[b], nb = ptr[b + 1] - p0;
+  double* out = blk + off2[b];
+  for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
+    const int i = e / nb, j = e % nb, di = dofs[p0 + i], dj = dofs[p0 + j];
+    double v;
+    if (fixed[di] || fixed[dj]) {
+      v = i == j ? 1.0 : 0.0;
+    } else {
+      int64_t lo = rp[di], hi = rp[di + 1];
+      while (lo < hi) {
+        const int64_t mid = (lo + hi) / 2;
+        if (col[mid] < dj) lo = mid + 1;
+        else hi = mid;
+      }
+      v = (lo < rp[di + 1] && col[lo] == dj) ? val[lo] : 0.0;
+    }
+    out[e] = v;
+  }
+}
+
+// In place: every block <- its inverse, by Cholesky L L^T in LDS (lane = row),
+// the columns of L^-1 by substitution (lane = column) and (L^-1)^T L^-1 (lane =
+// row). A non-positive pivot sets *bad.
+__global__ __launch_bounds__(64) void block_invert(const int32_t* __restrict__ ptr, const int64_t* __restrict__ off2,
+                                                   int nbmax, double* __restrict__ blk, int32_t* __restrict__ bad) {
+  extern __shared__ double sh[];
+  const int b = blockIdx.x, nb = ptr[b + 1] - ptr[b], t = threadIdx.x;
+  double* a = sh;                        // nb x nb, row-major
+  double* li = sh + (size_t)nbmax * nbmax;  // L^-1, row-major
+  double* g = blk + off2[b];
+  for (int e = t; e < nb * nb; e += blockDim.x) a[e] = g[e];
+  __syncthreads();
+  for (int k = 0; k < nb; ++k) {
+    if (t == k) {
+      const double d = a[k * nb + k];
+      if (!(d > 0.0)) atomicExch(bad, 1);
+      a[k * nb + k] = d > 0.0 ? sqrt(d) : 1.0;
+    }
+    __syncthreads();
+    if (t > k && t < nb) a[t * nb + k] /= a[k * nb + k];
+    __syncthreads();
+    if (t > k && t < nb)
+      for (int j = k + 1; j <= t; ++j) a[t * nb + j] = fma(-a[t * nb + k], a[j * nb + k], a[t * nb + j]);
+    __syncthreads();
+  }
+  if (t < nb) {  // column t of L^-1: L x = e_t
+    for (int j = 0; j < t; ++j) li[j * nb + t] = 0.0;
+    li[t * nb + t] = 1.0 / a[t * nb + t];
+    for (int j = t + 1; j < nb; ++j) {
+      double acc = 0.0;
+      for (int k = t; k < j; ++k) acc = fma(a[j * nb + k], li[k * nb + t], acc);
+      li[j * nb + t] = -acc / a[j * nb + j];
+    }
+  }
+  __syncthreads();
+  if (t < nb)  // row t of (L^-1)^T L^-1
+    for (int j = 0; j < nb; ++j) {
+      double acc = 0.0;
+      for (int k = t > j ? t : j; k < nb; ++k) acc = fma(li[k * nb + t], li[k * nb + j], acc);
+      g[t * nb + j] = acc;
+    }
+}
+
+// z = (block inverse) r on the free dofs, 0 on the fixed ones; one thread per
+// position of the block-ordered dof list
+__global__ void block_apply(const int32_t* __restrict__ ptr, const int32_t* __restrict__ dofs,
+                            const int32_t* __restrict__ pos_block, const int64_t* __restrict__ off2,
+                            const double* __restrict__ inv, const int32_t* __restrict__ fixed,
+                            const double* __restrict__ r, int64_t npos, double* __restrict__ z) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npos) return;
+  const int b = pos_block[p], p0 = ptr[b], nb = ptr[b + 1] - p0, i = (int)(p - p0), di = dofs[p];
+  if (fixed[di]) {
+    z[di] = 0.0;
+    return;
+  }
+  const double* row = inv + off2[b] + (int64_t)i * nb;
+  double acc = 0.0;
+  for (int j = 0; j < nb; ++j) acc = fma(row[j], r[dofs[p0 + j]], acc);
+  z[di] = acc;
+}
+
+__global__ void copy_f64(const double* __restrict__ a, int64_t n, double* __restrict__ b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
 __global__ void mark_above(const double* __restrict__ err, int n, const double* __restrict__ emax, double rel,
                            int32_t* __restrict__ flags, double* __restrict__ fcount) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -433,6 +545,11 @@ struct dpg_skeleton {
   dpg::DevArray<uint32_t> rperm, rseg;  // load vector
   dpg::DevArray<int64_t> row_ptr;
   dpg::DevArray<int32_t> col;
+  // block-Jacobi blocks (dpg_skeleton_set_blocks); none: point Jacobi
+  int nblocks = 0, nbmax = 0;
+  int64_t blk_total = 0;  // sum of the squared block sizes
+  dpg::DevArray<int32_t> blk_ptr, blk_dofs, pos_block;
+  dpg::DevArray<int64_t> blk_off2;
 };
 
 extern "C" {
@@ -448,6 +565,20 @@ int dpg_skeleton_numbering(const dpg_form_desc* form, const dpg_bucket_desc* buc
     dpg::Numbering nb = dpg::number_skeleton(L, ncells, cell_vertices);
     std::memcpy(gdof, nb.gdof.data(), nb.gdof.size() * sizeof(int32_t));
     return nb.nglobal;
+  });
+}
+
+int dpg_skeleton_entities(const dpg_form_desc* form, const dpg_bucket_desc* bucket, int ncells,
+                          const int32_t* cell_vertices, int32_t* entity) {
+  return dpg::guarded_status([&] {
+    if (!form || !bucket || ncells < 0 || (ncells > 0 && (!cell_vertices || !entity)))
+      throw std::invalid_argument("NULL argument or negative cell count");
+    dpg::ElementLayout L = dpg::make_layout(*form, *bucket);
+    if (L.T <= 0) throw std::invalid_argument("the form has no trace or flux variables");
+    if (ncells == 0) return 0;
+    dpg::Numbering nb = dpg::number_skeleton(L, ncells, cell_vertices);
+    std::memcpy(entity, nb.entity.data(), nb.entity.size() * sizeof(int32_t));
+    return nb.nentities;
   });
 }
 
@@ -554,6 +685,53 @@ int dpg_skeleton_assemble(const dpg_skeleton* sk, const double* schur, const dou
   });
 }
 
+int dpg_skeleton_set_blocks(dpg_skeleton* sk, const int32_t* entity, int nentities, void* stream) {
+  return dpg::guarded([&] {
+    if (!sk || nentities < 0 || (nentities > 0 && !entity)) throw std::invalid_argument("bad arguments");
+    hipStream_t st = (hipStream_t)stream;
+    sk->nblocks = sk->nbmax = 0;
+    sk->blk_total = 0;
+    if (nentities == 0) return;  // back to point Jacobi
+    const int64_t n = sk->nglobal;
+    // counting sort of the dofs by entity (ascending dof within an entity); empty entities dropped
+    std::vector<int32_t> count((size_t)nentities + 1, 0);
+    for (int64_t i = 0; i < n; ++i) {
+      if (entity[i] < 0 || entity[i] >= nentities) throw std::invalid_argument("entity id out of range");
+      ++count[(size_t)entity[i] + 1];
+    }
+    std::vector<int32_t> start((size_t)nentities + 1, 0);
+    for (int e = 0; e < nentities; ++e) start[(size_t)e + 1] = start[e] + count[(size_t)e + 1];
+    std::vector<int32_t> dofs((size_t)n), fill(start.begin(), start.end() - 1);
+    for (int64_t i = 0; i < n; ++i) dofs[(size_t)fill[(size_t)entity[i]]++] = (int32_t)i;
+    std::vector<int32_t> ptr(1, 0), pos_block((size_t)n);
+    std::vector<int64_t> off2(1, 0);
+    int nbmax = 0;
+    for (int e = 0; e < nentities; ++e) {
+      const int nb = start[(size_t)e + 1] - start[e];
+      if (nb == 0) continue;
+      if (nb > 64) throw std::invalid_argument("an entity holds more than 64 dofs (block-Jacobi limit)");
+      const int b = (int)ptr.size() - 1;
+      for (int k = start[e]; k < start[(size_t)e + 1]; ++k) pos_block[(size_t)k] = b;
+      ptr.push_back(start[(size_t)e + 1]);
+      off2.push_back(off2.back() + (int64_t)nb * nb);
+      nbmax = std::max(nbmax, nb);
+    }
+    sk->nblocks = (int)ptr.size() - 1;
+    sk->nbmax = nbmax;
+    sk->blk_total = off2.back();
+    sk->blk_ptr.resize(ptr.size());
+    sk->blk_dofs.resize(dofs.size());
+    sk->pos_block.resize(pos_block.size());
+    sk->blk_off2.resize(off2.size());
+    SK_CHECK(hipMemcpyAsync(sk->blk_ptr.p, ptr.data(), sizeof(int32_t) * ptr.size(), hipMemcpyHostToDevice, st));
+    SK_CHECK(hipMemcpyAsync(sk->blk_dofs.p, dofs.data(), sizeof(int32_t) * dofs.size(), hipMemcpyHostToDevice, st));
+    SK_CHECK(hipMemcpyAsync(sk->pos_block.p, pos_block.data(), sizeof(int32_t) * pos_block.size(),
+                            hipMemcpyHostToDevice, st));
+    SK_CHECK(hipMemcpyAsync(sk->blk_off2.p, off2.data(), sizeof(int64_t) * off2.size(), hipMemcpyHostToDevice, st));
+    SK_CHECK(hipStreamSynchronize(st));  // the host vectors go out of scope
+  });
+}
+
 int dpg_skeleton_solve(const dpg_skeleton* sk, const double* values, const double* rhs, const int32_t* fixed,
                        double* x, double tol, int maxit, int* iters, double* relres, void* stream) {
   return dpg::guarded([&] {
@@ -567,6 +745,32 @@ int dpg_skeleton_solve(const dpg_skeleton* sk, const double* values, const doubl
     dpg::spmv_free_rows<<<bw, 256, 0, st>>>(sk->row_ptr.p, sk->col.p, values, x, fixed, n, q.p);
     dpg::cg_start<<<bn, 256, 0, st>>>(rhs, q.p, d.p, fixed, n, r.p, z.p, p.p);
     SK_CHECK(hipGetLastError());
+    // block-Jacobi: the inverses of the entity blocks, z = B^-1 r instead of r / diag
+    const bool blocks = sk->nblocks > 0;
+    dpg::DevArray<double> binv(blocks ? (size_t)sk->blk_total : 0);
+    auto precondition = [&]() {
+      if (blocks)
+        dpg::block_apply<<<bn, 256, 0, st>>>(sk->blk_ptr.p, sk->blk_dofs.p, sk->pos_block.p, sk->blk_off2.p, binv.p,
+                                             fixed, r.p, n, z.p);
+      else
+        dpg::cg_precondition<<<bn, 256, 0, st>>>(r.p, d.p, fixed, n, z.p);
+    };
+    if (blocks) {
+      dpg::DevArray<int32_t> bad(1);
+      dpg::launch_fill_i32(bad.p, 1, 0, st);
+      dpg::block_extract<<<sk->nblocks, 64, 0, st>>>(sk->row_ptr.p, sk->col.p, values, fixed, sk->blk_ptr.p,
+                                                     sk->blk_dofs.p, sk->blk_off2.p, binv.p);
+      dpg::block_invert<<<sk->nblocks, 64, 2 * sizeof(double) * sk->nbmax * sk->nbmax, st>>>(
+          sk->blk_ptr.p, sk->blk_off2.p, sk->nbmax, binv.p, bad.p);
+      SK_CHECK(hipGetLastError());
+      int32_t hb = 0;
+      SK_CHECK(hipMemcpyAsync(&hb, bad.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+      SK_CHECK(hipStreamSynchronize(st));
+      if (hb) throw std::runtime_error("a diagonal block of the free system is not positive definite");
+      precondition();
+      dpg::copy_f64<<<bn, 256, 0, st>>>(z.p, n, p.p);
+      SK_CHECK(hipGetLastError());
+    }
     const double rr0 = dpg::reduce(r.p, r.p, n, 0, part, one, st);
     double rz = dpg::reduce(r.p, z.p, n, 0, part, one, st);
     double rr = rr0;
@@ -580,7 +784,7 @@ int dpg_skeleton_solve(const dpg_skeleton* sk, const double* values, const doubl
       rr = dpg::reduce(r.p, r.p, n, 0, part, one, st);
       ++it;
       if (rr <= tol * tol * rr0) break;
-      dpg::cg_precondition<<<bn, 256, 0, st>>>(r.p, d.p, fixed, n, z.p);
+      precondition();
       const double rz_new = dpg::reduce(r.p, z.p, n, 0, part, one, st);
       dpg::cg_direction<<<bn, 256, 0, st>>>(rz_new / rz, z.p, n, p.p);
       SK_CHECK(hipGetLastError());

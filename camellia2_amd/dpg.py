@@ -368,6 +368,15 @@ def skeleton_numbering(f, bucket, cell_vertices):
     return gdof, n
 
 
+def skeleton_entities(f, bucket, cell_vertices, nglobal):
+    """Mesh entity (vertex, edge, face) of every global dof of the same numbering,
+    and the number of entities (dpg_skeleton_entities)."""
+    cv = np.ascontiguousarray(cell_vertices, dtype=np.int32)
+    ent = np.zeros(int(nglobal), dtype=np.int32)
+    n = check(lib().dpg_skeleton_entities(ct.byref(f), ct.byref(bucket), cv.shape[0], _p32(cv), _p32(ent)))
+    return ent, n
+
+
 def skeleton_nodes(f, bucket):
     """Reference coordinates (T, dim) of the nodes of a cell's trace and flux dofs."""
     s = sizes(f, bucket)
@@ -442,6 +451,14 @@ class CondensedSystem:
         check(lib().dpg_skeleton_info(h, ct.byref(nnz), None, None, None))
         self.nnz = nnz.value
 
+    def set_blocks(self, entity, nentities, stream=None):
+        """Block-Jacobi preconditioning of solve() over the dofs of each entity
+        (dpg_skeleton_set_blocks); nentities = 0 returns to point Jacobi."""
+        e = np.ascontiguousarray(entity, dtype=np.int32)
+        assert e.shape == (self.nglobal,)
+        s = stream.cuda_stream if stream is not None else self._torch.cuda.current_stream().cuda_stream
+        check(lib().dpg_skeleton_set_blocks(self._h, _p32(e), int(nentities), ct.c_void_p(s)))
+
     def pattern(self, stream=None):
         """(row_ptr int64 (nglobal+1), col int32 (nnz)) as device tensors (copies)."""
         t = self._torch
@@ -471,7 +488,8 @@ class CondensedSystem:
 
     def solve(self, values, rhs, fixed, x, tol=1e-12, maxit=10000, stream=None):
         """In place on x: the free dofs of values (CSR) x = rhs with x[fixed] kept
-        (fixed: int32 0/1 per global dof); Jacobi-preconditioned CG on the device.
+        (fixed: int32 0/1 per global dof); Jacobi- (block-Jacobi after set_blocks)
+        preconditioned CG on the device.
         Returns (iterations, final relative residual)."""
         t = self._torch
         for a, dt in ((values, t.float64), (rhs, t.float64), (fixed, t.int32), (x, t.float64)):

@@ -443,12 +443,27 @@ int dpg_skeleton_pattern(const dpg_skeleton* sk, int64_t* row_ptr, int32_t* col,
  * be NULL to skip that part. */
 int dpg_skeleton_assemble(const dpg_skeleton* sk, const double* schur, const double* schur_rhs, double* values,
                           double* rhs, void* stream);
+/* Host. entity (nglobal) receives the mesh entity of every global dof of the
+ * numbering dpg_skeleton_numbering gives for the same cells: vertices first,
+ * then edges, then (3D) faces; a side's flux dofs share its entity with the
+ * trace dofs interior to it. Returns the number of entities or < 0 on error.
+ * The blocks of dpg_skeleton_set_blocks. */
+int dpg_skeleton_entities(const dpg_form_desc* form, const dpg_bucket_desc* bucket, int ncells,
+                          const int32_t* cell_vertices, int32_t* entity);
+/* Block-Jacobi preconditioning for dpg_skeleton_solve: the dofs of one entity
+ * (host array entity (nglobal) with ids in [0, nentities); at most 64 dofs per
+ * entity) form a block whose dense diagonal block of the matrix is inverted
+ * (Cholesky) at the start of every solve. nentities = 0 returns to point
+ * Jacobi. The reference's multigrid smoothers offer point and block Jacobi
+ * (GMGOperator.cpp:2241-2262, Ifpack_PointRelaxation / Ifpack_BlockRelaxation
+ * over a graph partition); here the blocks are the mesh entities. */
+int dpg_skeleton_set_blocks(dpg_skeleton* sk, const int32_t* entity, int nentities, void* stream);
 /* Solves the assembled system for its free dofs with the others prescribed:
  * rows of free dofs of A x = rhs, with x[d] given where fixed[d] = 1 (device
  * arrays of nglobal; x in/out, its free entries the initial guess). Jacobi-
- * preconditioned conjugate gradients (the free block is symmetric positive
- * definite) until ||r|| <= tol ||r_0|| or maxit iterations, every reduction
- * in a fixed order. Stands in for the global solve of TSolution::solve
+ * (or, after dpg_skeleton_set_blocks, block-Jacobi-) preconditioned conjugate
+ * gradients (the free block is symmetric positive definite) until
+ * ||r|| <= tol ||r_0|| or maxit iterations, every reduction in a fixed order. Stands in for the global solve of TSolution::solve
  * (Solution.cpp:3480, Solver.h:111-118: Amesos direct or an iterative solver).
  * iters and relres (final ||r|| / ||r_0||) may be NULL. */
 int dpg_skeleton_solve(const dpg_skeleton* sk, const double* values, const double* rhs, const int32_t* fixed,

@@ -138,3 +138,49 @@ def test_riesz_representation_solves_the_gram_system():
     print(f"riesz representation vs direct solve: {err:.2e}")
     assert err < 1e-9
     assert np.allclose(np.einsum("cn,cn->c", r, e), eta2, rtol=1e-9, atol=0)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_block_jacobi_solve_equals_direct_and_needs_fewer_iterations(dim):
+    """Block-Jacobi over mesh entities (a side's fluxes with the traces interior
+    to it, edge and vertex traces): the CG solution equals a direct solve of
+    the same system and takes fewer iterations than point Jacobi."""
+    if dim == 2:
+        f, b = dpg.form("poisson", 2), dpg.bucket(capi.QUAD, 3, 5)
+        nodes, par = dpg.mesh_quad(10, 9, 0.15, 131)
+    else:
+        f, b = dpg.form("poisson", 3), dpg.bucket(capi.HEX, 2, 3)
+        nodes, par = dpg.mesh_hex(4, 3, 3, 0.1, 141)
+    load = dpg.load_sine_modes(dim, 1, 4, 3, 0xC0FFEE + 151)
+    A = dpg.LocalAssembler(f, [b])
+    out, nbad = A.local_stiffness_and_rhs(0, dev(nodes), dev(par), None, load, want_schur=True)
+    A.close()
+    assert nbad == 0
+    cv = dpg.vertex_ids(nodes)
+    gdof, n = dpg.skeleton_numbering(f, b, cv)
+    sysm = dpg.CondensedSystem(gdof, n)
+    vals, rhs = sysm.assemble(out["schur"], out["schur_rhs"])
+    fixed = dpg.boundary_trace_mask(f, b, gdof, cv, n)
+    dfixed = dev(fixed)
+    x1 = torch.zeros(n, dtype=torch.float64, device="cuda:0")
+    it1, rel1 = sysm.solve(vals, rhs, dfixed, x1, tol=1e-12, maxit=20000)
+    ent, ne = dpg.skeleton_entities(f, b, cv, n)
+    sysm.set_blocks(ent, ne)
+    x2 = torch.zeros(n, dtype=torch.float64, device="cuda:0")
+    it2, rel2 = sysm.solve(vals, rhs, dfixed, x2, tol=1e-12, maxit=20000)
+    sysm.set_blocks(ent, 0)  # back to point Jacobi: the same iterations as before
+    x3 = torch.zeros(n, dtype=torch.float64, device="cuda:0")
+    it3, _ = sysm.solve(vals, rhs, dfixed, x3, tol=1e-12, maxit=20000)
+    rp, col = sysm.pattern()
+    Ah = sp.csr_matrix((vals.cpu().numpy(), col.cpu().numpy(), rp.cpu().numpy()), shape=(n, n))
+    free = np.flatnonzero(fixed == 0)
+    xd = np.zeros(n)
+    xd[free] = spla.spsolve(Ah[free][:, free].tocsc(), rhs.cpu().numpy()[free])
+    sysm.close()
+    e1 = np.abs(x1.cpu().numpy() - xd).max() / np.abs(xd).max()
+    e2 = np.abs(x2.cpu().numpy() - xd).max() / np.abs(xd).max()
+    print(f"dim {dim}: {n} dofs, {ne} entities; Jacobi {it1} iterations ({e1:.1e}), block-Jacobi {it2} ({e2:.1e})")
+    assert rel1 <= 1e-12 and rel2 <= 1e-12
+    assert e1 < 1e-9 and e2 < 1e-9
+    assert it2 < it1 and it3 == it1
+    assert torch.equal(x3, x1)

@@ -103,3 +103,83 @@ def test_numbering_rejects_bad_input():
         dpg.skeleton_numbering(f, b, np.array([[0, 1, 1, 2]], dtype=np.int32))
     with pytest.raises(capi.DPGError):
         dpg.skeleton_numbering(f, b, np.array([[0, 1, 2, -3]], dtype=np.int32))
+
+
+def check_entities(f, b, nodes, seed):
+    """dpg_skeleton_entities on a relabelled mesh: a vertex entity holds the
+    trace dofs at that vertex (one per trace variable, at its position); on
+    every side of every cell the flux dofs and the trace dofs interior to the
+    side share one entity, which is an edge (2D) or a face (3D); in 3D the trace
+    dofs on a face's edges (not its corners) belong to edge entities."""
+    C, V, dim = nodes.shape
+    cv = dpg.vertex_ids(nodes)
+    perm = np.random.default_rng(seed).permutation(cv.max() + 1).astype(np.int32)
+    cv = perm[cv] * 2 + 1
+    gdof, n = dpg.skeleton_numbering(f, b, cv)
+    ent, ne = dpg.skeleton_entities(f, b, cv, n)
+    assert ent.min() >= 0 and ent.max() < ne
+    nv = np.unique(cv).size
+    xi = dpg.skeleton_nodes(f, b)
+    Nv = vertex_basis(b.topology, xi)
+    x = np.einsum("tv,cvd->ctd", Nv, nodes)
+    sides = SIDES[b.topology]
+    blocks = [(f.trial_kind[v], v, si) for v in range(f.n_trial) if f.trial_kind[v] in (capi.TRACE, capi.FLUX)
+              for si in range(len(sides))]
+    k = b.h1_order
+    ct_, cf = (k + 1) ** (dim - 1), k ** (dim - 1)
+    ntrace = sum(1 for v in range(f.n_trial) if f.trial_kind[v] == capi.TRACE)
+    vpos = {}
+    for c in range(C):
+        off = 0
+        side_ent = {}
+        for kind, var, si in blocks:
+            card = ct_ if kind == capi.TRACE else cf
+            g = gdof[c, off:off + card]
+            e = ent[g]
+            if kind == capi.FLUX:
+                assert np.all(e == e[0]) and e[0] >= nv
+                side_ent.setdefault(si, set()).add(int(e[0]))
+            else:
+                if dim == 2:
+                    corner = np.array([0, k])
+                    interior = np.arange(1, k)
+                    for q in corner:
+                        assert e[q] < nv
+                        vpos.setdefault(int(e[q]), set()).add(tuple(np.round(x[c, off + q], 9)))
+                    side_ent.setdefault(si, set()).update(int(v) for v in e[interior])
+                else:
+                    a_, b_ = np.meshgrid(np.arange(k + 1), np.arange(k + 1), indexing="xy")
+                    a_, b_ = a_.ravel(), b_.ravel()
+                    is_corner = ((a_ == 0) | (a_ == k)) & ((b_ == 0) | (b_ == k))
+                    on_edge = ((a_ == 0) | (a_ == k) | (b_ == 0) | (b_ == k)) & ~is_corner
+                    inner = ~((a_ == 0) | (a_ == k) | (b_ == 0) | (b_ == k))
+                    for q in np.flatnonzero(is_corner):
+                        assert e[q] < nv
+                        vpos.setdefault(int(e[q]), set()).add(tuple(np.round(x[c, off + q], 9)))
+                    assert np.all(e[on_edge] >= nv)
+                    edge_ids = set(int(v) for v in e[on_edge])
+                    side_ent.setdefault(si, set()).update(int(v) for v in e[inner])
+                    assert not (edge_ids & set(int(v) for v in e[inner]))  # edges and the face differ
+            off += card
+        for si, es in side_ent.items():
+            assert len(es) == 1, (c, si, es)  # fluxes and interior traces of a side: one entity
+    for v, ps in vpos.items():
+        assert len(ps) == 1  # a vertex entity sits at one point
+    counts = np.bincount(ent[np.isin(ent, list(vpos))], minlength=1)
+    assert np.all(counts[list(vpos)] == ntrace)  # one dof per trace variable at a vertex
+    return ne
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_entities_quads_and_stokes(seed):
+    nodes, _ = dpg.mesh_quad(4, 3, 0.1, 80 + seed)
+    check_entities(dpg.form("poisson", 2), dpg.bucket(capi.QUAD, 4, 5), nodes, seed)
+    check_entities(dpg.form("stokes", 2), dpg.bucket(capi.QUAD, 3, 4), nodes, seed + 10)
+
+
+def test_entities_hexes():
+    nodes, _ = dpg.mesh_hex(2, 2, 3, 0.1, 90)
+    ne = check_entities(dpg.form("poisson", 3), dpg.bucket(capi.HEX, 3, 4), nodes, 5)
+    nv, ned = 3 * 3 * 4, 2 * 3 * 4 + 3 * 2 * 4 + 3 * 3 * 3
+    nf = 2 * 2 * 4 + 2 * 3 * 3 + 3 * 2 * 3
+    assert ne == nv + ned + nf
