@@ -893,11 +893,17 @@ __global__ __launch_bounds__(256) void ip_digits_kernel(const double* __restrict
           for (int k = 0; k < NPR; ++k) v = fma(w[k], r[e][k], v);
         }
         double rr = v * sx * 64.0;
+        // rint by the 1.5 * 2^52 shift (|rr| <= 64): the FP64 add rounds to nearest
+        // even like rint, leaves the integer in the low bits of the sum (two's
+        // complement) and needs no conversion instruction
+        constexpr double kShift = 6755399441055744.0;
 #pragma unroll
         for (int p = 0; p < S; ++p) {
-          const double d = rint(rr);
+          const double t = rr + kShift;
+          const double d = t - kShift;
           rr = (rr - d) * 128.0;
-          pk[p][e >> 2] |= ((uint32_t)(int)d & 0xFFu) << (8 * (e & 3));
+          const uint32_t lo = (uint32_t)__double2loint(t);
+          pk[p][e >> 2] |= (lo & 0xFFu) << (8 * (e & 3));
         }
       }
       const int chunk = (lt * QGp + q0) >> 5;
