@@ -820,34 +820,35 @@ __global__ __launch_bounds__(256) void gemm_gram_f16x3(GemmArgs g) { gemm_mixed_
 __global__ __launch_bounds__(256) void gemm_bmat_f16x3(GemmArgs g) { gemm_mixed_body<DPG_PREC_FP16X3>(g); }
 
 // ------------------------------------------- int8 digit split of the Gram
-// Producer: one thread per test dof j (rows N..NR-1 write zero digits). Pass 1
-// sums the squares of the operand column (the DPOEQU factor s_j, the same sum
-// as ip_operand_kernel's); pass 2 recomputes the values for 16 points at a
-// time, equilibrates them (|x| <= 1), splits each into kI8Digits signed
-// digits by exact steps (64x, then rint and 128 * remainder) and stores 16
-// bytes per plane: the 16 points are half of a 32-point k-chunk of one term.
+// Producer: two lanes per test dof j (rows N..NR-1 write zero digits), lane h
+// of the pair taking points 16h..16h+15 of every 32-point k-chunk, so that a
+// wave stores whole 32-byte rows (1 KB contiguous per plane). Pass 1 sums the
+// squares of the operand column (the DPOEQU factor s_j; the pair's halves are
+// added with one shuffle); pass 2 recomputes the values, equilibrates them
+// (|x| <= 1), splits each into kI8Digits signed digits by exact steps (64x,
+// then rint and 128 * remainder) and stores 16 bytes per plane.
 template <int NPR>
 __global__ __launch_bounds__(256) void ip_digits_kernel(const double* __restrict__ Wip, int n_lt, int nvar,
                                                         const double* __restrict__ R,
                                                         const int32_t* __restrict__ vod, int N, int QG, int QGp,
                                                         int NR, int8_t* __restrict__ dig, int64_t sd,
                                                         double* __restrict__ s, int64_t ss) {
-  constexpr int S = kI8Digits, QB = 16;
-  extern __shared__ double Wl[];  // QB points x (n_lt, nvar, NPR)
+  constexpr int S = kI8Digits, QB = 16, QC = 32;
+  extern __shared__ double Wl[];  // QC points x (n_lt, nvar, NPR)
   const int wslice = n_lt * nvar * NPR;
-  const int c = blockIdx.y, t = threadIdx.x;
-  const int j = blockIdx.x * blockDim.x + t;
+  const int c = blockIdx.y, t = threadIdx.x, h = t & 1;
+  const int j = blockIdx.x * (blockDim.x / 2) + (t >> 1);
   const bool active = j < N;
   const int var = active ? vod[j] : 0;
   double diag = 0.0;
-  for (int q0 = 0; q0 < QG; q0 += QB) {
-    const int nq = min(QB, QG - q0);
+  for (int q0 = 0; q0 < QG; q0 += QC) {
+    const int nq = min(QC, QG - q0);
     __syncthreads();
     const double* Wp = Wip + ((size_t)c * QG + q0) * wslice;
     for (int e = t; e < nq * wslice; e += blockDim.x) Wl[e] = Wp[e];
     __syncthreads();
     if (!active) continue;
-    for (int e = 0; e < nq; ++e) {
+    for (int e = QB * h; e < min(nq, QB * h + QB); ++e) {
       double r[NPR];
 #pragma unroll
       for (int k = 0; k < NPR; ++k) r[k] = R[((size_t)(q0 + e) * NPR + k) * N + j];
@@ -860,24 +861,26 @@ __global__ __launch_bounds__(256) void ip_digits_kernel(const double* __restrict
       }
     }
   }
+  diag += __shfl_xor(diag, 1);  // the pair's two halves (every lane takes part)
   const double sj = 1.0 / sqrt(diag);
-  if (active) s[(size_t)c * ss + j] = sj;
+  if (active && h == 0) s[(size_t)c * ss + j] = sj;
   const double sx = (active && diag > 0.0) ? sj : 0.0;  // an all-zero column stays zero
-  const int nch = n_lt * QGp / 32;
-  int8_t* dc = dig + (size_t)c * sd + (size_t)j * 32;
-  for (int q0 = 0; q0 < QGp; q0 += QB) {
-    const int nq = max(0, min(QB, QG - q0));
+  const int nch = n_lt * QGp / QC;
+  int8_t* dc = dig + (size_t)c * sd + (size_t)j * 32 + 16 * h;
+  for (int q0 = 0; q0 < QGp; q0 += QC) {
+    const int nq = max(0, min(QC, QG - q0));
     __syncthreads();
     const double* Wp = Wip + ((size_t)c * QG + q0) * wslice;
     for (int e = t; e < nq * wslice; e += blockDim.x) Wl[e] = Wp[e];
     __syncthreads();
     if (j >= NR) continue;
+    const int nh = max(0, min(QB, nq - QB * h));  // this lane's valid points
     double r[QB][NPR];
 #pragma unroll
     for (int e = 0; e < QB; ++e)
 #pragma unroll
-      for (int k = 0; k < NPR; ++k) r[e][k] = (active && e < nq) ? R[((size_t)(q0 + e) * NPR + k) * N + j] : 0.0;
-    const int h = (q0 / QB) & 1;
+      for (int k = 0; k < NPR; ++k)
+        r[e][k] = (active && e < nh) ? R[((size_t)(q0 + QB * h + e) * NPR + k) * N + j] : 0.0;
     for (int lt = 0; lt < n_lt; ++lt) {
       uint32_t pk[S][4];
 #pragma unroll
@@ -887,8 +890,8 @@ __global__ __launch_bounds__(256) void ip_digits_kernel(const double* __restrict
 #pragma unroll
       for (int e = 0; e < QB; ++e) {
         double v = 0.0;
-        if (e < nq) {  // points past QG are padding (and their LDS slots are not loaded)
-          const double* w = Wl + ((e * n_lt + lt) * nvar + var) * NPR;
+        if (e < nh) {  // points past QG are padding (and their LDS slots are not loaded)
+          const double* w = Wl + (((QB * h + e) * n_lt + lt) * nvar + var) * NPR;
 #pragma unroll
           for (int k = 0; k < NPR; ++k) v = fma(w[k], r[e][k], v);
         }
@@ -899,17 +902,17 @@ __global__ __launch_bounds__(256) void ip_digits_kernel(const double* __restrict
         constexpr double kShift = 6755399441055744.0;
 #pragma unroll
         for (int p = 0; p < S; ++p) {
-          const double t = rr + kShift;
-          const double d = t - kShift;
+          const double tt = rr + kShift;
+          const double d = tt - kShift;
           rr = (rr - d) * 128.0;
-          const uint32_t lo = (uint32_t)__double2loint(t);
+          const uint32_t lo = (uint32_t)__double2loint(tt);
           pk[p][e >> 2] |= (lo & 0xFFu) << (8 * (e & 3));
         }
       }
-      const int chunk = (lt * QGp + q0) >> 5;
+      const int chunk = (lt * QGp + q0) / QC;
 #pragma unroll
       for (int p = 0; p < S; ++p)
-        *reinterpret_cast<uint4*>(dc + (size_t)(p * nch + chunk) * NR * 32 + 16 * h) =
+        *reinterpret_cast<uint4*>(dc + (size_t)(p * nch + chunk) * NR * 32) =
             make_uint4(pk[p][0], pk[p][1], pk[p][2], pk[p][3]);
     }
   }
@@ -1551,8 +1554,8 @@ void launch_ip_digits(int dim, const double* Wip, int n_lt, int nvar, const doub
                       int QG, int QGp, int NR, int ncells, int8_t* dig, int64_t sd, double* s, int64_t ss,
                       hipStream_t st) {
   if (QGp % 32 != 0 || NR % 64 != 0 || NR < N) throw DeviceError("ip_digits: QGp must be a multiple of 32, NR of 64");
-  size_t shm = sizeof(double) * 16 * (size_t)n_lt * nvar * (dim + 1);
-  dim3 grid((NR + 255) / 256, ncells);
+  size_t shm = sizeof(double) * 32 * (size_t)n_lt * nvar * (dim + 1);
+  dim3 grid((NR + 127) / 128, ncells);  // two lanes per row
   if (dim == 2)
     ip_digits_kernel<3><<<grid, 256, shm, st>>>(Wip, n_lt, nvar, R, vod, N, QG, QGp, NR, dig, sd, s, ss);
   else
