@@ -1,7 +1,7 @@
 // dpg_oracle.cpp -- TEST INFRASTRUCTURE ONLY.
 //
 // FP64 CPU restatement of Camellia's DPG local assembly, used as the parity
-// checker by tests/, __graft_entry__.smoke() and the cpu_baseline leg of
+// reference by tests/, __graft_entry__.smoke() and the cpu_baseline leg of
 // bench.py. Nothing in the product library links, loads or calls this file.
 //
 // What it restates (reference file:line):
@@ -893,7 +893,7 @@ void cell_BGl_t(const dpg_form_desc* f, const Element& e, const Tabulator& tab, 
 }
 
 // Accumulation of B, G, l: FP64 (the reference's arithmetic) or, for the
-// extended-precision checker, 80-bit sums of the same FP64 point values,
+// extended-precision evaluation, 80-bit sums of the same FP64 point values,
 // rounded once at the end.
 static int g_accum_extended = 0;
 
@@ -1145,7 +1145,7 @@ int oracle_tri_rule(int n, double* xy, double* w) {
   return (int)ww.size();
 }
 
-// 1: accumulate B, G, l in 80-bit (extended-precision checker); 0: FP64
+// 1: accumulate B, G, l in 80-bit (extended-precision evaluation); 0: FP64
 int oracle_set_extended_accumulation(int on) {
   g_accum_extended = on ? 1 : 0;
   return 0;

@@ -3,7 +3,7 @@
 The oracle is the FP64 CPU restatement of the reference's DPG local assembly
 (see dpg_oracle.cpp for the reference file:line of every part). Only tests/,
 __graft_entry__.smoke() and the cpu_baseline leg of bench.py use it, and only
-as the checker / CPU baseline.
+as the parity reference / CPU baseline.
 """
 import ctypes as ct
 import os

@@ -364,7 +364,7 @@ def test_triangle_convection_diffusion_linear_solution_has_zero_residual(k, kt):
 
 
 def test_extended_accumulation_changes_only_rounding():
-    """The 80-bit checker sums the same FP64 point values: G, B, l agree with
+    """The 80-bit evaluation sums the same FP64 point values: G, B, l agree with
     the FP64 sums to rounding, and its K is within the FP64 floor of the oracle's."""
     nodes, par = _quad_mesh(2, 2, 0.1, 13)
     f = oracle.form("poisson", 2)
