@@ -231,30 +231,37 @@ __global__ __launch_bounds__(128) void point_kernel(int kind, const DevProgram* 
 
 // ----------------------------------------------------------- operand kernel
 // out[c*sc + b*sb + p*sp + i] = sum_r W[c][p][b][var(i)][r] * R[p][r][i]
+// One workgroup per (256 dofs, kOperandPoints points, cell): the weights of its
+// points are staged once, then every thread writes its dof's values point by point.
+constexpr int kOperandPoints = 8;
 template <typename T, int NPR>
 __global__ __launch_bounds__(256) void operand_kernel(const double* __restrict__ W, int ng, int nvar,
                                                       const double* __restrict__ R, const int32_t* __restrict__ vod,
                                                       int N, int npts, T* __restrict__ out, int64_t sc, int64_t sb,
                                                       int64_t sp) {
   extern __shared__ double Ws[];
-  int p = blockIdx.y, c = blockIdx.z;
-  int slice = ng * nvar * NPR;
-  const double* Wp = W + ((size_t)c * npts + p) * slice;
-  for (int e = threadIdx.x; e < slice; e += blockDim.x) Ws[e] = Wp[e];
+  const int p0 = blockIdx.y * kOperandPoints, c = blockIdx.z;
+  const int np = min(kOperandPoints, npts - p0);
+  const int slice = ng * nvar * NPR;
+  const double* Wp = W + ((size_t)c * npts + p0) * slice;
+  for (int e = threadIdx.x; e < np * slice; e += blockDim.x) Ws[e] = Wp[e];
   __syncthreads();
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  double r[NPR];
+  const int var = vod[i];
+  for (int pp = 0; pp < np; ++pp) {
+    const int p = p0 + pp;
+    double r[NPR];
 #pragma unroll
-  for (int k = 0; k < NPR; ++k) r[k] = R[((size_t)p * NPR + k) * N + i];
-  int var = vod[i];
-  T* o = out + (size_t)c * sc + (size_t)p * sp + i;
-  for (int b = 0; b < ng; ++b) {
-    const double* w = Ws + ((size_t)b * nvar + var) * NPR;
-    double acc = 0.0;
+    for (int k = 0; k < NPR; ++k) r[k] = R[((size_t)p * NPR + k) * N + i];
+    T* o = out + (size_t)c * sc + (size_t)p * sp + i;
+    for (int b = 0; b < ng; ++b) {
+      const double* w = Ws + ((size_t)pp * ng * nvar + (size_t)b * nvar + var) * NPR;
+      double acc = 0.0;
 #pragma unroll
-    for (int k = 0; k < NPR; ++k) acc = fma(w[k], r[k], acc);
-    o[(size_t)b * sb] = (T)acc;
+      for (int k = 0; k < NPR; ++k) acc = fma(w[k], r[k], acc);
+      o[(size_t)b * sb] = (T)acc;
+    }
   }
 }
 
@@ -1498,8 +1505,8 @@ void launch_point(int dim, int kind, const DevProgram* prog, const DevTables& ta
 void launch_operand(int dim, const double* W, int ng, int nvar, const double* R, const int32_t* vod, int N, int npts,
                     int ncells, double* out, int64_t sc, int64_t sb, int64_t sp, hipStream_t st) {
   if (ng == 0 || npts == 0) return;
-  dim3 grid((N + 255) / 256, npts, ncells);
-  size_t shm = sizeof(double) * ng * nvar * (dim + 1);
+  dim3 grid((N + 255) / 256, (npts + kOperandPoints - 1) / kOperandPoints, ncells);
+  size_t shm = sizeof(double) * kOperandPoints * ng * nvar * (dim + 1);
   if (dim == 2)
     operand_kernel<double, 3><<<grid, 256, shm, st>>>(W, ng, nvar, R, vod, N, npts, out, sc, sb, sp);
   else
