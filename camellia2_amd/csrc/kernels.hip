@@ -1206,17 +1206,20 @@ __global__ void diag_scale_factors(const double* __restrict__ A, int64_t lda, in
 }
 
 // A[r, j] *= sign * s'_r * s_j for the lower part of the square block [r0, D):
-// s'_r = s[r - r0] if r - r0 < n else 1; only columns j - r0 < ncols are scaled by s_j
-__global__ void apply_scaling(double* __restrict__ A, int64_t lda, int64_t sA, int r0, int D, int n,
-                              const double* __restrict__ s, int64_t ss, double sign) {
-  int c = blockIdx.z, jc = blockIdx.y;
-  int col = r0 + jc;
+// s'_r = s[r - r0] if r - r0 < n else 1; only columns j - r0 < ncols are scaled by s_j.
+// One wavefront per column (rows contiguous), 4 columns per workgroup.
+__global__ __launch_bounds__(256) void apply_scaling(double* __restrict__ A, int64_t lda, int64_t sA, int r0, int D,
+                                                     int n, int ncol, const double* __restrict__ s, int64_t ss,
+                                                     double sign) {
+  const int c = blockIdx.y, jc = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (jc >= ncol) return;
+  const int col = r0 + jc;
   const double* sc = s + (size_t)c * ss;
-  double sj = jc < n ? sc[jc] : 1.0;
+  const double sj = jc < n ? sc[jc] : 1.0;
   double* Ac = A + (size_t)c * sA + (size_t)col * lda;
-  for (int r = col + blockIdx.x * blockDim.x + threadIdx.x; r < D; r += gridDim.x * blockDim.x) {
-    int ir = r - r0;
-    double si = ir < n ? sc[ir] : 1.0;
+  for (int r = col + lane; r < D; r += 64) {
+    const int ir = r - r0;
+    const double si = ir < n ? sc[ir] : 1.0;
     Ac[r] = sign * Ac[r] * si * sj;
   }
 }
@@ -1662,7 +1665,7 @@ void launch_scaling(double* A, int64_t lda, int64_t sA, int r0, int D, int n, in
                     int ncells, double sign, hipStream_t st) {
   diag_scale_factors<<<dim3((n + 255) / 256, ncells), 256, 0, st>>>(A, lda, sA, r0, n, s, ss, sign > 0 ? 1 : -1);
   check_launch("diag_scale_factors");
-  apply_scaling<<<dim3((ncol + 255) / 256, ncol, ncells), 256, 0, st>>>(A, lda, sA, r0, D, n, s, ss, sign);
+  apply_scaling<<<dim3((ncol + 3) / 4, ncells), 256, 0, st>>>(A, lda, sA, r0, D, n, ncol, s, ss, sign);
   check_launch("apply_scaling");
 }
 
