@@ -470,6 +470,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-schur", action="store_true")
     ap.add_argument("--lanes", type=int, default=4, help="hp config: concurrent streams inside the graph")
+    ap.add_argument("--tiles", default="",
+                    help="GEMM tiles (64 or 32) for the Gram, B and Cholesky-update contractions, e.g. 64,32,64 "
+                         "(single-type configurations; default: chosen by the plan autotuner)")
     ap.add_argument("--sumfact", action="store_true",
                     help="opt-in sum-factorised G (fewer flops, K accurate only to ~1e-12..1e-1; not the default)")
     ap.add_argument("--no-peaks", action="store_true", help="skip the peak-rate microbenchmark")
@@ -501,8 +504,17 @@ def main():
     d_par = torch.from_numpy(par).to(dev)
     ws = args.workspace_gib * 2 ** 30
     A = dpg.LocalAssembler(f, [b], workspace_bytes=ws, precision=PRECISIONS[args.precision])
-    if args.sumfact:
-        A.set_tiles(0, 64, 64, 64, 1)
+    # GEMM tiles: given, or the plan's autotuner on the first cells (untimed set-up)
+    t_tune = None
+    if args.tiles:
+        tiles = [int(t) for t in args.tiles.split(",")]
+    elif args.sumfact:
+        tiles = list(A.tiles(0)[:3])
+    else:
+        t0 = time.perf_counter()
+        tiles = list(A.autotune(0, d_nodes, d_par, None, load, sample=min(C, 512))[:3])
+        t_tune = round(time.perf_counter() - t0, 2)
+    A.set_tiles(0, *tiles, 1 if args.sumfact else 0)
     out = {
         "K": torch.empty((C, info.M, info.M), dtype=torch.float64, device=dev),
         "rhs": torch.empty((C, info.M), dtype=torch.float64, device=dev),
@@ -531,8 +543,7 @@ def main():
     # one more step on a plan that times every launch with HIP events on the
     # launch stream (kept out of the timed steps above)
     Ap = dpg.LocalAssembler(f, [b], workspace_bytes=ws, profile=True, precision=PRECISIONS[args.precision])
-    if args.sumfact:
-        Ap.set_tiles(0, 64, 64, 64, 1)
+    Ap.set_tiles(0, *tiles, 1 if args.sumfact else 0)
     step(Ap)
     torch.cuda.synchronize()
     tm, _ = Ap.timings(reset=True)
@@ -569,7 +580,10 @@ def main():
         "data": "synthetic (seeded jittered mesh and sine-mode load)",
         "config": {"workload": cfg["name"], "cells": C, "N": info.N, "M": info.M, "T": info.T,
                    "outputs": "K,rhs" + (",schur,schur_rhs" if want_schur else ""),
-                   "precision": args.precision, "gram": "sum-factorised (opt-in)" if args.sumfact else "dense"},
+                   "precision": args.precision, "gram": "sum-factorised (opt-in)" if args.sumfact else "dense",
+                   "tiles": {"gram": tiles[0], "bmat": tiles[1], "update": tiles[2],
+                             "chosen_by": "--tiles" if args.tiles else ("plan default" if args.sumfact else
+                                                                         f"autotuner ({t_tune} s, untimed)")}},
         "gflops": round(info.flops_per_cell * cells_per_s / 1e9, 1),
         "algorithmic_flop_per_cell": info.flops_per_cell,
         "roofline": roofline,
@@ -588,6 +602,7 @@ def main():
         heads = {}
         for p in comps:
             Am = dpg.LocalAssembler(f, [b], workspace_bytes=ws, precision=PRECISIONS[p])
+            Am.set_tiles(0, *tiles, 0)  # the FP64 GEMMs of the mixed forms take the same tiles
             step(Am)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
