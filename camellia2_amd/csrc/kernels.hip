@@ -425,12 +425,14 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
       }
     }
   };
-  auto stage = [&](const double (&xr)[PER], const double (&yr)[PER], int k0) {
+  // weight of the segment the walker is in (kw: merged equal terms), 1 without
+  auto segw = [&]() { return (g.kw && seg < g.nseg) ? g.kw[(size_t)tile * g.nseg + seg] : 1.0; };
+  auto stage = [&](const double (&xr)[PER], const double (&yr)[PER], int k0, double wx) {
     __syncthreads();  // every wave is done reading the previous chunk
 #pragma unroll
     for (int r = 0; r < PER; ++r) {
       const bool vk = k0 + kk0 + r * RS < kend;
-      Xs[kk0 + r * RS][ii] = (vx && vk) ? xr[r] : 0.0;
+      Xs[kk0 + r * RS][ii] = (vx && vk) ? xr[r] * wx : 0.0;
       Ys[kk0 + r * RS][ii] = (vy && vk) ? yr[r] : 0.0;
     }
     __syncthreads();
@@ -453,7 +455,9 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
   };
   double xa_[PER], ya_[PER], xb_[PER], yb_[PER];
   int kc = skip_zero_segments(0);  // chunk to multiply next
+  double wa = segw();              // each register set carries its chunk's segment weight
   int kn = next_chunk(kc);         // the one after
+  double wb = segw();
   if (kc < kend) fetch(xa_, ya_, kc);
   if (kn < kend) fetch(xb_, yb_, kn);
   // two-chunk software pipeline, unrolled by two so that the register sets
@@ -464,15 +468,17 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
   const int si0 = i0 + wr * 32, sj0 = j0 + wc * 32;
   const bool busy = si0 < g.m && sj0 < g.n && (!g.lower || si0 + 31 >= sj0);
   while (kc < kend) {
-    stage(xa_, ya_, kc);
+    stage(xa_, ya_, kc, wa);
     int kf = next_chunk(kn);
+    wa = segw();
     if (kf < kend) fetch(xa_, ya_, kf);
     if (busy) multiply();
     kc = kn;
     kn = kf;
     if (kc >= kend) break;
-    stage(xb_, yb_, kc);
+    stage(xb_, yb_, kc, wb);
     kf = next_chunk(kn);
+    wb = segw();
     if (kf < kend) fetch(xb_, yb_, kf);
     if (busy) multiply();
     kc = kn;

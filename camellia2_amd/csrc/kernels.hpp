@@ -40,6 +40,10 @@ struct GemmArgs {
   // this tile's rows or columns and is skipped (adds exact zeros otherwise)
   const uint64_t* kmask;
   int64_t kseg;
+  // optional (with kmask): kw[tile*nseg + s] scales the staged X rows of segment s
+  // (a segment standing for several equal terms counts once per term)
+  const double* kw;
+  int32_t nseg;
 };
 
 void launch_point(int dim, int kind, const DevProgram* prog, const DevTables& tab, int npts, int Qs, int ncells,

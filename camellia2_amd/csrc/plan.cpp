@@ -205,7 +205,9 @@ struct BucketPlan {
   int64_t nDig = 0;   // int8 digit planes: bytes per cell
   // per lower tile of G (64- and 32-wide tiles): bit lt set when IP term lt
   // reaches both the tile's rows and its columns
-  const uint64_t* gmask[2] = {nullptr, nullptr};
+  const uint64_t* gmask[2] = {nullptr, nullptr};         // Gram k-segments per (64- or 32-) tile
+  const uint64_t* gmask_merged[2] = {nullptr, nullptr};  // the same with equal terms merged ...
+  const double* gweight[2] = {nullptr, nullptr};         // ... and the merged segments' weights
   // sum-factorised Gram (tensor-product cells, FP64): block pairs, pairs, 1D tables
   bool sf_ok = false;   // available for this bucket
   bool sf_on = false;   // used (opt-in)
@@ -512,13 +514,59 @@ void build_bucket(dpg_plan* plan, const dpg_bucket_desc& desc, BucketPlan& B) {
     std::vector<uint64_t> tv(f.n_test, 0);
     for (int i = 0; i < f.n_ipterms; ++i)
       for (int k = 0; k < f.ipterms[i].n; ++k) tv[f.ipterms[i].s[k].test_var] |= 1ull << i;
+    // IP terms a and b give the same operand values on the dofs of the test
+    // variables in `vars` when their summands on those variables are the same
+    // list (operator and coefficient, in order)
+    auto same_on = [&](int a, int b, uint64_t vars) {
+      auto restricted = [&](int t) {
+        std::vector<const dpg_lt_summand*> r;
+        for (int k = 0; k < f.ipterms[t].n; ++k)
+          if ((vars >> f.ipterms[t].s[k].test_var) & 1ull) r.push_back(&f.ipterms[t].s[k]);
+        return r;
+      };
+      const auto ra = restricted(a), rb = restricted(b);
+      if (ra.size() != rb.size()) return false;
+      for (size_t k = 0; k < ra.size(); ++k) {
+        const dpg_lt_summand &x = *ra[k], &y = *rb[k];
+        if (x.test_var != y.test_var || x.test_op != y.test_op || x.coef.scale != y.coef.scale ||
+            x.coef.coef_slot != y.coef.coef_slot || x.coef.coef_fn != y.coef.coef_fn ||
+            x.coef.normal_comp != y.coef.normal_comp)
+          return false;
+      }
+      return true;
+    };
+    const int nlt = B.prog.n_lt;
     for (int ts = 0; ts < 2; ++ts) {
       const int T = ts == 0 ? 64 : 32, nt = (L.N + T - 1) / T;
-      std::vector<uint64_t> reach(nt, 0), mask;  // terms reaching some dof of a T-block
-      for (int j = 0; j < L.N; ++j) reach[j / T] |= tv[L.var_of_test_dof[j]];
+      std::vector<uint64_t> reach(nt, 0), vars(nt, 0), mask, mmask;  // terms reaching some dof of a T-block
+      std::vector<double> weight;
+      for (int j = 0; j < L.N; ++j) {
+        reach[j / T] |= tv[L.var_of_test_dof[j]];
+        vars[j / T] |= 1ull << L.var_of_test_dof[j];
+      }
       for (int ti = 0; ti < nt; ++ti)
-        for (int tj = 0; tj <= ti; ++tj) mask.push_back(reach[ti] & reach[tj]);
+        for (int tj = 0; tj <= ti; ++tj) {
+          const uint64_t m = reach[ti] & reach[tj], u = vars[ti] | vars[tj];
+          mask.push_back(m);
+          // merged: terms equal on the variables of both blocks share one segment,
+          // whose staged rows are weighted by their count (the graph norm's
+          // (tau - grad v)_d and tau_d terms on tau-tau blocks, for instance)
+          uint64_t mm = m;
+          std::vector<double> w(nlt, 1.0);
+          for (int a = 0; a < nlt; ++a) {
+            if (!((mm >> a) & 1ull)) continue;
+            for (int b = a + 1; b < nlt; ++b)
+              if (((mm >> b) & 1ull) && same_on(a, b, u)) {
+                mm &= ~(1ull << b);
+                w[a] += 1.0;
+              }
+          }
+          mmask.push_back(mm);
+          weight.insert(weight.end(), w.begin(), w.end());
+        }
       B.gmask[ts] = upload(*plan->tables, mask);
+      B.gmask_merged[ts] = upload(*plan->tables, mmask);
+      B.gweight[ts] = upload(*plan->tables, weight);
     }
   }
   B.nTv = (int64_t)B.prog.n_vb * L.Q * B.Npad;
@@ -725,13 +773,16 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
   g.srow = s_ep;
   g.scol = s_ep;
   g.ss = B.nS;
-  g.kmask = B.gmask[B.tile[0] == 64 ? 0 : 1];
+  g.kmask = B.gmask_merged[B.tile[0] == 64 ? 0 : 1];
+  g.kw = B.gweight[B.tile[0] == 64 ? 0 : 1];
+  g.nseg = B.prog.n_lt;
   g.kseg = B.QGp;
   if (prec != DPG_PREC_FP64 && !i8) {
     // low-precision operands are pre-scaled (keeps FP16 in range): X(k,i) s_i, Y(k,j) s_j
     g.srow = g.scol = nullptr;
     g.px = g.py = s_ep;
     g.kmask = nullptr;
+    g.kw = nullptr;
   }
   if (i8) {
     // G = x^T x from the digit planes (x already carries the DPOEQU factors)
