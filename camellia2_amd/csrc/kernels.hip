@@ -466,7 +466,7 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
   // columns >= n, or strictly above the diagonal of a lower tile) stages its
   // share but issues no MFMAs, leaving the SIMD's matrix pipe to other waves
   const int si0 = i0 + wr * 32, sj0 = j0 + wc * 32;
-  const bool busy = si0 < g.m && sj0 < g.n && (!g.lower || si0 + 31 >= sj0);
+  const bool busy = si0 < g.m && sj0 < g.n && ((!g.lower && !g.skip_upper) || si0 + 31 >= sj0);
   while (kc < kend) {
     stage(xa_, ya_, kc, wa);
     int kf = next_chunk(kn);
@@ -525,7 +525,7 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int i = ir[bi], j = jc[v];
-        if (iv[bi] && jv[v] && (!g.lower || i >= j)) {
+        if (iv[bi] && jv[v] && ((!g.lower && !g.skip_upper) || i >= j)) {
           double val = g.alpha * acc[bj][bi][v];  // (alpha acc) s_i s_j: multiplying by 1 is exact
           val *= sr[bi];
           val *= sc[v];

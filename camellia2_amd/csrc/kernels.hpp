@@ -26,7 +26,8 @@ struct GemmArgs {
   int64_t ldc, sc;  // C(i,j) = C[i + j*ldc]
   int32_t m, n, k, lower;
   double alpha, beta;
-  int32_t tiles_m, ntiles, ncells, pad;
+  int32_t tiles_m, ntiles, ncells;
+  int32_t skip_upper;  // 1: C(i,j) with i < j is not needed (not computed for whole 32x32 sub-tiles, not written)
   // optional: scale C(i,j) by srow[i] * scol[j] (per-cell vectors, stride ss)
   const double* srow;
   const double* scol;

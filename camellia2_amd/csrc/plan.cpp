@@ -656,6 +656,7 @@ void factor_columns(dpg_plan* plan, const BucketPlan& B, int nb, const Work& w, 
       t.alpha = -1.0;
       t.beta = 1.0;
       t.ncells = nb;
+      t.skip_upper = 1;  // the panel's diagonal block: its factorisation reads the lower part only
       double kk = k0 - kbeg;
       e = P.start(st);
       launch_gemm(t, kGemmUpdate, st, B.tile[2]);
