@@ -126,3 +126,98 @@ def linear_trial_coefficients(nodes, par, k, a, eps=None, beta=None):
 
 
 from camellia2_amd.workloads import split_quads  # noqa: E402,F401
+
+
+# shards Hexahedron<8> (the reference's hex topology, CamelliaCellTools.cpp:1043-1122
+# parametrises a side through it): vertices and the side-node map, transcribed
+# from the shards cell-topology tables (shards is part of Trilinos and not
+# vendored in the reference). Face s is the image of the reference quad whose
+# vertex i maps to hex vertex HEX_FACES[s][i].
+HEX_VERTS = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
+                      [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]], dtype=float)
+HEX_FACES = [(0, 1, 5, 4), (1, 2, 6, 5), (2, 3, 7, 6), (0, 4, 7, 3), (0, 3, 2, 1), (4, 5, 6, 7)]
+
+
+def ref_vertices(nverts):
+    return HEX_VERTS if nverts == 8 else REF_VERTS[nverts]
+
+
+def side_points(nverts, side, t):
+    """reference (volume) coordinates of side points with side coordinates t (n, dim-1):
+    the affine map taking the reference line/quad vertices to the side's vertices"""
+    t = np.atleast_2d(t)
+    if nverts == 8:
+        n = HEX_VERTS[list(HEX_FACES[side])]
+        c0 = n.mean(axis=0)
+        c1 = 0.25 * (-n[0] + n[1] + n[2] - n[3])
+        c2 = 0.25 * (-n[0] - n[1] + n[2] + n[3])
+        return c0 + np.outer(t[:, 0], c1) + np.outer(t[:, 1], c2)
+    ref = REF_VERTS[nverts]
+    a, b = ref[side], ref[(side + 1) % nverts]
+    return 0.5 * (a + b) + np.outer(t[:, 0], 0.5 * (b - a))
+
+
+def vertex_map(nodes_c, xi):
+    """physical points of reference points xi (n, dim) under the C1 vertex map of one cell"""
+    V, dim = nodes_c.shape
+    ref = ref_vertices(V)
+    out = []
+    for p in np.atleast_2d(xi):
+        if V == 3:
+            w = np.array([1.0 - p[0] - p[1], p[0], p[1]])
+        else:
+            w = np.prod([(1 + ref[:, d] * p[d]) / 2 for d in range(dim)], axis=0)
+        out.append(w @ nodes_c)
+    return np.array(out)
+
+
+def tensor_nodes(g, dim):
+    """tensor points of the 1D nodes g, x fastest: (len(g)^dim, dim)"""
+    grids = np.meshgrid(*([g] * dim), indexing="ij")
+    return np.stack([gr.ravel(order="F") for gr in grids], axis=1) if dim > 1 else g[:, None]
+
+
+def nodal_trial_coefficients(f, k, nodes, par, value):
+    """Trial coefficients (C, M) of a quad/hex element of H1 order k by nodal
+    interpolation in the reference's dof layout (variables in order; fields at
+    the GLL tensor nodes of degree k-1, vector fields component-major; traces
+    (Cn(k)) and fluxes (Cn(k-1)) at the GLL nodes of every side, side after side).
+    value(var, comp, x, n) gives the variable's value at the physical point x
+    (n: the outward unit normal on sides, None in the volume); flux values are
+    multiplied by the side parity (LinearTerm.cpp:518-524)."""
+    from oracle import oracle
+    C, V, dim = nodes.shape
+    S = 2 * dim if V != 3 else 3
+    gf, gt = oracle.gll_nodes(k - 1), oracle.gll_nodes(k)
+    cols = []
+    for c in range(C):
+        xc = nodes[c]
+        cen = xc.mean(axis=0)
+        row = []
+        for v in range(f.n_trial):
+            kind = f.trial_kind[v]
+            if kind in (capi.FIELD, capi.FIELD_VECTOR):
+                X = vertex_map(xc, tensor_nodes(gf, dim))
+                for comp in range(dim if kind == capi.FIELD_VECTOR else 1):
+                    row += [value(v, comp, x, None) for x in X]
+                continue
+            g = gt if kind == capi.TRACE else gf
+            tn = tensor_nodes(g, dim - 1)
+            for s in range(S):
+                X = vertex_map(xc, side_points(V, s, tn))
+                # outward unit normal of the (planar) side
+                P = vertex_map(xc, side_points(V, s, np.zeros((1, dim - 1))))[0]  # side centre
+                if dim == 2:
+                    a, b = vertex_map(xc, side_points(V, s, np.array([[-1.0], [1.0]])))
+                    T = b - a
+                    n = np.array([T[1], -T[0]])
+                else:
+                    q = vertex_map(xc, side_points(V, s, np.array([[-1.0, -1.0], [1.0, -1.0], [-1.0, 1.0]])))
+                    n = np.cross(q[1] - q[0], q[2] - q[0])
+                n = n / np.linalg.norm(n)
+                if np.dot(n, P - cen) < 0:
+                    n = -n
+                sgn = par[c, s] if kind == capi.FLUX else 1.0
+                row += [sgn * value(v, 0, x, n) for x in X]
+        cols.append(row)
+    return np.array(cols)

@@ -6,10 +6,11 @@ import os
 import numpy as np
 import pytest
 
-from camellia2_amd import capi
+from camellia2_amd import capi, dpg
 from oracle import oracle
 
-from helpers import linear_trial_coefficients, split_quads
+from helpers import (HEX_FACES, HEX_VERTS, linear_trial_coefficients, nodal_trial_coefficients, side_points,
+                     split_quads, tensor_nodes)
 
 
 def load(golden_dir, name):
@@ -385,3 +386,228 @@ def test_extended_accumulation_changes_only_rounding():
     for k in ("K", "rhs", "S", "g"):
         rel = np.linalg.norm(o[k] - x[k]) / np.linalg.norm(x[k])
         assert rel < 1e-11, k
+
+
+# ---------------------------------------------------------------------------
+# Manufactured solutions that pin the oracle's Stokes VGP, CDR-with-reaction and
+# hex term tables: an exact solution that the trial space represents exactly
+# has l - B u = 0 on every cell (the TracesUltraweak invariant of
+# ConvectionDiffusionReactionFormulationTests.cpp:116-235; the steady Stokes
+# consistency check of StokesVGPFormulationTests.cpp:141-146), and a change of
+# one term's sign, of a coefficient function or of a side parity breaks it.
+
+def _residual(f, bk, nodes, par, u, coefs=None, load=None):
+    """per cell max |l - B u| / (max|B| max|u|)"""
+    o = oracle.assemble(f, bk, nodes, par, coefs=coefs, load=load, want=("B", "l"))
+    r = o["l"] - np.einsum("cmn,cm->cn", o["B"], u)
+    scale = np.abs(o["B"]).max(axis=(1, 2)) * np.abs(u).max(axis=1)
+    return np.abs(r).max(axis=1) / scale
+
+
+def _const_load(values):
+    load = capi.LoadDesc()
+    load.ncomp = len(values)
+    for i, v in enumerate(values):
+        load.constant[i] = v
+    return load
+
+
+def _stokes_solution(mu):
+    """divergence-free linear velocity, linear pressure (StokesVGPFormulation.cpp:156-408):
+    sigma = mu grad u, t_in = p n_i - sigma_i . n, f = grad p"""
+    a1, a2, b11, b12, b21 = 0.4, -0.7, 0.9, -1.3, 0.6
+    c0, c1, c2 = 0.2, -0.8, 1.1
+    vel = lambda x: np.array([a1 + b11 * x[0] + b12 * x[1], a2 + b21 * x[0] - b11 * x[1]])
+    grad = np.array([[b11, b12], [b21, -b11]])
+    pres = lambda x: c0 + c1 * x[0] + c2 * x[1]
+    sig = mu * grad
+
+    def value(v, comp, x, n):
+        if v in (0, 1):
+            return vel(x)[v]
+        if v == 2:
+            return pres(x)
+        if 3 <= v <= 6:
+            return sig[(v - 3) // 2, (v - 3) % 2]
+        if v in (7, 8):
+            return vel(x)[v - 7]
+        i = v - 9
+        return pres(x) * n[i] - sig[i] @ n
+    return value, (c1, c2)
+
+
+@pytest.mark.parametrize("mu", [1.0, 0.5])
+def test_stokes_vgp_linear_solution_has_zero_residual(mu):
+    """Stokes VGP at the Stokes workload's orders (H1 order 4, test 5) on jittered
+    quads with mixed parities: the exact linear solution gives l = B u (and a
+    zero energy error); flipping the sign of any one of the 20 bilinear terms
+    leaves a residual."""
+    nodes, par = _quad_mesh(3, 3, 0.1, 8)
+    par[:, 2] = -1
+    f = oracle.form("stokes", 2, mu)
+    bk = capi.BucketDesc(capi.QUAD, 4, 5, 0)
+    value, grad_p = _stokes_solution(mu)
+    load = _const_load(grad_p)
+    u = nodal_trial_coefficients(f, 4, nodes, par, value)
+    assert _residual(f, bk, nodes, par, u, load=load).max() < 1e-13
+    eta2 = oracle.energy_error(f, bk, nodes, par, u, load=load)
+    o = oracle.assemble(f, bk, nodes, par, load=load, want=("rhs",))
+    assert np.abs(eta2).max() < 1e-24 * max(1.0, np.abs(o["rhs"]).max() ** 2)
+    assert f.n_bterms == 20
+    for t in range(f.n_bterms):
+        g = oracle.form_like(f)
+        g.bterms[t].coef.scale = -g.bterms[t].coef.scale
+        assert _residual(g, bk, nodes, par, u, load=load).max() > 1e-4, t
+    # mu enters as 1/mu in (sigma/mu, tau): the solution of the other mu fails
+    wrong, _ = _stokes_solution(1.0 / mu if mu != 1.0 else 2.0)
+    uw = nodal_trial_coefficients(f, 4, nodes, par, wrong)
+    assert _residual(f, bk, nodes, par, uw, load=load).max() > 1e-4
+
+
+def _cdr_case(seed):
+    nodes, par = _quad_mesh(3, 3, 0.1, seed)
+    par[:, 1] = -1
+    C = nodes.shape[0]
+    eps = 10.0 ** np.linspace(-3, 0, C)
+    beta = 1.3 * np.array([np.cos(0.4), np.sin(0.4)])
+    alpha = np.linspace(0.2, 1.0, C)
+    return nodes, par, eps, beta, alpha
+
+
+def test_cdr_constant_solution_pins_reaction_and_convective_flux():
+    """ConvectionDiffusionReactionFormulation (ULTRAWEAK, 2D; terms of
+    ConvectionDiffusionReactionFormulation.cpp:88-94) with per-cell alpha: u = c
+    gives sigma = 0, sigma_n = -beta.n u * parity and f = alpha c. Each cell
+    is solved with its own load. The residual vanishes; it does not when the
+    reaction coefficient is doubled or the convective flux is dropped."""
+    nodes, par, eps, beta, alpha = _cdr_case(12)
+    f = oracle.form("cdr", 2)
+    bk = capi.BucketDesc(capi.QUAD, 3, 4, 0)
+    c0 = 0.75
+
+    def coeffs(flux_scale):
+        def value(v, comp, x, n):
+            if v in (0, 2):
+                return c0
+            if v == 1:
+                return 0.0
+            return -flux_scale * (beta @ n) * c0
+        return value
+
+    for c in range(nodes.shape[0]):
+        nd, pc = nodes[c:c + 1], par[c:c + 1]
+        co = np.array([[eps[c], beta[0], beta[1], alpha[c]]])
+        load = _const_load([alpha[c] * c0])
+        u = nodal_trial_coefficients(f, 3, nd, pc, coeffs(1.0))
+        assert _residual(f, bk, nd, pc, u, co, load)[0] < 1e-13, c
+        co2 = co.copy()
+        co2[0, 3] *= 2.0
+        assert _residual(f, bk, nd, pc, u, co2, load)[0] > 1e-4, c
+        u0 = nodal_trial_coefficients(f, 3, nd, pc, coeffs(0.0))
+        assert _residual(f, bk, nd, pc, u0, co, load)[0] > 1e-4, c
+
+
+def test_cdr_linear_solution_pins_sqrt_eps():
+    """The same formulation with alpha = 0 and u linear: sigma = sqrt(eps) grad u
+    (the scaled flux variable), sigma_n = (eps grad u - beta u).n * parity,
+    f = beta . grad u. The residual vanishes for eps in [1e-3, 1]; with
+    sigma = eps grad u (eps entering where sqrt(eps) belongs) it does not."""
+    nodes, par, eps, beta, _ = _cdr_case(13)
+    C = nodes.shape[0]
+    f = oracle.form("cdr", 2)
+    bk = capi.BucketDesc(capi.QUAD, 3, 4, 0)
+    a = np.array([0.3, -0.9, 1.4])
+    lin = lambda x: a[0] + a[1:] @ x
+    coefs = np.stack([eps, np.full(C, beta[0]), np.full(C, beta[1]), np.zeros(C)], axis=1)
+    load = _const_load([beta @ a[1:]])
+
+    def interp(sig_of_eps):
+        us = []
+        for c in range(C):
+            e = eps[c]
+
+            def value(v, comp, x, n, e=e):
+                if v in (0, 2):
+                    return lin(x)
+                if v == 1:
+                    return sig_of_eps(e) * a[1 + comp]
+                return (e * a[1:] @ n) - (beta @ n) * lin(x)
+            us.append(nodal_trial_coefficients(f, 3, nodes[c:c + 1], par[c:c + 1], value)[0])
+        return np.array(us)
+
+    res = _residual(f, bk, nodes, par, interp(np.sqrt), coefs, load)
+    assert res.max() < 1e-13
+    wrong = _residual(f, bk, nodes, par, interp(lambda e: e), coefs, load)
+    assert np.all(wrong[eps < 0.5] > 1e-5)
+
+
+def test_hex_side_maps_follow_shards_face_nodes():
+    """The shards Hexahedron<8> side-node table used by the tests: every side is
+    a face of the reference cube, its nodes run around that face and the side
+    parametrisation's normal (d xi/ds0 x d xi/ds1) points outward; and the
+    library's trace/flux dof nodes on each side (dpg_skeleton_nodes) are the
+    images of the side's GLL tensor nodes under that parametrisation
+    (CamelliaCellToolsTests.cpp:154-230: MapToReferenceSubcell agrees with the
+    node map)."""
+    for s, face in enumerate(HEX_FACES):
+        P = HEX_VERTS[list(face)]
+        const = [d for d in range(3) if np.all(P[:, d] == P[0, d])]
+        assert len(const) == 1
+        for i in range(4):
+            assert np.sum(P[i] != P[(i + 1) % 4]) == 1  # consecutive nodes share an edge
+        n = np.cross(P[1] - P[0], P[3] - P[0])
+        assert np.dot(n, P.mean(axis=0)) > 0, s
+    assert sorted(tuple(sorted(fc)) for fc in HEX_FACES) == sorted(
+        {tuple(sorted(np.nonzero(HEX_VERTS[:, d] == v)[0])) for d in range(3) for v in (-1, 1)})
+    f = oracle.form("poisson", 3)
+    k = 4
+    bk = capi.BucketDesc(capi.HEX, k, 5, 0)
+    xi = dpg.skeleton_nodes(f, bk)
+    gt, gf = oracle.gll_nodes(k), oracle.gll_nodes(k - 1)
+    nt, nfl = len(gt) ** 2, len(gf) ** 2
+    want = np.concatenate([side_points(8, s, tensor_nodes(gt, 2)) for s in range(6)]
+                          + [side_points(8, s, tensor_nodes(gf, 2)) for s in range(6)])
+    assert xi.shape == (6 * (nt + nfl), 3)
+    assert np.abs(xi - want).max() < 1e-14
+
+
+def _sheared_hexes():
+    nodes, par = dpg.mesh_hex(2, 2, 1, 0.0, 3)
+    A = np.array([[1.0, 0.3, -0.2], [0.1, 0.9, 0.25], [-0.15, 0.2, 1.1]])
+    return nodes @ A.T + np.array([0.1, -0.3, 0.2]), par
+
+
+def test_hex_poisson_linear_solution_has_zero_residual():
+    """3D Poisson ultraweak at the hex workload's orders (H1 order 4, test 5:
+    N = 666, M = 502) on sheared affine hexes (planar faces, constant normals, so
+    that sigma_n = grad u . n is a polynomial): u linear gives l = B u with
+    f = 0. Flipping one face parity (or using a face node map that starts at
+    another vertex) breaks it."""
+    nodes, par = _sheared_hexes()
+    f = oracle.form("poisson", 3)
+    k = 4
+    bk = capi.BucketDesc(capi.HEX, k, 5, 0)
+    a = np.array([0.5, -0.7, 1.2, 0.4])
+    lin = lambda x: a[0] + a[1:] @ x
+
+    def value(v, comp, x, n):
+        if v in (0, 2):
+            return lin(x)
+        if v == 1:
+            return a[1 + comp]
+        return a[1:] @ n
+
+    u = nodal_trial_coefficients(f, k, nodes, par, value)
+    assert u.shape[1] == oracle.sizes(f, bk)["M"] == 502
+    assert _residual(f, bk, nodes, par, u).max() < 1e-13
+    bad = par.copy()
+    bad[0, 1] = -bad[0, 1]
+    assert _residual(f, bk, nodes, bad, u)[0] > 1e-4
+    # the same trace values placed with face 5 parametrised from its second vertex
+    saved = HEX_FACES[5]
+    try:
+        HEX_FACES[5] = saved[1:] + saved[:1]
+        ur = nodal_trial_coefficients(f, k, nodes, par, value)
+    finally:
+        HEX_FACES[5] = saved
+    assert _residual(f, bk, nodes, par, ur).max() > 1e-4
