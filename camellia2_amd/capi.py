@@ -78,7 +78,7 @@ class Options(ct.Structure):
                 ("profile", ct.c_int32), ("workspace_bytes", ct.c_double)]
 
 
-STAGES = ["point", "operand", "gram", "bmat", "scale", "panel", "update", "extract"]
+STAGES = ["point", "operand", "gram", "bmat", "scale", "panel", "update", "extract", "adjoint"]
 NSTAGES = len(STAGES)
 
 
@@ -94,7 +94,8 @@ class Batch(ct.Structure):
 
 class Outputs(ct.Structure):
     _fields_ = [("K", ct.c_void_p), ("rhs", ct.c_void_p), ("G", ct.c_void_p), ("schur", ct.c_void_p),
-                ("schur_rhs", ct.c_void_p), ("info", ct.c_void_p)]
+                ("schur_rhs", ct.c_void_p), ("info", ct.c_void_p), ("factor", ct.c_void_p),
+                ("factor_scale", ct.c_void_p)]
 
 
 class Tiles(ct.Structure):
@@ -112,14 +113,16 @@ LIB_PATH = os.path.join(LIB_DIR, "libcamellia2_dpg.so")
 EXPORTS = [
     "dpg_version", "dpg_last_error", "dpg_form_poisson_uw", "dpg_form_cdr_uw",
     "dpg_form_stokes_vgp_uw", "dpg_form_convection_diffusion_uw", "dpg_form_graph_norm", "dpg_mesh_quad", "dpg_mesh_hex",
+    "dpg_mesh_quad_device", "dpg_mesh_hex_device",
     "dpg_load_sine_modes", "dpg_rng_uniform", "dpg_rng_normal", "dpg_bucket_sizes",
     "dpg_plan_create", "dpg_plan_destroy", "dpg_plan_clone_workspace", "dpg_plan_workspace", "dpg_plan_timings",
     "dpg_assemble",
-    "dpg_energy_error", "dpg_adjoint", "dpg_mesh_quad_graded", "dpg_plan_get_tiles", "dpg_plan_set_tiles",
+    "dpg_energy_error", "dpg_energy_error_factored", "dpg_adjoint", "dpg_mesh_quad_graded", "dpg_plan_get_tiles", "dpg_plan_set_tiles",
     "dpg_plan_autotune", "dpg_graph_create", "dpg_graph_launch", "dpg_graph_workspace", "dpg_graph_destroy",
     "dpg_skeleton_numbering", "dpg_skeleton_nodes", "dpg_skeleton_create", "dpg_skeleton_info",
     "dpg_skeleton_pattern", "dpg_skeleton_assemble", "dpg_skeleton_solve", "dpg_skeleton_entities",
     "dpg_skeleton_set_blocks", "dpg_mark_cells", "dpg_skeleton_destroy",
+    "dpg_coarse_create", "dpg_coarse_info", "dpg_coarse_destroy", "dpg_skeleton_solve_twolevel",
     "dpg_recover_fields", "dpg_riesz_representation", "dpg_side_parities", "dpg_sort_cells", "dpg_gather_cells",
 ]
 
@@ -143,6 +146,10 @@ def lib():
         L.dpg_mesh_quad.argtypes = [ct.c_int, ct.c_int, ct.c_double, ct.c_uint64, ct.c_void_p, ct.c_void_p]
         L.dpg_mesh_hex.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_double, ct.c_uint64, ct.c_void_p,
                                    ct.c_void_p]
+        L.dpg_mesh_quad_device.argtypes = [ct.c_int, ct.c_int, ct.c_double, ct.c_uint64, ct.c_void_p, ct.c_void_p,
+                                           ct.c_void_p]
+        L.dpg_mesh_hex_device.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_double, ct.c_uint64, ct.c_void_p,
+                                          ct.c_void_p, ct.c_void_p]
         L.dpg_mesh_quad_graded.argtypes = [ct.c_int, ct.c_int, ct.c_double, ct.c_uint64, ct.c_void_p,
                                            ct.c_void_p, ct.c_void_p]
         L.dpg_load_sine_modes.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_uint64, P(LoadDesc)]
@@ -157,8 +164,10 @@ def lib():
         L.dpg_assemble.argtypes = [ct.c_void_p, ct.c_int, P(Batch), P(Outputs), ct.c_void_p]
         L.dpg_energy_error.argtypes = [ct.c_void_p, ct.c_int, P(Batch), ct.c_void_p, ct.c_void_p,
                                        ct.c_void_p]
+        L.dpg_energy_error_factored.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p,
+                                                ct.c_void_p, ct.c_void_p]
         L.dpg_adjoint.argtypes = [ct.c_void_p, ct.c_int, P(Batch), ct.c_void_p, ct.c_void_p, ct.c_void_p,
-                                  ct.c_void_p]
+                                  ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]
         L.dpg_plan_get_tiles.argtypes = [ct.c_void_p, ct.c_int, P(Tiles)]
         L.dpg_plan_set_tiles.argtypes = [ct.c_void_p, ct.c_int, P(Tiles)]
         L.dpg_plan_autotune.argtypes = [ct.c_void_p, ct.c_int, P(Batch), ct.c_int, ct.c_void_p]
@@ -185,6 +194,13 @@ def lib():
         L.dpg_sort_cells.argtypes = [ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]
         L.dpg_gather_cells.argtypes = [ct.c_int, ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_int,
                                        ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p]
+        L.dpg_coarse_create.argtypes = [P(FormDesc), P(BucketDesc), ct.c_int, P(ct.c_int32), ct.c_void_p,
+                                        P(ct.c_void_p)]
+        L.dpg_coarse_info.argtypes = [ct.c_void_p, P(ct.c_int64), P(ct.c_int64), P(ct.c_int)]
+        L.dpg_coarse_destroy.argtypes = [ct.c_void_p]
+        L.dpg_skeleton_solve_twolevel.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                                  ct.c_void_p, ct.c_void_p, ct.c_double, ct.c_int, ct.c_int,
+                                                  P(ct.c_int), P(ct.c_double), ct.c_void_p]
         L.dpg_mark_cells.argtypes = [ct.c_void_p, ct.c_int, ct.c_double, ct.c_void_p, ct.c_void_p]
         L.dpg_riesz_representation.argtypes = [ct.c_void_p, ct.c_int, P(Batch), ct.c_void_p, ct.c_void_p,
                                                ct.c_void_p]

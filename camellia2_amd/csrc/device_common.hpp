@@ -15,6 +15,7 @@ struct DSummand {
 
 constexpr int kMaxSummands = 256;
 constexpr int kMaxGroups = 40;
+constexpr int kMaxTestVars = 16;
 
 // Compiled per-form "term program" evaluated at every cubature point.
 // Groups: IP scalar terms (lt), volume trial blocks (field components),
@@ -35,6 +36,7 @@ struct DevProgram {
   int32_t sb_card[kMaxGroups];
   int32_t nf;
   int32_t ncoef;
+  int32_t test_off[kMaxTestVars + 1];  // test dofs of variable v: [test_off[v], test_off[v + 1])
   DSummand s[kMaxSummands];
 };
 

@@ -15,58 +15,7 @@
 
 #include "../../include/dpg_capi.h"
 #include "errors.hpp"
-
-namespace dpg {
-
-class Rng {
- public:
-  explicit Rng(uint64_t seed) {
-    uint64_t x = seed;
-    for (int i = 0; i < 4; ++i) s_[i] = splitmix(x);
-  }
-  uint64_t next() {
-    uint64_t r = rotl(s_[1] * 5, 7) * 9;
-    uint64_t t = s_[1] << 17;
-    s_[2] ^= s_[0];
-    s_[3] ^= s_[1];
-    s_[1] ^= s_[2];
-    s_[0] ^= s_[3];
-    s_[2] ^= t;
-    s_[3] = rotl(s_[3], 45);
-    return r;
-  }
-  double uniform() { return (next() >> 11) * (1.0 / 9007199254740992.0); }  // [0,1)
-  double uniform(double lo, double hi) { return lo + (hi - lo) * uniform(); }
-  double normal() {
-    if (has_spare_) {
-      has_spare_ = false;
-      return spare_;
-    }
-    double u1 = 1.0 - uniform();  // (0,1]
-    double u2 = uniform();
-    double r = std::sqrt(-2.0 * std::log(u1));
-    spare_ = r * std::sin(2.0 * M_PI * u2);
-    has_spare_ = true;
-    return r * std::cos(2.0 * M_PI * u2);
-  }
-  int integer(int lo, int hi) {  // inclusive
-    return lo + (int)(uniform() * (hi - lo + 1));
-  }
-
- private:
-  static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
-  static uint64_t splitmix(uint64_t& x) {
-    uint64_t z = (x += 0x9e3779b97f4a7c15ULL);
-    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
-    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
-    return z ^ (z >> 31);
-  }
-  uint64_t s_[4];
-  bool has_spare_ = false;
-  double spare_ = 0.0;
-};
-
-}  // namespace dpg
+#include "rng.hpp"
 
 extern "C" {
 

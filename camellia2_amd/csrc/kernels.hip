@@ -536,6 +536,142 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
   }
 }
 
+// 128 x 64 output tile (rows of C x columns) per workgroup of 4 waves for the
+// tall panel updates of the factorisation (m = rows below the panel, n = the
+// panel's 64 columns): wave w owns rows 32w..32w+31 and all 64 columns, 2 x 4
+// MFMA blocks, so a 16-row k-chunk staged between two barriers feeds 32 MFMAs
+// per wave (16 in the 64 x 64 tile) and each X row is staged once for the
+// whole panel. Same staging scheme as above (fixed columns per thread, clamped
+// addresses, zeroing at LDS write, two chunks in flight) and the same k order
+// per output element, so the results are those of the square tiles bit for
+// bit. No k-segment masks, no lower-triangle tile walk (dense m x n).
+__device__ inline void gemm_tn_f64_wide_body(const GemmArgs& g) {
+  constexpr int TM = 128, TN = 64, NT = 256;
+  constexpr int PX = GKC * TM / NT, RX = NT / TM;  // X: 8 elements per thread, k-rows t/128 + 2r
+  constexpr int PY = GKC * TN / NT, RY = NT / TN;  // Y: 4 elements per thread, k-rows t/64 + 4r
+  int b = xcd_remap(blockIdx.x, gridDim.x);
+  int cell = b / g.ntiles, tile = b % g.ntiles;
+  if (cell >= g.ncells) return;
+  const int ti = tile % g.tiles_m, tj = tile / g.tiles_m;
+  const int i0 = ti * TM, j0 = tj * TN;
+  double* __restrict__ C = g.C + (size_t)cell * g.sc;
+  __shared__ double Xs[GKC][TM + 4];
+  __shared__ double Ys[GKC][TN + 4];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  d4 acc[4][2];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int bb = 0; bb < 2; ++bb) acc[a][bb] = d4{0.0, 0.0, 0.0, 0.0};
+  const int kend = g.k;
+  const int kx0 = t / TM, ix = t % TM, ky0 = t / TN, iy = t % TN;
+  const bool vx = i0 + ix < g.m, vy = j0 + iy < g.n;
+  const double* __restrict__ xp = g.X + (size_t)cell * g.sx + (vx ? i0 + ix : 0) + (size_t)kx0 * g.ldx;
+  const double* __restrict__ yp = g.Y + (size_t)cell * g.sy + (vy ? j0 + iy : 0) + (size_t)ky0 * g.ldy;
+  const int ldx = (int)g.ldx, ldy = (int)g.ldy;
+  auto fetch = [&](double (&xr)[PX], double (&yr)[PY], int k0) {
+    if (k0 + GKC <= kend) {
+#pragma unroll
+      for (int r = 0; r < PX; ++r) xr[r] = xp[(k0 + r * RX) * ldx];
+#pragma unroll
+      for (int r = 0; r < PY; ++r) yr[r] = yp[(k0 + r * RY) * ldy];
+    } else {
+#pragma unroll
+      for (int r = 0; r < PX; ++r) xr[r] = xp[(min(k0 + kx0 + r * RX, kend - 1) - kx0) * ldx];
+#pragma unroll
+      for (int r = 0; r < PY; ++r) yr[r] = yp[(min(k0 + ky0 + r * RY, kend - 1) - ky0) * ldy];
+    }
+  };
+  auto stage = [&](const double (&xr)[PX], const double (&yr)[PY], int k0) {
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < PX; ++r) Xs[kx0 + r * RX][ix] = (vx && k0 + kx0 + r * RX < kend) ? xr[r] : 0.0;
+#pragma unroll
+    for (int r = 0; r < PY; ++r) Ys[ky0 + r * RY][iy] = (vy && k0 + ky0 + r * RY < kend) ? yr[r] : 0.0;
+    __syncthreads();
+  };
+  auto multiply = [&]() {
+#pragma unroll
+    for (int k4 = 0; k4 < GKC; k4 += 4) {
+      const int kr = k4 + (lane >> 4), cc = lane & 15;
+      double xa[2], yb[4];
+      xa[0] = Xs[kr][w * 32 + cc];
+      xa[1] = Xs[kr][w * 32 + 16 + cc];
+#pragma unroll
+      for (int bj = 0; bj < 4; ++bj) yb[bj] = Ys[kr][bj * 16 + cc];
+#pragma unroll
+      for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+          acc[bj][bi] = __builtin_amdgcn_mfma_f64_16x16x4f64(yb[bj], xa[bi], acc[bj][bi], 0, 0, 0);
+    }
+  };
+  double xa_[PX], ya_[PY], xb_[PX], yb_[PY];
+  int kc = 0, kn = GKC;
+  if (kc < kend) fetch(xa_, ya_, kc);
+  if (kn < kend) fetch(xb_, yb_, kn);
+  const int si0 = i0 + w * 32;
+  const bool busy = si0 < g.m && j0 < g.n && (!g.skip_upper || si0 + 31 >= j0);
+  while (kc < kend) {
+    stage(xa_, ya_, kc);
+    int kf = kn + GKC;
+    if (kf < kend) fetch(xa_, ya_, kf);
+    if (busy) multiply();
+    kc = kn;
+    kn = kf;
+    if (kc >= kend) break;
+    stage(xb_, yb_, kc);
+    kf = kn + GKC;
+    if (kf < kend) fetch(xb_, yb_, kf);
+    if (busy) multiply();
+    kc = kn;
+    kn = kf;
+  }
+  // epilogue as in the square tile, in four column quarters (bj)
+  int ir[2];
+  bool iv[2];
+  double sr[2] = {1.0, 1.0};
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi) {
+    ir[bi] = si0 + bi * 16 + (lane & 15);
+    iv[bi] = ir[bi] < g.m;
+    if (g.srow) sr[bi] = g.srow[(size_t)cell * g.ss + (iv[bi] ? ir[bi] : 0)];
+  }
+#pragma unroll
+  for (int bj = 0; bj < 4; ++bj) {
+    int jc[4];
+    bool jv[4];
+    double sc[4] = {1.0, 1.0, 1.0, 1.0}, cold[2][4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      jc[v] = j0 + bj * 16 + (lane >> 4) + 4 * v;
+      jv[v] = jc[v] < g.n;
+      if (g.scol) sc[v] = g.scol[(size_t)cell * g.ss + (jv[v] ? jc[v] : 0)];
+    }
+    if (g.beta != 0.0) {
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          cold[bi][v] = C[(size_t)(iv[bi] ? ir[bi] : 0) + (size_t)(jv[v] ? jc[v] : 0) * g.ldc];
+    }
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = ir[bi], j = jc[v];
+        if (iv[bi] && jv[v] && (!g.skip_upper || i >= j)) {
+          double val = g.alpha * acc[bj][bi][v];
+          val *= sr[bi];
+          val *= sc[v];
+          if (g.beta != 0.0) val = fma(g.beta, cold[bi][v], val);
+          C[(size_t)i + (size_t)j * g.ldc] = val;
+        }
+      }
+  }
+}
+
 // one kernel symbol per role and tile so that profiles separate them
 __global__ __launch_bounds__(256) void gemm_gram_f64(GemmArgs g) { gemm_tn_f64_body<64>(g); }
 __global__ __launch_bounds__(256) void gemm_bmat_f64(GemmArgs g) { gemm_tn_f64_body<64>(g); }
@@ -543,6 +679,9 @@ __global__ __launch_bounds__(256) void gemm_update_f64(GemmArgs g) { gemm_tn_f64
 __global__ __launch_bounds__(64) void gemm_gram_f64_t32(GemmArgs g) { gemm_tn_f64_body<32>(g); }
 __global__ __launch_bounds__(64) void gemm_bmat_f64_t32(GemmArgs g) { gemm_tn_f64_body<32>(g); }
 __global__ __launch_bounds__(64) void gemm_update_f64_t32(GemmArgs g) { gemm_tn_f64_body<32>(g); }
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void gemm_update_f64_w128(GemmArgs g) {
+  gemm_tn_f64_wide_body(g);
+}
 
 // ------------------------------------------------- sum-factorised Gram
 // One workgroup per (block pair, cell). G_AB[i, j] with i = (i1, i2, i3),
@@ -1351,144 +1490,293 @@ __device__ inline double dcoef_value(const DSummand& sm, int k, const double* cc
   return v;
 }
 
-// Per-point contributions to d(r^T G^-1 r)/d(coef k) = -2 e^T dB/dk u - e^T dG/dk e:
-//   kind 0 (Gram points):  -w detJ * 2 * lt(e_h) * dlt/dk(e_h)        for every IP term
-//   kind 1 (volume):       -2 w detJ * dcoef/dk * op(e_h) * u_h|block  for every volume term
-//   kind 2 (sides):        -2 w |J t| * dcoef/dk * op(e_h) * u_h|side (x parity for fluxes)
-// e_h's operators come from the reference prims (recomputed, not stored) and
-// the transforms at the point; partial[c][p0 + p][k] are reduced afterwards.
-template <int DIM>
-__global__ __launch_bounds__(128) void adjoint_point_kernel(int kind, const DevProgram* __restrict__ prog,
-                                                            DevTables tab, int npts, int Qs, int ncells, int N,
-                                                            const double* __restrict__ nodes,
-                                                            const int8_t* __restrict__ parities,
-                                                            const double* __restrict__ coefs, int nsides,
-                                                            const double* __restrict__ e, int64_t se,
-                                                            const double* __restrict__ u, int M,
-                                                            double* __restrict__ partial, int64_t sp, int p0) {
-  int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= ncells * npts) return;
-  const int c = gid / npts, p = gid % npts;
-  const int V = prog->nverts;
-  constexpr int NPR = DIM + 1;
-  const int nvar = prog->nvar, ncoef = prog->ncoef;
-  const double* X = nodes + (size_t)c * V * DIM;
-  const double* cc = coefs + (size_t)c * ncoef;
-  const double* dN = kind == 0 ? tab.dN_ip : (kind == 1 ? tab.dN_vol : tab.dN_side);
-  const double* R = kind == 0 ? tab.R_ip : (kind == 1 ? tab.R_vol : tab.R_side);
-  PointGeo<DIM> g;
-  point_geometry<DIM>(X, dN + (size_t)p * V * DIM, V, g);
-  // reference prims of the test function e_h: E[var][r]
-  double E[DPG_MAX_VARS][NPR];
-  for (int v = 0; v < nvar; ++v)
-#pragma unroll
-    for (int r = 0; r < NPR; ++r) E[v][r] = 0.0;
-  const double* ec = e + (size_t)c * se;
-  for (int i = 0; i < N; ++i) {
-    const int v = tab.var_of_dof[i];
-    const double ei = ec[i];
-#pragma unroll
-    for (int r = 0; r < NPR; ++r) E[v][r] = fma(ei, R[((size_t)p * NPR + r) * N + i], E[v][r]);
-  }
-  const double* uc = u + (size_t)c * M;
-  double n[3] = {0.0, 0.0, 0.0};
-  double wm;
-  int s = 0;
-  if (kind == 0) {
-    wm = tab.w_ip[p] * g.detJ;
-  } else if (kind == 1) {
-    wm = tab.w_vol[p] * g.detJ;
-  } else {
-    s = p / Qs;
-    double T[2][3];
-    for (int a = 0; a < DIM - 1; ++a)
-      for (int d = 0; d < DIM; ++d) {
-        double acc = 0.0;
-        for (int ee = 0; ee < DIM; ++ee) acc += g.J[d][ee] * tab.side_tan[((size_t)s * 2 + a) * DIM + ee];
-        T[a][d] = acc;
-      }
-    if constexpr (DIM == 2) {
-      n[0] = T[0][1];
-      n[1] = -T[0][0];
-    } else {
-      n[0] = T[0][1] * T[1][2] - T[0][2] * T[1][1];
-      n[1] = T[0][2] * T[1][0] - T[0][0] * T[1][2];
-      n[2] = T[0][0] * T[1][1] - T[0][1] * T[1][0];
-    }
-    double len = 0.0;
-    for (int d = 0; d < DIM; ++d) len += n[d] * n[d];
-    len = sqrt(len);
-    for (int d = 0; d < DIM; ++d) n[d] /= len;
-    wm = tab.w_side[p] * len;
-  }
-  const double* nn = kind == 2 ? n : nullptr;
-  double grad[DPG_MAX_COEFS];
-  for (int k = 0; k < ncoef; ++k) grad[k] = 0.0;
-  double t[NPR];
-  auto op_of_e = [&](const DSummand& sm) {
-    op_to_prims<DIM>(sm.op, g, nn, t);
+// ------------------------------------------------- stored factor, adjoint
+// The factored first N columns of a cell's augmented matrix as a row-major
+// (N + M + 1) x N block F: rows 0..N-1 the lower factor Lt of S G S = Lt Lt^T
+// (entries above the diagonal 0), rows N..N+M-1 the rows of (Lt^-1 S B)^T,
+// row N+M (Lt^-1 S l)^T. Entries of Lt inside a diagonal 64-block come from
+// Ld, the others from A. 32x32 tiles through LDS: A read along its columns,
+// F written along its rows.
+__global__ __launch_bounds__(256) void extract_factor(const double* __restrict__ A, int64_t lda, int64_t sA, int N,
+                                                      int R, const double* __restrict__ Ld, int64_t sLd,
+                                                      double* __restrict__ F, int64_t sF) {
+  __shared__ double tile[32][33];
+  const int c = blockIdx.z, ti = blockIdx.y, tj = blockIdx.x;  // row tile, column tile of F
+  const double* Ac = A + (size_t)c * sA;
+  const double* Lc = Ld + (size_t)c * sLd;
+  double* Fc = F + (size_t)c * sF;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int yy = ty; yy < 32; yy += 8) {
+    const int i = ti * 32 + tx, j = tj * 32 + yy;
     double v = 0.0;
-#pragma unroll
-    for (int r = 0; r < NPR; ++r) v = fma(t[r], E[sm.var][r], v);
-    return v;
-  };
-  if (kind == 0) {
-    for (int lt = 0; lt < prog->n_lt; ++lt) {
-      double val = 0.0;
-      double dval[DPG_MAX_COEFS];
-      for (int k = 0; k < ncoef; ++k) dval[k] = 0.0;
-      for (int si = prog->lt_beg[lt]; si < prog->lt_beg[lt + 1]; ++si) {
-        const DSummand& sm = prog->s[si];
-        const double o = op_of_e(sm);
-        val = fma(coef_value(sm, cc, nullptr), o, val);
-        for (int k = 0; k < ncoef; ++k) dval[k] = fma(dcoef_value(sm, k, cc, nullptr), o, dval[k]);
-      }
-      for (int k = 0; k < ncoef; ++k) grad[k] -= 2.0 * wm * val * dval[k];
+    if (i < R && j < N && i >= j) {
+      const int kb = i & ~63;
+      v = (i < N && j >= kb) ? Lc[(size_t)i * 64 + (j - kb)] : Ac[(size_t)i + (size_t)j * lda];
     }
-  } else if (kind == 1) {
-    const double* phi = tab.field_vol + (size_t)p * prog->nf;
-    for (int gi = 0; gi < prog->n_vb; ++gi) {
-      double ug = 0.0;
-      const double* ub = uc + prog->vb_off[gi];
-      for (int j = 0; j < prog->nf; ++j) ug = fma(ub[j], phi[j], ug);
-      for (int si = prog->vb_beg[gi]; si < prog->vb_beg[gi + 1]; ++si) {
-        const DSummand& sm = prog->s[si];
-        if (sm.slot < 0) continue;
-        const double o = op_of_e(sm);
-        for (int k = 0; k < ncoef; ++k) grad[k] -= 2.0 * wm * dcoef_value(sm, k, cc, nullptr) * o * ug;
-      }
-    }
-  } else {
-    const int q = p % Qs;
-    for (int gi = 0; gi < prog->n_sb; ++gi) {
-      const int card = prog->sb_card[gi];
-      const double* tb = (prog->sb_flux[gi] ? tab.flux : tab.trace) + (size_t)q * card;
-      const double* ub = uc + prog->sb_off[gi] + (size_t)s * card;
-      double ug = 0.0;
-      for (int j = 0; j < card; ++j) ug = fma(ub[j], tb[j], ug);
-      if (prog->sb_flux[gi]) ug *= (double)parities[(size_t)c * nsides + s];
-      for (int si = prog->sb_beg[gi]; si < prog->sb_beg[gi + 1]; ++si) {
-        const DSummand& sm = prog->s[si];
-        if (sm.slot < 0) continue;
-        const double o = op_of_e(sm);
-        for (int k = 0; k < ncoef; ++k) grad[k] -= 2.0 * wm * dcoef_value(sm, k, cc, nn) * o * ug;
-      }
-    }
+    tile[tx][yy] = v;
   }
-  double* out = partial + (size_t)c * sp + (size_t)(p0 + p) * ncoef;
-  for (int k = 0; k < ncoef; ++k) out[k] = grad[k];
+  __syncthreads();
+  for (int yy = ty; yy < 32; yy += 8) {
+    const int i = ti * 32 + yy, j = tj * 32 + tx;
+    if (i < R && j < N) Fc[(size_t)i * N + j] = tile[yy][tx];
+  }
 }
 
-// grad[c][k] = sum over all points of partial[c][p][k], in point order
-__global__ void grad_reduce_kernel(const double* __restrict__ partial, int64_t sp, int npts, int ncoef, int ncells,
-                                   double* __restrict__ grad) {
-  int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= ncells * ncoef) return;
-  int c = gid / ncoef, k = gid % ncoef;
-  const double* pc = partial + (size_t)c * sp;
+// From a stored factor F: z = y_l - Y u = Lt^-1 S (l - B u) and eta2 = z^T z
+// (= r^T G^-1 r). One workgroup per cell, lanes over the test dofs; eta2 by a
+// fixed tree, so the same bits on every call. z and eta2 may be NULL.
+__global__ __launch_bounds__(256) void factored_residual(const double* __restrict__ F, int64_t sF, int N, int M,
+                                                         const double* __restrict__ u, double* __restrict__ z,
+                                                         int64_t sz, double* __restrict__ eta2) {
+  __shared__ double part[256];
+  const int c = blockIdx.x, t = threadIdx.x;
+  const double* Fc = F + (size_t)c * sF;
+  const double* uc = u + (size_t)c * M;
   double acc = 0.0;
-  for (int p = 0; p < npts; ++p) acc += pc[(size_t)p * ncoef + k];
-  grad[(size_t)c * ncoef + k] = acc;
+  for (int j = t; j < N; j += 256) {
+    double v = Fc[(size_t)(N + M) * N + j];
+    for (int m = 0; m < M; ++m) v = fma(-Fc[(size_t)(N + m) * N + j], uc[m], v);
+    if (z) z[(size_t)c * sz + j] = v;
+    acc = fma(v, v, acc);
+  }
+  part[t] = acc;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h) part[t] += part[t + h];
+    __syncthreads();
+  }
+  if (t == 0 && eta2) eta2[c] = part[0];
+}
+
+// X = S Lt^-T [Y | y_l | z] for the columns the adjoint needs (row-major
+// N x ncol): columns 0..nT-1 from the rows N.. of F (Y = Lt^-1 S B, nT = M or
+// 0), then y_l (row N+M of F) when with_y, then z when z is not NULL. The
+// columns are G^-1 B, G^-1 l and G^-1 (l - B u). One workgroup per cell, a
+// lane per column: substitution from the last row up with Lt's entries as
+// wave-uniform loads, then the scaling by s.
+__global__ __launch_bounds__(256) void factor_backsolve(const double* __restrict__ F, int64_t sF, int N, int M,
+                                                        const double* __restrict__ s, int64_t ss, int nT, int with_y,
+                                                        const double* __restrict__ z, int64_t sz,
+                                                        double* __restrict__ X, int64_t sX, int ncol) {
+  const int c = blockIdx.x;
+  const double* Fc = F + (size_t)c * sF;
+  const double* sc = s + (size_t)c * ss;
+  double* Xc = X + (size_t)c * sX;
+  for (int m = threadIdx.x; m < ncol; m += blockDim.x) {
+    const double* b = m < nT ? Fc + (size_t)(N + m) * N
+                             : ((with_y && m == nT) ? Fc + (size_t)(N + M) * N : z + (size_t)c * sz);
+    for (int j = N - 1; j >= 0; --j) {
+      double x = b[j];
+      for (int i = j + 1; i < N; ++i) x = fma(-Fc[(size_t)i * N + j], Xc[(size_t)i * ncol + m], x);
+      Xc[(size_t)j * ncol + m] = x / Fc[(size_t)j * N + j];
+    }
+    for (int j = 0; j < N; ++j) Xc[(size_t)j * ncol + m] *= sc[j];
+  }
+}
+
+// Adjoint of the local assembly with respect to the per-cell coefficients:
+//   grad[c][k] = d/dcoef_k ( <Kbar, K> + <rbar, rhs> + eta2(u) ),
+// every part present only when its cotangent (Kbar, rbar) or u is given. With
+// T = G^-1 B, y = G^-1 l, e = G^-1 (l - B u) (the columns of X) and
+// Ks = sym(Kbar):
+//   Bbar = 2 T Ks + y rbar^T - 2 e u^T,   Gbar = -T Ks T^T - sym(T rbar y^T) - e e^T,
+//   d/dk = <Bbar, dB/dk> + <Gbar, dG/dk>   (l does not depend on the coefficients).
+// B and G are sums over cubature points of w * coef * op(test) * trial and of
+// w * lt(test) lt(test), so at a point only the test operators applied to the
+// functions whose test-basis coefficients are the columns of X are needed,
+// P[m] = op(sum_i X[i][m] psi_i)(x_q): the reference prims of those functions
+// are contracted from the tables (recomputed, not stored), then transformed:
+//   Gram point, IP term lt:      2 w dlt_k(X)^T Gp lt(X)
+//   volume / side term (coef):   w dcoef_k op(X)^T Bp phi
+// with Gp, Bp the cotangents in the coordinates of X's columns and phi the
+// trial basis values at the point (flux values times the side parity). One
+// workgroup per cell, lanes over the columns, the points in a fixed order and
+// one fixed-tree reduction per coefficient at the end: the same bits on every
+// call. Column layout: T at [0, nT), y at cy (or -1), e at ce (or -1).
+template <int DIM>
+__global__ __launch_bounds__(256) void adjoint_kernel(const DevProgram* __restrict__ prog, DevTables tab, int N,
+                                                      int M, int QG, int Q, int Qs, int nsides,
+                                                      const double* __restrict__ nodes,
+                                                      const int8_t* __restrict__ parities,
+                                                      const double* __restrict__ coefs,
+                                                      const double* __restrict__ X, int64_t sX, int ncol, int nT,
+                                                      int cy, int ce, const double* __restrict__ Kbar,
+                                                      const double* __restrict__ rbar,
+                                                      const double* __restrict__ u, double* __restrict__ grad) {
+  extern __shared__ double lds[];
+  __shared__ double red[256];
+  constexpr int NPR = DIM + 1;
+  const int c = blockIdx.x, t = threadIdx.x;
+  const int V = prog->nverts, nvar = prog->nvar, ncoef = prog->ncoef;
+  double* Z = lds;                                 // Z[(v * NPR + r) * ncol + m]
+  double* Pl = lds + (size_t)nvar * NPR * ncol;    // P[m] of one IP term
+  const double* Xn = nodes + (size_t)c * V * DIM;
+  const double* cc = coefs + (size_t)c * ncoef;
+  const double* Xc = X + (size_t)c * sX;
+  const double* Kc = Kbar ? Kbar + (size_t)c * M * M : nullptr;
+  const double* rc = rbar ? rbar + (size_t)c * M : nullptr;
+  const double* uc = u ? u + (size_t)c * M : nullptr;
+  double acc[DPG_MAX_COEFS];
+  for (int k = 0; k < DPG_MAX_COEFS; ++k) acc[k] = 0.0;
+  const int npts = QG + Q + nsides * Qs;
+  for (int pp = 0; pp < npts; ++pp) {
+    const int kind = pp < QG ? 0 : (pp < QG + Q ? 1 : 2);
+    const int p = kind == 0 ? pp : (kind == 1 ? pp - QG : pp - QG - Q);
+    const double* dN = kind == 0 ? tab.dN_ip : (kind == 1 ? tab.dN_vol : tab.dN_side);
+    const double* R = kind == 0 ? tab.R_ip : (kind == 1 ? tab.R_vol : tab.R_side);
+    PointGeo<DIM> g;
+    point_geometry<DIM>(Xn, dN + (size_t)p * V * DIM, V, g);
+    double n[3] = {0.0, 0.0, 0.0};
+    double wm;
+    int side = 0;
+    if (kind == 0) {
+      wm = tab.w_ip[p] * g.detJ;
+    } else if (kind == 1) {
+      wm = tab.w_vol[p] * g.detJ;
+    } else {
+      side = p / Qs;
+      double Tg[2][3];
+      for (int a = 0; a < DIM - 1; ++a)
+        for (int d = 0; d < DIM; ++d) {
+          double s = 0.0;
+          for (int e = 0; e < DIM; ++e) s += g.J[d][e] * tab.side_tan[((size_t)side * 2 + a) * DIM + e];
+          Tg[a][d] = s;
+        }
+      if constexpr (DIM == 2) {
+        n[0] = Tg[0][1];
+        n[1] = -Tg[0][0];
+      } else {
+        n[0] = Tg[0][1] * Tg[1][2] - Tg[0][2] * Tg[1][1];
+        n[1] = Tg[0][2] * Tg[1][0] - Tg[0][0] * Tg[1][2];
+        n[2] = Tg[0][0] * Tg[1][1] - Tg[0][1] * Tg[1][0];
+      }
+      double len = 0.0;
+      for (int d = 0; d < DIM; ++d) len += n[d] * n[d];
+      len = sqrt(len);
+      for (int d = 0; d < DIM; ++d) n[d] /= len;
+      wm = tab.w_side[p] * len;
+    }
+    const double* nn = kind == 2 ? n : nullptr;
+    // reference prims of the column functions at this point
+    __syncthreads();
+    for (int m = t; m < ncol; m += 256)
+      for (int v = 0; v < nvar; ++v) {
+        double zr[NPR];
+#pragma unroll
+        for (int r = 0; r < NPR; ++r) zr[r] = 0.0;
+        for (int i = prog->test_off[v]; i < prog->test_off[v + 1]; ++i) {
+          const double xi = Xc[(size_t)i * ncol + m];
+#pragma unroll
+          for (int r = 0; r < NPR; ++r) zr[r] = fma(R[((size_t)p * NPR + r) * N + i], xi, zr[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < NPR; ++r) Z[((size_t)v * NPR + r) * ncol + m] = zr[r];
+      }
+    __syncthreads();
+    double tr[NPR];
+    auto op_col = [&](const DSummand& sm, int m) {
+      op_to_prims<DIM>(sm.op, g, nn, tr);
+      double v = 0.0;
+#pragma unroll
+      for (int r = 0; r < NPR; ++r) v = fma(tr[r], Z[((size_t)sm.var * NPR + r) * ncol + m], v);
+      return v;
+    };
+    if (kind == 0) {
+      for (int lt = 0; lt < prog->n_lt; ++lt) {
+        bool dep = false;
+        for (int si = prog->lt_beg[lt]; si < prog->lt_beg[lt + 1]; ++si) dep |= prog->s[si].slot >= 0;
+        if (!dep) continue;  // the term's weight has no coefficient: no contribution
+        for (int m = t; m < ncol; m += 256) {
+          double v = 0.0;
+          for (int si = prog->lt_beg[lt]; si < prog->lt_beg[lt + 1]; ++si)
+            v = fma(coef_value(prog->s[si], cc, nullptr), op_col(prog->s[si], m), v);
+          Pl[m] = v;
+        }
+        __syncthreads();
+        for (int m = t; m < ncol; m += 256) {
+          // h = (Gbar in column coordinates) P
+          double h = 0.0;
+          if (m < nT) {
+            if (Kc)
+              for (int q = 0; q < M; ++q) h = fma(-0.5 * (Kc[(size_t)m * M + q] + Kc[(size_t)q * M + m]), Pl[q], h);
+            if (rc) h = fma(-0.5 * rc[m], Pl[cy], h);
+          } else if (m == cy) {
+            for (int q = 0; q < M; ++q) h = fma(-0.5 * rc[q], Pl[q], h);
+          } else if (m == ce) {
+            h = -Pl[ce];
+          }
+#pragma unroll
+          for (int k = 0; k < DPG_MAX_COEFS; ++k) {
+            if (k >= ncoef) continue;
+            double dv = 0.0;
+            for (int si = prog->lt_beg[lt]; si < prog->lt_beg[lt + 1]; ++si) {
+              const DSummand& sm = prog->s[si];
+              if (sm.slot == k) dv = fma(dcoef_value(sm, k, cc, nullptr), op_col(sm, m), dv);
+            }
+            acc[k] = fma(2.0 * wm * dv, h, acc[k]);
+          }
+        }
+        __syncthreads();
+      }
+    } else {
+      // trial blocks at this point: volume field blocks, or the side's trace/flux blocks
+      const int nblk = kind == 1 ? prog->n_vb : prog->n_sb;
+      for (int gi = 0; gi < nblk; ++gi) {
+        const int sb = kind == 1 ? prog->vb_beg[gi] : prog->sb_beg[gi];
+        const int se = kind == 1 ? prog->vb_beg[gi + 1] : prog->sb_beg[gi + 1];
+        bool dep = false;
+        for (int si = sb; si < se; ++si) dep |= prog->s[si].slot >= 0;
+        if (!dep) continue;
+        int off, card;
+        const double* phi;
+        double par = 1.0;
+        if (kind == 1) {
+          off = prog->vb_off[gi];
+          card = prog->nf;
+          phi = tab.field_vol + (size_t)p * prog->nf;
+        } else {
+          card = prog->sb_card[gi];
+          off = prog->sb_off[gi] + side * card;
+          phi = (prog->sb_flux[gi] ? tab.flux : tab.trace) + (size_t)(p % Qs) * card;
+          if (prog->sb_flux[gi]) par = (double)parities[(size_t)c * nsides + side];
+        }
+        for (int m = t; m < ncol; m += 256) {
+          // gb = (Bbar in column coordinates) phi
+          double gb = 0.0;
+          if (m < nT) {
+            if (Kc)
+              for (int j = 0; j < card; ++j)
+                gb = fma(Kc[(size_t)m * M + off + j] + Kc[(size_t)(off + j) * M + m], phi[j], gb);
+          } else if (m == cy) {
+            for (int j = 0; j < card; ++j) gb = fma(rc[off + j], phi[j], gb);
+          } else if (m == ce) {
+            for (int j = 0; j < card; ++j) gb = fma(-2.0 * uc[off + j], phi[j], gb);
+          }
+          gb *= par * wm;
+          for (int si = sb; si < se; ++si) {
+            const DSummand& sm = prog->s[si];
+            if (sm.slot < 0) continue;
+            const double d = dcoef_value(sm, sm.slot, cc, nn) * op_col(sm, m) * gb;
+#pragma unroll
+            for (int k = 0; k < DPG_MAX_COEFS; ++k)
+              if (k == sm.slot) acc[k] += d;
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < DPG_MAX_COEFS; ++k) {
+    if (k >= ncoef) break;
+    __syncthreads();
+    red[t] = acc[k];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+      if (t < h) red[t] += red[t + h];
+      __syncthreads();
+    }
+    if (t == 0) grad[(size_t)c * ncoef + k] = red[0];
+  }
 }
 
 // ------------------------------------------------------------ host launchers
@@ -1547,7 +1835,19 @@ void launch_ip_operand(int dim, const double* Wip, int n_lt, int nvar, const dou
 
 void launch_gemm(GemmArgs g, int role, hipStream_t st, int tile) {
   if (g.m <= 0 || g.n <= 0 || g.ncells <= 0) return;
-  if (tile != 64 && tile != 32) throw DeviceError("gemm tile must be 64 or 32");
+  if (tile != 64 && tile != 32 && tile != 128) throw DeviceError("gemm tile must be 128, 64 or 32");
+  // the 128 x 64 tile serves dense (not lower-triangle) updates without k masks;
+  // other calls of a 128 choice take the 64 x 64 tile
+  if (tile == 128 && (role != kGemmUpdate || g.lower || g.kmask || g.kw)) tile = 64;
+  if (tile == 128) {
+    g.tiles_m = (g.m + 127) / 128;
+    g.ntiles = g.tiles_m * ((g.n + 63) / 64);
+    int64_t nb = (int64_t)g.ntiles * g.ncells;
+    if (nb > 0x7fffffff) throw DeviceError("gemm grid too large");
+    gemm_update_f64_w128<<<dim3((unsigned)nb), 256, 0, st>>>(g);
+    check_launch("gemm_update_f64_w128");
+    return;
+  }
   g.tiles_m = (g.m + tile - 1) / tile;
   int tiles_n = (g.n + tile - 1) / tile;
   g.ntiles = g.lower ? g.tiles_m * (g.tiles_m + 1) / 2 : g.tiles_m * tiles_n;
@@ -1700,25 +2000,47 @@ void launch_backsolve(const double* A, int64_t lda, int64_t sA, int N, int row_r
   check_launch("backsolve_kernel");
 }
 
-void launch_adjoint_points(int dim, int kind, const DevProgram* prog, const DevTables& tab, int npts, int Qs,
-                           int ncells, int N, const double* nodes, const int8_t* par, const double* coefs, int nsides,
-                           const double* e, int64_t se, const double* u, int M, double* partial, int64_t sp, int p0,
-                           hipStream_t st) {
-  if (npts <= 0) return;
-  dim3 grid((ncells * npts + 127) / 128);
-  if (dim == 2)
-    adjoint_point_kernel<2><<<grid, 128, 0, st>>>(kind, prog, tab, npts, Qs, ncells, N, nodes, par, coefs, nsides, e,
-                                                  se, u, M, partial, sp, p0);
-  else
-    adjoint_point_kernel<3><<<grid, 128, 0, st>>>(kind, prog, tab, npts, Qs, ncells, N, nodes, par, coefs, nsides, e,
-                                                  se, u, M, partial, sp, p0);
-  check_launch("adjoint_point_kernel");
+void launch_extract_factor(const double* A, int64_t lda, int64_t sA, int N, int M, const double* Ld, int64_t sLd,
+                           double* F, int64_t sF, int ncells, hipStream_t st) {
+  if (ncells <= 0 || N <= 0) return;
+  const int R = N + M + 1;
+  extract_factor<<<dim3((N + 31) / 32, (R + 31) / 32, ncells), 256, 0, st>>>(A, lda, sA, N, R, Ld, sLd, F, sF);
+  check_launch("extract_factor");
 }
 
-void launch_grad_reduce(const double* partial, int64_t sp, int npts, int ncoef, int ncells, double* grad,
-                        hipStream_t st) {
-  grad_reduce_kernel<<<dim3((ncells * ncoef + 255) / 256), 256, 0, st>>>(partial, sp, npts, ncoef, ncells, grad);
-  check_launch("grad_reduce_kernel");
+void launch_factored_residual(const double* F, int64_t sF, int N, int M, const double* u, double* z, int64_t sz,
+                              double* eta2, int ncells, hipStream_t st) {
+  if (ncells <= 0) return;
+  factored_residual<<<ncells, 256, 0, st>>>(F, sF, N, M, u, z, sz, eta2);
+  check_launch("factored_residual");
+}
+
+void launch_factor_backsolve(const double* F, int64_t sF, int N, int M, const double* s, int64_t ss, int nT,
+                             int with_y, const double* z, int64_t sz, double* X, int64_t sX, int ncol, int ncells,
+                             hipStream_t st) {
+  if (ncells <= 0 || ncol <= 0) return;
+  factor_backsolve<<<ncells, 256, 0, st>>>(F, sF, N, M, s, ss, nT, with_y, z, sz, X, sX, ncol);
+  check_launch("factor_backsolve");
+}
+
+size_t adjoint_lds_bytes(int nvar, int dim, int ncol) {
+  return sizeof(double) * ((size_t)nvar * (dim + 1) * ncol + ncol);
+}
+
+void launch_adjoint(int dim, const DevProgram* prog, const DevTables& tab, int N, int M, int QG, int Q, int Qs,
+                    int nsides, int nvar, const double* nodes, const int8_t* par, const double* coefs,
+                    const double* X, int64_t sX, int ncol, int nT, int cy, int ce, const double* Kbar,
+                    const double* rbar, const double* u, double* grad, int ncells, hipStream_t st) {
+  if (ncells <= 0) return;
+  const size_t lds = adjoint_lds_bytes(nvar, dim, ncol);
+  if (lds > 64 * 1024) throw DeviceError("adjoint: LDS request above 64 KiB");
+  if (dim == 3)
+    adjoint_kernel<3><<<ncells, 256, lds, st>>>(prog, tab, N, M, QG, Q, Qs, nsides, nodes, par, coefs, X, sX, ncol,
+                                                nT, cy, ce, Kbar, rbar, u, grad);
+  else
+    adjoint_kernel<2><<<ncells, 256, lds, st>>>(prog, tab, N, M, QG, Q, Qs, nsides, nodes, par, coefs, X, sX, ncol,
+                                                nT, cy, ce, Kbar, rbar, u, grad);
+  check_launch("adjoint_kernel");
 }
 
 void launch_extract_row(const double* A, int64_t lda, int64_t sA, int row, int r0, int n, double sign, double* out,

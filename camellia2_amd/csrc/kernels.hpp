@@ -109,18 +109,29 @@ void launch_residual_row(double* A, int64_t lda, int64_t sA, int N, int M, int r
                          const double* u, int ncells, hipStream_t st);
 void launch_extract_sym(const double* A, int64_t lda, int64_t sA, int r0, int n, double sign, double* out, int ncells,
                         hipStream_t st);
-// adjoint: e = S Lt^-T y (y = residual row of A), per-point gradient partials, reduction
 // field recovery after a condensed solve: S K_FF S and the row s (rhs_F - K_FT u_T) into A
 void launch_recover_setup(const double* K, const double* rhs, const double* u, int M, int F, double* A, int64_t lda,
                           int64_t sA, double* s, int64_t ss, int ncells, hipStream_t st);
+// e = S Lt^-T y (y = residual row of A): the Riesz representation and the field back solve
 void launch_backsolve(const double* A, int64_t lda, int64_t sA, int N, int row_r, const double* Ld, int64_t sLd,
                       const double* s, int64_t ss, double* e, int64_t se, int ncells, hipStream_t st);
-void launch_adjoint_points(int dim, int kind, const DevProgram* prog, const DevTables& tab, int npts, int Qs,
-                           int ncells, int N, const double* nodes, const int8_t* par, const double* coefs, int nsides,
-                           const double* e, int64_t se, const double* u, int M, double* partial, int64_t sp, int p0,
-                           hipStream_t st);
-void launch_grad_reduce(const double* partial, int64_t sp, int npts, int ncoef, int ncells, double* grad,
-                        hipStream_t st);
+// the factored first N columns of the augmented matrices as row-major (N+M+1, N)
+// blocks (Lt, (Lt^-1 S B)^T, (Lt^-1 S l)^T), per-cell stride sF
+void launch_extract_factor(const double* A, int64_t lda, int64_t sA, int N, int M, const double* Ld, int64_t sLd,
+                           double* F, int64_t sF, int ncells, hipStream_t st);
+// z = Lt^-1 S (l - B u) and eta2 = z^T z from stored factors (z, eta2 optional)
+void launch_factored_residual(const double* F, int64_t sF, int N, int M, const double* u, double* z, int64_t sz,
+                              double* eta2, int ncells, hipStream_t st);
+// X (N x ncol per cell) = S Lt^-T [Y (nT columns) | y_l (with_y) | z (z != NULL)]
+void launch_factor_backsolve(const double* F, int64_t sF, int N, int M, const double* s, int64_t ss, int nT,
+                             int with_y, const double* z, int64_t sz, double* X, int64_t sX, int ncol, int ncells,
+                             hipStream_t st);
+size_t adjoint_lds_bytes(int nvar, int dim, int ncol);
+// grad (C, ncoef) of <Kbar, K> + <rbar, rhs> + eta2(u) from X = [G^-1 B | G^-1 l | G^-1 r]
+void launch_adjoint(int dim, const DevProgram* prog, const DevTables& tab, int N, int M, int QG, int Q, int Qs,
+                    int nsides, int nvar, const double* nodes, const int8_t* par, const double* coefs,
+                    const double* X, int64_t sX, int ncol, int nT, int cy, int ce, const double* Kbar,
+                    const double* rbar, const double* u, double* grad, int ncells, hipStream_t st);
 void launch_extract_row(const double* A, int64_t lda, int64_t sA, int row, int r0, int n, double sign, double* out,
                         int ncells, hipStream_t st);
 

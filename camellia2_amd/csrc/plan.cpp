@@ -195,8 +195,11 @@ struct BucketPlan {
   int npr = 0, D = 0;
   int64_t lda = 0, Npad = 0;
   // per-cell element counts (doubles) of the workspace regions
-  int64_t nA = 0, nAt = 0, nTv = 0, nTs = 0, nWip = 0, nWvol = 0, nWrhs = 0, nWside = 0, nS = 0, nLd = 0, nLi = 0, nE = 0, nP = 0;
+  int64_t nA = 0, nAt = 0, nTv = 0, nTs = 0, nWip = 0, nWvol = 0, nWrhs = 0, nWside = 0, nS = 0, nLd = 0, nLi = 0;
   int64_t per_cell_bytes = 0;
+  // adjoint only, after the regions above: stored factor, X = [G^-1 B | G^-1 l | G^-1 r], z
+  int64_t nFac = 0, nX = 0, nZ = 0;
+  int64_t per_cell_bytes_adj = 0;
   double flops = 0.0;
   double gram_flops = 0.0;  // sum over IP terms of QG * n(n+1), n = dofs the term touches
   int tile[3] = {64, 64, 64};  // FP64 GEMM tile edge per role: Gram, B blocks, Cholesky updates
@@ -276,6 +279,9 @@ void compile_program(const dpg_form_desc& f, BucketPlan& B) {
   P.vb_beg[P.n_vb] = ns;
   P.nf = B.L.nf;
   P.ncoef = f.n_coefs;
+  static_assert(kMaxTestVars == DPG_MAX_VARS, "test variable tables");
+  for (int v = 0; v < f.n_test; ++v) P.test_off[v] = B.L.test_off[v];
+  P.test_off[f.n_test] = B.L.N;
   for (int g = 0; g < P.n_vb; ++g) P.vb_off[g] = B.L.blocks[B.vb_block[g]].offset;
   // side groups: trace / flux variables
   B.sb_var.clear();
@@ -578,20 +584,23 @@ void build_bucket(dpg_plan* plan, const dpg_bucket_desc& desc, BucketPlan& B) {
   B.nS = round_up(std::max(L.N, L.M), 8);
   B.nLd = (int64_t)round_up(L.N + L.F, 64) * 64;  // factored diagonal blocks of both stages
   B.nLi = 64 * 64;                                 // current diagonal block, column-major
-  B.nE = round_up(L.N, 8);                          // adjoint: e = G^-1 r
-  B.nP = (int64_t)(L.QG + L.Q + S * L.Qs) * std::max(1, f.n_coefs);  // adjoint: per-point partials
-  int64_t tot = B.nA + B.nAt + B.nTv + B.nTs + B.nWip + B.nWvol + B.nWrhs + B.nWside + B.nS + B.nLd + B.nLi +
-                B.nE + B.nP;
+  int64_t tot = B.nA + B.nAt + B.nTv + B.nTs + B.nWip + B.nWvol + B.nWrhs + B.nWside + B.nS + B.nLd + B.nLi;
   B.per_cell_bytes = tot * (int64_t)sizeof(double);
+  B.nFac = round_up((int64_t)(L.N + L.M + 1) * L.N, 8);
+  B.nX = round_up((int64_t)L.N * (L.M + 2), 8);
+  B.nZ = round_up(L.N, 8);
+  B.per_cell_bytes_adj = B.per_cell_bytes + (B.nFac + B.nX + B.nZ) * (int64_t)sizeof(double);
 }
 
 struct Work {
-  double *A, *At, *Tv, *Ts, *Wip, *Wvol, *Wrhs, *Wside, *s, *Ld, *Li, *E, *Pt;
+  double *A, *At, *Tv, *Ts, *Wip, *Wvol, *Wrhs, *Wside, *s, *Ld, *Li;
+  double *Fac = nullptr, *X = nullptr, *Z = nullptr;  // adjoint regions (carve with adj)
   int32_t* info;  // per-cell LAPACK INFO of the batch
 };
 
-// per-cell regions of a batch of nb cells, carved from `base`
-Work carve(void* base, int32_t* info, const BucketPlan& B, int nb) {
+// per-cell regions of a batch of nb cells, carved from `base` (with adj, also
+// the adjoint's regions: B.per_cell_bytes_adj per cell)
+Work carve(void* base, int32_t* info, const BucketPlan& B, int nb, bool adj = false) {
   Work w;
   w.info = info;
   double* p = (double*)base;
@@ -611,15 +620,18 @@ Work carve(void* base, int32_t* info, const BucketPlan& B, int nb) {
   w.s = take(B.nS);
   w.Ld = take(B.nLd);
   w.Li = take(B.nLi);
-  w.E = take(B.nE);
-  w.Pt = take(B.nP);
+  if (adj) {
+    w.Fac = take(B.nFac);
+    w.X = take(B.nX);
+    w.Z = take(B.nZ);
+  }
   return w;
 }
 
-int batch_cells(dpg_plan* plan, const BucketPlan& B, int ncells) {
+int batch_cells(dpg_plan* plan, const BucketPlan& B, int ncells, int64_t per_cell_bytes = 0) {
   if (plan->opt.max_batch_cells > 0) return std::min(std::min(ncells, plan->opt.max_batch_cells), 65535);
   double budget = plan->opt.workspace_bytes > 0 ? plan->opt.workspace_bytes : 16.0 * (1ull << 30);
-  int64_t nb = (int64_t)(budget / (double)B.per_cell_bytes);
+  int64_t nb = (int64_t)(budget / (double)(per_cell_bytes > 0 ? per_cell_bytes : B.per_cell_bytes));
   nb = std::max<int64_t>(1, std::min<int64_t>(nb, ncells));
   return (int)std::min<int64_t>(nb, 65535);  // cells are a grid y/z dimension
 }
@@ -910,9 +922,16 @@ void assemble_batch(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, in
   const int N = B.L.N, M = B.L.M, F = B.L.F, T = B.L.T;
   stage_factor(plan, B, bt, c0, nb, w, nullptr, o.G, st);
   hipEvent_t e = plan->prof.start(st);
+  if (o.factor) {
+    // before the Schur stage, which re-uses the scale vector
+    launch_extract_factor(w.A, B.lda, B.nA, N, M, w.Ld, B.nLd, o.factor + (size_t)c0 * (N + M + 1) * N,
+                          (int64_t)(N + M + 1) * N, nb, st);
+    launch_extract_row(w.s, 1, B.nS, 0, 0, N, 1.0, o.factor_scale + (size_t)c0 * N, nb, st);
+  }
   if (o.K) launch_extract_sym(w.A, B.lda, B.nA, N, M, -1.0, o.K + (size_t)c0 * M * M, nb, st);
   if (o.rhs) launch_extract_row(w.A, B.lda, B.nA, N + M, N, M, -1.0, o.rhs + (size_t)c0 * M, nb, st);
-  plan->prof.stop(e, st, DPG_STAGE_EXTRACT, 0.0, nb * ((double)M * M * 2.0 + M * 2.0) * 8.0);
+  plan->prof.stop(e, st, DPG_STAGE_EXTRACT, 0.0,
+                  nb * ((double)M * M * 2.0 + M * 2.0 + (o.factor ? 2.0 * (N + M + 1.0) * N + 2.0 * N : 0.0)) * 8.0);
   if (o.schur || o.schur_rhs) {
     stage_schur(plan, B, nb, w, st);
     e = plan->prof.start(st);
@@ -938,8 +957,14 @@ void check_batch(const dpg_plan* plan, int bucket, const dpg_batch* batch) {
       throw std::invalid_argument("bad load mode count");
 }
 
-void prepare_workspace(dpg_plan* plan, const BucketPlan& B, int nb) {
-  plan->ws.ensure((size_t)B.per_cell_bytes * nb);
+void check_outputs(const dpg_outputs* o) {
+  if (!o) throw std::invalid_argument("outputs is NULL");
+  if ((o->factor == nullptr) != (o->factor_scale == nullptr))
+    throw std::invalid_argument("factor and factor_scale go together");
+}
+
+void prepare_workspace(dpg_plan* plan, const BucketPlan& B, int nb, int64_t per_cell_bytes = 0) {
+  plan->ws.ensure((size_t)(per_cell_bytes > 0 ? per_cell_bytes : B.per_cell_bytes) * nb);
   plan->info.ensure(sizeof(int32_t) * std::max(nb, 1));
 }
 
@@ -1045,7 +1070,7 @@ int dpg_plan_timings(dpg_plan* plan, dpg_timings* out, int reset) {
 int dpg_assemble(dpg_plan* plan, int bucket, const dpg_batch* batch, const dpg_outputs* outputs, void* stream) {
   return dpg::guarded_status([&] {
     dpg::check_batch(plan, bucket, batch);
-    if (!outputs) throw std::invalid_argument("outputs is NULL");
+    dpg::check_outputs(outputs);
     if (batch->ncells == 0) return 0;
     hipStream_t st = (hipStream_t)stream;
     const dpg::BucketPlan& B = plan->buckets[bucket];
@@ -1122,43 +1147,119 @@ int dpg_recover_fields(dpg_plan* plan, int bucket, int ncells, const double* K, 
   });
 }
 
-int dpg_adjoint(dpg_plan* plan, int bucket, const dpg_batch* batch, const double* u, double* eta2, double* grad,
+int dpg_energy_error_factored(dpg_plan* plan, int bucket, int ncells, const double* factor, const double* u,
+                              double* eta2, void* stream) {
+  return dpg::guarded([&] {
+    if (!plan) throw std::invalid_argument("plan is NULL");
+    if (bucket < 0 || bucket >= (int)plan->buckets.size()) throw std::invalid_argument("bucket index out of range");
+    if (ncells < 0) throw std::invalid_argument("ncells < 0");
+    if (ncells == 0) return;
+    if (!factor || !u || !eta2) throw std::invalid_argument("factor, u and eta2 are required");
+    const dpg::ElementLayout& L = plan->buckets[bucket].L;
+    for (int c0 = 0; c0 < ncells; c0 += 65535) {
+      const int nb = std::min(65535, ncells - c0);
+      dpg::launch_factored_residual(factor + (size_t)c0 * (L.N + L.M + 1) * L.N, (int64_t)(L.N + L.M + 1) * L.N,
+                                    L.N, L.M, u + (size_t)c0 * L.M, nullptr, 0, eta2 + c0, nb, (hipStream_t)stream);
+    }
+  });
+}
+
+int dpg_adjoint(dpg_plan* plan, int bucket, const dpg_batch* batch, const double* u, const double* Kbar,
+                const double* rhsbar, const double* factor, const double* factor_scale, double* eta2, double* grad,
                 void* stream) {
   return dpg::guarded_status([&] {
     dpg::check_batch(plan, bucket, batch);
     if (grad && plan->form.n_coefs <= 0) throw std::invalid_argument("the form has no per-cell coefficients");
     if (plan->opt.precision != DPG_PREC_FP64) throw std::invalid_argument("the adjoint runs in FP64 only");
-    if (batch->ncells == 0) return 0;  // an empty batch needs no buffers
-    if (!u || !eta2) throw std::invalid_argument("u and eta2 are required");
+    if ((factor == nullptr) != (factor_scale == nullptr))
+      throw std::invalid_argument("factor and factor_scale go together");
+    if (eta2 && !u) throw std::invalid_argument("eta2 needs u");
+    if (batch->ncells == 0 || (!grad && !eta2)) return 0;  // nothing to compute
     hipStream_t st = (hipStream_t)stream;
     const dpg::BucketPlan& B = plan->buckets[bucket];
     const dpg::ElementLayout& L = B.L;
     const int N = L.N, M = L.M, S = L.rc.nsides, dim = L.dim, V = L.rc.nverts, nc = plan->form.n_coefs;
+    // columns of X: G^-1 B (for Kbar or rhsbar), G^-1 l (rhsbar), G^-1 r (u)
+    const int nT = (Kbar || rhsbar) ? M : 0;
+    const int cy = rhsbar ? nT : -1;
+    const int ce = u ? nT + (rhsbar ? 1 : 0) : -1;
+    const int ncol = nT + (rhsbar ? 1 : 0) + (u ? 1 : 0);
+    if (grad && dpg::adjoint_lds_bytes(plan->form.n_test, dim, ncol) > 64 * 1024)
+      throw std::invalid_argument("adjoint: the element type is too large for the adjoint kernel's LDS");
+    const int64_t sF = (int64_t)(N + M + 1) * N;
+    const bool stored = factor != nullptr;
+    // algorithmic work per cell of the adjoint stage: back solves (N^2 per
+    // column), the column functions' prims at every point (2 N npr per column),
+    // the Kbar products of the terms whose weight has a coefficient (2 M^2 per
+    // IP term and Gram point, 2 M card per trial block and point); bytes: the
+    // factor, the cotangents and u read, the gradient written
+    double adj_flops = (double)N * N * ncol + 2.0 * N * (dim + 1) * ncol * (L.QG + L.Q + S * L.Qs);
+    if (Kbar) {
+      const dpg::DevProgram& P = B.prog;
+      auto has_coef = [&](int b0, int b1) {
+        for (int i = b0; i < b1; ++i)
+          if (P.s[i].slot >= 0) return true;
+        return false;
+      };
+      for (int lt = 0; lt < P.n_lt; ++lt)
+        if (has_coef(P.lt_beg[lt], P.lt_beg[lt + 1])) adj_flops += 2.0 * M * M * L.QG;
+      for (int g = 0; g < P.n_vb; ++g)
+        if (has_coef(P.vb_beg[g], P.vb_beg[g + 1])) adj_flops += 2.0 * M * P.nf * L.Q;
+      for (int g = 0; g < P.n_sb; ++g)
+        if (has_coef(P.sb_beg[g], P.sb_beg[g + 1])) adj_flops += 2.0 * M * P.sb_card[g] * S * L.Qs;
+    }
+    const double adj_bytes = 8.0 * ((double)sF + N + (Kbar ? (double)M * M : 0.0) + (rhsbar ? M : 0) + (u ? M : 0) +
+                                    (double)nc * (dim * V + 1));
+    // with a stored factor only X and z are needed
+    const int64_t per_cell = stored ? (B.nX + B.nZ) * (int64_t)sizeof(double) : B.per_cell_bytes_adj;
     int C = batch->ncells;
-    int maxb = dpg::batch_cells(plan, B, C);
-    dpg::prepare_workspace(plan, B, maxb);
+    int maxb = dpg::batch_cells(plan, B, C, per_cell);
+    dpg::prepare_workspace(plan, B, maxb, per_cell);
     int bad = 0;
     for (int c0 = 0; c0 < C; c0 += maxb) {
       int nb = std::min(maxb, C - c0);
-      dpg::Work w = dpg::carve(plan->ws.p, (int32_t*)plan->info.p, B, nb);
-      dpg::stage_factor(plan, B, *batch, c0, nb, w, u, nullptr, st);
-      dpg::launch_extract_row(w.A, B.lda, B.nA, B.D - 1, B.D - 1, 1, -1.0, eta2 + c0, nb, st);
+      dpg::Work w{};
+      const double *F, *s;
+      int64_t ss;
+      if (stored) {
+        w.X = (double*)plan->ws.p;
+        w.Z = w.X + B.nX * nb;
+        F = factor + (size_t)c0 * sF;
+        s = factor_scale + (size_t)c0 * N;
+        ss = N;
+        if (u)
+          dpg::launch_factored_residual(F, sF, N, M, u + (size_t)c0 * M, w.Z, B.nZ, eta2 ? eta2 + c0 : nullptr, nb,
+                                        st);
+      } else {
+        w = dpg::carve(plan->ws.p, (int32_t*)plan->info.p, B, nb, true);
+        // the forward factorisation with the residual row r = l - B u: its last
+        // row then holds z = Lt^-1 S r and its trailing entry -eta2
+        dpg::stage_factor(plan, B, *batch, c0, nb, w, u, nullptr, st);
+        dpg::launch_extract_factor(w.A, B.lda, B.nA, N, M, w.Ld, B.nLd, w.Fac, B.nFac, nb, st);
+        F = w.Fac;
+        s = w.s;
+        ss = B.nS;
+        if (u) {
+          dpg::launch_extract_row(w.A, B.lda, B.nA, B.D - 1, 0, N, 1.0, w.Z, nb, st);
+          if (eta2) dpg::launch_extract_row(w.A, B.lda, B.nA, B.D - 1, B.D - 1, 1, -1.0, eta2 + c0, nb, st);
+        }
+      }
+      const int64_t fstride = stored ? sF : B.nFac;
+      // z of extract_row is packed with stride N; factored_residual wrote it with stride nZ
+      const int64_t zstride = stored ? B.nZ : N;
       if (grad) {
-        const double* nodes = batch->nodes + (size_t)c0 * V * dim;
-        const int8_t* par = batch->parities + (size_t)c0 * S;
-        const double* coefs = batch->coefs + (size_t)c0 * nc;
-        const double* uc = u + (size_t)c0 * M;
-        dpg::launch_backsolve(w.A, B.lda, B.nA, N, B.D - 1, w.Ld, B.nLd, w.s, B.nS, w.E, B.nE, nb, st);
-        dpg::launch_adjoint_points(dim, 0, B.prog_d, B.tab, L.QG, L.Qs, nb, N, nodes, par, coefs, S, w.E, B.nE, uc, M,
-                                   w.Pt, B.nP, 0, st);
-        dpg::launch_adjoint_points(dim, 1, B.prog_d, B.tab, L.Q, L.Qs, nb, N, nodes, par, coefs, S, w.E, B.nE, uc, M,
-                                   w.Pt, B.nP, L.QG, st);
-        dpg::launch_adjoint_points(dim, 2, B.prog_d, B.tab, S * L.Qs, L.Qs, nb, N, nodes, par, coefs, S, w.E, B.nE,
-                                   uc, M, w.Pt, B.nP, L.QG + L.Q, st);
-        dpg::launch_grad_reduce(w.Pt, B.nP, L.QG + L.Q + S * L.Qs, nc, nb, grad + (size_t)c0 * nc, st);
+        hipEvent_t e = plan->prof.start(st);
+        dpg::launch_factor_backsolve(F, fstride, N, M, s, ss, nT, rhsbar ? 1 : 0, u ? w.Z : nullptr, zstride, w.X,
+                                     (int64_t)N * ncol, ncol, nb, st);
+        dpg::launch_adjoint(dim, B.prog_d, B.tab, N, M, L.QG, L.Q, L.Qs, S, plan->form.n_test,
+                            batch->nodes + (size_t)c0 * V * dim, batch->parities + (size_t)c0 * S,
+                            batch->coefs + (size_t)c0 * nc, w.X, (int64_t)N * ncol, ncol, nT, cy, ce,
+                            Kbar ? Kbar + (size_t)c0 * M * M : nullptr, rhsbar ? rhsbar + (size_t)c0 * M : nullptr,
+                            u ? u + (size_t)c0 * M : nullptr, grad + (size_t)c0 * nc, nb, st);
+        plan->prof.stop(e, st, DPG_STAGE_ADJOINT, nb * adj_flops, nb * adj_bytes);
       }
       plan->prof.collect(st, nb);
-      bad += dpg::count_failures(w, nb, nullptr, st);
+      if (!stored) bad += dpg::count_failures(w, nb, nullptr, st);
     }
     return bad;
   });
@@ -1205,7 +1306,8 @@ int dpg_plan_set_tiles(dpg_plan* plan, int bucket, const dpg_tiles* tiles) {
     if (bucket < 0 || bucket >= (int)plan->buckets.size()) throw std::invalid_argument("bucket index out of range");
     int v[3] = {tiles->gram, tiles->bmat, tiles->update};
     for (int r = 0; r < 3; ++r)
-      if (v[r] != 64 && v[r] != 32) throw std::invalid_argument("tile edges must be 64 or 32");
+      if (v[r] != 64 && v[r] != 32 && !(r == 2 && v[r] == 128))
+        throw std::invalid_argument("tile edges must be 64 or 32 (128 for the updates)");
     if (tiles->sumfact != 0 && tiles->sumfact != 1) throw std::invalid_argument("sumfact must be 0 or 1");
     if (tiles->sumfact && !plan->buckets[bucket].sf_ok)
       throw std::invalid_argument("the sum-factorised G is not available for this bucket");
@@ -1230,12 +1332,14 @@ int dpg_plan_autotune(dpg_plan* plan, int bucket, const dpg_batch* batch, int sa
     dpg_timings saved = plan->prof.acc;
     plan->prof.on = true;
     const int stage_of[3] = {DPG_STAGE_GRAM, DPG_STAGE_BMAT, DPG_STAGE_UPDATE};
-    const int cand[2] = {64, 32};
+    // candidate tile edges per role: 64 and 32 for all, and for the updates the
+    // 128 x 64 panel-update tile
+    const int cand[3][3] = {{64, 64, 64}, {32, 32, 32}, {64, 64, 128}};
     // the G path (dense or the opt-in sum-factorised one) is kept as set
-    double ms[2][3] = {{0}};
+    double ms[3][3] = {{0}};
     try {
-      for (int c = 0; c < 2; ++c) {
-        for (int r = 0; r < 3; ++r) B.tile[r] = cand[c];
+      for (int c = 0; c < 3; ++c) {
+        for (int r = 0; r < 3; ++r) B.tile[r] = cand[c][r];
         for (int pass = 0; pass < 2; ++pass) {  // pass 0 warms up
           plan->prof.acc = dpg_timings{};
           dpg::stage_factor(plan, B, *batch, 0, ns, w, nullptr, nullptr, st);
@@ -1249,7 +1353,11 @@ int dpg_plan_autotune(dpg_plan* plan, int bucket, const dpg_batch* batch, int sa
       plan->prof.acc = saved;
       throw;
     }
-    for (int r = 0; r < 3; ++r) B.tile[r] = ms[1][r] < ms[0][r] ? cand[1] : cand[0];
+    for (int r = 0; r < 2; ++r) B.tile[r] = ms[1][r] < ms[0][r] ? 32 : 64;
+    int best = 0;
+    for (int c = 1; c < 3; ++c)
+      if (ms[c][2] < ms[best][2]) best = c;
+    B.tile[2] = cand[best][2];
     plan->prof.on = on;
     plan->prof.acc = saved;
   });

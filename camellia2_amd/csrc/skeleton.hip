@@ -462,6 +462,99 @@ __global__ void mark_above(const double* __restrict__ err, int n, const double* 
   fcount[i] = f;
 }
 
+// --------------------------------------------- two-level (p-multigrid) solve
+// Coarse blocks of every cell: Sc = P^T D S D P, with D the free mask of the
+// cell's fine dofs (rows and columns of prescribed dofs dropped) and P the
+// local prolongation (T x Tc), the same for every cell. One workgroup per
+// cell, W = D S D P in LDS; Sc is computed for a >= b and mirrored, so it is
+// symmetric bit for bit.
+__global__ __launch_bounds__(256) void coarse_blocks(const double* __restrict__ S, int T, int Tc,
+                                                     const double* __restrict__ P, const int32_t* __restrict__ gdof,
+                                                     const int32_t* __restrict__ fixed, double* __restrict__ Sc) {
+  extern __shared__ double lds[];
+  double* W = lds;                   // T x Tc
+  double* m = lds + (size_t)T * Tc;  // free flags (0 / 1) of the cell's fine dofs
+  const int c = blockIdx.x, t = threadIdx.x;
+  const double* Sin = S + (size_t)c * T * T;
+  for (int i = t; i < T; i += 256) m[i] = fixed[gdof[(size_t)c * T + i]] ? 0.0 : 1.0;
+  __syncthreads();
+  for (int e = t; e < T * Tc; e += 256) {
+    const int i = e / Tc, a = e % Tc;
+    double acc = 0.0;
+    if (m[i] != 0.0)
+      for (int j = 0; j < T; ++j) acc = fma(Sin[(size_t)i * T + j] * m[j], P[(size_t)j * Tc + a], acc);
+    W[e] = acc;
+  }
+  __syncthreads();
+  double* out = Sc + (size_t)c * Tc * Tc;
+  for (int e = t; e < Tc * Tc; e += 256) {
+    const int a = e / Tc, b = e % Tc, hi = a > b ? a : b, lo = a > b ? b : a;
+    double acc = 0.0;
+    for (int i = 0; i < T; ++i) acc = fma(P[(size_t)i * Tc + hi], W[(size_t)i * Tc + lo], acc);
+    out[e] = acc;
+  }
+}
+
+// y[i] = sum over row i of val * x[col] (y[i] += ... with add), 0 where
+// mask[i] is set (mask may be NULL); one thread per row
+__global__ void csr_apply(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                          const double* __restrict__ val, const double* __restrict__ x,
+                          const int32_t* __restrict__ mask, int64_t n, int add, double* __restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double acc = 0.0;
+  if (!(mask && mask[i]))
+    for (int k = rp[i]; k < rp[i + 1]; ++k) acc = fma(val[k], x[col[k]], acc);
+  y[i] = add ? y[i] + acc : acc;
+}
+
+// coarse dofs whose row of the coarse matrix vanishes (all their fine support
+// prescribed) are held at 0 like prescribed dofs
+__global__ void zero_diag_mask(const double* __restrict__ d, int64_t n, int32_t* __restrict__ mask) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) mask[i] = d[i] > 0.0 ? 0 : 1;
+}
+
+// conjugate gradients on the coarse matrix with the scalars kept on the
+// device (s[0] = r.z, s[1] = p.q, s[2] = the new r.z): no host round trips
+__global__ void dcg_step(const double* __restrict__ s, const double* __restrict__ p, const double* __restrict__ q,
+                         int64_t n, double* __restrict__ x, double* __restrict__ r) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double alpha = s[1] > 0.0 ? s[0] / s[1] : 0.0;
+  x[i] = fma(alpha, p[i], x[i]);
+  r[i] = fma(-alpha, q[i], r[i]);
+}
+
+__global__ void dcg_direction(const double* __restrict__ s, const double* __restrict__ z, int64_t n,
+                              double* __restrict__ p) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double beta = s[0] > 0.0 ? s[2] / s[0] : 0.0;
+  p[i] = fma(beta, p[i], z[i]);
+}
+
+__global__ void dcg_shift(double* __restrict__ s) { s[0] = s[2]; }
+
+// r - y on the free rows, 0 on the prescribed ones
+__global__ void sub_free(const double* __restrict__ r, const double* __restrict__ y, const int32_t* __restrict__ fixed,
+                         int64_t n, double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = fixed[i] ? 0.0 : r[i] - y[i];
+}
+
+__global__ void fill_f64(double* __restrict__ a, int64_t n, double v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// a += b
+__global__ void add_f64(const double* __restrict__ b, int64_t n, double* __restrict__ a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] += b[i];
+}
+
 template <class T>
 struct DevArray {
   T* p = nullptr;
@@ -534,6 +627,47 @@ double reduce(const double* a, const double* b, int64_t n, int op, DevArray<doub
 
 unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)((n + per - 1) / per); }
 
+// reference coordinates (T, dim) of the nodes of a layout's trailing dofs
+std::vector<double> side_node_coords(const ElementLayout& L) {
+  const int dim = L.dim;
+  std::vector<double> xi((size_t)L.T * dim);
+  for (const TrialBlock& b : side_blocks(L)) {
+    const int n = side_degree(L, b);
+    const std::vector<double> t = gll_points(n);
+    const double* p = &L.rc.side_param[(size_t)b.side * 3 * dim];
+    for (int k = 0; k < b.card; ++k) {
+      const int a = k % (n + 1), bb = k / (n + 1);
+      double* x = &xi[(size_t)(b.offset - L.F + k) * dim];
+      for (int d = 0; d < dim; ++d) x[d] = p[d] + p[dim + d] * t[a] + (dim == 3 ? p[2 * dim + d] * t[bb] : 0.0);
+    }
+  }
+  return xi;
+}
+
+// Local prolongation (T x Tc, row-major) from the lowest-order layout Lc to L:
+// a coarse trace dof is a vertex of its side and its function on the side is
+// the cell's vertex function prod_d (1 + v_d xi_d) / 2 (reference cell
+// [-1,1]^d), evaluated at the fine nodes of the same variable and side; a
+// coarse flux dof is the constant 1 on its side.
+std::vector<double> local_prolongation(const ElementLayout& L, const ElementLayout& Lc) {
+  const int dim = L.dim, T = L.T, Tc = Lc.T;
+  const std::vector<double> xf = side_node_coords(L), xc = side_node_coords(Lc);
+  std::vector<double> P((size_t)T * Tc, 0.0);
+  for (const TrialBlock& b : side_blocks(L))
+    for (const TrialBlock& bc : side_blocks(Lc)) {
+      if (b.var != bc.var || b.side != bc.side) continue;
+      for (int i = 0; i < b.card; ++i)
+        for (int j = 0; j < bc.card; ++j) {
+          const int fi = b.offset - L.F + i, cj = bc.offset - Lc.F + j;
+          double w = 1.0;
+          if (b.kind == DPG_TRACE)
+            for (int d = 0; d < dim; ++d) w *= 0.5 * (1.0 + xc[(size_t)cj * dim + d] * xf[(size_t)fi * dim + d]);
+          P[(size_t)fi * Tc + cj] = w;
+        }
+    }
+  return P;
+}
+
 }  // namespace
 }  // namespace dpg
 
@@ -550,6 +684,17 @@ struct dpg_skeleton {
   int64_t blk_total = 0;  // sum of the squared block sizes
   dpg::DevArray<int32_t> blk_ptr, blk_dofs, pos_block;
   dpg::DevArray<int64_t> blk_off2;
+};
+
+// the lowest-order level of the condensed system (dpg_coarse_create)
+struct dpg_coarse {
+  int ncells = 0, T = 0, Tc = 0;
+  int64_t nfine = 0, ncoarse = 0;
+  dpg::DevArray<int32_t> gdof;  // (C, T) fine global dofs of every cell
+  dpg::DevArray<double> Pl;     // (T, Tc) local prolongation
+  dpg::DevArray<int32_t> p_rp, p_col, pt_rp, pt_col;  // global P (fine rows) and P^T (coarse rows), CSR
+  dpg::DevArray<double> p_val, pt_val;
+  std::unique_ptr<dpg_skeleton> csk;  // coarse pattern and summation order
 };
 
 extern "C" {
@@ -586,18 +731,8 @@ int dpg_skeleton_nodes(const dpg_form_desc* form, const dpg_bucket_desc* bucket,
   return dpg::guarded([&] {
     if (!form || !bucket || !xi) throw std::invalid_argument("NULL argument");
     dpg::ElementLayout L = dpg::make_layout(*form, *bucket);
-    const int dim = L.dim;
-    for (const dpg::TrialBlock& b : dpg::side_blocks(L)) {
-      const int n = dpg::side_degree(L, b);
-      const std::vector<double> t = dpg::gll_points(n);
-      const double* p = &L.rc.side_param[(size_t)b.side * 3 * dim];
-      for (int k = 0; k < b.card; ++k) {
-        const int a = k % (n + 1), bb = k / (n + 1);
-        double* x = xi + (size_t)(b.offset - L.F + k) * dim;
-        for (int d = 0; d < dim; ++d)
-          x[d] = p[d] + p[dim + d] * t[a] + (dim == 3 ? p[2 * dim + d] * t[bb] : 0.0);
-      }
-    }
+    const std::vector<double> x = dpg::side_node_coords(L);
+    std::memcpy(xi, x.data(), x.size() * sizeof(double));
   });
 }
 
@@ -812,6 +947,222 @@ int dpg_mark_cells(const double* err, int ncells, double rel_threshold, int32_t*
 
 int dpg_skeleton_destroy(dpg_skeleton* sk) {
   return dpg::guarded([&] { delete sk; });
+}
+
+int dpg_coarse_create(const dpg_form_desc* form, const dpg_bucket_desc* bucket, int ncells,
+                      const int32_t* cell_vertices, void* stream, dpg_coarse** out) {
+  return dpg::guarded([&] {
+    if (!form || !bucket || !out || ncells <= 0 || !cell_vertices) throw std::invalid_argument("bad arguments");
+    *out = nullptr;
+    if (bucket->topology != DPG_QUAD && bucket->topology != DPG_HEX)
+      throw std::invalid_argument("two-level preconditioning is built for quads and hexes");
+    hipStream_t st = (hipStream_t)stream;
+    const dpg::ElementLayout L = dpg::make_layout(*form, *bucket);
+    dpg_bucket_desc cb = *bucket;
+    cb.h1_order = 1;  // traces linear on the sides, fluxes constant
+    const dpg::ElementLayout Lc = dpg::make_layout(*form, cb);
+    if (L.T <= 0) throw std::invalid_argument("the form has no trace or flux variables");
+    const int C = ncells, T = L.T, Tc = Lc.T;
+    const dpg::Numbering nf = dpg::number_skeleton(L, C, cell_vertices);
+    const dpg::Numbering nc = dpg::number_skeleton(Lc, C, cell_vertices);
+    const std::vector<double> Pl = dpg::local_prolongation(L, Lc);
+    // global P: every fine dof's row from the first cell that holds it (the
+    // coarse functions are conforming, so every cell gives the same row: checked)
+    std::vector<int32_t> first((size_t)nf.nglobal, -1);  // (cell * T + local) of the first holder
+    for (int c = 0; c < C; ++c)
+      for (int i = 0; i < T; ++i) {
+        const int32_t g = nf.gdof[(size_t)c * T + i];
+        if (first[g] < 0) {
+          first[g] = c * T + i;
+          continue;
+        }
+        const int c0 = first[g] / T, i0 = first[g] % T;
+        for (int j = 0; j < Tc; ++j) {
+          const double v = Pl[(size_t)i * Tc + j];
+          if (v == 0.0) continue;
+          const int32_t gc = nc.gdof[(size_t)c * Tc + j];
+          bool found = false;
+          for (int j0 = 0; j0 < Tc; ++j0)
+            if (nc.gdof[(size_t)c0 * Tc + j0] == gc && std::fabs(Pl[(size_t)i0 * Tc + j0] - v) <= 1e-13) found = true;
+          if (!found) throw std::logic_error("coarse functions disagree between the cells of a shared dof");
+        }
+      }
+    std::vector<int32_t> rp((size_t)nf.nglobal + 1, 0), col;
+    std::vector<double> val;
+    for (int64_t g = 0; g < nf.nglobal; ++g) {
+      const int c = first[g] / T, i = first[g] % T;
+      for (int j = 0; j < Tc; ++j) {
+        const double v = Pl[(size_t)i * Tc + j];
+        if (v == 0.0) continue;
+        col.push_back(nc.gdof[(size_t)c * Tc + j]);
+        val.push_back(v);
+      }
+      rp[(size_t)g + 1] = (int32_t)col.size();
+    }
+    // P^T by counting
+    std::vector<int32_t> trp((size_t)nc.nglobal + 1, 0), tcol(col.size());
+    std::vector<double> tval(col.size());
+    for (int32_t j : col) ++trp[(size_t)j + 1];
+    for (int64_t j = 0; j < nc.nglobal; ++j) trp[(size_t)j + 1] += trp[(size_t)j];
+    std::vector<int32_t> fill(trp.begin(), trp.end() - 1);
+    for (int64_t g = 0; g < nf.nglobal; ++g)
+      for (int k = rp[g]; k < rp[(size_t)g + 1]; ++k) {
+        const int32_t q = fill[(size_t)col[k]]++;
+        tcol[q] = (int32_t)g;
+        tval[q] = val[k];
+      }
+    std::unique_ptr<dpg_coarse> co(new dpg_coarse());
+    co->ncells = C;
+    co->T = T;
+    co->Tc = Tc;
+    co->nfine = nf.nglobal;
+    co->ncoarse = nc.nglobal;
+    auto up = [&](auto& dst, const auto& src) {
+      dst.resize(src.size());
+      SK_CHECK(hipMemcpyAsync(dst.p, src.data(), sizeof(src[0]) * src.size(), hipMemcpyHostToDevice, st));
+    };
+    up(co->gdof, nf.gdof);
+    up(co->Pl, Pl);
+    up(co->p_rp, rp);
+    up(co->p_col, col);
+    up(co->p_val, val);
+    up(co->pt_rp, trp);
+    up(co->pt_col, tcol);
+    up(co->pt_val, tval);
+    dpg_skeleton* csk = nullptr;
+    if (dpg_skeleton_create(C, Tc, nc.gdof.data(), nc.nglobal, stream, &csk) != 0)
+      throw std::runtime_error(std::string("coarse skeleton: ") + dpg_last_error());
+    co->csk.reset(csk);
+    SK_CHECK(hipStreamSynchronize(st));  // the host vectors go out of scope
+    *out = co.release();
+  });
+}
+
+int dpg_coarse_info(const dpg_coarse* co, int64_t* nglobal_coarse, int64_t* nnz_coarse, int* T_coarse) {
+  return dpg::guarded([&] {
+    if (!co) throw std::invalid_argument("NULL coarse level");
+    if (nglobal_coarse) *nglobal_coarse = co->ncoarse;
+    if (nnz_coarse) *nnz_coarse = co->csk->nnz;
+    if (T_coarse) *T_coarse = co->Tc;
+  });
+}
+
+int dpg_coarse_destroy(dpg_coarse* co) {
+  return dpg::guarded([&] { delete co; });
+}
+
+int dpg_skeleton_solve_twolevel(const dpg_skeleton* sk, const dpg_coarse* co, const double* schur,
+                                const double* values, const double* rhs, const int32_t* fixed, double* x, double tol,
+                                int maxit, int coarse_iters, int* iters, double* relres, void* stream) {
+  return dpg::guarded([&] {
+    if (!sk || !co || !schur || !values || !rhs || !fixed || !x) throw std::invalid_argument("NULL argument");
+    if (!(tol >= 0.0) || maxit < 0 || coarse_iters < 0) throw std::invalid_argument("bad tolerance or counts");
+    if (sk->ncells != co->ncells || sk->T != co->T || sk->nglobal != co->nfine)
+      throw std::invalid_argument("the coarse level belongs to another mesh or element type");
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t n = sk->nglobal, nc = co->ncoarse;
+    const int C = co->ncells, T = co->T, Tc = co->Tc;
+    const size_t lds = sizeof(double) * ((size_t)T * Tc + T);
+    if (lds > 64 * 1024) throw std::invalid_argument("coarse blocks: the element type is too large");
+    const dpg_skeleton& cs = *co->csk;
+    // coarse matrix A_c = P^T A_ff P from the cells' Schur blocks
+    dpg::DevArray<double> Sc((size_t)C * Tc * Tc), vc(cs.nnz), dc(nc);
+    dpg::DevArray<int32_t> cmask(nc);
+    dpg::coarse_blocks<<<C, 256, lds, st>>>(schur, T, Tc, co->Pl.p, co->gdof.p, fixed, Sc.p);
+    dpg::ordered_segment_sum<<<(unsigned)(((uint64_t)cs.nnz + 255) / 256), 256, 0, st>>>(Sc.p, cs.perm.p, cs.seg.p,
+                                                                                          cs.nnz, vc.p);
+    const unsigned bc = dpg::blocks_for(nc), bcw = dpg::blocks_for(nc * 64);
+    dpg::csr_diagonal<<<bc, 256, 0, st>>>(cs.row_ptr.p, cs.col.p, vc.p, nc, dc.p);
+    dpg::zero_diag_mask<<<bc, 256, 0, st>>>(dc.p, nc, cmask.p);
+    SK_CHECK(hipGetLastError());
+    // fine smoother: block Jacobi over the entities (or point Jacobi)
+    dpg::DevArray<double> r(n), z(n), p(n), q(n), d(n), t1(n), t2(n), part(dpg::kRedBlocks), one(1);
+    const unsigned bn = dpg::blocks_for(n), bw = dpg::blocks_for(n * 64);
+    dpg::csr_diagonal<<<bn, 256, 0, st>>>(sk->row_ptr.p, sk->col.p, values, n, d.p);
+    const bool blocks = sk->nblocks > 0;
+    dpg::DevArray<double> binv(blocks ? (size_t)sk->blk_total : 0);
+    if (blocks) {
+      dpg::DevArray<int32_t> bad(1);
+      dpg::launch_fill_i32(bad.p, 1, 0, st);
+      dpg::block_extract<<<sk->nblocks, 64, 0, st>>>(sk->row_ptr.p, sk->col.p, values, fixed, sk->blk_ptr.p,
+                                                     sk->blk_dofs.p, sk->blk_off2.p, binv.p);
+      dpg::block_invert<<<sk->nblocks, 64, 2 * sizeof(double) * sk->nbmax * sk->nbmax, st>>>(
+          sk->blk_ptr.p, sk->blk_off2.p, sk->nbmax, binv.p, bad.p);
+      SK_CHECK(hipGetLastError());
+      int32_t hb = 0;
+      SK_CHECK(hipMemcpyAsync(&hb, bad.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+      SK_CHECK(hipStreamSynchronize(st));
+      if (hb) throw std::runtime_error("a diagonal block of the free system is not positive definite");
+    }
+    auto smooth = [&](const double* rin, double* zout) {
+      if (blocks)
+        dpg::block_apply<<<bn, 256, 0, st>>>(sk->blk_ptr.p, sk->blk_dofs.p, sk->pos_block.p, sk->blk_off2.p, binv.p,
+                                             fixed, rin, n, zout);
+      else
+        dpg::cg_precondition<<<bn, 256, 0, st>>>(rin, d.p, fixed, n, zout);
+    };
+    // coarse solve: coarse_iters steps of Jacobi-preconditioned CG from 0, scalars on the device
+    dpg::DevArray<double> rc(nc), ec(nc), zc(nc), pc(nc), qc(nc), sc(4);
+    auto dot_dev = [&](const double* a, const double* b2, int64_t len, double* slot) {
+      dpg::reduce_partial<<<dpg::kRedBlocks, 256, 0, st>>>(a, b2, len, 0, part.p);
+      dpg::reduce_final<<<1, 256, 0, st>>>(part.p, dpg::kRedBlocks, 0, slot);
+    };
+    auto coarse_solve = [&]() {  // ec ~ A_c^-1 rc (rc is overwritten)
+      dpg::cg_precondition<<<bc, 256, 0, st>>>(rc.p, dc.p, cmask.p, nc, zc.p);
+      dpg::copy_f64<<<bc, 256, 0, st>>>(zc.p, nc, pc.p);
+      dot_dev(rc.p, zc.p, nc, sc.p + 0);
+      for (int k = 0; k < coarse_iters; ++k) {
+        dpg::spmv_free_rows<<<bcw, 256, 0, st>>>(cs.row_ptr.p, cs.col.p, vc.p, pc.p, cmask.p, nc, qc.p);
+        dot_dev(pc.p, qc.p, nc, sc.p + 1);
+        dpg::dcg_step<<<bc, 256, 0, st>>>(sc.p, pc.p, qc.p, nc, ec.p, rc.p);
+        dpg::cg_precondition<<<bc, 256, 0, st>>>(rc.p, dc.p, cmask.p, nc, zc.p);
+        dot_dev(rc.p, zc.p, nc, sc.p + 2);
+        dpg::dcg_direction<<<bc, 256, 0, st>>>(sc.p, zc.p, nc, pc.p);
+        dpg::dcg_shift<<<1, 1, 0, st>>>(sc.p);
+      }
+    };
+    // two-level preconditioner, symmetric: smooth, coarse correction, smooth
+    auto precondition = [&](const double* rin, double* zout) {
+      smooth(rin, zout);
+      dpg::spmv_free_rows<<<bw, 256, 0, st>>>(sk->row_ptr.p, sk->col.p, values, zout, fixed, n, t1.p);
+      dpg::sub_free<<<bn, 256, 0, st>>>(rin, t1.p, fixed, n, t2.p);
+      dpg::csr_apply<<<bc, 256, 0, st>>>(co->pt_rp.p, co->pt_col.p, co->pt_val.p, t2.p, cmask.p, nc, 0, rc.p);
+      dpg::fill_f64<<<bc, 256, 0, st>>>(ec.p, nc, 0.0);
+      coarse_solve();
+      dpg::csr_apply<<<bn, 256, 0, st>>>(co->p_rp.p, co->p_col.p, co->p_val.p, ec.p, fixed, n, 1, zout);
+      dpg::spmv_free_rows<<<bw, 256, 0, st>>>(sk->row_ptr.p, sk->col.p, values, zout, fixed, n, t1.p);
+      dpg::sub_free<<<bn, 256, 0, st>>>(rin, t1.p, fixed, n, t2.p);
+      smooth(t2.p, t1.p);
+      dpg::add_f64<<<bn, 256, 0, st>>>(t1.p, n, zout);
+      SK_CHECK(hipGetLastError());
+    };
+    // flexible conjugate gradients (the coarse solve is inexact)
+    dpg::spmv_free_rows<<<bw, 256, 0, st>>>(sk->row_ptr.p, sk->col.p, values, x, fixed, n, q.p);
+    dpg::sub_free<<<bn, 256, 0, st>>>(rhs, q.p, fixed, n, r.p);
+    const double rr0 = dpg::reduce(r.p, r.p, n, 0, part, one, st);
+    double rr = rr0;
+    int it = 0;
+    if (rr0 > 0.0) {
+      precondition(r.p, z.p);
+      dpg::copy_f64<<<bn, 256, 0, st>>>(z.p, n, p.p);
+      while (it < maxit && rr > tol * tol * rr0) {
+        dpg::spmv_free_rows<<<bw, 256, 0, st>>>(sk->row_ptr.p, sk->col.p, values, p.p, fixed, n, q.p);
+        const double pq = dpg::reduce(p.p, q.p, n, 0, part, one, st);
+        if (!(pq > 0.0)) throw std::runtime_error("the free block of the system is not positive definite");
+        const double pr = dpg::reduce(p.p, r.p, n, 0, part, one, st);
+        dpg::cg_step<<<bn, 256, 0, st>>>(pr / pq, p.p, q.p, n, x, r.p);
+        rr = dpg::reduce(r.p, r.p, n, 0, part, one, st);
+        ++it;
+        if (rr <= tol * tol * rr0) break;
+        precondition(r.p, z.p);
+        const double zq = dpg::reduce(z.p, q.p, n, 0, part, one, st);
+        dpg::cg_direction<<<bn, 256, 0, st>>>(-zq / pq, z.p, n, p.p);
+        SK_CHECK(hipGetLastError());
+      }
+    }
+    if (iters) *iters = it;
+    if (relres) *relres = rr0 > 0.0 ? std::sqrt(rr / rr0) : 0.0;
+  });
 }
 
 }  // extern "C"

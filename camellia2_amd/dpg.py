@@ -66,6 +66,30 @@ def mesh_hex(nx, ny, nz, jitter, seed):
     return nodes, par
 
 
+def mesh_quad_device(nx, ny, jitter, seed, stream=None):
+    """mesh_quad generated on the device (dpg_mesh_quad_device): the same bits, as device tensors."""
+    import torch
+
+    C = nx * ny
+    nodes = torch.empty((C, 4, 2), dtype=torch.float64, device="cuda:0")
+    par = torch.empty((C, 4), dtype=torch.int8, device="cuda:0")
+    s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    check(lib().dpg_mesh_quad_device(nx, ny, jitter, seed, nodes.data_ptr(), par.data_ptr(), ct.c_void_p(s)))
+    return nodes, par
+
+
+def mesh_hex_device(nx, ny, nz, jitter, seed, stream=None):
+    """mesh_hex generated on the device (dpg_mesh_hex_device): the same bits, as device tensors."""
+    import torch
+
+    C = nx * ny * nz
+    nodes = torch.empty((C, 8, 3), dtype=torch.float64, device="cuda:0")
+    par = torch.empty((C, 6), dtype=torch.int8, device="cuda:0")
+    s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    check(lib().dpg_mesh_hex_device(nx, ny, nz, jitter, seed, nodes.data_ptr(), par.data_ptr(), ct.c_void_p(s)))
+    return nodes, par
+
+
 def mesh_quad_graded(n0, levels, jitter, seed):
     """hp mesh graded toward x = 1: (nodes (C,4,2), parities (C,4), refinement level (C,))."""
     L = lib()
@@ -173,7 +197,7 @@ class LocalAssembler:
         check(lib().dpg_plan_autotune(self._plan, bucket, ct.byref(b), sample, ct.c_void_p(s)))
         return self.tiles(bucket)
 
-    def _outputs(self, bucket, C, dev, out, want_G, want_schur, want_info):
+    def _outputs(self, bucket, C, dev, out, want_G, want_schur, want_info, want_factor=False):
         t = self._torch
         info = self.info[bucket]
         o = out if out is not None else {}
@@ -189,8 +213,12 @@ class LocalAssembler:
             o["schur_rhs"] = t.empty((C, info.T), dtype=t.float64, device=dev)
         if want_info and "info" not in o:
             o["info"] = t.zeros((C,), dtype=t.int32, device=dev)
+        if want_factor and "factor" not in o:
+            o["factor"] = t.empty((C, info.N + info.M + 1, info.N), dtype=t.float64, device=dev)
+        if want_factor and "factor_scale" not in o:
+            o["factor_scale"] = t.empty((C, info.N), dtype=t.float64, device=dev)
         outs = capi.Outputs()
-        for key in ("K", "rhs", "G", "schur", "schur_rhs", "info"):
+        for key in ("K", "rhs", "G", "schur", "schur_rhs", "info", "factor", "factor_scale"):
             setattr(outs, key, o[key].data_ptr() if key in o else None)
         return o, outs
 
@@ -210,12 +238,15 @@ class LocalAssembler:
         return b
 
     def local_stiffness_and_rhs(self, bucket, nodes, parities, coefs=None, load=None, want_G=False,
-                                want_schur=False, want_info=True, out=None, stream=None):
-        """K (C,M,M), rhs (C,M) [, G (C,N,N), schur (C,T,T), schur_rhs (C,T)] on the device.
+                                want_schur=False, want_info=True, out=None, stream=None, want_factor=False):
+        """K (C,M,M), rhs (C,M) [, G (C,N,N), schur (C,T,T), schur_rhs (C,T),
+        factor (C,N+M+1,N) and factor_scale (C,N) kept for energy_error_factored
+        and adjoint] on the device.
 
         `nodes`, `parities`, `coefs` are device tensors. Returns (outputs dict, #non-SPD cells)."""
         t = self._torch
-        o, outs = self._outputs(bucket, nodes.shape[0], nodes.device, out, want_G, want_schur, want_info)
+        o, outs = self._outputs(bucket, nodes.shape[0], nodes.device, out, want_G, want_schur, want_info,
+                                want_factor)
         b = self._batch(nodes, parities, coefs, load)
         s = stream.cuda_stream if stream is not None else t.cuda.current_stream().cuda_stream
         nbad = check(lib().dpg_assemble(self._plan, bucket, ct.byref(b), ct.byref(outs), ct.c_void_p(s)))
@@ -258,18 +289,44 @@ class LocalAssembler:
                                              ct.c_void_p(e.data_ptr()), ct.c_void_p(s)))
         return e
 
-    def energy_error_and_gradient(self, bucket, nodes, parities, u, coefs, load=None, stream=None):
-        """(eta2 (C,), d eta2 / d coefs (C, n_coefs)) from the adjoint kernels (dpg_adjoint)."""
+    def energy_error_factored(self, bucket, factor, u, stream=None):
+        """eta2 (C,) from the factor a local_stiffness_and_rhs(..., want_factor=True)
+        kept (dpg_energy_error_factored): no re-assembly, no refactorisation."""
+        t = self._torch
+        C = u.shape[0]
+        for a in (factor, u):
+            assert a.is_cuda and a.dtype == t.float64 and a.is_contiguous() and a.shape[0] == C
+        eta2 = t.empty((C,), dtype=t.float64, device=u.device)
+        s = stream.cuda_stream if stream is not None else t.cuda.current_stream().cuda_stream
+        check(lib().dpg_energy_error_factored(self._plan, bucket, C, ct.c_void_p(factor.data_ptr()),
+                                              ct.c_void_p(u.data_ptr()), ct.c_void_p(eta2.data_ptr()),
+                                              ct.c_void_p(s)))
+        return eta2
+
+    def adjoint(self, bucket, nodes, parities, coefs, load=None, u=None, Kbar=None, rhsbar=None, factor=None,
+                factor_scale=None, want_eta2=False, stream=None):
+        """grad (C, n_coefs) of <Kbar, K> + <rhsbar, rhs> + eta2(u) per cell with
+        respect to the cell's coefficients (dpg_adjoint; each part only when its
+        input is given). With factor/factor_scale from local_stiffness_and_rhs(...,
+        want_factor=True) nothing is re-assembled. Returns grad, or (eta2, grad)
+        with want_eta2 (needs u)."""
         t = self._torch
         C = nodes.shape[0]
-        eta2 = t.empty((C,), dtype=t.float64, device=nodes.device)
         grad = t.empty((C, self.form.n_coefs), dtype=t.float64, device=nodes.device)
-        assert u.is_cuda and u.dtype == t.float64 and u.is_contiguous()
+        eta2 = t.empty((C,), dtype=t.float64, device=nodes.device) if want_eta2 else None
+        for a in (u, Kbar, rhsbar, factor, factor_scale):
+            if a is not None:
+                assert a.is_cuda and a.dtype == t.float64 and a.is_contiguous() and a.shape[0] == C
         b = self._batch(nodes, parities, coefs, load)
         s = stream.cuda_stream if stream is not None else t.cuda.current_stream().cuda_stream
-        check(lib().dpg_adjoint(self._plan, bucket, ct.byref(b), ct.c_void_p(u.data_ptr()),
-                                ct.c_void_p(eta2.data_ptr()), ct.c_void_p(grad.data_ptr()), ct.c_void_p(s)))
-        return eta2, grad
+        vp = lambda a: ct.c_void_p(a.data_ptr()) if a is not None else None  # noqa: E731
+        check(lib().dpg_adjoint(self._plan, bucket, ct.byref(b), vp(u), vp(Kbar), vp(rhsbar), vp(factor),
+                                vp(factor_scale), vp(eta2), vp(grad), ct.c_void_p(s)))
+        return (eta2, grad) if want_eta2 else grad
+
+    def energy_error_and_gradient(self, bucket, nodes, parities, u, coefs, load=None, stream=None):
+        """(eta2 (C,), d eta2 / d coefs (C, n_coefs)) from the adjoint kernels (dpg_adjoint)."""
+        return self.adjoint(bucket, nodes, parities, coefs, load, u=u, want_eta2=True, stream=stream)
 
 
 class AssemblyGraph:
@@ -344,6 +401,38 @@ class EnergyErrorFunction:
             def backward(ctx, g):
                 (grad,) = ctx.saved_tensors
                 return g[:, None] * grad
+
+        return _F.apply(coefs)
+
+
+class LocalAssemblyFunction:
+    """torch.autograd.Function over the local assembly: (K, rhs) of every cell
+    as functions of the per-cell coefficients (C, n_coefs). The forward keeps
+    the factorisation, the backward is dpg_adjoint with Kbar, rhsbar = the
+    incoming gradients (no re-assembly). Used to check the adjoint against
+    finite differences; forward and backward both run in the HIP library."""
+
+    @staticmethod
+    def apply(assembler, bucket, nodes, parities, coefs, load=None):
+        import torch
+
+        class _F(torch.autograd.Function):
+            @staticmethod
+            def forward(ctx, c):
+                cd = c.detach().contiguous()
+                out, nbad = assembler.local_stiffness_and_rhs(bucket, nodes, parities, cd, load, want_factor=True)
+                if nbad:
+                    raise RuntimeError(f"{nbad} cells with a non-SPD Gram matrix")
+                ctx.save_for_backward(cd, out["factor"], out["factor_scale"])
+                return out["K"], out["rhs"]
+
+            @staticmethod
+            def backward(ctx, gK, grhs):
+                cd, fac, fs = ctx.saved_tensors
+                kb = gK.contiguous() if gK is not None else None
+                rb = grhs.contiguous() if grhs is not None else None
+                return assembler.adjoint(bucket, nodes, parities, cd, load, Kbar=kb, rhsbar=rb, factor=fac,
+                                         factor_scale=fs)
 
         return _F.apply(coefs)
 
@@ -501,6 +590,26 @@ class CondensedSystem:
                                        int(maxit), ct.byref(it), ct.byref(rel), ct.c_void_p(s)))
         return it.value, rel.value
 
+    def solve_twolevel(self, coarse, schur, values, rhs, fixed, x, tol=1e-12, maxit=10000, coarse_iters=20,
+                       stream=None):
+        """solve() preconditioned by two levels (dpg_skeleton_solve_twolevel): the
+        entity block-Jacobi (or point Jacobi) smoother and a correction on the
+        lowest-order level `coarse` (a CoarseLevel of the same mesh), whose
+        matrix comes from the cells' Schur blocks `schur` (C, T, T); flexible CG.
+        Returns (iterations, final relative residual)."""
+        t = self._torch
+        for a, dt in ((schur, t.float64), (values, t.float64), (rhs, t.float64), (fixed, t.int32),
+                      (x, t.float64)):
+            assert a.is_cuda and a.dtype == dt and a.is_contiguous()
+        it, rel = ct.c_int(), ct.c_double()
+        s = stream.cuda_stream if stream is not None else t.cuda.current_stream().cuda_stream
+        check(lib().dpg_skeleton_solve_twolevel(self._h, coarse._h, ct.c_void_p(schur.data_ptr()),
+                                                ct.c_void_p(values.data_ptr()), ct.c_void_p(rhs.data_ptr()),
+                                                ct.c_void_p(fixed.data_ptr()), ct.c_void_p(x.data_ptr()), float(tol),
+                                                int(maxit), int(coarse_iters), ct.byref(it), ct.byref(rel),
+                                                ct.c_void_p(s)))
+        return it.value, rel.value
+
     def close(self):
         if self._h:
             lib().dpg_skeleton_destroy(self._h)
@@ -575,3 +684,33 @@ def gather_cells(perm, nodes, parities, coefs=None, stream=None):
                                  ct.c_void_p(coefs.data_ptr() if coefs is not None else 0),
                                  ct.c_void_p(out_c.data_ptr() if out_c is not None else 0), ct.c_void_p(s)))
     return out_n, out_p, out_c
+
+
+class CoarseLevel:
+    """The lowest-order trace and flux level of a conforming quad or hex mesh
+    for two-level preconditioning (dpg_coarse_create): coarse numbering,
+    prolongation and the coarse matrix pattern."""
+
+    def __init__(self, f, bucket, cell_vertices, stream=None):
+        import torch
+
+        cv = np.ascontiguousarray(cell_vertices, dtype=np.int32)
+        s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        h = ct.c_void_p()
+        check(lib().dpg_coarse_create(ct.byref(f), ct.byref(bucket), cv.shape[0], _p32(cv), ct.c_void_p(s),
+                                      ct.byref(h)))
+        self._h = h
+        n, nnz, tc = ct.c_int64(), ct.c_int64(), ct.c_int()
+        check(lib().dpg_coarse_info(h, ct.byref(n), ct.byref(nnz), ct.byref(tc)))
+        self.nglobal, self.nnz, self.T = n.value, nnz.value, tc.value
+
+    def close(self):
+        if self._h:
+            lib().dpg_coarse_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

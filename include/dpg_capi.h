@@ -177,7 +177,8 @@ enum {
   DPG_STAGE_PANEL = 5,   /* diagonal-block Cholesky + panel solve     */
   DPG_STAGE_UPDATE = 6,  /* trailing updates (MFMA)                   */
   DPG_STAGE_EXTRACT = 7, /* K, rhs, G, Schur copies to the outputs    */
-  DPG_NSTAGES = 8
+  DPG_STAGE_ADJOINT = 8, /* adjoint: back solves and point contractions */
+  DPG_NSTAGES = 9
 };
 
 /* accumulated per-stage kernel times (HIP events on the call's stream) with
@@ -209,6 +210,14 @@ typedef struct {
   double* schur;     /* (C, T, T) */
   double* schur_rhs; /* (C, T)    */
   int32_t* info;     /* (C)       */
+  /* The factorisation kept for dpg_energy_error_factored and dpg_adjoint
+   * (both or neither): with S = diag(factor_scale) the DPOEQU factors and
+   * S G S = L L^T the Cholesky factor of the equilibrated Gram matrix,
+   *   factor (C, N + M + 1, N) row-major: rows 0..N-1 L (lower; 0 above the
+   *          diagonal), rows N..N+M-1 (L^-1 S B)^T, row N+M (L^-1 S l)^T
+   *   factor_scale (C, N): the diagonal of S. */
+  double* factor;
+  double* factor_scale;
 } dpg_outputs;
 
 typedef struct dpg_plan dpg_plan;
@@ -253,6 +262,14 @@ int dpg_mesh_hex(int nx, int ny, int nz, double jitter, uint64_t seed, double* n
  * only the count. nodes (C,4,2) CCW, parities (C,4), level (C) or NULL. */
 int dpg_mesh_quad_graded(int n0, int levels, double jitter, uint64_t seed, double* nodes, int8_t* parities,
                          int32_t* level);
+/* The same two meshes generated on the device (device arrays nodes, parities,
+ * stream-ordered; synchronises the stream): bit for bit the host generators'
+ * output. Each vertex reaches its jitter draws in the generator's stream by a
+ * jump ahead of the linear xoshiro256** state. */
+int dpg_mesh_quad_device(int nx, int ny, double jitter, uint64_t seed, double* nodes, int8_t* parities,
+                         void* stream);
+int dpg_mesh_hex_device(int nx, int ny, int nz, double jitter, uint64_t seed, double* nodes, int8_t* parities,
+                        void* stream);
 /* sine-mode load: nmodes modes per component, amp ~ N(0,1), k ~ U{1..kmax} */
 int dpg_load_sine_modes(int spatial_dim, int ncomp, int nmodes, int kmax, uint64_t seed, dpg_load_desc* out);
 /* xoshiro256** seeded by splitmix64: n uniforms on [lo,hi) / n normals (Box-Muller) */
@@ -298,13 +315,30 @@ int dpg_assemble(dpg_plan* plan, int bucket, const dpg_batch* batch, const dpg_o
 int dpg_energy_error(dpg_plan* plan, int bucket, const dpg_batch* batch, const double* u,
                      double* eta2, void* stream);
 
-/* Adjoint of the energy-error indicator with respect to the per-cell PDE
- * coefficients (no counterpart in the reference, which has no derivatives):
- * eta2[c] = r^T G^-1 r as dpg_energy_error, and grad[c][k] = d eta2[c] / d coefs[c][k]
- *   = -2 e^T (dB/dk) u - e^T (dG/dk) e,  e = G^-1 r,
- * with e from the factor of the same pass (no refactorisation) and the basis
- * values recomputed at the quadrature points. grad is (C, n_coefs) or NULL. */
-int dpg_adjoint(dpg_plan* plan, int bucket, const dpg_batch* batch, const double* u, double* eta2, double* grad,
+/* The energy error from a stored factorisation (outputs->factor of a
+ * dpg_assemble of the same cells): eta2[c] = |y_l - Y u|^2 with Y = L^-1 S B
+ * and y_l = L^-1 S l the factor's rows, i.e. r^T G^-1 r with r = l - B u as in
+ * TSolution::computeErrorRepresentation (Solution.cpp:2688-2722) -- without
+ * re-assembling G and B or refactoring. u (C, M), eta2 (C). */
+int dpg_energy_error_factored(dpg_plan* plan, int bucket, int ncells, const double* factor, const double* u,
+                              double* eta2, void* stream);
+
+/* Adjoint of the local assembly with respect to the per-cell PDE coefficients
+ * (no counterpart in the reference, which has no derivatives). For every cell
+ *   L_c = <Kbar_c, K_c> + <rhsbar_c, rhs_c> + eta2_c(u_c),
+ * each part present only when its input (Kbar (C, M, M), rhsbar (C, M), u
+ * (C, M)) is not NULL, and grad[c][k] = d L_c / d coefs[c][k] (C, n_coefs):
+ *   with T = G^-1 B, y = G^-1 l, e = G^-1 (l - B u), Ks = sym(Kbar),
+ *   Bbar = 2 T Ks + y rhsbar^T - 2 e u^T,  Gbar = -T Ks T^T - sym(T rhsbar y^T) - e e^T,
+ *   grad = <Bbar, dB/dcoef> + <Gbar, dG/dcoef>,
+ * the basis values recomputed at the quadrature points. eta2 (C, needs u) is
+ * r^T G^-1 r as dpg_energy_error. factor / factor_scale: the factorisation a
+ * dpg_assemble of the same cells kept (outputs->factor): then nothing is
+ * re-assembled or refactored; NULL: the call assembles and factors G, B, l
+ * itself. grad, eta2 may be NULL. Returns the count of non-SPD cells (0 with a
+ * stored factor). */
+int dpg_adjoint(dpg_plan* plan, int bucket, const dpg_batch* batch, const double* u, const double* Kbar,
+                const double* rhsbar, const double* factor, const double* factor_scale, double* eta2, double* grad,
                 void* stream);
 
 /* ---- hp meshes: tile selection and the captured all-bucket launch --------
@@ -321,13 +355,16 @@ int dpg_adjoint(dpg_plan* plan, int bucket, const dpg_batch* batch, const double
 typedef struct {
   int32_t gram;    /* G = A^T A (dense path)                        */
   int32_t bmat;    /* B blocks                                      */
-  int32_t update;  /* Cholesky panel/trailing updates               */
+  int32_t update;  /* Cholesky panel/trailing updates; also 128: a
+                      128 x 64 tile for the panel updates (the
+                      trailing update then takes 64)                */
   int32_t sumfact; /* 1: sum-factorised G (opt-in, see above)       */
 } dpg_tiles;
 int dpg_plan_get_tiles(const dpg_plan* plan, int bucket, dpg_tiles* out);
 int dpg_plan_set_tiles(dpg_plan* plan, int bucket, const dpg_tiles* tiles);
-/* Autotuner: times both tile edges for every role on the first `sample` cells
- * of `batch` (HIP events, one warm-up pass each) and keeps the faster per role.
+/* Autotuner: times both tile edges for every role (and the 128 x 64 tile for
+ * the updates) on the first `sample` cells of `batch` (HIP events, one warm-up
+ * pass each) and keeps the fastest per role.
  * Tile edges never change results (every output accumulates in the same k
  * order). The G path is left as set. Blocks until done; `stream` runs the timings. */
 int dpg_plan_autotune(dpg_plan* plan, int bucket, const dpg_batch* batch, int sample, void* stream);
@@ -474,6 +511,33 @@ int dpg_skeleton_solve(const dpg_skeleton* sk, const double* values, const doubl
  * err = sqrt(eta2) from dpg_energy_error. Returns the number of marked cells. */
 int dpg_mark_cells(const double* err, int ncells, double rel_threshold, int32_t* flags, void* stream);
 int dpg_skeleton_destroy(dpg_skeleton* sk);
+
+/* Two-level (p-multigrid) preconditioning of the condensed system, the
+ * multilevel solve the reference offers through GMGSolver (GMGSolver.h:23,
+ * GMGOperator: a coarse space of lower polynomial order, block-Jacobi
+ * smoothing, GMGOperator.cpp:2241-2262). The coarse level is the lowest-order
+ * trace and flux space of the same mesh (H1 traces linear on every side: one
+ * dof per vertex; L2 fluxes constant per side), numbered as
+ * dpg_skeleton_numbering numbers H1 order 1; the prolongation P interpolates
+ * the coarse functions at the fine nodes. Quads and hexes. Host and device
+ * set-up, once per mesh; cell_vertices (C, V) on the host as for
+ * dpg_skeleton_numbering. */
+typedef struct dpg_coarse dpg_coarse;
+int dpg_coarse_create(const dpg_form_desc* form, const dpg_bucket_desc* bucket, int ncells,
+                      const int32_t* cell_vertices, void* stream, dpg_coarse** out);
+/* coarse global dofs, nonzeros of the coarse matrix, coarse dofs per cell (any may be NULL) */
+int dpg_coarse_info(const dpg_coarse* co, int64_t* nglobal_coarse, int64_t* nnz_coarse, int* T_coarse);
+int dpg_coarse_destroy(dpg_coarse* co);
+/* dpg_skeleton_solve with the two-level preconditioner: the coarse matrix
+ * P^T A P of the free dofs is assembled from the cells' Schur blocks schur
+ * (C, T, T), the matrix of values; flexible conjugate gradients, each
+ * preconditioner application = block-Jacobi smoothing (the entity blocks of
+ * dpg_skeleton_set_blocks, else point Jacobi), a coarse correction by
+ * coarse_iters Jacobi-CG steps on the coarse matrix, and a second smoothing
+ * step. Same arguments and stopping rule as dpg_skeleton_solve. */
+int dpg_skeleton_solve_twolevel(const dpg_skeleton* sk, const dpg_coarse* co, const double* schur,
+                                const double* values, const double* rhs, const int32_t* fixed, double* x, double tol,
+                                int maxit, int coarse_iters, int* iters, double* relres, void* stream);
 
 #ifdef __cplusplus
 }

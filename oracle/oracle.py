@@ -208,6 +208,25 @@ def extended_schur(K, rhs, F):
     return S, g
 
 
+def eta2_extended(form, bucket, nodes, parities, u, coefs=None, load=None):
+    """eta2 = r^T G^-1 r per cell with G, B, l summed in 80-bit from the same
+    FP64 point values, r = l - B u formed in 80-bit (rounded once to FP64) and
+    the solve in 80-bit: the reference's energy error evaluated more accurately
+    than FP64 reaches, for measuring the FP64 outputs against."""
+    lib().oracle_set_extended_accumulation(1)
+    try:
+        ox = assemble(form, bucket, nodes, parities, coefs, load, want=("B", "G", "l"))
+    finally:
+        lib().oracle_set_extended_accumulation(0)
+    out = np.zeros(nodes.shape[0])
+    for c in range(nodes.shape[0]):
+        r = (ox["l"][c].astype(np.longdouble) -
+             ox["B"][c].T.astype(np.longdouble) @ np.asarray(u[c]).astype(np.longdouble)).astype(np.float64)
+        Kr, _ = extended_K(ox["G"][c], r[None, :], ox["l"][c])
+        out[c] = Kr[0, 0]
+    return out
+
+
 def with_extended(form, bucket, nodes, parities, coefs=None, load=None, schur=False):
     """Oracle outputs plus extended-precision K, rhs (and S, g): G, B, l summed
     in 80-bit from the same FP64 point values (basis values, weights, Jacobians)

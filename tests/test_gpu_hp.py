@@ -138,17 +138,23 @@ def test_tile_edges_give_identical_results_and_autotune_picks_one():
         A.set_tiles(j, 32, 32, 32, 0)
         b, _ = A.local_stiffness_and_rhs(j, *args, want_schur=True, want_G=True)
         b = {key: v.cpu().numpy() for key, v in b.items()}
+        A.set_tiles(j, 64, 64, 128, 0)  # 128 x 64 panel-update tile
+        c, _ = A.local_stiffness_and_rhs(j, *args, want_schur=True, want_G=True)
+        c = {key: v.cpu().numpy() for key, v in c.items()}
         for key in ("K", "rhs", "schur", "schur_rhs", "G"):
             assert np.array_equal(a[key], b[key]), key
+            assert np.array_equal(a[key], c[key]), key
         eg = rel_fro(sfr["G"], a["G"])
         ek = rel_fro(sfr["K"], a["K"])
         print(f"type {g['topology'], g['k'], g['kt']}: sum-factorised vs dense G {eg:.2e}, K {ek:.2e}")
         assert eg < 1e-14
         t = A.autotune(j, *args, sample=g["ncells"])
         print(f"  autotuned {t}")
-        assert all(v in (32, 64) for v in t[:3]) and t[3] == 0
+        assert t[0] in (32, 64) and t[1] in (32, 64) and t[2] in (32, 64, 128) and t[3] == 0
     with pytest.raises(capi.DPGError):
         A.set_tiles(0, 48, 64, 64)
+    with pytest.raises(capi.DPGError):
+        A.set_tiles(0, 128, 64, 64)  # the 128 tile is for the updates only
     A.close()
     At = dpg.LocalAssembler(f, [dpg.bucket(capi.TRI, 3, 4)])
     assert At.tiles(0) == (64, 64, 64, 0)

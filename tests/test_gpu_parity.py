@@ -110,6 +110,24 @@ def test_poisson_2d_other_orders(k, kt):
     check_against_oracle(poisson2d(3, 3, k=k, kt=kt), want_G=True, want_schur=True)
 
 
+@pytest.mark.parametrize("case", ["c4", "c2"])
+def test_update_tiles_give_identical_results(case):
+    """The 128 x 64 panel-update tile, the 64 x 64 and the 32 x 32 tiles
+    accumulate every output in the same k order: the same bits."""
+    f, b, nodes, par, coefs, load = {"c4": lambda: poisson3d(2, 2, 1), "c2": lambda: stokes2d(3, 3)}[case]()
+    A = dpg.LocalAssembler(f, [b])
+    res = []
+    for t in (64, 32, 128):
+        A.set_tiles(0, 64, 64, t, 0)
+        o, nbad = A.local_stiffness_and_rhs(0, dev(nodes), dev(par), dev(coefs), load, want_schur=True)
+        assert nbad == 0
+        res.append({k: v.clone() for k, v in o.items()})
+    for r in res[1:]:
+        for k in ("K", "rhs", "schur", "schur_rhs"):
+            assert torch.equal(r[k], res[0][k]), k
+    A.close()
+
+
 def test_batches_give_identical_results():
     case = poisson2d(6, 5)
     a, _ = gpu_assemble(*case, want_schur=True)

@@ -1,6 +1,7 @@
-"""Input preparation on the device: side parities of conforming meshes from
-their vertex ids (the generators' parities, the reference's rule), and
-element-type bucketing with the gather of the cells' inputs."""
+"""Input preparation on the device: the seeded jittered meshes generated on
+the device (bit for bit the host generators), side parities of conforming
+meshes from their vertex ids (the generators' parities, the reference's rule),
+and element-type bucketing with the gather of the cells' inputs."""
 import numpy as np
 import pytest
 
@@ -81,3 +82,30 @@ def test_bucketing_the_hp_mesh_by_element_type():
         sel = np.flatnonzero(key == k)
         assert np.array_equal(n2[o[b]:o[b + 1]].cpu().numpy(), q["nodes"][sel])
         assert np.array_equal(c2[o[b]:o[b + 1]].cpu().numpy(), q["coefs"][sel])
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (2, 1), (7, 5), (16, 16), (64, 64), (128, 128), (301, 17)])
+def test_device_quad_mesh_equals_the_host_generator(shape):
+    nx, ny = shape
+    for seed, jitter in ((0xC0FFEE + 1, 0.10), (12345, 0.49), (7, 0.0)):
+        hn, hp = dpg.mesh_quad(nx, ny, jitter, seed)
+        dn, dp = dpg.mesh_quad_device(nx, ny, jitter, seed)
+        torch.cuda.synchronize()
+        assert np.array_equal(dn.cpu().numpy(), hn), (shape, seed)
+        assert np.array_equal(dp.cpu().numpy(), hp), (shape, seed)
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (2, 3, 4), (5, 4, 3), (32, 32, 32), (40, 7, 9)])
+def test_device_hex_mesh_equals_the_host_generator(shape):
+    hn, hp = dpg.mesh_hex(*shape, 0.05, 0xC0FFEE + 4)
+    dn, dp = dpg.mesh_hex_device(*shape, 0.05, 0xC0FFEE + 4)
+    torch.cuda.synchronize()
+    assert np.array_equal(dn.cpu().numpy(), hn), shape
+    assert np.array_equal(dp.cpu().numpy(), hp), shape
+
+
+def test_device_mesh_rejects_bad_arguments():
+    with pytest.raises(capi.DPGError):
+        dpg.mesh_quad_device(0, 4, 0.1, 1)
+    with pytest.raises(capi.DPGError):
+        dpg.mesh_hex_device(2, 2, 2, 0.7, 1)

@@ -184,3 +184,76 @@ def test_block_jacobi_solve_equals_direct_and_needs_fewer_iterations(dim):
     assert e1 < 1e-9 and e2 < 1e-9
     assert it2 < it1 and it3 == it1
     assert torch.equal(x3, x1)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_two_level_solve_equals_direct_and_needs_fewer_iterations(dim):
+    """Two levels (entity block-Jacobi smoothing + a correction on the
+    lowest-order trace/flux space of the same mesh): the flexible-CG solution
+    equals a direct solve, and it takes fewer iterations than block-Jacobi CG;
+    a linear exact solution prescribed on the boundary comes back exactly."""
+    if dim == 2:
+        f, b = dpg.form("poisson", 2), dpg.bucket(capi.QUAD, 4, 5)
+        nodes, par = dpg.mesh_quad(16, 14, 0.15, 161)
+    else:
+        f, b = dpg.form("poisson", 3), dpg.bucket(capi.HEX, 3, 4)
+        nodes, par = dpg.mesh_hex(5, 4, 4, 0.1, 171)
+    load = dpg.load_sine_modes(dim, 1, 4, 3, 0xC0FFEE + 181)
+    A = dpg.LocalAssembler(f, [b])
+    out, nbad = A.local_stiffness_and_rhs(0, dev(nodes), dev(par), None, load, want_schur=True)
+    A.close()
+    assert nbad == 0
+    cv = dpg.vertex_ids(nodes)
+    gdof, n = dpg.skeleton_numbering(f, b, cv)
+    sysm = dpg.CondensedSystem(gdof, n)
+    vals, rhs = sysm.assemble(out["schur"], out["schur_rhs"])
+    fixed = dpg.boundary_trace_mask(f, b, gdof, cv, n)
+    dfixed = dev(fixed)
+    ent, ne = dpg.skeleton_entities(f, b, cv, n)
+    sysm.set_blocks(ent, ne)
+    co = dpg.CoarseLevel(f, b, cv)
+    x1 = torch.zeros(n, dtype=torch.float64, device="cuda:0")
+    it1, rel1 = sysm.solve(vals, rhs, dfixed, x1, tol=1e-12, maxit=20000)
+    x2 = torch.zeros(n, dtype=torch.float64, device="cuda:0")
+    it2, rel2 = sysm.solve_twolevel(co, out["schur"], vals, rhs, dfixed, x2, tol=1e-12, maxit=20000)
+    rp, col = sysm.pattern()
+    Ah = sp.csr_matrix((vals.cpu().numpy(), col.cpu().numpy(), rp.cpu().numpy()), shape=(n, n))
+    free = np.flatnonzero(fixed == 0)
+    xd = np.zeros(n)
+    xd[free] = spla.spsolve(Ah[free][:, free].tocsc(), rhs.cpu().numpy()[free])
+    e1 = np.abs(x1.cpu().numpy() - xd).max() / np.abs(xd).max()
+    e2 = np.abs(x2.cpu().numpy() - xd).max() / np.abs(xd).max()
+    print(f"dim {dim}: {n} dofs, coarse {co.nglobal} dofs / {co.nnz} nonzeros; block-Jacobi CG {it1} iterations "
+          f"({e1:.1e}), two-level {it2} ({e2:.1e})")
+    assert rel1 <= 1e-12 and rel2 <= 1e-12
+    assert e1 < 1e-9 and e2 < 1e-9
+    assert it2 < it1
+    co.close()
+    sysm.close()
+
+
+def test_two_level_solve_returns_a_linear_exact_solution():
+    f = dpg.form("poisson", 2)
+    b = dpg.bucket(capi.QUAD, 3, 5)
+    nodes, par = dpg.mesh_quad(9, 7, 0.15, 191)
+    exact = linear_trial_coefficients(nodes, par, 3, np.array([0.3, -1.2, 0.7]))
+    s = dpg.sizes(f, b)
+    A = dpg.LocalAssembler(f, [b])
+    out, nbad = A.local_stiffness_and_rhs(0, dev(nodes), dev(par), None, None, want_schur=True)
+    A.close()
+    assert nbad == 0
+    cv = dpg.vertex_ids(nodes)
+    gdof, n = dpg.skeleton_numbering(f, b, cv)
+    xg = np.zeros(n)
+    xg[gdof.ravel()] = exact[:, s.F:].ravel()
+    sysm = dpg.CondensedSystem(gdof, n)
+    vals, rhs = sysm.assemble(out["schur"], out["schur_rhs"])
+    fixed = dpg.boundary_trace_mask(f, b, gdof, cv, n)
+    x = dev(np.where(fixed == 1, xg, 0.0))
+    co = dpg.CoarseLevel(f, b, cv)
+    it, rel = sysm.solve_twolevel(co, out["schur"], vals, rhs, dev(fixed), x, tol=1e-13, maxit=5000)
+    print(f"two-level: {it} iterations, relres {rel:.1e}")
+    assert rel <= 1e-13
+    assert np.abs(x.cpu().numpy() - xg).max() <= 1e-9 * np.abs(xg).max()
+    co.close()
+    sysm.close()
