@@ -306,6 +306,10 @@ def bench_hp(args, cfg):
     gws = G.workspace_bytes()
     G.close()
     A.close()
+    hp_system = None
+    if not args.no_assembly:
+        hp_system = bench_hp_system(args, f, groups, buckets, out_e, load)
+        progress("hp condensed system done")
 
     # per-stage kernel times of the tuned per-bucket launches (HIP events)
     Ap = dpg.LocalAssembler(f, buckets, workspace_bytes=ws, profile=True)
@@ -317,11 +321,11 @@ def bench_hp(args, cfg):
     Ap.close()
     stages = {k: {"ms": round(v["ms"], 3), "launches": v["launches"], "gflop": round(v["flops"] / 1e9, 3),
                   "gbyte": round(v["bytes"] / 1e9, 3)} for k, v in tm.items()}
-    dom = max(["gram", "update", "panel"], key=lambda k: tm.get(k, {"ms": 0.0})["ms"])
+    dom = max(["gram", "update", "panel", "solve"], key=lambda k: tm.get(k, {"ms": 0.0})["ms"])
     d = tm[dom]
     achieved = d["flops"] / (d["ms"] * 1e-3)
     roofline = {"bound": "mfma", "kernel": {"gram": "gemm_gram_f64", "update": "gemm_update_f64",
-                                            "panel": "potrf_wave+panel_trsm"}[dom],
+                                            "panel": "potrf_wave+panel_trsm", "solve": "update_solve_f64"}[dom],
                 "achieved": round(achieved / 1e12, 3), "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK["fp64"], 4), "traffic": None,
                 "avg_launch_ms": round(d["ms"] / d["launches"], 4), "flop_per_launch": d["flops"] / d["launches"],
@@ -358,6 +362,7 @@ def bench_hp(args, cfg):
                      "tiles": {("tri" if g["topology"] == capi.TRI else "quad") + f"_k{g['k']}_kt{g['kt']}": list(t)
                                for g, t in zip(groups, tiles)}},
         "graph_workspace_gib": round(gws / 2 ** 30, 2),
+        "global_assembly": hp_system,
         "roofline": roofline,
         "stages": stages,
     }
@@ -471,13 +476,14 @@ def bench_c5(args, cfg):
                   "gbyte": round(v["bytes"] / 1e9, 4)} for k, v in tm.items()}
     dom = max(tm, key=lambda k: tm[k]["ms"])
     d = tm[dom]
-    mfma = dom in ("gram", "update", "bmat")
+    mfma = dom in ("gram", "update", "bmat", "solve")
     if mfma or d["flops"] / max(d["bytes"], 1.0) > PEAK["fp64"] / PEAK["hbm"]:
         achieved, peak, unit, bound = d["flops"] / (d["ms"] * 1e-3), PEAK["fp64"], "TFLOP/s", "mfma"
     else:
         achieved, peak, unit, bound = d["bytes"] / (d["ms"] * 1e-3), PEAK["hbm"], "GB/s", "hbm"
     scale = 1e12 if unit == "TFLOP/s" else 1e9
     kernels = {"gram": "gemm_gram_f64", "update": "gemm_update_f64", "bmat": "gemm_bmat_f64",
+               "solve": "update_solve_f64",
                "panel": "potrf_wave+panel_trsm", "adjoint": "factor_backsolve+adjoint_kernel",
                "operand": "ip_operand_kernel+operand_kernel", "point": "point_kernel",
                "extract": "extract_sym+extract_factor", "scale": "apply_scaling"}
@@ -531,6 +537,62 @@ def bench_c5(args, cfg):
         progress("accuracy against the oracle and the 80-bit evaluation done")
     attach_peaks(result, args.peaks)
     print(json.dumps(result))
+
+
+def bench_hp_system(args, f, groups, buckets, out, load, maxit=8000, tol=1e-10):
+    """The condensed system of the hp mesh (hanging sides, order jumps,
+    triangles): constrained numbering (host), assembly of every element type's
+    Schur blocks, block-Jacobi CG with the trace fixed to 0 on the boundary,
+    the local trace/flux coefficients and field recovery, each timed."""
+    import torch
+
+    from camellia2_amd import dpg
+
+    pts = np.concatenate([g["nodes"].reshape(-1, 2) for g in groups])
+    X, inv = np.unique(pts, axis=0, return_inverse=True)
+    C = sum(g["ncells"] for g in groups)
+    cv = np.full((C, 4), -1, dtype=np.int32)
+    cb = np.zeros(C, dtype=np.int32)
+    pos = row = 0
+    for b, g in enumerate(groups):
+        n, V = g["nodes"].shape[0], g["nodes"].shape[1]
+        cv[row:row + n, :V] = inv[pos:pos + n * V].reshape(n, V)
+        cb[row:row + n] = b
+        pos += n * V
+        row += n
+    t0 = time.perf_counter()
+    hs = dpg.HpSystem(f, buckets, cb, cv, X)
+    t_setup = time.perf_counter() - t0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    vals, rhs = hs.assemble([o["schur"] for o in out], [o["schur_rhs"] for o in out])
+    torch.cuda.synchronize()
+    t_asm = time.perf_counter() - t0
+    fixed, _, _, _ = hs.dofs()
+    x = torch.zeros(hs.nglobal, dtype=torch.float64, device="cuda:0")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    iters, rel = hs.solve(vals, rhs, torch.from_numpy(fixed).to("cuda:0"), x, tol=tol, maxit=maxit)
+    torch.cuda.synchronize()
+    t_cg = time.perf_counter() - t0
+    u = [torch.zeros((g["ncells"], dpg.sizes(f, bk).M), dtype=torch.float64, device="cuda:0")
+         for g, bk in zip(groups, buckets)]
+    A = dpg.LocalAssembler(f, buckets, workspace_bytes=args.workspace_gib * 2 ** 30)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hs.local(x, u)
+    nbad = sum(A.recover_fields(b, out[b]["K"], out[b]["rhs"], u[b]) for b in range(len(buckets)))
+    torch.cuda.synchronize()
+    t_rec = time.perf_counter() - t0
+    A.close()
+    res = {"cells": C, "global_dofs": hs.nglobal, "nnz": hs.nnz, "entities": hs.nentities,
+           "setup_host_s": round(t_setup, 3), "assembly_ms": round(t_asm * 1e3, 3), "cg_iterations": iters,
+           "cg_relres": rel, "cg_s": round(t_cg, 3), "local_and_recovery_ms": round(t_rec * 1e3, 3),
+           "recover_non_spd": nbad,
+           "note": "minimum rule over hanging sides and order jumps (dpg_hp_system); CG with block Jacobi over "
+                   f"vertices and sides to relres {tol:g} (at most {maxit} iterations), trace 0 on the boundary"}
+    hs.close()
+    return res
 
 
 def bench_solve_loop(args, f, b, nodes, cv, gdof, n, sysm, vals, rhs, solve, schur, maxit=8000, tol=1e-10):
@@ -848,10 +910,11 @@ def main():
     progress("profiled step done")
     stages = {k: {"ms": round(v["ms"], 3), "launches": v["launches"], "gflop": round(v["flops"] / 1e9, 3),
                   "gbyte": round(v["bytes"] / 1e9, 3)} for k, v in tm.items()}
-    dom = max(["gram", "update"], key=lambda k: tm.get(k, {"ms": 0.0})["ms"])
+    dom = max(["gram", "update", "solve"], key=lambda k: tm.get(k, {"ms": 0.0})["ms"])
     d = tm[dom]
     achieved = d["flops"] / (d["ms"] * 1e-3)
-    roofline = {"bound": "mfma", "kernel": {"gram": "gemm_gram_f64", "update": "gemm_update_f64"}[dom],
+    roofline = {"bound": "mfma",
+                "kernel": {"gram": "gemm_gram_f64", "update": "gemm_update_f64", "solve": "update_solve_f64"}[dom],
                 "achieved": round(achieved / 1e12, 3), "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK["fp64"], 4), "traffic": None,
                 "avg_launch_ms": round(d["ms"] / d["launches"], 4),

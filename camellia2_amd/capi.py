@@ -78,7 +78,7 @@ class Options(ct.Structure):
                 ("profile", ct.c_int32), ("workspace_bytes", ct.c_double)]
 
 
-STAGES = ["point", "operand", "gram", "bmat", "scale", "panel", "update", "extract", "adjoint"]
+STAGES = ["point", "operand", "gram", "bmat", "scale", "panel", "update", "extract", "adjoint", "solve"]
 NSTAGES = len(STAGES)
 
 
@@ -119,10 +119,13 @@ EXPORTS = [
     "dpg_assemble",
     "dpg_energy_error", "dpg_energy_error_factored", "dpg_adjoint", "dpg_mesh_quad_graded", "dpg_plan_get_tiles", "dpg_plan_set_tiles",
     "dpg_plan_autotune", "dpg_graph_create", "dpg_graph_launch", "dpg_graph_workspace", "dpg_graph_destroy",
-    "dpg_skeleton_numbering", "dpg_skeleton_nodes", "dpg_skeleton_create", "dpg_skeleton_info",
+    "dpg_skeleton_numbering", "dpg_skeleton_nodes", "dpg_skeleton_create", "dpg_skeleton_create_ragged",
+    "dpg_skeleton_info",
     "dpg_skeleton_pattern", "dpg_skeleton_assemble", "dpg_skeleton_solve", "dpg_skeleton_entities",
     "dpg_skeleton_set_blocks", "dpg_mark_cells", "dpg_skeleton_destroy",
     "dpg_coarse_create", "dpg_coarse_info", "dpg_coarse_destroy", "dpg_skeleton_solve_twolevel",
+    "dpg_hp_numbering", "dpg_hp_system_create", "dpg_hp_system_info", "dpg_hp_system_dofs", "dpg_hp_system_assemble",
+    "dpg_hp_system_local", "dpg_hp_system_destroy",
     "dpg_recover_fields", "dpg_riesz_representation", "dpg_side_parities", "dpg_sort_cells", "dpg_gather_cells",
 ]
 
@@ -182,6 +185,8 @@ def lib():
         L.dpg_skeleton_nodes.argtypes = [P(FormDesc), P(BucketDesc), P(ct.c_double)]
         L.dpg_skeleton_create.argtypes = [ct.c_int, ct.c_int, P(ct.c_int32), ct.c_int, ct.c_void_p,
                                           P(ct.c_void_p)]
+        L.dpg_skeleton_create_ragged.argtypes = [ct.c_int, P(ct.c_int32), P(ct.c_int32), ct.c_int, ct.c_void_p,
+                                                 P(ct.c_void_p)]
         L.dpg_skeleton_info.argtypes = [ct.c_void_p, P(ct.c_int64), P(ct.c_int64), P(ct.c_void_p),
                                         P(ct.c_void_p)]
         L.dpg_skeleton_pattern.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]
@@ -201,6 +206,17 @@ def lib():
         L.dpg_skeleton_solve_twolevel.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p,
                                                   ct.c_void_p, ct.c_void_p, ct.c_double, ct.c_int, ct.c_int,
                                                   P(ct.c_int), P(ct.c_double), ct.c_void_p]
+        L.dpg_hp_numbering.argtypes = [P(FormDesc), P(BucketDesc), ct.c_int, ct.c_int, P(ct.c_int32), P(ct.c_int32),
+                                       ct.c_int, P(ct.c_double), P(ct.c_int64), P(ct.c_int32), P(ct.c_int32),
+                                       P(ct.c_int32), P(ct.c_int32), P(ct.c_double)]
+        L.dpg_hp_system_create.argtypes = [P(FormDesc), P(BucketDesc), ct.c_int, ct.c_int, P(ct.c_int32),
+                                           P(ct.c_int32), ct.c_int, P(ct.c_double), ct.c_void_p, P(ct.c_void_p)]
+        L.dpg_hp_system_info.argtypes = [ct.c_void_p, P(ct.c_int64), P(ct.c_int64), P(ct.c_void_p)]
+        L.dpg_hp_system_dofs.argtypes = [ct.c_void_p, P(ct.c_int32), P(ct.c_int32), P(ct.c_int32), P(ct.c_int32)]
+        L.dpg_hp_system_assemble.argtypes = [ct.c_void_p, P(ct.c_void_p), P(ct.c_void_p), ct.c_void_p, ct.c_void_p,
+                                             ct.c_void_p]
+        L.dpg_hp_system_local.argtypes = [ct.c_void_p, ct.c_void_p, P(ct.c_void_p), ct.c_void_p]
+        L.dpg_hp_system_destroy.argtypes = [ct.c_void_p]
         L.dpg_mark_cells.argtypes = [ct.c_void_p, ct.c_int, ct.c_double, ct.c_void_p, ct.c_void_p]
         L.dpg_riesz_representation.argtypes = [ct.c_void_p, ct.c_int, P(Batch), ct.c_void_p, ct.c_void_p,
                                                ct.c_void_p]

@@ -380,7 +380,7 @@ RefCell make_ref_cell(int topo) {
   return c;
 }
 
-ElementLayout make_layout(const dpg_form_desc& f, const dpg_bucket_desc& b) {
+ElementLayout make_layout(const dpg_form_desc& f, const dpg_bucket_desc& b, bool skeleton_only) {
   ElementLayout L;
   L.topo = b.topology;
   L.rc = make_ref_cell(b.topology);
@@ -405,7 +405,8 @@ ElementLayout make_layout(const dpg_form_desc& f, const dpg_bucket_desc& b) {
     int kind = f.trial_kind[v];
     if (kind == DPG_FIELD || kind == DPG_FIELD_VECTOR) {
       if (sawSide) throw std::invalid_argument("field variables must precede traces and fluxes");
-      if (L.field_deg < 1 && L.dim > 0 && L.k < 2) throw std::invalid_argument("field order must be >= 1");
+      if (!skeleton_only && L.field_deg < 1 && L.dim > 0 && L.k < 2)
+        throw std::invalid_argument("field order must be >= 1");
       int nc = kind == DPG_FIELD_VECTOR ? L.dim : 1;
       for (int c = 0; c < nc; ++c) {
         L.blocks.push_back({v, c, -1, kind, off, L.nf});
@@ -415,7 +416,7 @@ ElementLayout make_layout(const dpg_form_desc& f, const dpg_bucket_desc& b) {
     } else if (kind == DPG_TRACE || kind == DPG_FLUX) {
       sawSide = true;
       int card = kind == DPG_TRACE ? L.ntr : L.nfl;
-      if (kind == DPG_FLUX && L.flux_deg < 1) throw std::invalid_argument("flux order must be >= 1");
+      if (!skeleton_only && kind == DPG_FLUX && L.flux_deg < 1) throw std::invalid_argument("flux order must be >= 1");
       for (int s = 0; s < L.rc.nsides; ++s) {
         L.blocks.push_back({v, 0, s, kind, off, card});
         off += card;

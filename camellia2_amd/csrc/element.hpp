@@ -105,7 +105,10 @@ struct ElementLayout {
 };
 
 // Throws std::invalid_argument on an unsupported form/bucket.
-ElementLayout make_layout(const dpg_form_desc& form, const dpg_bucket_desc& bucket);
+// skeleton_only: the layout's block structure for numbering the trace and flux
+// dofs only, which also admits H1 order 1 (constant fields and fluxes; the
+// assembly itself needs order >= 2)
+ElementLayout make_layout(const dpg_form_desc& form, const dpg_bucket_desc& bucket, bool skeleton_only = false);
 
 // Algorithmic flop count per cell (the roofline numerator):
 //   F_G  = sum_lt 2 QG (sum of the cardinalities of the test vars of lt)^2

@@ -233,7 +233,6 @@ __global__ __launch_bounds__(128) void point_kernel(int kind, const DevProgram* 
 // out[c*sc + b*sb + p*sp + i] = sum_r W[c][p][b][var(i)][r] * R[p][r][i]
 // One workgroup per (256 dofs, kOperandPoints points, cell): the weights of its
 // points are staged once, then every thread writes its dof's values point by point.
-constexpr int kOperandPoints = 8;
 template <typename T, int NPR>
 __global__ __launch_bounds__(256) void operand_kernel(const double* __restrict__ W, int ng, int nvar,
                                                       const double* __restrict__ R, const int32_t* __restrict__ vod,
@@ -324,6 +323,15 @@ __global__ __launch_bounds__(256) void ip_operand_kernel(const double* __restric
   for (int lt = 0; lt < n_lt; ++lt)
     for (int q = QG; q < QGp; ++q) oc[((size_t)lt * QGp + q) * ld] = 0.0;
   s[(size_t)c * ss + j] = 1.0 / sqrt(diag);
+}
+
+// lane `k`'s value of a double (k per lane, or wave-uniform), through the
+// LDS crossbar (ds_bpermute: no LDS allocation, result in vector registers)
+__device__ inline double wave_bcast(double v, int k) {
+  long long b = __builtin_bit_cast(long long, v);
+  int lo = __builtin_amdgcn_ds_bpermute(k << 2, (int)(b & 0xffffffffLL));
+  int hi = __builtin_amdgcn_ds_bpermute(k << 2, (int)(b >> 32));
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
 }
 
 // ------------------------------------------------------------------- GEMM
@@ -545,18 +553,33 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
 // addresses, zeroing at LDS write, two chunks in flight) and the same k order
 // per output element, so the results are those of the square tiles bit for
 // bit. No k-segment masks, no lower-triangle tile walk (dense m x n).
+//
+// SOLVE (rows below a panel's diagonal block): the updated tile is then solved
+// in registers against the diagonal factor, X = C L^-T, before it is stored,
+// so the panel is written once instead of written, read and written again.
+// The solve is blocked by the four 16-column MFMA blocks: inside a block the
+// substitution runs column by column as the reference DTRSM does (x_s = c_s /
+// L_ss, then c_m -= L_ms x_s for the later columns m of the block), with x_s
+// moved between the four lane groups that hold a row's columns through the
+// LDS crossbar; the later blocks then take X_block L[later, block]^T as MFMAs
+// (a rank-4 update per k-slice, the B operand being the solved accumulator
+// element itself). L stays in LDS (the staging buffers, reused).
+template <bool SOLVE>
 __device__ inline void gemm_tn_f64_wide_body(const GemmArgs& g) {
   constexpr int TM = 128, TN = 64, NT = 256;
   constexpr int PX = GKC * TM / NT, RX = NT / TM;  // X: 8 elements per thread, k-rows t/128 + 2r
   constexpr int PY = GKC * TN / NT, RY = NT / TN;  // Y: 4 elements per thread, k-rows t/64 + 4r
+  constexpr int LXS = TM + 4, LYS = TN + 4, LP = 68;  // LDS row pitches (L: one column of the factor per row)
+  constexpr int NSTAGE = GKC * (LXS + LYS), NL = SOLVE ? 64 * LP : 0;
   int b = xcd_remap(blockIdx.x, gridDim.x);
   int cell = b / g.ntiles, tile = b % g.ntiles;
   if (cell >= g.ncells) return;
   const int ti = tile % g.tiles_m, tj = tile / g.tiles_m;
   const int i0 = ti * TM, j0 = tj * TN;
   double* __restrict__ C = g.C + (size_t)cell * g.sc;
-  __shared__ double Xs[GKC][TM + 4];
-  __shared__ double Ys[GKC][TN + 4];
+  __shared__ double smem[NSTAGE > NL ? NSTAGE : NL];
+  double (*Xs)[LXS] = reinterpret_cast<double (*)[LXS]>(smem);
+  double (*Ys)[LYS] = reinterpret_cast<double (*)[LYS]>(smem + GKC * LXS);
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   d4 acc[4][2];
 #pragma unroll
@@ -635,7 +658,82 @@ __device__ inline void gemm_tn_f64_wide_body(const GemmArgs& g) {
   for (int bi = 0; bi < 2; ++bi) {
     ir[bi] = si0 + bi * 16 + (lane & 15);
     iv[bi] = ir[bi] < g.m;
-    if (g.srow) sr[bi] = g.srow[(size_t)cell * g.ss + (iv[bi] ? ir[bi] : 0)];
+    if (!SOLVE && g.srow) sr[bi] = g.srow[(size_t)cell * g.ss + (iv[bi] ? ir[bi] : 0)];
+  }
+  if constexpr (SOLVE) {
+    // the updated tile C = beta C + alpha acc (0 outside the matrix), in place
+    // of the accumulators
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj) {
+      int jc[4];
+      bool jv[4];
+      double cold[2][4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        jc[v] = j0 + bj * 16 + (lane >> 4) + 4 * v;
+        jv[v] = jc[v] < g.n;
+      }
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          cold[bi][v] = C[(size_t)(iv[bi] ? ir[bi] : 0) + (size_t)(jv[v] ? jc[v] : 0) * g.ldc];
+      __builtin_amdgcn_s_waitcnt(0x0F70);
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          acc[bj][bi][v] = (iv[bi] && jv[v]) ? fma(g.beta, cold[bi][v], g.alpha * acc[bj][bi][v]) : 0.0;
+    }
+    // the diagonal factor into LDS (the staging buffers are free once every
+    // wave has left the k loop): row j of Ls = column j of L
+    __syncthreads();
+    double* Ls = smem;
+    const double* __restrict__ Lc = g.Lt + (size_t)cell * 64 * 64;
+    for (int e = t; e < 64 * 64; e += NT) Ls[(e >> 6) * LP + (e & 63)] = Lc[e];
+    __syncthreads();
+    const int q = lane >> 4, rowl = lane & 15;
+#pragma unroll
+    for (int bj0 = 0; bj0 < 4; ++bj0) {
+      // substitution over the 16 columns of block bj0; column s of the block is
+      // element s/4 of lane group s%4
+#pragma unroll
+      for (int s = 0; s < 16; ++s) {
+        const int vs = s >> 2, qs = s & 3, js = bj0 * 16 + s;
+        const double dinv = Ls[js * LP + js];
+        double lm[4];
+#pragma unroll
+        for (int v = vs; v < 4; ++v) lm[v] = Ls[js * LP + bj0 * 16 + q + 4 * v];
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi) {
+          const double xs = wave_bcast(acc[bj0][bi][vs], rowl + 16 * qs) * dinv;
+          acc[bj0][bi][vs] = q == qs ? xs : (q > qs ? fma(-lm[vs], xs, acc[bj0][bi][vs]) : acc[bj0][bi][vs]);
+#pragma unroll
+          for (int v = vs + 1; v < 4; ++v) acc[bj0][bi][v] = fma(-lm[v], xs, acc[bj0][bi][v]);
+        }
+      }
+      // the later blocks: C[:, bj1] -= X[:, bj0] L[bj1, bj0]^T, a rank-4 MFMA
+      // per k-slice v0 (columns bj0*16 + 4 v0 + k, k = lane group)
+#pragma unroll
+      for (int bj1 = bj0 + 1; bj1 < 4; ++bj1)
+#pragma unroll
+        for (int v0 = 0; v0 < 4; ++v0) {
+          const double la = -Ls[(bj0 * 16 + 4 * v0 + q) * LP + bj1 * 16 + rowl];
+#pragma unroll
+          for (int bi = 0; bi < 2; ++bi)
+            acc[bj1][bi] = __builtin_amdgcn_mfma_f64_16x16x4f64(la, acc[bj0][bi][v0], acc[bj1][bi], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = ir[bi], j = j0 + bj * 16 + (lane >> 4) + 4 * v;
+          if (iv[bi] && j < g.n) C[(size_t)i + (size_t)j * g.ldc] = acc[bj][bi][v];
+        }
+    return;
   }
 #pragma unroll
   for (int bj = 0; bj < 4; ++bj) {
@@ -680,7 +778,10 @@ __global__ __launch_bounds__(64) void gemm_gram_f64_t32(GemmArgs g) { gemm_tn_f6
 __global__ __launch_bounds__(64) void gemm_bmat_f64_t32(GemmArgs g) { gemm_tn_f64_body<32>(g); }
 __global__ __launch_bounds__(64) void gemm_update_f64_t32(GemmArgs g) { gemm_tn_f64_body<32>(g); }
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void gemm_update_f64_w128(GemmArgs g) {
-  gemm_tn_f64_wide_body(g);
+  gemm_tn_f64_wide_body<false>(g);
+}
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void update_solve_f64(GemmArgs g) {
+  gemm_tn_f64_wide_body<true>(g);
 }
 
 // ------------------------------------------------- sum-factorised Gram
@@ -1189,14 +1290,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void g
 // makes K up to 20x less accurate than the substitution does.
 constexpr int PNB = 64;
 
-// lane `k`'s value of a double, in every lane (k wave-uniform), through the
-// LDS crossbar (ds_bpermute: no LDS allocation, result in vector registers)
-__device__ inline double wave_bcast(double v, int k) {
-  long long b = __builtin_bit_cast(long long, v);
-  int lo = __builtin_amdgcn_ds_bpermute(k << 2, (int)(b & 0xffffffffLL));
-  int hi = __builtin_amdgcn_ds_bpermute(k << 2, (int)(b >> 32));
-  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
-}
 
 // Diagonal block (kb <= NB <= 64 columns of a 64-column panel) of one cell per
 // wavefront (4 cells per workgroup), in registers: lane i holds row i of the
@@ -1803,7 +1896,8 @@ void launch_operand(int dim, const double* W, int ng, int nvar, const double* R,
                     int ncells, double* out, int64_t sc, int64_t sb, int64_t sp, hipStream_t st) {
   if (ng == 0 || npts == 0) return;
   dim3 grid((N + 255) / 256, (npts + kOperandPoints - 1) / kOperandPoints, ncells);
-  size_t shm = sizeof(double) * kOperandPoints * ng * nvar * (dim + 1);
+  size_t shm = operand_lds_bytes(ng, nvar, dim);
+  if (shm > kMaxLds) throw DeviceError("operand_kernel: LDS request above 64 KiB");
   if (dim == 2)
     operand_kernel<double, 3><<<grid, 256, shm, st>>>(W, ng, nvar, R, vod, N, npts, out, sc, sb, sp);
   else
@@ -1824,7 +1918,8 @@ void launch_rhs(int dim, const double* Wrhs, int nvar, const double* R, const in
 void launch_ip_operand(int dim, const double* Wip, int n_lt, int nvar, const double* R, const int32_t* vod, int N,
                        int QG, int QGp, int ncells, double* out, int64_t sc, int64_t ld, double* s, int64_t ss,
                        hipStream_t st) {
-  size_t shm = sizeof(double) * (size_t)n_lt * nvar * (dim + 1);
+  size_t shm = ip_operand_lds_bytes(n_lt, nvar, dim);
+  if (shm > kMaxLds) throw DeviceError("ip_operand_kernel: LDS request above 64 KiB");
   dim3 grid((N + 255) / 256, ncells);
   if (dim == 2)
     ip_operand_kernel<3><<<grid, 256, shm, st>>>(Wip, n_lt, nvar, R, vod, N, QG, QGp, out, sc, ld, s, ss);
@@ -1870,7 +1965,8 @@ void launch_ip_digits(int dim, const double* Wip, int n_lt, int nvar, const doub
                       int QG, int QGp, int NR, int ncells, int8_t* dig, int64_t sd, double* s, int64_t ss,
                       hipStream_t st) {
   if (QGp % 32 != 0 || NR % 64 != 0 || NR < N) throw DeviceError("ip_digits: QGp must be a multiple of 32, NR of 64");
-  size_t shm = sizeof(double) * 32 * (size_t)n_lt * nvar * (dim + 1);
+  size_t shm = ip_digits_lds_bytes(n_lt, nvar, dim);
+  if (shm > kMaxLds) throw DeviceError("ip_digits_kernel: LDS request above 64 KiB");
   dim3 grid((NR + 127) / 128, ncells);  // two lanes per row
   if (dim == 2)
     ip_digits_kernel<3><<<grid, 256, shm, st>>>(Wip, n_lt, nvar, R, vod, N, QG, QGp, NR, dig, sd, s, ss);
@@ -1917,14 +2013,32 @@ void launch_gemm_mixed(GemmArgs g, int precision, int role, hipStream_t st) {
   check_launch("gemm_mixed");
 }
 
-void launch_panel(double* A, int64_t lda, int64_t sA, int D, int k0, int kb, double* Ld, int64_t sLd, double* Lt,
-                  int ncells, int32_t* info, int info_base, hipStream_t st) {
+void launch_potrf(double* A, int64_t lda, int64_t sA, int k0, int kb, double* Ld, int64_t sLd, double* Lt, int ncells,
+                  int32_t* info, int info_base, hipStream_t st) {
+  if (kb <= 0 || kb > PNB) throw DeviceError("potrf_wave: panel width must be 1..64");
   const dim3 grid((ncells + 3) / 4);
   if (kb <= 8) potrf_wave<8><<<grid, 256, 0, st>>>(A, lda, sA, k0, kb, Ld, sLd, Lt, info, info_base, ncells);
   else if (kb <= 16) potrf_wave<16><<<grid, 256, 0, st>>>(A, lda, sA, k0, kb, Ld, sLd, Lt, info, info_base, ncells);
   else if (kb <= 32) potrf_wave<32><<<grid, 256, 0, st>>>(A, lda, sA, k0, kb, Ld, sLd, Lt, info, info_base, ncells);
   else potrf_wave<64><<<grid, 256, 0, st>>>(A, lda, sA, k0, kb, Ld, sLd, Lt, info, info_base, ncells);
   check_launch("potrf_wave");
+}
+
+void launch_update_solve(GemmArgs g, hipStream_t st) {
+  if (g.m <= 0 || g.ncells <= 0) return;
+  if (g.n <= 0 || g.n > 64 || !g.Lt || g.lower || g.skip_upper || g.kmask || g.kw || g.srow || g.scol)
+    throw DeviceError("update_solve_f64: a dense panel of at most 64 columns with its diagonal factor");
+  g.tiles_m = (g.m + 127) / 128;
+  g.ntiles = g.tiles_m;
+  int64_t nb = (int64_t)g.ntiles * g.ncells;
+  if (nb > 0x7fffffff) throw DeviceError("gemm grid too large");
+  update_solve_f64<<<dim3((unsigned)nb), 256, 0, st>>>(g);
+  check_launch("update_solve_f64");
+}
+
+void launch_panel(double* A, int64_t lda, int64_t sA, int D, int k0, int kb, double* Ld, int64_t sLd, double* Lt,
+                  int ncells, int32_t* info, int info_base, hipStream_t st) {
+  launch_potrf(A, lda, sA, k0, kb, Ld, sLd, Lt, ncells, info, info_base, st);
   int rows = D - k0 - kb;
   if (rows <= 0) return;
   const dim3 tg((rows + 255) / 256, ncells);

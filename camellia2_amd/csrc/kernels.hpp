@@ -45,6 +45,9 @@ struct GemmArgs {
   // (a segment standing for several equal terms counts once per term)
   const double* kw;
   int32_t nseg;
+  // update_solve_f64 only: the panel's diagonal factor as potrf_wave leaves it
+  // (per cell 64 x 64, Lt[j*64 + m] = L[m][j] below the diagonal, 1 / L[j][j] on it)
+  const double* Lt;
 };
 
 void launch_point(int dim, int kind, const DevProgram* prog, const DevTables& tab, int npts, int Qs, int ncells,
@@ -60,6 +63,17 @@ void launch_ip_operand(int dim, const double* Wip, int n_lt, int nvar, const dou
                        int QG, int QGp, int ncells, double* out, int64_t sc, int64_t ld, double* s, int64_t ss,
                        hipStream_t st);
 enum { kGemmGram = 0, kGemmBmat = 1, kGemmUpdate = 2 };
+// points whose weights one operand workgroup stages in LDS, and the LDS bytes
+// of the weight-staging kernels (at most kMaxLds per workgroup)
+constexpr int kOperandPoints = 8;
+constexpr size_t kMaxLds = 64 * 1024;
+inline size_t operand_lds_bytes(int ng, int nvar, int dim) {
+  return sizeof(double) * kOperandPoints * (size_t)ng * nvar * (dim + 1);
+}
+inline size_t ip_operand_lds_bytes(int n_lt, int nvar, int dim) { return sizeof(double) * (size_t)n_lt * nvar * (dim + 1); }
+inline size_t ip_digits_lds_bytes(int n_lt, int nvar, int dim) {
+  return sizeof(double) * 32 * (size_t)n_lt * nvar * (dim + 1);
+}
 void launch_gemm(GemmArgs g, int role, hipStream_t st, int tile = 64);
 void launch_fill_i32(int32_t* p, int n, int32_t v, hipStream_t st);
 // sum-factorised Gram matrix, unscaled, lower triangle of A's first N rows/columns
@@ -102,6 +116,13 @@ void launch_gram_i8(I8GramArgs a, hipStream_t st);
 // the reciprocals of its diagonal
 void launch_panel(double* A, int64_t lda, int64_t sA, int D, int k0, int kb, double* Ld, int64_t sLd, double* Lt,
                   int ncells, int32_t* info, int info_base, hipStream_t st);
+// the diagonal block's factorisation alone (potrf_wave), for launch_update_solve
+void launch_potrf(double* A, int64_t lda, int64_t sA, int k0, int kb, double* Ld, int64_t sLd, double* Lt, int ncells,
+                  int32_t* info, int info_base, hipStream_t st);
+// rows below a panel's diagonal block: C = beta C + alpha X^T Y (the panel
+// update, g.k may be 0) and then C <- C L^-T with the diagonal factor g.Lt, in
+// one kernel (128 x 64 tiles, g.n <= 64)
+void launch_update_solve(GemmArgs g, hipStream_t st);
 void launch_zero_trailing(double* A, int64_t lda, int64_t sA, int r0, int D, int ncells, hipStream_t st);
 void launch_scaling(double* A, int64_t lda, int64_t sA, int r0, int D, int n, int ncol, double* s, int64_t ss,
                     int ncells, double sign, hipStream_t st);

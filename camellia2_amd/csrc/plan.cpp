@@ -117,6 +117,18 @@ void validate_form(const dpg_form_desc& f) {
     check_op(f.rhs[i].test_var, f.rhs[i].test_op, false);
     if (f.rhs[i].load_comp < 0 || f.rhs[i].load_comp > 2) throw std::invalid_argument("bad load component");
   }
+  // LDS of the kernels that stage per-point weights: volume groups (field
+  // components), side groups (trace and flux variables), IP terms
+  int n_vb = 0, n_sb = 0;
+  for (int v = 0; v < f.n_trial; ++v) {
+    const int kind = f.trial_kind[v];
+    if (kind == DPG_FIELD) n_vb += 1;
+    else if (kind == DPG_FIELD_VECTOR) n_vb += f.spatial_dim;
+    else n_sb += 1;
+  }
+  const int dim = f.spatial_dim, nvar = f.n_test;
+  if (operand_lds_bytes(std::max(n_vb, n_sb), nvar, dim) > kMaxLds || ip_operand_lds_bytes(f.n_ipterms, nvar, dim) > kMaxLds)
+    throw std::invalid_argument("the form has too many test variables x terms for the operand kernels' LDS (64 KiB)");
 }
 
 }  // namespace
@@ -542,6 +554,10 @@ void build_bucket(dpg_plan* plan, const dpg_bucket_desc& desc, BucketPlan& B) {
       return true;
     };
     const int nlt = B.prog.n_lt;
+    // the merged segments below are whole IP terms: one term per segment, every
+    // segment a whole number of the GEMM's 16-row k-chunks
+    if (nlt != f.n_ipterms || B.QGp % 16 != 0)
+      throw std::logic_error("Gram operand segments must be whole IP terms of whole k-chunks");
     for (int ts = 0; ts < 2; ++ts) {
       const int T = ts == 0 ? 64 : 32, nt = (L.N + T - 1) / T;
       std::vector<uint64_t> reach(nt, 0), vars(nt, 0), mask, mmask;  // terms reaching some dof of a T-block
@@ -647,10 +663,65 @@ void factor_columns(dpg_plan* plan, const BucketPlan& B, int nb, const Work& w, 
   Profiler& P = plan->prof;
   const int64_t sA = B.nA, lda = B.lda;
   int32_t* info = w.info;
+  // update tile 128: the rows below each diagonal block are updated and solved
+  // in one kernel (update_solve_f64) after the block's own update and POTRF
+  const bool fused = B.tile[2] == 128;
   for (int k0 = kbeg; k0 < kend; k0 += 64) {
     int kb = std::min(64, kend - k0);
     double rows = B.D - k0;
     hipEvent_t e;
+    if (fused) {
+      const double kk = k0 - kbeg, below = rows - kb;
+      if (k0 > kbeg) {  // the diagonal block's update (its lower part)
+        GemmArgs t{};
+        t.X = w.A + k0 + (size_t)kbeg * lda;
+        t.ldx = lda;
+        t.sx = sA;
+        t.Y = t.X;
+        t.ldy = lda;
+        t.sy = sA;
+        t.C = w.A + k0 + (size_t)k0 * lda;
+        t.ldc = lda;
+        t.sc = sA;
+        t.m = kb;
+        t.n = kb;
+        t.k = k0 - kbeg;
+        t.alpha = -1.0;
+        t.beta = 1.0;
+        t.ncells = nb;
+        t.skip_upper = 1;
+        e = P.start(st);
+        launch_gemm(t, kGemmUpdate, st, 64);
+        P.stop(e, st, DPG_STAGE_UPDATE, nb * 2.0 * (kb * (kb + 1) / 2.0) * kk, nb * (kb * kk + kb * kb * 2.0) * 8.0);
+      }
+      e = P.start(st);
+      launch_potrf(w.A, lda, sA, k0, kb, w.Ld, B.nLd, w.Li, nb, info, 0, st);
+      P.stop(e, st, DPG_STAGE_PANEL, nb * (double)kb * kb * kb / 3.0, nb * (double)kb * kb * 3.0 * 8.0);
+      if (below > 0) {
+        GemmArgs t{};
+        t.X = w.A + k0 + kb + (size_t)kbeg * lda;
+        t.ldx = lda;
+        t.sx = sA;
+        t.Y = w.A + k0 + (size_t)kbeg * lda;
+        t.ldy = lda;
+        t.sy = sA;
+        t.C = w.A + k0 + kb + (size_t)k0 * lda;
+        t.ldc = lda;
+        t.sc = sA;
+        t.m = (int)below;
+        t.n = kb;
+        t.k = k0 - kbeg;
+        t.alpha = -1.0;
+        t.beta = 1.0;
+        t.ncells = nb;
+        t.Lt = w.Li;
+        e = P.start(st);
+        launch_update_solve(t, st);
+        P.stop(e, st, DPG_STAGE_SOLVE, nb * (2.0 * below * kb * kk + below * (double)kb * kb),
+               nb * (below * kk + kb * kk + below * kb * 2.0 + kb * 64.0) * 8.0);
+      }
+      continue;
+    }
     if (k0 > kbeg) {
       GemmArgs t{};
       t.X = w.A + k0 + (size_t)kbeg * lda;
@@ -1017,6 +1088,9 @@ int dpg_plan_create(const dpg_form_desc* form, const dpg_bucket_desc* buckets, i
     if (options) plan->opt = *options;
     if (plan->opt.precision < DPG_PREC_FP64 || plan->opt.precision > DPG_PREC_INT8SPLIT)
       throw std::invalid_argument("unknown precision");
+    if (plan->opt.precision == DPG_PREC_INT8SPLIT &&
+        dpg::ip_digits_lds_bytes(form->n_ipterms, form->n_test, form->spatial_dim) > dpg::kMaxLds)
+      throw std::invalid_argument("the form has too many test variables x IP terms for the int8 digit kernel's LDS");
     if (plan->opt.refine != 0)
       throw std::invalid_argument("refine: a refinement of G^-1 B cannot correct an error that sits in G itself "
                                   "(the factorisation is FP64); not supported");
@@ -1347,6 +1421,8 @@ int dpg_plan_autotune(dpg_plan* plan, int bucket, const dpg_batch* batch, int sa
           plan->prof.collect(st, ns);
         }
         for (int r = 0; r < 3; ++r) ms[c][r] = plan->prof.acc.ms[stage_of[r]];
+        // the update tile also decides how the panel solve runs (fused or not)
+        ms[c][2] += plan->prof.acc.ms[DPG_STAGE_PANEL] + plan->prof.acc.ms[DPG_STAGE_SOLVE];
       }
     } catch (...) {
       plan->prof.on = on;

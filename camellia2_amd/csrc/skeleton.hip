@@ -226,6 +226,26 @@ __global__ void expand_keys(const int32_t* __restrict__ gdof, int T, int64_t ngl
   idx[p] = (uint32_t)p;
 }
 
+// the same for cells with different numbers of dofs: cell c holds dofs
+// [off[c], off[c+1]) and its contributions are its n_c x n_c block, row-major,
+// from position off2[c] (off2 = prefix sums of n_c^2)
+__global__ void expand_keys_ragged(const int32_t* __restrict__ off, const int32_t* __restrict__ dofs,
+                                   const uint64_t* __restrict__ off2, int ncells, int64_t nglobal, uint64_t E,
+                                   uint64_t* __restrict__ keys, uint32_t* __restrict__ idx) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= E) return;
+  int lo = 0, hi = ncells - 1;  // the cell with off2[c] <= p < off2[c+1]
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (off2[mid] <= p) lo = mid;
+    else hi = mid - 1;
+  }
+  const int c = lo, nc = off[c + 1] - off[c];
+  const uint64_t rem = p - off2[c], i = rem / nc, j = rem - i * nc;
+  keys[p] = (uint64_t)dofs[off[c] + i] * (uint64_t)nglobal + (uint64_t)dofs[off[c] + j];
+  idx[p] = (uint32_t)p;
+}
+
 __global__ void expand_rows(const int32_t* __restrict__ gdof, uint64_t E, uint64_t* __restrict__ keys,
                             uint32_t* __restrict__ idx) {
   const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -555,6 +575,21 @@ __global__ void add_f64(const double* __restrict__ b, int64_t n, double* __restr
   if (i < n) a[i] += b[i];
 }
 
+// a *= s
+__global__ void scale_f64(double* __restrict__ a, int64_t n, double s) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] *= s;
+}
+
+// a start vector for power iteration: a fixed pseudo-random value in [-1, 1)
+// on the free rows (Weyl sequence of the row index), 0 on the prescribed ones
+__global__ void seed_free(const int32_t* __restrict__ fixed, int64_t n, double* __restrict__ v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t h = (uint32_t)((uint64_t)(i + 1) * 2654435761ull);
+  v[i] = fixed[i] ? 0.0 : (double)h * (2.0 / 4294967296.0) - 1.0;
+}
+
 template <class T>
 struct DevArray {
   T* p = nullptr;
@@ -633,7 +668,7 @@ std::vector<double> side_node_coords(const ElementLayout& L) {
   std::vector<double> xi((size_t)L.T * dim);
   for (const TrialBlock& b : side_blocks(L)) {
     const int n = side_degree(L, b);
-    const std::vector<double> t = gll_points(n);
+    const std::vector<double> t = n > 0 ? gll_points(n) : std::vector<double>{0.0};  // degree 0: the side centre
     const double* p = &L.rc.side_param[(size_t)b.side * 3 * dim];
     for (int k = 0; k < b.card; ++k) {
       const int a = k % (n + 1), bb = k / (n + 1);
@@ -697,6 +732,68 @@ struct dpg_coarse {
   std::unique_ptr<dpg_skeleton> csk;  // coarse pattern and summation order
 };
 
+namespace dpg {
+namespace {
+
+// CSR pattern and summation orders of sum_c P_c^T S_c P_c for cells with the
+// global dofs dofs[off[c] .. off[c+1]) (host arrays); T > 0 when every cell has
+// T dofs (the contributions are then (c, i, j) in order), 0 for the ragged form.
+std::unique_ptr<dpg_skeleton> build_skeleton(int ncells, const int32_t* off, const int32_t* dofs, int nglobal, int T,
+                                             hipStream_t st) {
+  const int64_t ndof = off[ncells];
+  uint64_t E = 0;
+  std::vector<uint64_t> off2((size_t)ncells + 1, 0);
+  for (int c = 0; c < ncells; ++c) {
+    const uint64_t nc = (uint64_t)(off[c + 1] - off[c]);
+    off2[(size_t)c + 1] = off2[c] + nc * nc;
+  }
+  E = off2[ncells];
+  if (E > 0xFFFFFFFFull) throw std::invalid_argument("the number of contributions must be below 2^32");
+  for (int64_t q = 0; q < ndof; ++q)
+    if (dofs[q] < 0 || dofs[q] >= nglobal) throw std::invalid_argument("global dof index out of range");
+  std::unique_ptr<dpg_skeleton> sk(new dpg_skeleton());
+  sk->ncells = ncells;
+  sk->T = T;
+  sk->nglobal = nglobal;
+  DevArray<int32_t> dg((size_t)ndof), doff((size_t)ncells + 1);
+  DevArray<uint64_t> doff2((size_t)ncells + 1);
+  SK_CHECK(hipMemcpyAsync(dg.p, dofs, sizeof(int32_t) * dg.n, hipMemcpyHostToDevice, st));
+  SK_CHECK(hipMemcpyAsync(doff.p, off, sizeof(int32_t) * doff.n, hipMemcpyHostToDevice, st));
+  SK_CHECK(hipMemcpyAsync(doff2.p, off2.data(), sizeof(uint64_t) * doff2.n, hipMemcpyHostToDevice, st));
+  {  // matrix pattern and summation order
+    DevArray<uint64_t> keys(E), uk;
+    DevArray<uint32_t> idx(E);
+    if (T > 0)
+      expand_keys<<<(unsigned)((E + 255) / 256), 256, 0, st>>>(dg.p, T, nglobal, E, keys.p, idx.p);
+    else
+      expand_keys_ragged<<<(unsigned)((E + 255) / 256), 256, 0, st>>>(doff.p, dg.p, doff2.p, ncells, nglobal, E,
+                                                                       keys.p, idx.p);
+    SK_CHECK(hipGetLastError());
+    const uint64_t maxkey = (uint64_t)nglobal * (uint64_t)nglobal - 1;
+    sort_and_segment(keys, idx, E, maxkey, sk->perm, uk, sk->seg, sk->nnz, st);
+    sk->col.resize(sk->nnz);
+    sk->row_ptr.resize((size_t)nglobal + 1);
+    csr_from_keys<<<(unsigned)(((uint64_t)sk->nnz + 1 + 255) / 256), 256, 0, st>>>(uk.p, sk->nnz, nglobal, sk->col.p,
+                                                                                  sk->row_ptr.p);
+    SK_CHECK(hipGetLastError());
+  }
+  {  // load vector: every global dof has at least one contribution
+    const uint64_t E2 = (uint64_t)ndof;
+    DevArray<uint64_t> keys(E2), uk;
+    DevArray<uint32_t> idx(E2);
+    expand_rows<<<(unsigned)((E2 + 255) / 256), 256, 0, st>>>(dg.p, E2, keys.p, idx.p);
+    SK_CHECK(hipGetLastError());
+    uint32_t runs = 0;
+    sort_and_segment(keys, idx, E2, (uint64_t)nglobal - 1, sk->rperm, uk, sk->rseg, runs, st);
+    if ((int64_t)runs != nglobal) throw std::invalid_argument("some global dofs belong to no cell");
+  }
+  SK_CHECK(hipStreamSynchronize(st));
+  return sk;
+}
+
+}  // namespace
+}  // namespace dpg
+
 extern "C" {
 
 int dpg_skeleton_numbering(const dpg_form_desc* form, const dpg_bucket_desc* bucket, int ncells,
@@ -740,42 +837,22 @@ int dpg_skeleton_create(int ncells, int T, const int32_t* gdof, int nglobal, voi
   return dpg::guarded([&] {
     if (!out || ncells <= 0 || T <= 0 || nglobal <= 0 || !gdof) throw std::invalid_argument("bad arguments");
     *out = nullptr;
-    const uint64_t E = (uint64_t)ncells * T * T;
-    if (E > 0xFFFFFFFFull) throw std::invalid_argument("ncells * T * T must be below 2^32");
-    for (size_t q = 0; q < (size_t)ncells * T; ++q)
-      if (gdof[q] < 0 || gdof[q] >= nglobal) throw std::invalid_argument("gdof entry out of range");
-    hipStream_t st = (hipStream_t)stream;
-    std::unique_ptr<dpg_skeleton> sk(new dpg_skeleton());
-    sk->ncells = ncells;
-    sk->T = T;
-    sk->nglobal = nglobal;
-    dpg::DevArray<int32_t> dg((size_t)ncells * T);
-    SK_CHECK(hipMemcpyAsync(dg.p, gdof, sizeof(int32_t) * dg.n, hipMemcpyHostToDevice, st));
-    {  // matrix pattern and summation order
-      dpg::DevArray<uint64_t> keys(E), uk;
-      dpg::DevArray<uint32_t> idx(E);
-      dpg::expand_keys<<<(unsigned)((E + 255) / 256), 256, 0, st>>>(dg.p, T, nglobal, E, keys.p, idx.p);
-      SK_CHECK(hipGetLastError());
-      const uint64_t maxkey = (uint64_t)nglobal * (uint64_t)nglobal - 1;
-      dpg::sort_and_segment(keys, idx, E, maxkey, sk->perm, uk, sk->seg, sk->nnz, st);
-      sk->col.resize(sk->nnz);
-      sk->row_ptr.resize((size_t)nglobal + 1);
-      dpg::csr_from_keys<<<(unsigned)(((uint64_t)sk->nnz + 1 + 255) / 256), 256, 0, st>>>(uk.p, sk->nnz, nglobal,
-                                                                                         sk->col.p, sk->row_ptr.p);
-      SK_CHECK(hipGetLastError());
-    }
-    {  // load vector: every global dof has at least one contribution
-      const uint64_t E2 = (uint64_t)ncells * T;
-      dpg::DevArray<uint64_t> keys(E2), uk;
-      dpg::DevArray<uint32_t> idx(E2);
-      dpg::expand_rows<<<(unsigned)((E2 + 255) / 256), 256, 0, st>>>(dg.p, E2, keys.p, idx.p);
-      SK_CHECK(hipGetLastError());
-      uint32_t runs = 0;
-      dpg::sort_and_segment(keys, idx, E2, (uint64_t)nglobal - 1, sk->rperm, uk, sk->rseg, runs, st);
-      if ((int64_t)runs != nglobal) throw std::invalid_argument("some global dofs belong to no cell");
-    }
-    SK_CHECK(hipStreamSynchronize(st));
-    *out = sk.release();
+    std::vector<int32_t> off((size_t)ncells + 1);
+    for (int c = 0; c <= ncells; ++c) off[c] = c * T;
+    if ((int64_t)ncells * T > 0x7fffffffLL) throw std::invalid_argument("ncells * T must be below 2^31");
+    *out = dpg::build_skeleton(ncells, off.data(), gdof, nglobal, T, (hipStream_t)stream).release();
+  });
+}
+
+int dpg_skeleton_create_ragged(int ncells, const int32_t* offsets, const int32_t* dofs, int nglobal, void* stream,
+                               dpg_skeleton** out) {
+  return dpg::guarded([&] {
+    if (!out || ncells <= 0 || nglobal <= 0 || !offsets || !dofs) throw std::invalid_argument("bad arguments");
+    *out = nullptr;
+    if (offsets[0] != 0) throw std::invalid_argument("offsets[0] must be 0");
+    for (int c = 0; c < ncells; ++c)
+      if (offsets[c + 1] <= offsets[c]) throw std::invalid_argument("every cell needs at least one dof");
+    *out = dpg::build_skeleton(ncells, offsets, dofs, nglobal, 0, (hipStream_t)stream).release();
   });
 }
 
@@ -960,7 +1037,7 @@ int dpg_coarse_create(const dpg_form_desc* form, const dpg_bucket_desc* bucket, 
     const dpg::ElementLayout L = dpg::make_layout(*form, *bucket);
     dpg_bucket_desc cb = *bucket;
     cb.h1_order = 1;  // traces linear on the sides, fluxes constant
-    const dpg::ElementLayout Lc = dpg::make_layout(*form, cb);
+    const dpg::ElementLayout Lc = dpg::make_layout(*form, cb, true);
     if (L.T <= 0) throw std::invalid_argument("the form has no trace or flux variables");
     const int C = ncells, T = L.T, Tc = Lc.T;
     const dpg::Numbering nf = dpg::number_skeleton(L, C, cell_vertices);
@@ -1101,6 +1178,36 @@ int dpg_skeleton_solve_twolevel(const dpg_skeleton* sk, const dpg_coarse* co, co
       else
         dpg::cg_precondition<<<bn, 256, 0, st>>>(rin, d.p, fixed, n, zout);
     };
+    // Damping of the smoother. The symmetric two-level preconditioner below is
+    // positive definite only where the smoother S converges on its own,
+    // omega * lambda_max(S A) < 2. Undamped, the entity blocks reach
+    // lambda_max(S A) ~ 4 on quads and ~ 6 on hexes (a block couples to its
+    // neighbours through every cell it touches), point Jacobi 10-50. The
+    // largest eigenvalue is estimated by power iteration on S A (norm ratios,
+    // 30 steps) and omega, its reciprocal, is folded into the block inverses or
+    // the diagonal.
+    {
+      dpg::DevArray<double> v(n), w(n), y(n);
+      dpg::seed_free<<<bn, 256, 0, st>>>(fixed, n, v.p);
+      double lam = 0.0;
+      for (int k = 0; k < 30; ++k) {
+        dpg::spmv_free_rows<<<bw, 256, 0, st>>>(sk->row_ptr.p, sk->col.p, values, v.p, fixed, n, w.p);
+        smooth(w.p, y.p);
+        const double yy = dpg::reduce(y.p, y.p, n, 0, part, one, st), vv = dpg::reduce(v.p, v.p, n, 0, part, one, st);
+        if (!(vv > 0.0) || !(yy > 0.0) || !std::isfinite(yy)) break;
+        lam = std::sqrt(yy / vv);
+        dpg::copy_f64<<<bn, 256, 0, st>>>(y.p, n, v.p);
+        dpg::scale_f64<<<bn, 256, 0, st>>>(v.p, n, 1.0 / std::sqrt(yy));
+      }
+      if (lam > 0.0 && std::isfinite(lam)) {
+        if (blocks)
+          dpg::scale_f64<<<dpg::blocks_for(sk->blk_total), 256, 0, st>>>(binv.p, sk->blk_total, 1.0 / lam);
+        else
+          dpg::scale_f64<<<bn, 256, 0, st>>>(d.p, n, lam);
+      }
+      SK_CHECK(hipGetLastError());
+      SK_CHECK(hipStreamSynchronize(st));  // v, w, y go out of scope
+    }
     // coarse solve: coarse_iters steps of Jacobi-preconditioned CG from 0, scalars on the device
     dpg::DevArray<double> rc(nc), ec(nc), zc(nc), pc(nc), qc(nc), sc(4);
     auto dot_dev = [&](const double* a, const double* b2, int64_t len, double* slot) {

@@ -593,7 +593,7 @@ class CondensedSystem:
     def solve_twolevel(self, coarse, schur, values, rhs, fixed, x, tol=1e-12, maxit=10000, coarse_iters=20,
                        stream=None):
         """solve() preconditioned by two levels (dpg_skeleton_solve_twolevel): the
-        entity block-Jacobi (or point Jacobi) smoother and a correction on the
+        damped entity block-Jacobi (or point Jacobi) smoother and a correction on the
         lowest-order level `coarse` (a CoarseLevel of the same mesh), whose
         matrix comes from the cells' Schur blocks `schur` (C, T, T); flexible CG.
         Returns (iterations, final relative residual)."""
@@ -707,6 +707,117 @@ class CoarseLevel:
     def close(self):
         if self._h:
             lib().dpg_coarse_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _hp_cells(cell_bucket, cell_vertices, vertex_coords):
+    cb = np.ascontiguousarray(cell_bucket, dtype=np.int32)
+    cv = np.full((cb.shape[0], 4), -1, dtype=np.int32)
+    cvi = np.asarray(cell_vertices)
+    cv[:, :cvi.shape[1]] = cvi
+    X = np.ascontiguousarray(vertex_coords, dtype=np.float64)
+    return cb, cv, X
+
+
+def hp_numbering(f, buckets, cell_bucket, cell_vertices, vertex_coords):
+    """The constrained numbering of an hp mesh on the host (dpg_hp_numbering):
+    (nglobal, per-cell global dof lists (offsets, dofs), and M as CSR over the
+    cells' local trace/flux rows: row_ptr, entry_dof, entry_weight).
+    vertex_coords (nvertices, 2) holds the coordinates of the vertex ids."""
+    cb, cv, X = _hp_cells(cell_bucket, cell_vertices, vertex_coords)
+    arr = (capi.BucketDesc * len(buckets))(*buckets)
+    sz = np.zeros(4, dtype=np.int64)
+    args = (ct.byref(f), arr, len(buckets), cb.shape[0], _p32(cb), _p32(cv), X.shape[0],
+            X.ctypes.data_as(ct.POINTER(ct.c_double)), sz.ctypes.data_as(ct.POINTER(ct.c_int64)))
+    check(lib().dpg_hp_numbering(*args, None, None, None, None, None))
+    off = np.zeros(cb.shape[0] + 1, dtype=np.int32)
+    dofs = np.zeros(int(sz[1]), dtype=np.int32)
+    rp = np.zeros(int(sz[2]) + 1, dtype=np.int32)
+    ed = np.zeros(int(sz[3]), dtype=np.int32)
+    ew = np.zeros(int(sz[3]))
+    check(lib().dpg_hp_numbering(*args, _p32(off), _p32(dofs), _p32(rp), _p32(ed),
+                                 ew.ctypes.data_as(ct.POINTER(ct.c_double))))
+    return int(sz[0]), off, dofs, rp, ed, ew
+
+
+class HpSystem:
+    """The condensed system of a 2D hp mesh (quads and triangles, hanging sides,
+    one element type per cell) on the device (dpg_hp_system_*): the minimum
+    rule's constrained numbering, assembly of the element types' Schur blocks
+    and the local trace/flux coefficients of a global vector."""
+
+    def __init__(self, f, buckets, cell_bucket, cell_vertices, vertex_coords, stream=None):
+        import torch
+
+        self._torch = torch
+        self.buckets = list(buckets)
+        cb, cv, X = _hp_cells(cell_bucket, cell_vertices, vertex_coords)
+        self.cell_bucket = cb
+        arr = (capi.BucketDesc * len(self.buckets))(*self.buckets)
+        s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        h = ct.c_void_p()
+        check(lib().dpg_hp_system_create(ct.byref(f), arr, len(self.buckets), cb.shape[0], _p32(cb), _p32(cv),
+                                         X.shape[0], X.ctypes.data_as(ct.POINTER(ct.c_double)), ct.c_void_p(s),
+                                         ct.byref(h)))
+        self._h = h
+        n, ne, sk = ct.c_int64(), ct.c_int64(), ct.c_void_p()
+        check(lib().dpg_hp_system_info(h, ct.byref(n), ct.byref(ne), ct.byref(sk)))
+        self.nglobal, self.nentities, self._sk = n.value, ne.value, sk
+        nnz = ct.c_int64()
+        check(lib().dpg_skeleton_info(sk, ct.byref(nnz), None, None, None))
+        self.nnz = nnz.value
+        self.sizes = [sizes(f, b) for b in self.buckets]
+
+    def dofs(self):
+        """(fixed, entity, rep_cell, rep_local) host int32 arrays of nglobal"""
+        out = [np.zeros(self.nglobal, dtype=np.int32) for _ in range(4)]
+        check(lib().dpg_hp_system_dofs(self._h, *[_p32(a) for a in out]))
+        return tuple(out)
+
+    def assemble(self, schur, schur_rhs, stream=None):
+        """CSR values (nnz) and rhs (nglobal) from the per-bucket Schur blocks (lists of device tensors)."""
+        t = self._torch
+        nb = len(self.buckets)
+        sp = (ct.c_void_p * nb)(*[x.data_ptr() if x is not None else None for x in schur])
+        gp = (ct.c_void_p * nb)(*[x.data_ptr() if x is not None else None for x in schur_rhs])
+        vals = t.empty(self.nnz, dtype=t.float64, device="cuda:0")
+        rhs = t.empty(self.nglobal, dtype=t.float64, device="cuda:0")
+        s = stream.cuda_stream if stream is not None else t.cuda.current_stream().cuda_stream
+        check(lib().dpg_hp_system_assemble(self._h, sp, gp, ct.c_void_p(vals.data_ptr()),
+                                           ct.c_void_p(rhs.data_ptr()), ct.c_void_p(s)))
+        return vals, rhs
+
+    def local(self, x, u, stream=None):
+        """writes the trace/flux entries of every u[b] (C_b, M_b) from the global vector x"""
+        t = self._torch
+        nb = len(self.buckets)
+        up = (ct.c_void_p * nb)(*[a.data_ptr() if a is not None else None for a in u])
+        s = stream.cuda_stream if stream is not None else t.cuda.current_stream().cuda_stream
+        check(lib().dpg_hp_system_local(self._h, ct.c_void_p(x.data_ptr()), up, ct.c_void_p(s)))
+
+    def solve(self, values, rhs, fixed, x, tol=1e-12, maxit=10000, blocks=True, stream=None):
+        """CG on the assembled system (dpg_skeleton_solve on its skeleton), block
+        Jacobi over the mesh's vertices and sides unless blocks is False."""
+        t = self._torch
+        s = stream.cuda_stream if stream is not None else t.cuda.current_stream().cuda_stream
+        if blocks:
+            _, ent, _, _ = self.dofs()
+            check(lib().dpg_skeleton_set_blocks(self._sk, _p32(ent), int(self.nentities), ct.c_void_p(s)))
+        it, rel = ct.c_int(), ct.c_double()
+        check(lib().dpg_skeleton_solve(self._sk, ct.c_void_p(values.data_ptr()), ct.c_void_p(rhs.data_ptr()),
+                                       ct.c_void_p(fixed.data_ptr()), ct.c_void_p(x.data_ptr()), float(tol),
+                                       int(maxit), ct.byref(it), ct.byref(rel), ct.c_void_p(s)))
+        return it.value, rel.value
+
+    def close(self):
+        if self._h:
+            lib().dpg_hp_system_destroy(self._h)
             self._h = None
 
     def __del__(self):

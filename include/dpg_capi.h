@@ -178,7 +178,8 @@ enum {
   DPG_STAGE_UPDATE = 6,  /* trailing updates (MFMA)                   */
   DPG_STAGE_EXTRACT = 7, /* K, rhs, G, Schur copies to the outputs    */
   DPG_STAGE_ADJOINT = 8, /* adjoint: back solves and point contractions */
-  DPG_NSTAGES = 9
+  DPG_STAGE_SOLVE = 9,   /* panel update fused with the panel solve (MFMA) */
+  DPG_NSTAGES = 10
 };
 
 /* accumulated per-stage kernel times (HIP events on the call's stream) with
@@ -355,9 +356,11 @@ int dpg_adjoint(dpg_plan* plan, int bucket, const dpg_batch* batch, const double
 typedef struct {
   int32_t gram;    /* G = A^T A (dense path)                        */
   int32_t bmat;    /* B blocks                                      */
-  int32_t update;  /* Cholesky panel/trailing updates; also 128: a
-                      128 x 64 tile for the panel updates (the
-                      trailing update then takes 64)                */
+  int32_t update;  /* Cholesky panel/trailing updates; also 128: the
+                      rows below each diagonal block are updated
+                      and solved against it in one kernel on
+                      128 x 64 tiles (the diagonal blocks and the
+                      trailing update then take 64)                 */
   int32_t sumfact; /* 1: sum-factorised G (opt-in, see above)       */
 } dpg_tiles;
 int dpg_plan_get_tiles(const dpg_plan* plan, int bucket, dpg_tiles* out);
@@ -467,6 +470,13 @@ int dpg_skeleton_nodes(const dpg_form_desc* form, const dpg_bucket_desc* bucket,
  * the order in which each nonzero sums its contributions. Needs
  * C * T * T < 2^32. Synchronises `stream`. */
 int dpg_skeleton_create(int ncells, int T, const int32_t* gdof, int nglobal, void* stream, dpg_skeleton** out);
+/* The same for cells with different numbers of global dofs (hp meshes after
+ * their constraints, dpg_hp_system): cell c holds the global dofs
+ * dofs[offsets[c] .. offsets[c+1]) (host arrays); dpg_skeleton_assemble then
+ * reads each cell's n_c x n_c block, row-major, one after the other (and n_c
+ * load entries per cell). */
+int dpg_skeleton_create_ragged(int ncells, const int32_t* offsets, const int32_t* dofs, int nglobal, void* stream,
+                               dpg_skeleton** out);
 /* nnz, global size and the device CSR pattern (owned by the skeleton; any
  * argument may be NULL) */
 int dpg_skeleton_info(const dpg_skeleton* sk, int64_t* nnz, int64_t* nglobal, const int64_t** row_ptr,
@@ -512,6 +522,58 @@ int dpg_skeleton_solve(const dpg_skeleton* sk, const double* values, const doubl
 int dpg_mark_cells(const double* err, int ncells, double rel_threshold, int32_t* flags, void* stream);
 int dpg_skeleton_destroy(dpg_skeleton* sk);
 
+/* ------------------------------------------------------------------------
+ * Condensed system of a 2D hp mesh: quads and triangles, hanging sides of a
+ * 2:1 refinement, one element type (bucket) per cell. The minimum rule of
+ * GDAMinimumRule::interpretLocalData (GDAMinimumRule.cpp:714-780) with the
+ * side constraints of BasisReconciliation (BasisReconciliation.cpp:259):
+ * every side of the mesh (a side of two cells, a boundary side, or the side of
+ * a coarse cell whose neighbours are two fine cells) carries the smallest
+ * order of its cells; H1 traces have a dof per regular vertex and order - 1 per
+ * side, a hanging vertex takes the coarse side's value there; L2 fluxes have
+ * `order` dofs per side. Each cell's trace and flux dofs are the global
+ * functions at their nodes (local = M_c global); the cells' Schur blocks
+ * enter as M_c^T S_c M_c.
+ * ---------------------------------------------------------------------- */
+typedef struct dpg_hp_system dpg_hp_system;
+/* Host + device set-up once per mesh: cell_bucket (C) indexes buckets,
+ * cell_vertices (C, 4) the global vertex ids in [0, nvertices) in each cell's
+ * vertex order (triangles: 3, the fourth ignored), vertex_coords (nvertices, 2)
+ * their coordinates (a hanging vertex is the midpoint of its coarse side). */
+int dpg_hp_system_create(const dpg_form_desc* form, const dpg_bucket_desc* buckets, int nbuckets, int ncells,
+                         const int32_t* cell_bucket, const int32_t* cell_vertices, int nvertices,
+                         const double* vertex_coords, void* stream, dpg_hp_system** out);
+/* Host only: the constrained numbering itself. sizes[4] receives the global
+ * dof count, the total of the cells' global dof lists, the number of local
+ * trace/flux rows (sum of T over the cells) and of (row, global dof, weight)
+ * entries; with the arrays non-NULL (sizes from a first call): offsets (C + 1)
+ * and dofs (the global dofs each cell touches), row_ptr (rows + 1), entry_dof
+ * and entry_weight (entries) -- local = M global, rows in cell order then the
+ * cell's trace/flux dof order. */
+int dpg_hp_numbering(const dpg_form_desc* form, const dpg_bucket_desc* buckets, int nbuckets, int ncells,
+                     const int32_t* cell_bucket, const int32_t* cell_vertices, int nvertices,
+                     const double* vertex_coords, int64_t* sizes, int32_t* offsets, int32_t* dofs, int32_t* row_ptr,
+                     int32_t* entry_dof, double* entry_weight);
+/* global dofs, mesh entities (vertices and sides, the blocks for
+ * dpg_skeleton_set_blocks) and the skeleton of the assembled matrix (owned by
+ * the hp system; for dpg_skeleton_solve, _set_blocks, _pattern, _info). */
+int dpg_hp_system_info(const dpg_hp_system* hs, int64_t* nglobal, int64_t* nentities, dpg_skeleton** sk);
+/* Host arrays (nglobal each, any may be NULL): fixed = 1 on the trace dofs of
+ * boundary sides; entity of every dof; a (cell, local trace/flux index) whose
+ * local dof equals the global dof (-1 when none does). */
+int dpg_hp_system_dofs(const dpg_hp_system* hs, int32_t* fixed, int32_t* entity, int32_t* rep_cell,
+                       int32_t* rep_local);
+/* values (nnz) and rhs (nglobal) from the buckets' Schur blocks: schur[b]
+ * (C_b, T_b, T_b) and schur_rhs[b] (C_b, T_b) of the cells of bucket b in
+ * increasing cell order (host arrays of device pointers). Stream-ordered. */
+int dpg_hp_system_assemble(dpg_hp_system* hs, const double* const* schur, const double* const* schur_rhs,
+                           double* values, double* rhs, void* stream);
+/* the cells' local trace and flux coefficients from a global vector x: the
+ * trailing T_b entries of every row of u[b] (C_b, M_b) are written (for
+ * dpg_recover_fields). */
+int dpg_hp_system_local(const dpg_hp_system* hs, const double* x, double* const* u, void* stream);
+int dpg_hp_system_destroy(dpg_hp_system* hs);
+
 /* Two-level (p-multigrid) preconditioning of the condensed system, the
  * multilevel solve the reference offers through GMGSolver (GMGSolver.h:23,
  * GMGOperator: a coarse space of lower polynomial order, block-Jacobi
@@ -531,10 +593,13 @@ int dpg_coarse_destroy(dpg_coarse* co);
 /* dpg_skeleton_solve with the two-level preconditioner: the coarse matrix
  * P^T A P of the free dofs is assembled from the cells' Schur blocks schur
  * (C, T, T), the matrix of values; flexible conjugate gradients, each
- * preconditioner application = block-Jacobi smoothing (the entity blocks of
- * dpg_skeleton_set_blocks, else point Jacobi), a coarse correction by
- * coarse_iters Jacobi-CG steps on the coarse matrix, and a second smoothing
- * step. Same arguments and stopping rule as dpg_skeleton_solve. */
+ * preconditioner application = damped block-Jacobi smoothing (the entity
+ * blocks of dpg_skeleton_set_blocks, else point Jacobi; damping 1 / lambda_max
+ * of the smoothed operator, estimated by power iteration at the start of the
+ * call, which keeps the preconditioner positive definite), a coarse
+ * correction by coarse_iters Jacobi-CG steps on the coarse matrix, and a
+ * second smoothing step. Same arguments and stopping rule as
+ * dpg_skeleton_solve. */
 int dpg_skeleton_solve_twolevel(const dpg_skeleton* sk, const dpg_coarse* co, const double* schur,
                                 const double* values, const double* rhs, const int32_t* fixed, double* x, double tol,
                                 int maxit, int coarse_iters, int* iters, double* relres, void* stream);

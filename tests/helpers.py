@@ -221,3 +221,51 @@ def nodal_trial_coefficients(f, k, nodes, par, value):
                 row += [sgn * value(v, 0, x, n) for x in X]
         cols.append(row)
     return np.array(cols)
+
+
+def hp_mesh(n0, levels, tri_cols, pmax=3, dpmax=2):
+    """cells of the graded hp mesh in one list (quads, then triangles): nodes
+    per topology, the bucket of every cell, the buckets, global vertex ids and
+    the vertices' coordinates"""
+    from camellia2_amd.workloads import cd_hp
+
+    _, cells, _ = cd_hp(n0, levels, tri_cols=tri_cols, pmax=pmax, dpmax=dpmax)
+    order = []  # (topology, index within the topology)
+    types = {}
+    for topo in (capi.QUAD, capi.TRI):
+        if topo not in cells:
+            continue
+        c = cells[topo]
+        for i in range(c["nodes"].shape[0]):
+            key = (topo, int(c["p"][i]) + 1, int(c["p"][i] + c["dp"][i]))
+            types.setdefault(key, len(types))
+            order.append((topo, i, types[key]))
+    buckets = [None] * len(types)
+    for (topo, k, kt), b in types.items():
+        buckets[b] = dpg.bucket(topo, k, kt)
+    # global vertex ids from the coordinates (shared vertices are identical)
+    pts = np.concatenate([cells[t]["nodes"].reshape(-1, 2) for t in (capi.QUAD, capi.TRI) if t in cells])
+    X, inv = np.unique(pts, axis=0, return_inverse=True)
+    cv = np.full((len(order), 4), -1, dtype=np.int32)
+    pos, row = 0, 0
+    for t in (capi.QUAD, capi.TRI):
+        if t not in cells:
+            continue
+        V = cells[t]["nodes"].shape[1]
+        n = cells[t]["nodes"].shape[0]
+        cv[row:row + n, :V] = inv[pos:pos + n * V].reshape(n, V)
+        pos += n * V
+        row += n
+    cell_bucket = np.array([o[2] for o in order], dtype=np.int32)
+    return cells, order, buckets, cell_bucket, cv, X
+
+
+def per_bucket(cells, order, nb):
+    """the nodes, parities and global cell ids of every bucket's cells, in cell order"""
+    out = []
+    for b in range(nb):
+        ids = [g for g, o in enumerate(order) if o[2] == b]
+        nodes = np.stack([cells[order[g][0]]["nodes"][order[g][1]] for g in ids])
+        par = np.stack([cells[order[g][0]]["parities"][order[g][1]] for g in ids])
+        out.append(dict(ids=ids, nodes=nodes, parities=par))
+    return out

@@ -173,3 +173,18 @@ def test_graded_mesh_is_a_consistent_hp_mesh():
                 assert lvl[cc] == lvl[c] - 1 and par[cc, cs_] == 1 and par[c, s] == -1
                 nhang += 1
     assert nhang == 2 * n0 * (2 ** levels - 1)  # two fine sides per refined coarse side
+
+
+def test_form_too_large_for_the_operand_lds_is_refused():
+    """The operand kernels stage the weights of 8 points x groups x test
+    variables x prims in LDS: a form beyond 64 KiB is refused when it is
+    validated, with a message, not at launch."""
+    f = dpg.form("poisson", 3)
+    f.n_test = 16
+    for v in range(2, 16):
+        f.test_space[v] = capi.HGRAD
+    f.n_trial = 16
+    for v in range(4, 16):
+        f.trial_kind[v] = capi.FIELD_VECTOR
+    with pytest.raises(capi.DPGError, match="LDS"):
+        dpg.sizes(f, dpg.bucket(capi.HEX, 2, 3))

@@ -143,7 +143,14 @@ def test_tile_edges_give_identical_results_and_autotune_picks_one():
         c = {key: v.cpu().numpy() for key, v in c.items()}
         for key in ("K", "rhs", "schur", "schur_rhs", "G"):
             assert np.array_equal(a[key], b[key]), key
-            assert np.array_equal(a[key], c[key]), key
+        # the fused panel update and solve (update tile 128) solves blockwise:
+        # the same G, the other outputs under the floor rule
+        assert np.array_equal(a["G"], c["G"])
+        o, x = oracle.with_extended(oracle.form_like(f), g["bucket"], g["nodes"], g["parities"], g["coefs"], load,
+                                    schur=True)
+        for key, ok in (("K", "K"), ("rhs", "rhs"), ("schur", "S"), ("schur_rhs", "g")):
+            floor = rel_fro(o[ok], x[ok])
+            assert rel_fro(c[key], x[ok]) <= max(1e-12, 4.0 * floor), key
         eg = rel_fro(sfr["G"], a["G"])
         ek = rel_fro(sfr["K"], a["K"])
         print(f"type {g['topology'], g['k'], g['kt']}: sum-factorised vs dense G {eg:.2e}, K {ek:.2e}")

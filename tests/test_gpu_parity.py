@@ -22,10 +22,12 @@ def dev(a):
     return torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
 
 
-def gpu_assemble(f, b, nodes, par, coefs, load, want_G=False, want_schur=False, sumfact=None, **plan_kw):
+def gpu_assemble(f, b, nodes, par, coefs, load, want_G=False, want_schur=False, sumfact=None, tiles=None, **plan_kw):
     A = dpg.LocalAssembler(f, [b], **plan_kw)
     if sumfact is not None:
         A.set_tiles(0, 64, 64, 64, sumfact)
+    if tiles is not None:
+        A.set_tiles(0, *tiles, 0)
     out, nbad = A.local_stiffness_and_rhs(0, dev(nodes), dev(par), dev(coefs), load, want_G=want_G,
                                           want_schur=want_schur)
     torch.cuda.synchronize()
@@ -34,14 +36,15 @@ def gpu_assemble(f, b, nodes, par, coefs, load, want_G=False, want_schur=False, 
     return res, nbad
 
 
-def check_against_oracle(case, want_G=False, want_schur=False, sumfact=None, **plan_kw):
+def check_against_oracle(case, want_G=False, want_schur=False, sumfact=None, tiles=None, **plan_kw):
     """GPU vs oracle on the same inputs. The reference algorithm's own FP64
     accuracy on these inputs (`floor`: the oracle's FP64 outputs vs an
     evaluation that sums G, B, l in 80-bit from the same FP64 point values and
     factors in 80-bit) can exceed 1e-12 when G is ill conditioned, so the bound
     is max(1e-12, 4 * floor) on the GPU's distance from the 80-bit result."""
     f, b, nodes, par, coefs, load = case
-    g, nbad = gpu_assemble(f, b, nodes, par, coefs, load, want_G, want_schur, sumfact=sumfact, **plan_kw)
+    g, nbad = gpu_assemble(f, b, nodes, par, coefs, load, want_G, want_schur, sumfact=sumfact, tiles=tiles,
+                           **plan_kw)
     o, x = oracle.with_extended(oracle.form_like(f), b, nodes, par, coefs, load, schur=want_schur)
     assert nbad == 0 and np.all(g["info"] == 0)
     pairs = [("K", "K"), ("rhs", "rhs")] + ([("schur", "S"), ("schur_rhs", "g")] if want_schur else [])
@@ -72,8 +75,10 @@ def test_c5_cdr_2d_per_cell_coefficients():
 
 
 def test_c2_stokes_2d():
-    """C2 orders (p=3, traces 4, test 5) on a 4x4 patch."""
-    check_against_oracle(stokes2d(4, 4), want_schur=True)
+    """C2 orders (p=3, traces 4, test 5) on a 4x4 patch; G included: the Stokes
+    graph norm has IP terms equal on pairs of test variables, which the Gram
+    GEMM merges into one weighted k-segment (G must still equal the oracle's)."""
+    check_against_oracle(stokes2d(4, 4), want_G=True, want_schur=True)
 
 
 def test_c4_poisson_3d_hexes():
@@ -112,20 +117,37 @@ def test_poisson_2d_other_orders(k, kt):
 
 @pytest.mark.parametrize("case", ["c4", "c2"])
 def test_update_tiles_give_identical_results(case):
-    """The 128 x 64 panel-update tile, the 64 x 64 and the 32 x 32 tiles
-    accumulate every output in the same k order: the same bits."""
+    """The 64 x 64 and the 32 x 32 update tiles accumulate every output in the
+    same k order: the same bits. The fused panel update and solve (update tile
+    128) solves the panel blockwise, with MFMA sums between the 16-column
+    blocks: the same G, and K, rhs and the Schur complement under the floor
+    rule (test_fused_panel_solve_against_the_oracle)."""
     f, b, nodes, par, coefs, load = {"c4": lambda: poisson3d(2, 2, 1), "c2": lambda: stokes2d(3, 3)}[case]()
     A = dpg.LocalAssembler(f, [b])
     res = []
     for t in (64, 32, 128):
         A.set_tiles(0, 64, 64, t, 0)
-        o, nbad = A.local_stiffness_and_rhs(0, dev(nodes), dev(par), dev(coefs), load, want_schur=True)
+        o, nbad = A.local_stiffness_and_rhs(0, dev(nodes), dev(par), dev(coefs), load, want_schur=True,
+                                            want_G=True)
         assert nbad == 0
         res.append({k: v.clone() for k, v in o.items()})
-    for r in res[1:]:
-        for k in ("K", "rhs", "schur", "schur_rhs"):
-            assert torch.equal(r[k], res[0][k]), k
+    for k in ("K", "rhs", "schur", "schur_rhs", "G"):
+        assert torch.equal(res[1][k], res[0][k]), k
+    assert torch.equal(res[2]["G"], res[0]["G"])
+    for k in ("K", "rhs", "schur", "schur_rhs"):
+        print(f"{k:10s} fused panel solve vs 64 x 64 tiles {rel_fro(res[2][k].cpu().numpy(), res[0][k].cpu().numpy()):.2e}")
     A.close()
+
+
+@pytest.mark.parametrize("case", ["c4", "hex_low", "c2", "c5", "c1"])
+def test_fused_panel_solve_against_the_oracle(case):
+    """Update tile 128: every panel's rows below the diagonal block are
+    updated and solved against the block's factor in one kernel
+    (update_solve_f64), the solve blocked by 16 columns with the later blocks'
+    updates on MFMA. Same bar as the other FP64 outputs."""
+    cases = {"c4": lambda: poisson3d(2, 2, 1), "hex_low": lambda: poisson3d(2, 2, 2, k=2, kt=3),
+             "c2": lambda: stokes2d(3, 3), "c5": lambda: cdr2d(6, 6), "c1": lambda: poisson2d(5, 4)}
+    check_against_oracle(cases[case](), want_G=True, want_schur=True, tiles=(64, 64, 128))
 
 
 def test_batches_give_identical_results():
