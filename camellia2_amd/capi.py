@@ -15,6 +15,7 @@ MAX_LT_SUMMANDS = 8
 MAX_RHS_TERMS = 8
 MAX_LOAD_MODES = 32
 MAX_COEFS = 8
+NOALLOC_SCRATCH_BYTES = 256 << 20  # DPG_NOALLOC_SCRATCH_BYTES
 
 QUAD, HEX, TRI = 0, 1, 2
 FIELD, FIELD_VECTOR, TRACE, FLUX = 0, 1, 2, 3
@@ -116,7 +117,7 @@ EXPORTS = [
     "dpg_mesh_quad_device", "dpg_mesh_hex_device",
     "dpg_load_sine_modes", "dpg_rng_uniform", "dpg_rng_normal", "dpg_bucket_sizes",
     "dpg_plan_create", "dpg_plan_destroy", "dpg_plan_clone_workspace", "dpg_plan_workspace", "dpg_plan_timings",
-    "dpg_assemble",
+    "dpg_assemble", "dpg_assemble_noalloc",
     "dpg_energy_error", "dpg_energy_error_factored", "dpg_adjoint", "dpg_mesh_quad_graded", "dpg_plan_get_tiles", "dpg_plan_set_tiles",
     "dpg_plan_autotune", "dpg_graph_create", "dpg_graph_launch", "dpg_graph_workspace", "dpg_graph_destroy",
     "dpg_skeleton_numbering", "dpg_skeleton_nodes", "dpg_skeleton_create", "dpg_skeleton_create_ragged",
@@ -165,6 +166,7 @@ def lib():
         L.dpg_plan_workspace.argtypes = [ct.c_void_p, P(ct.c_size_t)]
         L.dpg_plan_timings.argtypes = [ct.c_void_p, P(Timings), ct.c_int]
         L.dpg_assemble.argtypes = [ct.c_void_p, ct.c_int, P(Batch), P(Outputs), ct.c_void_p]
+        L.dpg_assemble_noalloc.argtypes = [ct.c_void_p, ct.c_int, P(Batch), P(Outputs), ct.c_void_p]
         L.dpg_energy_error.argtypes = [ct.c_void_p, ct.c_int, P(Batch), ct.c_void_p, ct.c_void_p,
                                        ct.c_void_p]
         L.dpg_energy_error_factored.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p,

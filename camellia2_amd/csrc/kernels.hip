@@ -2054,6 +2054,27 @@ __global__ void fill_i32(int32_t* __restrict__ p, int n, int32_t v) {
   if (i < n) p[i] = v;
 }
 
+// One workgroup: copies info[0..n) to out (when out is not NULL) and writes the
+// number of non-zero entries to *count (per-thread partial sums, then a fixed
+// tree in LDS).
+__global__ __launch_bounds__(256) void info_count(const int32_t* __restrict__ info, int n, int32_t* __restrict__ out,
+                                                  int32_t* __restrict__ count) {
+  __shared__ int32_t part[256];
+  int c = 0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    int32_t v = info[i];
+    if (out) out[i] = v;
+    c += v != 0;
+  }
+  part[threadIdx.x] = c;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *count = part[0];
+}
+
 void launch_gram_sf(int dim, const SfBlockPair* bps, int nbp, const SfPair* pairs, const double* F, int nF, int q,
                     const double* Wip, int n_lt, int na, int QG, double* A, int64_t lda, int64_t sA, int ncells,
                     size_t lds_bytes, hipStream_t st) {
@@ -2072,6 +2093,11 @@ void launch_fill_i32(int32_t* p, int n, int32_t v, hipStream_t st) {
   if (n <= 0) return;
   fill_i32<<<(n + 255) / 256, 256, 0, st>>>(p, n, v);
   check_launch("fill_i32");
+}
+
+void launch_info_count(const int32_t* info, int n, int32_t* out, int32_t* count, hipStream_t st) {
+  info_count<<<1, 256, 0, st>>>(info, n, out, count);
+  check_launch("info_count");
 }
 
 void launch_zero_trailing(double* A, int64_t lda, int64_t sA, int r0, int D, int ncells, hipStream_t st) {

@@ -76,6 +76,8 @@ inline size_t ip_digits_lds_bytes(int n_lt, int nvar, int dim) {
 }
 void launch_gemm(GemmArgs g, int role, hipStream_t st, int tile = 64);
 void launch_fill_i32(int32_t* p, int n, int32_t v, hipStream_t st);
+// copies info (n) to out (optional) and writes the count of non-zero entries to *count
+void launch_info_count(const int32_t* info, int n, int32_t* out, int32_t* count, hipStream_t st);
 // sum-factorised Gram matrix, unscaled, lower triangle of A's first N rows/columns
 void launch_gram_sf(int dim, const SfBlockPair* bps, int nbp, const SfPair* pairs, const double* F, int nF, int q,
                     const double* Wip, int n_lt, int na, int QG, double* A, int64_t lda, int64_t sA, int ncells,

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -232,6 +233,34 @@ struct BucketPlan {
   const SfPair* sf_pairs = nullptr;
   const double* sf_F = nullptr;
 };
+
+// ---- static scratch of dpg_assemble_noalloc ---------------------------------
+// Workspace of the entry point that allocates nothing: a file-scope device
+// array, one copy per GPU for the life of the process, shared by every call on
+// that GPU. Host code never passes `scratch` itself to a kernel (on the host the
+// name is only a stand-in); it uses the device address resolved per device by
+// static_scratch().
+static __device__ __align__(256) unsigned char scratch[DPG_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxScratchDevices] = {};
+}  // namespace
+
+// Device address of `scratch` on `device`, which must be the current device.
+// Resolved once per device (hipGetSymbolAddress) and kept.
+unsigned char* static_scratch(int device) {
+  if (device < 0 || device >= kMaxScratchDevices) throw std::invalid_argument("device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw DeviceError("hipGetSymbolAddress returned NULL for the static scratch");
+    g_scratch_ptr[device] = (unsigned char*)p;
+  }
+  return g_scratch_ptr[device];
+}
 
 }  // namespace dpg
 
@@ -1095,6 +1124,7 @@ int dpg_plan_create(const dpg_form_desc* form, const dpg_bucket_desc* buckets, i
       throw std::invalid_argument("refine: a refinement of G^-1 B cannot correct an error that sits in G itself "
                                   "(the factorisation is FP64); not supported");
     HIP_CHECK(hipGetDevice(&plan->device));
+    dpg::static_scratch(plan->device);  // set-up of dpg_assemble_noalloc's workspace on this device
     plan->prof.on = plan->opt.profile != 0;
     plan->buckets.resize(nbuckets);
     for (int i = 0; i < nbuckets; ++i) dpg::build_bucket(plan.get(), buckets[i], plan->buckets[i]);
@@ -1159,6 +1189,55 @@ int dpg_assemble(dpg_plan* plan, int bucket, const dpg_batch* batch, const dpg_o
       plan->prof.collect(st, nb);
       // the next batch reuses the workspace in stream order
       if (outputs->info) bad += dpg::count_failures(w, nb, outputs->info + c0, st);
+    }
+    return bad;
+  });
+}
+
+int dpg_assemble_noalloc(dpg_plan* plan, int bucket, const dpg_batch* batch, const dpg_outputs* outputs,
+                         void* stream) {
+  return dpg::guarded_status([&] {
+    dpg::check_batch(plan, bucket, batch);
+    dpg::check_outputs(outputs);
+    if (batch->ncells == 0) return 0;
+    int dev = -1;
+    HIP_CHECK(hipGetDevice(&dev));
+    if (dev != plan->device) throw std::invalid_argument("the current device is not the plan's device");
+    hipStream_t st = (hipStream_t)stream;
+    const dpg::BucketPlan& B = plan->buckets[bucket];
+    // scratch layout: INFO of a batch (at most 65535 cells), the failure count,
+    // then the per-cell workspace regions of the batch
+    constexpr size_t kInfoBytes = 65536 * sizeof(int32_t), kCountBytes = 256;
+    constexpr size_t kWsOffset = kInfoBytes + kCountBytes;
+    const size_t room = (size_t)DPG_NOALLOC_SCRATCH_BYTES - kWsOffset;
+    if ((size_t)B.per_cell_bytes > room)
+      throw std::invalid_argument("dpg_assemble_noalloc: one cell of this element type needs " +
+                                  std::to_string(B.per_cell_bytes) + " bytes of workspace, the static scratch holds " +
+                                  std::to_string(room) + " (DPG_NOALLOC_SCRATCH_BYTES); use dpg_assemble");
+    int C = batch->ncells;
+    int maxb = (int)std::min<size_t>(room / (size_t)B.per_cell_bytes, (size_t)dpg::batch_cells(plan, B, C));
+    unsigned char* base = dpg::static_scratch(dev);
+    int32_t* info = (int32_t*)base;
+    int32_t* count = (int32_t*)(base + kInfoBytes);
+    // no per-launch timing events in this path
+    struct ProfOff {
+      dpg::Profiler& p;
+      bool was;
+      explicit ProfOff(dpg::Profiler& q) : p(q), was(q.on) { p.on = false; }
+      ~ProfOff() { p.on = was; }
+    } prof_off(plan->prof);
+    int bad = 0;
+    for (int c0 = 0; c0 < C; c0 += maxb) {
+      int nb = std::min(maxb, C - c0);
+      dpg::Work w = dpg::carve(base + kWsOffset, info, B, nb);
+      dpg::assemble_batch(plan, B, *batch, c0, nb, w, *outputs, st);
+      if (outputs->info) {
+        dpg::launch_info_count(info, nb, outputs->info + c0, count, st);
+        int32_t h = 0;
+        HIP_CHECK(hipMemcpyAsync(&h, count, sizeof(h), hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        bad += h;
+      }
     }
     return bad;
   });

@@ -311,6 +311,23 @@ int dpg_plan_timings(dpg_plan* plan, dpg_timings* out, int reset);
 int dpg_assemble(dpg_plan* plan, int bucket, const dpg_batch* batch, const dpg_outputs* outputs,
                  void* stream);
 
+/* dpg_assemble without any device allocation. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same inputs,
+ * outputs, results and return value as dpg_assemble (outputs->G excepted:
+ * supported as there), but the workspace is a static device array of
+ * DPG_NOALLOC_SCRATCH_BYTES that the library holds on every GPU it runs on until
+ * the process ends, instead of the plan's workspace: the cells are assembled in
+ * batches that fit it. A call whose element type needs more than the scratch
+ * for a single cell fails with a negative status and a message (dpg_assemble
+ * has no such limit). No per-launch timings are recorded. All calls on one GPU
+ * share the scratch, whatever the plan: calls that overlap in time on one GPU
+ * (other host threads, other streams) race and give wrong results. The current
+ * device must be the plan's device. */
+#define DPG_NOALLOC_SCRATCH_BYTES (256u << 20)
+int dpg_assemble_noalloc(dpg_plan* plan, int bucket, const dpg_batch* batch, const dpg_outputs* outputs,
+                         void* stream);
+
 /* Energy-error indicator of TSolution::rankLocalEnergyError (Solution.cpp:2653-2798):
  * r = l - B u,  G e = r,  eta2[c] = r^T e  (= e^T G e; Camellia reports sqrt). */
 int dpg_energy_error(dpg_plan* plan, int bucket, const dpg_batch* batch, const double* u,
