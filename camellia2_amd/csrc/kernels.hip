@@ -1683,9 +1683,19 @@ __global__ __launch_bounds__(256) void factor_backsolve(const double* __restrict
 //   volume / side term (coef):   w dcoef_k op(X)^T Bp phi
 // with Gp, Bp the cotangents in the coordinates of X's columns and phi the
 // trial basis values at the point (flux values times the side parity). One
-// workgroup per cell, lanes over the columns, the points in a fixed order and
-// one fixed-tree reduction per coefficient at the end: the same bits on every
-// call. Column layout: T at [0, nT), y at cy (or -1), e at ce (or -1).
+// workgroup per cell; its waves take the points in turn (npw of them, as many
+// as their LDS buffers allow), lanes over the columns, each wave's points in a
+// fixed order and one fixed-tree reduction per coefficient at the end: the
+// same bits on every call. Column layout: T at [0, nT), y at cy (or -1), e at
+// ce (or -1).
+// LDS written by some lanes of a wave and read by others: ordered at wavefront
+// scope (no workgroup barrier: the other waves work on other points)
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int DIM>
 __global__ __launch_bounds__(256) void adjoint_kernel(const DevProgram* __restrict__ prog, DevTables tab, int N,
                                                       int M, int QG, int Q, int Qs, int nsides,
@@ -1695,24 +1705,29 @@ __global__ __launch_bounds__(256) void adjoint_kernel(const DevProgram* __restri
                                                       const double* __restrict__ X, int64_t sX, int ncol, int nT,
                                                       int cy, int ce, const double* __restrict__ Kbar,
                                                       const double* __restrict__ rbar,
-                                                      const double* __restrict__ u, double* __restrict__ grad) {
+                                                      const double* __restrict__ u, double* __restrict__ grad,
+                                                      int npw) {
   extern __shared__ double lds[];
   __shared__ double red[256];
   constexpr int NPR = DIM + 1;
-  const int c = blockIdx.x, t = threadIdx.x;
+  // the wave index as a scalar: the point loop, its tables and the program are
+  // then uniform (scalar loads)
+  const int c = blockIdx.x, t = threadIdx.x, w = __builtin_amdgcn_readfirstlane(t >> 6), lane = t & 63;
   const int V = prog->nverts, nvar = prog->nvar, ncoef = prog->ncoef;
-  double* Z = lds;                                 // Z[(v * NPR + r) * ncol + m]
-  double* Pl = lds + (size_t)nvar * NPR * ncol;    // P[m] of one IP term
   const double* Xn = nodes + (size_t)c * V * DIM;
   const double* cc = coefs + (size_t)c * ncoef;
   const double* Xc = X + (size_t)c * sX;
   const double* Kc = Kbar ? Kbar + (size_t)c * M * M : nullptr;
   const double* rc = rbar ? rbar + (size_t)c * M : nullptr;
   const double* uc = u ? u + (size_t)c * M : nullptr;
+  // wave w's own buffers (the waves take the points in turn: npw of them)
+  double* Z = lds + (size_t)w * ((size_t)nvar * NPR * ncol + ncol);  // Z[(v * NPR + r) * ncol + m]
+  double* Pl = Z + (size_t)nvar * NPR * ncol;                        // P[m] of one IP term
+  auto ks = [&](int i, int j) { return Kc[(size_t)i * M + j] + Kc[(size_t)j * M + i]; };
   double acc[DPG_MAX_COEFS];
   for (int k = 0; k < DPG_MAX_COEFS; ++k) acc[k] = 0.0;
   const int npts = QG + Q + nsides * Qs;
-  for (int pp = 0; pp < npts; ++pp) {
+  for (int pp = w; w < npw && pp < npts; pp += npw) {
     const int kind = pp < QG ? 0 : (pp < QG + Q ? 1 : 2);
     const int p = kind == 0 ? pp : (kind == 1 ? pp - QG : pp - QG - Q);
     const double* dN = kind == 0 ? tab.dN_ip : (kind == 1 ? tab.dN_vol : tab.dN_side);
@@ -1751,8 +1766,8 @@ __global__ __launch_bounds__(256) void adjoint_kernel(const DevProgram* __restri
     }
     const double* nn = kind == 2 ? n : nullptr;
     // reference prims of the column functions at this point
-    __syncthreads();
-    for (int m = t; m < ncol; m += 256)
+    wave_lds_sync();
+    for (int m = lane; m < ncol; m += 64)
       for (int v = 0; v < nvar; ++v) {
         double zr[NPR];
 #pragma unroll
@@ -1765,7 +1780,7 @@ __global__ __launch_bounds__(256) void adjoint_kernel(const DevProgram* __restri
 #pragma unroll
         for (int r = 0; r < NPR; ++r) Z[((size_t)v * NPR + r) * ncol + m] = zr[r];
       }
-    __syncthreads();
+    wave_lds_sync();
     double tr[NPR];
     auto op_col = [&](const DSummand& sm, int m) {
       op_to_prims<DIM>(sm.op, g, nn, tr);
@@ -1779,19 +1794,19 @@ __global__ __launch_bounds__(256) void adjoint_kernel(const DevProgram* __restri
         bool dep = false;
         for (int si = prog->lt_beg[lt]; si < prog->lt_beg[lt + 1]; ++si) dep |= prog->s[si].slot >= 0;
         if (!dep) continue;  // the term's weight has no coefficient: no contribution
-        for (int m = t; m < ncol; m += 256) {
+        for (int m = lane; m < ncol; m += 64) {
           double v = 0.0;
           for (int si = prog->lt_beg[lt]; si < prog->lt_beg[lt + 1]; ++si)
             v = fma(coef_value(prog->s[si], cc, nullptr), op_col(prog->s[si], m), v);
           Pl[m] = v;
         }
-        __syncthreads();
-        for (int m = t; m < ncol; m += 256) {
+        wave_lds_sync();
+        for (int m = lane; m < ncol; m += 64) {
           // h = (Gbar in column coordinates) P
           double h = 0.0;
           if (m < nT) {
             if (Kc)
-              for (int q = 0; q < M; ++q) h = fma(-0.5 * (Kc[(size_t)m * M + q] + Kc[(size_t)q * M + m]), Pl[q], h);
+              for (int q = 0; q < M; ++q) h = fma(-0.5 * ks(m, q), Pl[q], h);
             if (rc) h = fma(-0.5 * rc[m], Pl[cy], h);
           } else if (m == cy) {
             for (int q = 0; q < M; ++q) h = fma(-0.5 * rc[q], Pl[q], h);
@@ -1809,7 +1824,7 @@ __global__ __launch_bounds__(256) void adjoint_kernel(const DevProgram* __restri
             acc[k] = fma(2.0 * wm * dv, h, acc[k]);
           }
         }
-        __syncthreads();
+        wave_lds_sync();
       }
     } else {
       // trial blocks at this point: volume field blocks, or the side's trace/flux blocks
@@ -1833,13 +1848,13 @@ __global__ __launch_bounds__(256) void adjoint_kernel(const DevProgram* __restri
           phi = (prog->sb_flux[gi] ? tab.flux : tab.trace) + (size_t)(p % Qs) * card;
           if (prog->sb_flux[gi]) par = (double)parities[(size_t)c * nsides + side];
         }
-        for (int m = t; m < ncol; m += 256) {
+        for (int m = lane; m < ncol; m += 64) {
           // gb = (Bbar in column coordinates) phi
           double gb = 0.0;
           if (m < nT) {
             if (Kc)
               for (int j = 0; j < card; ++j)
-                gb = fma(Kc[(size_t)m * M + off + j] + Kc[(size_t)(off + j) * M + m], phi[j], gb);
+                gb = fma(ks(m, off + j), phi[j], gb);
           } else if (m == cy) {
             for (int j = 0; j < card; ++j) gb = fma(rc[off + j], phi[j], gb);
           } else if (m == ce) {
@@ -2172,14 +2187,16 @@ void launch_adjoint(int dim, const DevProgram* prog, const DevTables& tab, int N
                     const double* X, int64_t sX, int ncol, int nT, int cy, int ce, const double* Kbar,
                     const double* rbar, const double* u, double* grad, int ncells, hipStream_t st) {
   if (ncells <= 0) return;
-  const size_t lds = adjoint_lds_bytes(nvar, dim, ncol);
-  if (lds > 64 * 1024) throw DeviceError("adjoint: LDS request above 64 KiB");
+  const size_t per_wave = adjoint_lds_bytes(nvar, dim, ncol), cap = 64 * 1024;
+  if (per_wave > cap) throw DeviceError("adjoint: LDS request above 64 KiB");
+  const int npw = (int)std::min<size_t>(4, cap / per_wave);  // waves sharing the points
+  const size_t lds = per_wave * npw;
   if (dim == 3)
     adjoint_kernel<3><<<ncells, 256, lds, st>>>(prog, tab, N, M, QG, Q, Qs, nsides, nodes, par, coefs, X, sX, ncol,
-                                                nT, cy, ce, Kbar, rbar, u, grad);
+                                                nT, cy, ce, Kbar, rbar, u, grad, npw);
   else
     adjoint_kernel<2><<<ncells, 256, lds, st>>>(prog, tab, N, M, QG, Q, Qs, nsides, nodes, par, coefs, X, sX, ncol,
-                                                nT, cy, ce, Kbar, rbar, u, grad);
+                                                nT, cy, ce, Kbar, rbar, u, grad, npw);
   check_launch("adjoint_kernel");
 }
 
