@@ -796,7 +796,9 @@ def library_dgemm_tflops(n=8192, reps=10):
 
 
 def attach_peaks(result, peaks):
-    """measured peaks, and the dominant kernel's fraction of the measured FP64 MFMA rate"""
+    """measured peaks, and the dominant kernel's fraction of the highest measured
+    FP64 MFMA rate (the library DGEMM on 8192^2 sustains more than a register-only
+    MFMA loop does, so it is the ceiling when it was measured)"""
     if peaks is None:
         return
     result["measured_peaks"] = peaks
@@ -804,8 +806,11 @@ def attach_peaks(result, peaks):
     if r["unit"] == "GB/s":
         if peaks.get("hbm_copy_gbs", 0) > 0:
             r["frac_of_measured_peak"] = round(r["achieved"] / peaks["hbm_copy_gbs"], 4)
-    elif peaks.get("fp64_mfma_tflops", 0) > 0:
-        r["frac_of_measured_peak"] = round(r["achieved"] / peaks["fp64_mfma_tflops"], 4)
+    else:
+        ceiling = max(peaks.get("fp64_mfma_tflops", 0), peaks.get("library_dgemm_tflops", 0) or 0)
+        if ceiling > 0:
+            r["frac_of_measured_peak"] = round(r["achieved"] / ceiling, 4)
+            r["measured_peak_tflops"] = ceiling
 
 
 def main():
@@ -822,7 +827,7 @@ def main():
     ap.add_argument("--no-schur", action="store_true")
     ap.add_argument("--lanes", type=int, default=4, help="hp config: concurrent streams inside the graph")
     ap.add_argument("--tiles", default="",
-                    help="GEMM tiles (64 or 32) for the Gram, B and Cholesky-update contractions, e.g. 64,32,64 "
+                    help="GEMM tiles (64 or 32; 128 for the updates) for the Gram, B and Cholesky-update contractions, e.g. 64,32,128 "
                          "(single-type configurations; default: chosen by the plan autotuner)")
     ap.add_argument("--sumfact", action="store_true",
                     help="opt-in sum-factorised G (fewer flops, K accurate only to ~1e-12..1e-1; not the default)")

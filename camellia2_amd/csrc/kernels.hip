@@ -358,16 +358,19 @@ __device__ inline void tile_coords(int tile, int lower, int tiles_m, int& ti, in
   }
 }
 
-// TT x TT output tile per workgroup of (TT/32)^2 waves; each wave owns a 32x32
-// sub-tile (2x2 MFMA blocks). TT = 64 for large operands, TT = 32 for buckets
-// whose dimensions would leave most of a 64-tile empty.
+// TT x TT output tile per workgroup of (TT/WT)^2 waves; each wave owns a WT x WT
+// sub-tile (WT/16 x WT/16 MFMA blocks). TT = 64 (WT 32) for large operands,
+// TT = 32 for buckets whose dimensions would leave most of a 64-tile empty.
+// (A 128-tile of four 64 x 64 waves was measured slower on every role: its
+// ~230 registers per lane leave two waves per SIMD, too few to hide the loads.)
 // k runs in chunks of GKC rows staged through LDS; each thread stages one fixed
 // column of X and of Y (rows t/TT + r*NT/TT of a chunk), from a clamped,
 // always valid address, zeroed outside the matrix (no branches per element).
 // Global loads run two chunks ahead of the MFMAs (two register sets).
-template <int TT>
+template <int TT, int WT = 32>
 __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
-  constexpr int NT = (TT / 32) * (TT / 32) * 64;  // threads
+  constexpr int WB = WT / 16;                     // MFMA blocks per wave and dimension
+  constexpr int NT = (TT / WT) * (TT / WT) * 64;  // threads
   constexpr int PER = GKC * TT / NT;              // staged elements per thread and operand
   constexpr int RS = NT / TT;                     // k-row step between a thread's elements
   constexpr int LD = TT + 4;
@@ -380,12 +383,12 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
   double* __restrict__ C = g.C + (size_t)cell * g.sc;
   __shared__ double Xs[GKC][LD];
   __shared__ double Ys[GKC][LD];
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6, wr = w / (TT / 32), wc = w % (TT / 32);
-  d4 acc[2][2];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, wr = w / (TT / WT), wc = w % (TT / WT);
+  d4 acc[WB][WB];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < WB; ++a)
 #pragma unroll
-    for (int bb = 0; bb < 2; ++bb) acc[a][bb] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int bb = 0; bb < WB; ++bb) acc[a][bb] = d4{0.0, 0.0, 0.0, 0.0};
   const int kend = g.k;
   const int kk0 = t / TT, ii = t % TT;
   const bool vx = i0 + ii < g.m, vy = j0 + ii < g.n;
@@ -449,15 +452,16 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
 #pragma unroll
     for (int k4 = 0; k4 < GKC; k4 += 4) {
       const int kr = k4 + (lane >> 4), cc = lane & 15;
-      double xa[2], yb[2];
-      xa[0] = Xs[kr][wr * 32 + cc];
-      xa[1] = Xs[kr][wr * 32 + 16 + cc];
-      yb[0] = Ys[kr][wc * 32 + cc];
-      yb[1] = Ys[kr][wc * 32 + 16 + cc];
+      double xa[WB], yb[WB];
 #pragma unroll
-      for (int bj = 0; bj < 2; ++bj)
+      for (int q = 0; q < WB; ++q) {
+        xa[q] = Xs[kr][wr * WT + q * 16 + cc];
+        yb[q] = Ys[kr][wc * WT + q * 16 + cc];
+      }
 #pragma unroll
-        for (int bi = 0; bi < 2; ++bi)
+      for (int bj = 0; bj < WB; ++bj)
+#pragma unroll
+        for (int bi = 0; bi < WB; ++bi)
           acc[bj][bi] = __builtin_amdgcn_mfma_f64_16x16x4f64(yb[bj], xa[bi], acc[bj][bi], 0, 0, 0);
     }
   };
@@ -470,11 +474,11 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
   if (kn < kend) fetch(xb_, yb_, kn);
   // two-chunk software pipeline, unrolled by two so that the register sets
   // alternate without copies
-  // a wave whose 32x32 sub-tile holds no stored element (all rows >= m, all
+  // a wave whose WT x WT sub-tile holds no stored element (all rows >= m, all
   // columns >= n, or strictly above the diagonal of a lower tile) stages its
   // share but issues no MFMAs, leaving the SIMD's matrix pipe to other waves
-  const int si0 = i0 + wr * 32, sj0 = j0 + wc * 32;
-  const bool busy = si0 < g.m && sj0 < g.n && ((!g.lower && !g.skip_upper) || si0 + 31 >= sj0);
+  const int si0 = i0 + wr * WT, sj0 = j0 + wc * WT;
+  const bool busy = si0 < g.m && sj0 < g.n && ((!g.lower && !g.skip_upper) || si0 + WT - 1 >= sj0);
   while (kc < kend) {
     stage(xa_, ya_, kc, wa);
     int kf = next_chunk(kn);
@@ -494,33 +498,34 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
   }
   // v_mfma_f64_16x16x4 result layout: D[a][b] in lane b + 16*(a%4), element a/4,
   // with a the A-operand (Y, C column j) and b the B-operand (X, C row i) index.
-  // This thread's rows are i0 + wr*32 + bi*16 + (lane & 15) and its columns
-  // j0 + wc*32 + bj*16 + (lane >> 4) + 4v. The epilogue runs in two halves (bj):
+  // This thread's rows are i0 + wr*WT + bi*16 + (lane & 15) and its columns
+  // j0 + wc*WT + bj*16 + (lane >> 4) + 4v. The epilogue runs in WB parts (bj):
   // each issues all its loads (scales, C when beta != 0) from clamped addresses
   // before using any, one memory round trip per half instead of one per element.
-  int ir[2];
-  bool iv[2];
-  double sr[2] = {1.0, 1.0};
+  int ir[WB];
+  bool iv[WB];
+  double sr[WB];
 #pragma unroll
-  for (int bi = 0; bi < 2; ++bi) {
-    ir[bi] = i0 + wr * 32 + bi * 16 + (lane & 15);
+  for (int bi = 0; bi < WB; ++bi) {
+    sr[bi] = 1.0;
+    ir[bi] = i0 + wr * WT + bi * 16 + (lane & 15);
     iv[bi] = ir[bi] < g.m;
     if (g.srow) sr[bi] = g.srow[(size_t)cell * g.ss + (iv[bi] ? ir[bi] : 0)];
   }
 #pragma unroll
-  for (int bj = 0; bj < 2; ++bj) {
+  for (int bj = 0; bj < WB; ++bj) {
     int jc[4];
     bool jv[4];
-    double sc[4] = {1.0, 1.0, 1.0, 1.0}, cold[2][4];
+    double sc[4] = {1.0, 1.0, 1.0, 1.0}, cold[WB][4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      jc[v] = j0 + wc * 32 + bj * 16 + (lane >> 4) + 4 * v;
+      jc[v] = j0 + wc * WT + bj * 16 + (lane >> 4) + 4 * v;
       jv[v] = jc[v] < g.n;
       if (g.scol) sc[v] = g.scol[(size_t)cell * g.ss + (jv[v] ? jc[v] : 0)];
     }
     if (g.beta != 0.0) {
 #pragma unroll
-      for (int bi = 0; bi < 2; ++bi)
+      for (int bi = 0; bi < WB; ++bi)
 #pragma unroll
         for (int v = 0; v < 4; ++v)
           cold[bi][v] = C[(size_t)(iv[bi] ? ir[bi] : 0) + (size_t)(jv[v] ? jc[v] : 0) * g.ldc];
@@ -529,7 +534,7 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
     // the compiler re-waits in every conditional store below, stores included
     __builtin_amdgcn_s_waitcnt(0x0F70);
 #pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
+    for (int bi = 0; bi < WB; ++bi)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int i = ir[bi], j = jc[v];

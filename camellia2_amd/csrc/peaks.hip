@@ -100,15 +100,16 @@ int main() {
   const int cus = prop.multiProcessorCount;
   const int max_blocks = cus * 8;  // up to 8 workgroups of 4 waves per CU
   double* out = nullptr;
-  CHECK(hipMalloc(&out, (size_t)max_blocks * 256 * 8 * 4 * sizeof(double)));
+  CHECK(hipMalloc(&out, (size_t)max_blocks * 256 * 16 * 4 * sizeof(double)));
 
   // fp64 MFMA: 2*16*16*4 flops per instruction and wave
   double mfma_best = 0.0;
   int mfma_wps = 0;
   for (int per_cu : {1, 2, 4, 8}) {
     const int blocks = cus * per_cu, iters = 20000;
-    const float ms = best_ms([&] { mfma_f64<4><<<blocks, 256>>>(out, iters, 1.0, 2.0); });
-    const double tf = (double)blocks * 4 * iters * 4 * 2.0 * 16 * 16 * 4 / (ms * 1e-3) / 1e12;
+    // 16 chains per wave: fewer sustain less here (4: 49, 8: 49, 16: 56 TFLOP/s)
+    const float ms = best_ms([&] { mfma_f64<16><<<blocks, 256>>>(out, iters / 4, 1.0, 2.0); });
+    const double tf = (double)blocks * 4 * (iters / 4) * 16 * 2.0 * 16 * 16 * 4 / (ms * 1e-3) / 1e12;
     if (tf > mfma_best) {
       mfma_best = tf;
       mfma_wps = per_cu;  // waves per SIMD: 4 waves per workgroup over 4 SIMDs
