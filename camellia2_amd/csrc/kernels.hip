@@ -381,8 +381,12 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
   tile_coords(tile, g.lower, g.tiles_m, ti, tj);
   const int i0 = ti * TT, j0 = tj * TT;
   double* __restrict__ C = g.C + (size_t)cell * g.sc;
-  __shared__ double Xs[GKC][LD];
-  __shared__ double Ys[GKC][LD];
+  // two buffers for the 64-tile (chunk i is read from one while chunk i+1 is
+  // written to the other); one for the 32-tile of one wave, whose small
+  // workgroups would otherwise be limited by LDS rather than by registers
+  constexpr int NBUF = TT >= 64 ? 2 : 1;
+  __shared__ double Xs[NBUF][GKC][LD];
+  __shared__ double Ys[NBUF][GKC][LD];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, wr = w / (TT / WT), wc = w % (TT / WT);
   d4 acc[WB][WB];
 #pragma unroll
@@ -438,25 +442,26 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
   };
   // weight of the segment the walker is in (kw: merged equal terms), 1 without
   auto segw = [&]() { return (g.kw && seg < g.nseg) ? g.kw[(size_t)tile * g.nseg + seg] : 1.0; };
-  auto stage = [&](const double (&xr)[PER], const double (&yr)[PER], int k0, double wx) {
-    __syncthreads();  // every wave is done reading the previous chunk
+  auto stage = [&](const double (&xr)[PER], const double (&yr)[PER], int k0, double wx, int buf) {
+    buf &= NBUF - 1;
+    if constexpr (NBUF == 1) __syncthreads();  // every wave is done reading the buffer
 #pragma unroll
     for (int r = 0; r < PER; ++r) {
       const bool vk = k0 + kk0 + r * RS < kend;
-      Xs[kk0 + r * RS][ii] = (vx && vk) ? xr[r] * wx : 0.0;
-      Ys[kk0 + r * RS][ii] = (vy && vk) ? yr[r] : 0.0;
+      Xs[buf][kk0 + r * RS][ii] = (vx && vk) ? xr[r] * wx : 0.0;
+      Ys[buf][kk0 + r * RS][ii] = (vy && vk) ? yr[r] : 0.0;
     }
-    __syncthreads();
   };
-  auto multiply = [&]() {
+  auto multiply = [&](int buf) {
+    buf &= NBUF - 1;
 #pragma unroll
     for (int k4 = 0; k4 < GKC; k4 += 4) {
       const int kr = k4 + (lane >> 4), cc = lane & 15;
       double xa[WB], yb[WB];
 #pragma unroll
       for (int q = 0; q < WB; ++q) {
-        xa[q] = Xs[kr][wr * WT + q * 16 + cc];
-        yb[q] = Ys[kr][wc * WT + q * 16 + cc];
+        xa[q] = Xs[buf][kr][wr * WT + q * 16 + cc];
+        yb[q] = Ys[buf][kr][wc * WT + q * 16 + cc];
       }
 #pragma unroll
       for (int bj = 0; bj < WB; ++bj)
@@ -472,29 +477,39 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
   double wb = segw();
   if (kc < kend) fetch(xa_, ya_, kc);
   if (kn < kend) fetch(xb_, yb_, kn);
-  // two-chunk software pipeline, unrolled by two so that the register sets
-  // alternate without copies
+  // software pipeline, unrolled by two so that the register sets and the LDS
+  // buffers alternate without copies: while chunk i is multiplied from one
+  // buffer, chunk i+1 goes from registers to the other and chunk i+2 is loaded
+  // into the registers chunk i+1 left; one barrier per chunk
   // a wave whose WT x WT sub-tile holds no stored element (all rows >= m, all
   // columns >= n, or strictly above the diagonal of a lower tile) stages its
   // share but issues no MFMAs, leaving the SIMD's matrix pipe to other waves
   const int si0 = i0 + wr * WT, sj0 = j0 + wc * WT;
   const bool busy = si0 < g.m && sj0 < g.n && ((!g.lower && !g.skip_upper) || si0 + WT - 1 >= sj0);
-  while (kc < kend) {
-    stage(xa_, ya_, kc, wa);
-    int kf = next_chunk(kn);
+  if (kc < kend) {
+    stage(xa_, ya_, kc, wa, 0);
+    int kf = next_chunk(kn);  // buffer 0: kc; set b: kn; set a: kf
     wa = segw();
     if (kf < kend) fetch(xa_, ya_, kf);
-    if (busy) multiply();
-    kc = kn;
-    kn = kf;
-    if (kc >= kend) break;
-    stage(xb_, yb_, kc, wb);
-    kf = next_chunk(kn);
-    wb = segw();
-    if (kf < kend) fetch(xb_, yb_, kf);
-    if (busy) multiply();
-    kc = kn;
-    kn = kf;
+    __syncthreads();
+    while (true) {
+      if (busy) multiply(0);
+      if (kn >= kend) break;
+      stage(xb_, yb_, kn, wb, 1);
+      int kg = next_chunk(kf);
+      wb = segw();
+      if (kg < kend) fetch(xb_, yb_, kg);
+      __syncthreads();  // buffer 1: kn; set a: kf; set b: kg
+      if (busy) multiply(1);
+      if (kf >= kend) break;
+      stage(xa_, ya_, kf, wa, 0);
+      int kh = next_chunk(kg);
+      wa = segw();
+      if (kh < kend) fetch(xa_, ya_, kh);
+      __syncthreads();  // buffer 0: kf; set b: kg; set a: kh
+      kn = kg;
+      kf = kh;
+    }
   }
   // v_mfma_f64_16x16x4 result layout: D[a][b] in lane b + 16*(a%4), element a/4,
   // with a the A-operand (Y, C column j) and b the B-operand (X, C row i) index.
@@ -575,7 +590,7 @@ __device__ inline void gemm_tn_f64_wide_body(const GemmArgs& g) {
   constexpr int PX = GKC * TM / NT, RX = NT / TM;  // X: 8 elements per thread, k-rows t/128 + 2r
   constexpr int PY = GKC * TN / NT, RY = NT / TN;  // Y: 4 elements per thread, k-rows t/64 + 4r
   constexpr int LXS = TM + 4, LYS = TN + 4, LP = 68;  // LDS row pitches (L: one column of the factor per row)
-  constexpr int NSTAGE = GKC * (LXS + LYS), NL = SOLVE ? 64 * LP : 0;
+  constexpr int NSTAGE = 2 * GKC * (LXS + LYS), NL = SOLVE ? 64 * LP : 0;  // two staging buffers
   int b = xcd_remap(blockIdx.x, gridDim.x);
   int cell = b / g.ntiles, tile = b % g.ntiles;
   if (cell >= g.ncells) return;
@@ -583,8 +598,8 @@ __device__ inline void gemm_tn_f64_wide_body(const GemmArgs& g) {
   const int i0 = ti * TM, j0 = tj * TN;
   double* __restrict__ C = g.C + (size_t)cell * g.sc;
   __shared__ double smem[NSTAGE > NL ? NSTAGE : NL];
-  double (*Xs)[LXS] = reinterpret_cast<double (*)[LXS]>(smem);
-  double (*Ys)[LYS] = reinterpret_cast<double (*)[LYS]>(smem + GKC * LXS);
+  double (*Xs)[GKC][LXS] = reinterpret_cast<double (*)[GKC][LXS]>(smem);
+  double (*Ys)[GKC][LYS] = reinterpret_cast<double (*)[GKC][LYS]>(smem + 2 * GKC * LXS);
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   d4 acc[4][2];
 #pragma unroll
@@ -610,23 +625,21 @@ __device__ inline void gemm_tn_f64_wide_body(const GemmArgs& g) {
       for (int r = 0; r < PY; ++r) yr[r] = yp[(min(k0 + ky0 + r * RY, kend - 1) - ky0) * ldy];
     }
   };
-  auto stage = [&](const double (&xr)[PX], const double (&yr)[PY], int k0) {
-    __syncthreads();
+  auto stage = [&](const double (&xr)[PX], const double (&yr)[PY], int k0, int buf) {
 #pragma unroll
-    for (int r = 0; r < PX; ++r) Xs[kx0 + r * RX][ix] = (vx && k0 + kx0 + r * RX < kend) ? xr[r] : 0.0;
+    for (int r = 0; r < PX; ++r) Xs[buf][kx0 + r * RX][ix] = (vx && k0 + kx0 + r * RX < kend) ? xr[r] : 0.0;
 #pragma unroll
-    for (int r = 0; r < PY; ++r) Ys[ky0 + r * RY][iy] = (vy && k0 + ky0 + r * RY < kend) ? yr[r] : 0.0;
-    __syncthreads();
+    for (int r = 0; r < PY; ++r) Ys[buf][ky0 + r * RY][iy] = (vy && k0 + ky0 + r * RY < kend) ? yr[r] : 0.0;
   };
-  auto multiply = [&]() {
+  auto multiply = [&](int buf) {
 #pragma unroll
     for (int k4 = 0; k4 < GKC; k4 += 4) {
       const int kr = k4 + (lane >> 4), cc = lane & 15;
       double xa[2], yb[4];
-      xa[0] = Xs[kr][w * 32 + cc];
-      xa[1] = Xs[kr][w * 32 + 16 + cc];
+      xa[0] = Xs[buf][kr][w * 32 + cc];
+      xa[1] = Xs[buf][kr][w * 32 + 16 + cc];
 #pragma unroll
-      for (int bj = 0; bj < 4; ++bj) yb[bj] = Ys[kr][bj * 16 + cc];
+      for (int bj = 0; bj < 4; ++bj) yb[bj] = Ys[buf][kr][bj * 16 + cc];
 #pragma unroll
       for (int bj = 0; bj < 4; ++bj)
 #pragma unroll
@@ -640,20 +653,29 @@ __device__ inline void gemm_tn_f64_wide_body(const GemmArgs& g) {
   if (kn < kend) fetch(xb_, yb_, kn);
   const int si0 = i0 + w * 32;
   const bool busy = si0 < g.m && j0 < g.n && (!g.skip_upper || si0 + 31 >= j0);
-  while (kc < kend) {
-    stage(xa_, ya_, kc);
-    int kf = kn + GKC;
+  // two LDS buffers and two register sets, one barrier per chunk (as in the
+  // square tile)
+  if (kc < kend) {
+    stage(xa_, ya_, kc, 0);
+    int kf = kn + GKC;  // buffer 0: kc; set b: kn; set a: kf
     if (kf < kend) fetch(xa_, ya_, kf);
-    if (busy) multiply();
-    kc = kn;
-    kn = kf;
-    if (kc >= kend) break;
-    stage(xb_, yb_, kc);
-    kf = kn + GKC;
-    if (kf < kend) fetch(xb_, yb_, kf);
-    if (busy) multiply();
-    kc = kn;
-    kn = kf;
+    __syncthreads();
+    while (true) {
+      if (busy) multiply(0);
+      if (kn >= kend) break;
+      stage(xb_, yb_, kn, 1);
+      const int kg = kf + GKC;
+      if (kg < kend) fetch(xb_, yb_, kg);
+      __syncthreads();  // buffer 1: kn; set a: kf; set b: kg
+      if (busy) multiply(1);
+      if (kf >= kend) break;
+      stage(xa_, ya_, kf, 0);
+      const int kh = kg + GKC;
+      if (kh < kend) fetch(xa_, ya_, kh);
+      __syncthreads();  // buffer 0: kf; set b: kg; set a: kh
+      kn = kg;
+      kf = kh;
+    }
   }
   // epilogue as in the square tile, in four column quarters (bj)
   int ir[2];
