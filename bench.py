@@ -829,8 +829,10 @@ def main():
     ap.add_argument("--tiles", default="",
                     help="GEMM tiles (64 or 32; 128 for the updates) for the Gram, B and Cholesky-update contractions, e.g. 64,32,128 "
                          "(single-type configurations; default: chosen by the plan autotuner)")
-    ap.add_argument("--sumfact", action="store_true",
-                    help="opt-in sum-factorised G (fewer flops, K accurate only to ~1e-12..1e-1; not the default)")
+    ap.add_argument("--gram", default="auto", choices=["auto", "dense", "sumfact"],
+                    help="G from the dense MFMA Gram or the sum-factorised kernel (tensor cells); auto: the plan "
+                         "autotuner's choice, which takes the sum-factorised G only where it is faster and moves K "
+                         "no more than a second rounding of the dense G does")
     ap.add_argument("--no-peaks", action="store_true", help="skip the peak-rate microbenchmark")
     ap.add_argument("--no-assembly", action="store_true",
                     help="skip the condensed global system (numbering, CSR pattern, assembly)")
@@ -868,16 +870,18 @@ def main():
     A = dpg.LocalAssembler(f, [b], workspace_bytes=ws, precision=PRECISIONS[args.precision])
     # GEMM tiles: given, or the plan's autotuner on the first cells (untimed set-up)
     t_tune = None
+    sf = 0
+    if args.gram == "auto" or not args.tiles:
+        t0 = time.perf_counter()
+        tuned = A.autotune(0, d_nodes, d_par, None, load, sample=min(C, 512))
+        t_tune = round(time.perf_counter() - t0, 2)
+        tiles, sf = list(tuned[:3]), int(tuned[3])
+        progress(f"autotuner chose tiles {tiles}, sum-factorised G {sf}")
     if args.tiles:
         tiles = [int(t) for t in args.tiles.split(",")]
-    elif args.sumfact:
-        tiles = list(A.tiles(0)[:3])
-    else:
-        t0 = time.perf_counter()
-        tiles = list(A.autotune(0, d_nodes, d_par, None, load, sample=min(C, 512))[:3])
-        t_tune = round(time.perf_counter() - t0, 2)
-        progress(f"autotuner chose tiles {tiles}")
-    A.set_tiles(0, *tiles, 1 if args.sumfact else 0)
+    if args.gram != "auto":
+        sf = 1 if args.gram == "sumfact" else 0
+    A.set_tiles(0, *tiles, sf)
     out = {
         "K": torch.empty((C, info.M, info.M), dtype=torch.float64, device=dev),
         "rhs": torch.empty((C, info.M), dtype=torch.float64, device=dev),
@@ -907,7 +911,7 @@ def main():
     # one more step on a plan that times every launch with HIP events on the
     # launch stream (kept out of the timed steps above)
     Ap = dpg.LocalAssembler(f, [b], workspace_bytes=ws, profile=True, precision=PRECISIONS[args.precision])
-    Ap.set_tiles(0, *tiles, 1 if args.sumfact else 0)
+    Ap.set_tiles(0, *tiles, sf)
     step(Ap)
     torch.cuda.synchronize()
     tm, _ = Ap.timings(reset=True)
@@ -946,10 +950,10 @@ def main():
         "data": "synthetic (seeded jittered mesh and sine-mode load)",
         "config": {"workload": cfg["name"], "cells": C, "N": info.N, "M": info.M, "T": info.T,
                    "outputs": "K,rhs" + (",schur,schur_rhs" if want_schur else ""),
-                   "precision": args.precision, "gram": "sum-factorised (opt-in)" if args.sumfact else "dense",
+                   "precision": args.precision,
+                   "gram": ("sum-factorised" if sf else "dense") + (" (autotuner)" if args.gram == "auto" else ""),
                    "tiles": {"gram": tiles[0], "bmat": tiles[1], "update": tiles[2],
-                             "chosen_by": "--tiles" if args.tiles else ("plan default" if args.sumfact else
-                                                                         f"autotuner ({t_tune} s, untimed)")}},
+                             "chosen_by": "--tiles" if args.tiles else f"autotuner ({t_tune} s, untimed)"}},
         "gflops": round(info.flops_per_cell * cells_per_s / 1e9, 1),
         "algorithmic_flop_per_cell": info.flops_per_cell,
         "roofline": roofline,
@@ -972,7 +976,7 @@ def main():
         return h
 
     Ah = dpg.LocalAssembler(f, [b], workspace_bytes=ws, precision=PRECISIONS[args.precision])
-    Ah.set_tiles(0, *tiles, 1 if args.sumfact else 0)
+    Ah.set_tiles(0, *tiles, sf)
     heads = {args.precision: heads_of(Ah)}
     Ah.close()
     # mixed-precision forms of the same step, timed the same way, with their

@@ -1914,6 +1914,36 @@ __global__ __launch_bounds__(256) void adjoint_kernel(const DevProgram* __restri
   }
 }
 
+// Per cell c of n doubles each: out[2c] = sum (a - b)^2, out[2c+1] = sum b^2
+// (one workgroup per cell, a fixed reduction tree).
+__global__ __launch_bounds__(256) void cell_diff_norms(const double* __restrict__ a, const double* __restrict__ b,
+                                                       int64_t n, double* __restrict__ out) {
+  __shared__ double sd[256], sb[256];
+  const int c = blockIdx.x, t = threadIdx.x;
+  const double* ac = a + (size_t)c * n;
+  const double* bc = b + (size_t)c * n;
+  double d2 = 0.0, b2 = 0.0;
+  for (int64_t i = t; i < n; i += 256) {
+    const double d = ac[i] - bc[i];
+    d2 = fma(d, d, d2);
+    b2 = fma(bc[i], bc[i], b2);
+  }
+  sd[t] = d2;
+  sb[t] = b2;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h) {
+      sd[t] += sd[t + h];
+      sb[t] += sb[t + h];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    out[2 * (size_t)c] = sd[0];
+    out[2 * (size_t)c + 1] = sb[0];
+  }
+}
+
 // ------------------------------------------------------------ host launchers
 static void check_launch(const char* what) {
   hipError_t e = hipGetLastError();
@@ -2225,6 +2255,12 @@ void launch_adjoint(int dim, const DevProgram* prog, const DevTables& tab, int N
     adjoint_kernel<2><<<ncells, 256, lds, st>>>(prog, tab, N, M, QG, Q, Qs, nsides, nodes, par, coefs, X, sX, ncol,
                                                 nT, cy, ce, Kbar, rbar, u, grad, npw);
   check_launch("adjoint_kernel");
+}
+
+void launch_cell_diff_norms(const double* a, const double* b, int64_t n, int ncells, double* out, hipStream_t st) {
+  if (ncells <= 0) return;
+  cell_diff_norms<<<ncells, 256, 0, st>>>(a, b, n, out);
+  check_launch("cell_diff_norms");
 }
 
 void launch_extract_row(const double* A, int64_t lda, int64_t sA, int row, int r0, int n, double sign, double* out,

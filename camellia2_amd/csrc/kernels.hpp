@@ -130,6 +130,8 @@ void launch_scaling(double* A, int64_t lda, int64_t sA, int r0, int D, int n, in
                     int ncells, double sign, hipStream_t st);
 void launch_residual_row(double* A, int64_t lda, int64_t sA, int N, int M, int r0, int row_l, int row_r,
                          const double* u, int ncells, hipStream_t st);
+// per cell of n doubles: out[2c] = |a_c - b_c|^2, out[2c+1] = |b_c|^2
+void launch_cell_diff_norms(const double* a, const double* b, int64_t n, int ncells, double* out, hipStream_t st);
 void launch_extract_sym(const double* A, int64_t lda, int64_t sA, int r0, int n, double sign, double* out, int ncells,
                         hipStream_t st);
 // field recovery after a condensed solve: S K_FF S and the row s (rhs_F - K_FT u_T) into A

@@ -226,7 +226,8 @@ struct BucketPlan {
   const double* gweight[2] = {nullptr, nullptr};         // ... and the merged segments' weights
   // sum-factorised Gram (tensor-product cells, FP64): block pairs, pairs, 1D tables
   bool sf_ok = false;   // available for this bucket
-  bool sf_on = false;   // used (opt-in)
+  bool sf_on = false;   // used (dpg_plan_set_tiles, or the autotuner's accuracy-gated choice)
+  bool merge_off = false;  // dense G without merged equal terms (the autotuner's second rounding of G)
   int sf_nbp = 0, sf_nF = 0, sf_q = 0;
   size_t sf_lds = 0;
   const SfBlockPair* sf_bps = nullptr;
@@ -494,7 +495,7 @@ void build_sum_factorisation(dpg_plan* plan, BucketPlan& B) {
   B.sf_q = q;
   B.sf_lds = lds;
   B.sf_ok = true;
-  B.sf_on = false;  // opt-in only (dpg_plan_set_tiles): see the accuracy note on dpg_tiles.sumfact
+  B.sf_on = false;  // off until dpg_plan_set_tiles or dpg_plan_autotune switches it on (see dpg_tiles.sumfact)
 }
 
 void build_bucket(dpg_plan* plan, const dpg_bucket_desc& desc, BucketPlan& B) {
@@ -886,8 +887,9 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
   g.srow = s_ep;
   g.scol = s_ep;
   g.ss = B.nS;
-  g.kmask = B.gmask_merged[B.tile[0] == 64 ? 0 : 1];
-  g.kw = B.gweight[B.tile[0] == 64 ? 0 : 1];
+  const int ts = B.tile[0] == 64 ? 0 : 1;
+  g.kmask = B.merge_off ? B.gmask[ts] : B.gmask_merged[ts];
+  g.kw = B.merge_off ? nullptr : B.gweight[ts];
   g.nseg = B.prog.n_lt;
   g.kseg = B.QGp;
   if (prec != DPG_PREC_FP64 && !i8) {
@@ -1040,6 +1042,58 @@ void assemble_batch(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, in
       launch_extract_row(w.A, B.lda, B.nA, N + M, N + F, T, 1.0, o.schur_rhs + (size_t)c0 * T, nb, st);
     plan->prof.stop(e, st, DPG_STAGE_EXTRACT, 0.0, nb * ((double)T * T * 2.0 + T * 2.0) * 8.0);
   }
+}
+
+// The sum-factorised G (tensor-product cells) against the dense one, on the
+// first cells of a batch, with the tiles already chosen: it is switched on when
+// it is faster over the whole pipeline and moves K no more than 4 times what a
+// second rounding of the dense G does (the same G with its equal IP terms summed
+// separately instead of merged, i.e. another FP64 summation order of the same
+// products). On the 32^3-hex workload it passes (K 9e-11 from an 80-bit
+// evaluation against 6.6e-10 for the dense G: far fewer roundings per entry);
+// where expanding the graph norm's squares into separate metric terms cancels
+// (2D Stokes at h = 1/128: K 1e-1 off) it fails and G stays dense.
+void choose_gram_path(dpg_plan* plan, BucketPlan& B, const dpg_batch& bt, int ns, const Work& w, hipStream_t st) {
+  const int N = B.L.N, M = B.L.M;
+  const int ng = std::min(ns, 128);
+  const int64_t mm = (int64_t)M * M;
+  DeviceBuffer kd, kx, nrm;
+  kd.ensure(sizeof(double) * ng * mm);
+  kx.ensure(sizeof(double) * ng * mm);
+  nrm.ensure(sizeof(double) * 2 * ng);
+  std::vector<double> h(2 * (size_t)ng);
+  auto run = [&](double* K) {  // warm pass, timed pass; K of the timed pass
+    double t = 0.0;
+    for (int pass = 0; pass < 2; ++pass) {
+      plan->prof.acc = dpg_timings{};
+      stage_factor(plan, B, bt, 0, ng, w, nullptr, nullptr, st);
+      launch_extract_sym(w.A, B.lda, B.nA, N, M, -1.0, K, ng, st);
+      stage_schur(plan, B, ng, w, st);
+      plan->prof.collect(st, ng);
+      t = 0.0;
+      for (int k = 0; k < DPG_NSTAGES; ++k) t += plan->prof.acc.ms[k];
+    }
+    return t;
+  };
+  auto worst = [&]() {  // max over cells of |K_x - K_d| / |K_d|
+    launch_cell_diff_norms((const double*)kx.p, (const double*)kd.p, mm, ng, (double*)nrm.p, st);
+    HIP_CHECK(hipMemcpyAsync(h.data(), nrm.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    double e = 0.0;
+    for (int c = 0; c < ng; ++c) e = std::max(e, std::sqrt(h[2 * c] / std::max(h[2 * c + 1], 1e-300)));
+    return e;
+  };
+  B.sf_on = false;
+  B.merge_off = false;
+  const double t_dense = run((double*)kd.p);
+  B.merge_off = true;
+  run((double*)kx.p);
+  B.merge_off = false;
+  const double e_dense = worst();
+  B.sf_on = true;
+  const double t_sf = run((double*)kx.p);
+  const double e_sf = worst();
+  B.sf_on = t_sf < t_dense && e_sf <= 4.0 * e_dense + 1e-15;
 }
 
 void check_batch(const dpg_plan* plan, int bucket, const dpg_batch* batch) {
@@ -1488,7 +1542,9 @@ int dpg_plan_autotune(dpg_plan* plan, int bucket, const dpg_batch* batch, int sa
     // candidate tile edges per role: 64 and 32 for all, and for the updates the
     // 128 x 64 panel-update tile
     const int cand[3][3] = {{64, 64, 64}, {32, 32, 32}, {64, 64, 128}};
-    // the G path (dense or the opt-in sum-factorised one) is kept as set
+    // tiles are timed on the dense G; then, for tensor-product cells in FP64,
+    // the sum-factorised G is tried against it (choose_gram_path)
+    B.sf_on = false;
     double ms[3][3] = {{0}};
     try {
       for (int c = 0; c < 3; ++c) {
@@ -1503,16 +1559,18 @@ int dpg_plan_autotune(dpg_plan* plan, int bucket, const dpg_batch* batch, int sa
         // the update tile also decides how the panel solve runs (fused or not)
         ms[c][2] += plan->prof.acc.ms[DPG_STAGE_PANEL] + plan->prof.acc.ms[DPG_STAGE_SOLVE];
       }
+      for (int r = 0; r < 2; ++r) B.tile[r] = ms[1][r] < ms[0][r] ? 32 : 64;
+      int best = 0;
+      for (int c = 1; c < 3; ++c)
+        if (ms[c][2] < ms[best][2]) best = c;
+      B.tile[2] = cand[best][2];
+      if (B.sf_ok && plan->opt.precision == DPG_PREC_FP64) dpg::choose_gram_path(plan, B, *batch, ns, w, st);
     } catch (...) {
+      B.merge_off = false;
       plan->prof.on = on;
       plan->prof.acc = saved;
       throw;
     }
-    for (int r = 0; r < 2; ++r) B.tile[r] = ms[1][r] < ms[0][r] ? 32 : 64;
-    int best = 0;
-    for (int c = 1; c < 3; ++c)
-      if (ms[c][2] < ms[best][2]) best = c;
-    B.tile[2] = cand[best][2];
     plan->prof.on = on;
     plan->prof.acc = saved;
   });

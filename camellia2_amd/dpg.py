@@ -190,7 +190,9 @@ class LocalAssembler:
         check(lib().dpg_plan_set_tiles(self._plan, bucket, ct.byref(capi.Tiles(gram, bmat, update, sumfact))))
 
     def autotune(self, bucket, nodes, parities, coefs=None, load=None, sample=256, stream=None):
-        """Times both tile edges per GEMM role on `sample` cells and keeps the faster; returns the choice."""
+        """Times both tile edges per GEMM role on `sample` cells and keeps the faster, then takes the
+        sum-factorised G where it is faster and accurate (dpg_plan_autotune); returns the choice
+        (gram, bmat, update, sumfact)."""
         t = self._torch
         b = self._batch(nodes, parities, coefs, load)
         s = stream.cuda_stream if stream is not None else t.cuda.current_stream().cuda_stream

@@ -365,11 +365,13 @@ int dpg_adjoint(dpg_plan* plan, int bucket, const dpg_batch* batch, const double
 
 /* FP64 contraction choices of one bucket: GEMM tile edge (64 or 32) per role,
  * and whether G comes from the sum-factorised kernel (quads and hexes only).
- * sumfact = 1 is opt-in: it contracts G = sum_q M(q) R R direction by direction
- * (about 60x fewer flops than A^T A on the 32^3 hex workload) but, no longer a
- * Gram product of computed vectors, its rounding is amplified by G's
- * conditioning: K moves by 2e-12 (32^3-hex orders) up to 1e-1 (Stokes at h =
- * 1/128) from the dense result. The autotuner never switches it on. */
+ * sumfact = 1 contracts G = sum_q M(q) R R direction by direction (about 60x
+ * fewer flops than A^T A on the 32^3 hex workload). Its accuracy depends on the
+ * form and the mesh: on the 32^3 hexes K is 9e-11 from an 80-bit evaluation
+ * (the dense G: 6.6e-10; far fewer roundings per entry), but where expanding
+ * the graph norm's squares into separate metric terms cancels it is far worse
+ * (2D Stokes at h = 1/128: K 1e-1 from the dense result). dpg_plan_autotune
+ * switches it on only where it is faster and accurate (see there). */
 typedef struct {
   int32_t gram;    /* G = A^T A (dense path)                        */
   int32_t bmat;    /* B blocks                                      */
@@ -378,15 +380,20 @@ typedef struct {
                       and solved against it in one kernel on
                       128 x 64 tiles (the diagonal blocks and the
                       trailing update then take 64)                 */
-  int32_t sumfact; /* 1: sum-factorised G (opt-in, see above)       */
+  int32_t sumfact; /* 1: sum-factorised G (see above)               */
 } dpg_tiles;
 int dpg_plan_get_tiles(const dpg_plan* plan, int bucket, dpg_tiles* out);
 int dpg_plan_set_tiles(dpg_plan* plan, int bucket, const dpg_tiles* tiles);
 /* Autotuner: times both tile edges for every role (and the 128 x 64 tile for
  * the updates) on the first `sample` cells of `batch` (HIP events, one warm-up
- * pass each) and keeps the fastest per role.
- * Tile edges never change results (every output accumulates in the same k
- * order). The G path is left as set. Blocks until done; `stream` runs the timings. */
+ * pass each) and keeps the fastest per role. Tile edges never change results
+ * (every output accumulates in the same k order).
+ * Then, for quads and hexes in FP64, the G path: the sum-factorised G is taken
+ * when the whole pipeline runs faster with it and, on the first (at most 128)
+ * sample cells, it moves K by no more than 4 times what a second rounding of
+ * the dense G does (the dense G with its equal IP terms summed separately
+ * instead of merged: the same products in another FP64 order); otherwise G is
+ * dense. Blocks until done; `stream` runs the timings. */
 int dpg_plan_autotune(dpg_plan* plan, int bucket, const dpg_batch* batch, int sample, void* stream);
 
 /* one element type's share of a mesh: its cells, coefficients and outputs */
