@@ -89,14 +89,24 @@ __device__ inline void op_to_prims(int op, const PointGeo<DIM>& g, const double*
   if (op == DPG_OP_VALUE) {
     t[0] = 1.0;
   } else if (op >= DPG_OP_DX && op < DPG_OP_DX + DIM) {
-    int d = op - DPG_OP_DX;
+    // the direction selected among compile-time indices: a run-time index into
+    // g would put the whole geometry in scratch memory
+    const int d = op - DPG_OP_DX;
 #pragma unroll
-    for (int e = 0; e < DIM; ++e) t[1 + e] = g.Jinv[e][d];
+    for (int dd = 0; dd < DIM; ++dd)
+      if (dd == d) {
+#pragma unroll
+        for (int e = 0; e < DIM; ++e) t[1 + e] = g.Jinv[e][dd];
+      }
   } else if (op >= DPG_OP_X && op < DPG_OP_X + DIM) {
-    int d = op - DPG_OP_X;
+    const int d = op - DPG_OP_X;
     double id = 1.0 / g.detJ;
 #pragma unroll
-    for (int e = 0; e < DIM; ++e) t[e] = g.J[d][e] * id;
+    for (int dd = 0; dd < DIM; ++dd)
+      if (dd == d) {
+#pragma unroll
+        for (int e = 0; e < DIM; ++e) t[e] = g.J[dd][e] * id;
+      }
   } else if (op == DPG_OP_DIV) {
     t[DIM] = 1.0 / g.detJ;
   } else if (op == DPG_OP_DOT_NORMAL && n) {
@@ -1610,6 +1620,21 @@ __device__ inline double dcoef_value(const DSummand& sm, int k, const double* cc
   return v;
 }
 
+// The same with the side normal held in registers (n0, n1, n2 selected, never
+// addressed: a pointer to a local array would put it in scratch memory)
+__device__ inline double normal_comp(bool hasn, double n0, double n1, double n2, int i) {
+  return hasn ? (i == 0 ? n0 : (i == 1 ? n1 : n2)) : 0.0;
+}
+__device__ inline double dcoef_value_n(const DSummand& sm, int k, const double* cc, bool hasn, double n0, double n1,
+                                       double n2) {
+  if (sm.slot != k) return 0.0;
+  double v = sm.scale;
+  double a = cc[k];
+  v *= (sm.fn == DPG_FN_SQRT) ? 0.5 / sqrt(a) : (sm.fn == DPG_FN_INV ? -1.0 / (a * a) : 1.0);
+  if (sm.normal >= 0) v *= normal_comp(hasn, n0, n1, n2, sm.normal);
+  return v;
+}
+
 // ------------------------------------------------- stored factor, adjoint
 // The factored first N columns of a cell's augmented matrix as a row-major
 // (N + M + 1) x N block F: rows 0..N-1 the lower factor Lt of S G S = Lt Lt^T
@@ -1791,7 +1816,7 @@ __global__ __launch_bounds__(256) void adjoint_kernel(const DevProgram* __restri
       for (int d = 0; d < DIM; ++d) n[d] /= len;
       wm = tab.w_side[p] * len;
     }
-    const double* nn = kind == 2 ? n : nullptr;
+    const bool hasn = kind == 2;
     // reference prims of the column functions at this point
     wave_lds_sync();
     for (int m = lane; m < ncol; m += 64)
@@ -1810,7 +1835,17 @@ __global__ __launch_bounds__(256) void adjoint_kernel(const DevProgram* __restri
     wave_lds_sync();
     double tr[NPR];
     auto op_col = [&](const DSummand& sm, int m) {
-      op_to_prims<DIM>(sm.op, g, nn, tr);
+      op_to_prims<DIM>(sm.op, g, nullptr, tr);
+      if (hasn && sm.op == DPG_OP_DOT_NORMAL) {  // (J n)^T / detJ with the normal from registers
+        const double id = 1.0 / g.detJ;
+#pragma unroll
+        for (int e = 0; e < DIM; ++e) {
+          double sn = n[0] * g.J[0][e];
+#pragma unroll
+          for (int d = 1; d < DIM; ++d) sn += n[d] * g.J[d][e];
+          tr[e] = sn * id;
+        }
+      }
       double v = 0.0;
 #pragma unroll
       for (int r = 0; r < NPR; ++r) v = fma(tr[r], Z[((size_t)sm.var * NPR + r) * ncol + m], v);
@@ -1840,16 +1875,22 @@ __global__ __launch_bounds__(256) void adjoint_kernel(const DevProgram* __restri
           } else if (m == ce) {
             h = -Pl[ce];
           }
+          // d/dcoef_k of the term's value, every k in one pass over the summands
+          // (each k's sum keeps the summand order)
+          double dv[DPG_MAX_COEFS];
 #pragma unroll
-          for (int k = 0; k < DPG_MAX_COEFS; ++k) {
-            if (k >= ncoef) continue;
-            double dv = 0.0;
-            for (int si = prog->lt_beg[lt]; si < prog->lt_beg[lt + 1]; ++si) {
-              const DSummand& sm = prog->s[si];
-              if (sm.slot == k) dv = fma(dcoef_value(sm, k, cc, nullptr), op_col(sm, m), dv);
-            }
-            acc[k] = fma(2.0 * wm * dv, h, acc[k]);
+          for (int k = 0; k < DPG_MAX_COEFS; ++k) dv[k] = 0.0;
+          for (int si = prog->lt_beg[lt]; si < prog->lt_beg[lt + 1]; ++si) {
+            const DSummand& sm = prog->s[si];
+            if (sm.slot < 0) continue;
+            const double d = dcoef_value(sm, sm.slot, cc, nullptr), o = op_col(sm, m);
+#pragma unroll
+            for (int k = 0; k < DPG_MAX_COEFS; ++k)
+              if (k == sm.slot) dv[k] = fma(d, o, dv[k]);
           }
+#pragma unroll
+          for (int k = 0; k < DPG_MAX_COEFS; ++k)
+            if (k < ncoef) acc[k] = fma(2.0 * wm * dv[k], h, acc[k]);
         }
         wave_lds_sync();
       }
@@ -1891,7 +1932,8 @@ __global__ __launch_bounds__(256) void adjoint_kernel(const DevProgram* __restri
           for (int si = sb; si < se; ++si) {
             const DSummand& sm = prog->s[si];
             if (sm.slot < 0) continue;
-            const double d = dcoef_value(sm, sm.slot, cc, nn) * op_col(sm, m) * gb;
+            const double d = dcoef_value_n(sm, sm.slot, cc, hasn, n[0], n[1], DIM == 3 ? n[2] : 0.0) *
+                             op_col(sm, m) * gb;
 #pragma unroll
             for (int k = 0; k < DPG_MAX_COEFS; ++k)
               if (k == sm.slot) acc[k] += d;

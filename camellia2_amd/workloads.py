@@ -1,8 +1,9 @@
 """Seeded synthetic workloads shared by the benchmark and the tests.
 
-hp convection-diffusion: the graded quad mesh of dpg_mesh_quad_graded with a
-band of triangles (the level-0 quads of the first coarse columns, each split
-along its diagonal) and a per-cell field degree p and test enrichment dp;
+hp convection-diffusion: the graded quad mesh of dpg_mesh_quad_graded with
+triangles (the level-0 quads of the first coarse columns and every quad of
+the first coarse rows, at all refinement levels, each split along its
+diagonal) and a per-cell field degree p and test enrichment dp;
 cells of equal (topology, p, dp) form one element type (bucket), as
 Camellia's ElementType keys them (GlobalDofAssignment.cpp:615-644: trial
 order p+1, test order p+dp)."""
@@ -28,9 +29,11 @@ def split_quads(nodes, par):
     return tri, tp
 
 
-def cd_hp(n0=16, levels=10, seed=SEED_C3, jitter=0.05, pmax=6, dpmax=3, tri_cols=4):
-    """hp-adaptive 2D convection-diffusion: n0=16, levels=10, tri_cols=4 gives
-    49,296 quads and 128 triangles (49,424 cells).
+def cd_hp(n0=16, levels=10, seed=SEED_C3, jitter=0.05, pmax=6, dpmax=3, tri_cols=4, tri_rows=1):
+    """hp-adaptive 2D convection-diffusion: n0=16, levels=10, tri_cols=4,
+    tri_rows=1 gives 46,215 quads and 6,290 triangles (52,505 cells, 12%
+    triangles, 36 element types); tri_rows=0 gives 49,296 quads and 128
+    triangles.
 
     Mesh graded toward the outflow edge x = 1; per-cell p ~ U{1..pmax} and
     dp ~ U{1..dpmax} (H1 order p+1, test degree p+dp); per-cell eps
@@ -48,7 +51,8 @@ def cd_hp(n0=16, levels=10, seed=SEED_C3, jitter=0.05, pmax=6, dpmax=3, tri_cols
     theta = dpg.rng_uniform(seed + 4, 0.0, 2 * np.pi, 1)[0]
     coefs = np.stack([eps, np.full(C, np.cos(theta)), np.full(C, np.sin(theta))], axis=1)
     load = dpg.load_sine_modes(2, 1, 8, 6, SEED_C1 + 100)
-    band = (lvl == 0) & (nodes[:, :, 0].mean(axis=1) < tri_cols / n0)
+    cx, cy = nodes[:, :, 0].mean(axis=1), nodes[:, :, 1].mean(axis=1)
+    band = ((lvl == 0) & (cx < tri_cols / n0)) | (cy < tri_rows / n0)
     q = ~band
     cells = {capi.QUAD: dict(nodes=nodes[q], parities=par[q], coefs=coefs[q], p=p[q], dp=dp[q])}
     if band.any():

@@ -229,7 +229,7 @@ def hp_mesh(n0, levels, tri_cols, pmax=3, dpmax=2):
     the vertices' coordinates"""
     from camellia2_amd.workloads import cd_hp
 
-    _, cells, _ = cd_hp(n0, levels, tri_cols=tri_cols, pmax=pmax, dpmax=dpmax)
+    _, cells, _ = cd_hp(n0, levels, tri_cols=tri_cols, pmax=pmax, dpmax=dpmax, tri_rows=0)
     order = []  # (topology, index within the topology)
     types = {}
     for topo in (capi.QUAD, capi.TRI):

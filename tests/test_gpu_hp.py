@@ -19,7 +19,7 @@ def dev(a):
 
 
 def c3_case(n0, levels, tri_cols=1):
-    f, cells, load = cd_hp(n0, levels, tri_cols=tri_cols)
+    f, cells, load = cd_hp(n0, levels, tri_cols=tri_cols, tri_rows=0)
     return f, hp_groups(cells), load
 
 

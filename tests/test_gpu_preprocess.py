@@ -69,7 +69,7 @@ def test_bucketing_and_gather_match_numpy():
 def test_bucketing_the_hp_mesh_by_element_type():
     """The graded hp mesh's cells grouped by (topology, p, dp) on the device give
     the same element types and cell counts as the host grouping."""
-    f, cells, load = cd_hp(4, 4, tri_cols=1)
+    f, cells, load = cd_hp(4, 4, tri_cols=1, tri_rows=0)
     q = cells[capi.QUAD]
     key = (q["p"] * 8 + q["dp"]).astype(np.int32)
     perm, offsets, bkeys = dpg.sort_cells(dev(key))
