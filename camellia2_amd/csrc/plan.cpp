@@ -1049,14 +1049,16 @@ void assemble_batch(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, in
 // it is faster over the whole pipeline and moves K no more than 4 times what a
 // second rounding of the dense G does (the same G with its equal IP terms summed
 // separately instead of merged, i.e. another FP64 summation order of the same
-// products). On the 32^3-hex workload it passes (K 9e-11 from an 80-bit
+// products). A form without mergeable terms has no second rounding (the two
+// dense K agree bit for bit), and then the sum-factorised G is not taken. On the 32^3-hex workload it passes (K 9e-11 from an 80-bit
 // evaluation against 6.6e-10 for the dense G: far fewer roundings per entry);
 // where expanding the graph norm's squares into separate metric terms cancels
 // (2D Stokes at h = 1/128: K 1e-1 off) it fails and G stays dense.
 void choose_gram_path(dpg_plan* plan, BucketPlan& B, const dpg_batch& bt, int ns, const Work& w, hipStream_t st) {
   const int N = B.L.N, M = B.L.M;
-  const int ng = std::min(ns, 128);
   const int64_t mm = (int64_t)M * M;
+  // up to 128 cells, and at most 512 MiB per copy of K
+  const int ng = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)ns, 128, (512ll << 20) / (8 * mm)}));
   DeviceBuffer kd, kx, nrm;
   kd.ensure(sizeof(double) * ng * mm);
   kx.ensure(sizeof(double) * ng * mm);
