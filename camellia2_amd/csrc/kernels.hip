@@ -335,12 +335,37 @@ __global__ __launch_bounds__(256) void ip_operand_kernel(const double* __restric
   s[(size_t)c * ss + j] = 1.0 / sqrt(diag);
 }
 
-// lane `k`'s value of a double (k per lane, or wave-uniform), through the
-// LDS crossbar (ds_bpermute: no LDS allocation, result in vector registers)
+// lane `k`'s value of a double (k per lane), through the LDS crossbar
+// (ds_bpermute: no LDS allocation, result in vector registers)
 __device__ inline double wave_bcast(double v, int k) {
   long long b = __builtin_bit_cast(long long, v);
   int lo = __builtin_amdgcn_ds_bpermute(k << 2, (int)(b & 0xffffffffLL));
   int hi = __builtin_amdgcn_ds_bpermute(k << 2, (int)(b >> 32));
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
+// the value of lane (lane & 15) + 16 qs of a double in every lane: row qs of
+// the wave's four 16-lane rows broadcast to all rows by two row swaps,
+// v_permlane16_swap (rows 0,1 and 2,3) then v_permlane32_swap (halves); qs is
+// a constant after unrolling, so the choices fold away
+__device__ inline int row_bcast32(int x, int qs) {
+  const auto a = __builtin_amdgcn_permlane16_swap(x, x, false, false);  // [0]: rows 0,0,2,2  [1]: 1,1,3,3
+  const int y = (qs & 1) ? a[1] : a[0];
+  const auto b = __builtin_amdgcn_permlane32_swap(y, y, false, false);  // [0]: rows 0,1,0,1  [1]: 2,3,2,3
+  return (qs & 2) ? b[1] : b[0];
+}
+__device__ inline double row_bcast(double v, int qs) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = row_bcast32((int)(b & 0xffffffffLL), qs), hi = row_bcast32((int)(b >> 32), qs);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
+// lane `k`'s value of a double for a wave-uniform k, read into scalar
+// registers (v_readlane: no trip through the LDS crossbar)
+__device__ inline double wave_read(double v, int k) {
+  long long b = __builtin_bit_cast(long long, v);
+  int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), k);
+  int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
   return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
 }
 
@@ -590,8 +615,8 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
 // The solve is blocked by the four 16-column MFMA blocks: inside a block the
 // substitution runs column by column as the reference DTRSM does (x_s = c_s /
 // L_ss, then c_m -= L_ms x_s for the later columns m of the block), with x_s
-// moved between the four lane groups that hold a row's columns through the
-// LDS crossbar; the later blocks then take X_block L[later, block]^T as MFMAs
+// moved between the four lane groups that hold a row's columns by two row
+// swaps (row_bcast); the later blocks then take X_block L[later, block]^T as MFMAs
 // (a rank-4 update per k-slice, the B operand being the solved accumulator
 // element itself). L stays in LDS (the staging buffers, reused).
 template <bool SOLVE>
@@ -743,7 +768,7 @@ __device__ inline void gemm_tn_f64_wide_body(const GemmArgs& g) {
         for (int v = vs; v < 4; ++v) lm[v] = Ls[js * LP + bj0 * 16 + q + 4 * v];
 #pragma unroll
         for (int bi = 0; bi < 2; ++bi) {
-          const double xs = wave_bcast(acc[bj0][bi][vs], rowl + 16 * qs) * dinv;
+          const double xs = row_bcast(acc[bj0][bi][vs], qs) * dinv;
           acc[bj0][bi][vs] = q == qs ? xs : (q > qs ? fma(-lm[vs], xs, acc[bj0][bi][vs]) : acc[bj0][bi][vs]);
 #pragma unroll
           for (int v = vs + 1; v < 4; ++v) acc[bj0][bi][v] = fma(-lm[v], xs, acc[bj0][bi][v]);
@@ -1363,7 +1388,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void p
 #pragma unroll
   for (int j = 0; j < NB; ++j) {
     __builtin_amdgcn_sched_barrier(0);
-    double d = wave_bcast(a[j], j);
+    double d = wave_read(a[j], j);
     const bool bad = !(d > 0.0);
     fail = (fail < 0 && bad) ? j : fail;
     d = bad ? __builtin_nan("") : d;
@@ -1372,7 +1397,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void p
     a[j] = lf == j ? sd : (lf > j ? lij : a[j]);
 #pragma unroll
     for (int k = j + 1; k < NB; ++k) {
-      a[k] = fma(-lij, wave_bcast(a[j], k), a[k]);
+      a[k] = fma(-lij, wave_read(a[j], k), a[k]);
       // materialise the update now: left to itself the compiler sinks it to a[k]'s
       // next use and keeps every broadcast value live in scalar registers meanwhile
       asm volatile("" : "+v"(a[k]));
