@@ -224,6 +224,15 @@ struct BucketPlan {
   const uint64_t* gmask[2] = {nullptr, nullptr};         // Gram k-segments per (64- or 32-) tile
   const uint64_t* gmask_merged[2] = {nullptr, nullptr};  // the same with equal terms merged ...
   const double* gweight[2] = {nullptr, nullptr};         // ... and the merged segments' weights
+  // Exact zeros left of the B rows: a trial variable's row of B is +0 in every
+  // cell at the columns of the test variables no bilinear term couples it with
+  // (their point weights are written as zeros), and its solved row stays +0 up
+  // to its first coupled column. zmask: per lower tile of the trailing block
+  // (64- and 32-wide) the k-segments (zseg columns each) some product in the
+  // tile needs.
+  const uint64_t* zmask[2] = {nullptr, nullptr};
+  double zkeep[2] = {1.0, 1.0};  // share of the trailing GEMM's tile x k work the masks keep
+  int zseg = 16;
   // sum-factorised Gram (tensor-product cells, FP64): block pairs, pairs, 1D tables
   bool sf_ok = false;   // available for this bucket
   bool sf_on = false;   // used (dpg_plan_set_tiles, or the autotuner's accuracy-gated choice)
@@ -621,6 +630,41 @@ void build_bucket(dpg_plan* plan, const dpg_bucket_desc& desc, BucketPlan& B) {
       B.gweight[ts] = upload(*plan->tables, weight);
     }
   }
+  {
+    // first column each augmented row can be non-zero in (rounded down to the
+    // 16-column k-chunk); the load and residual rows are taken as dense
+    const int N = L.N, D = B.D;
+    std::vector<int> z(D, 0);
+    for (const TrialBlock& blk : L.blocks) {
+      int zc = N;
+      for (int i = 0; i < f.n_bterms; ++i)
+        if (f.bterms[i].trial_var == blk.var) zc = std::min(zc, L.test_off[f.bterms[i].test_var]);
+      zc = zc / 16 * 16;
+      for (int r = 0; r < blk.card; ++r) z[N + blk.offset + r] = zc;
+    }
+    B.zseg = 16 * (int)((N + 16 * 64 - 1) / (16 * 64));  // at most 64 segments
+    const int n = D - N, nseg = (N + B.zseg - 1) / B.zseg;
+    for (int ts = 0; ts < 2; ++ts) {
+      const int T = ts == 0 ? 64 : 32, nt = (n + T - 1) / T;
+      std::vector<int> zt(nt, N);  // smallest z over each T-block of the trailing rows
+      for (int r = 0; r < n; ++r) zt[r / T] = std::min(zt[r / T], z[N + r]);
+      std::vector<uint64_t> mask;
+      double kept = 0.0;
+      for (int ti = 0; ti < nt; ++ti)
+        for (int tj = 0; tj <= ti; ++tj) {
+          const int k0 = std::max(zt[ti], zt[tj]);  // products left of it are 0 x (something)
+          uint64_t m = 0;
+          for (int sg = 0; sg < nseg; ++sg)
+            if ((sg + 1) * B.zseg > k0) {
+              m |= 1ull << sg;
+              kept += std::min(N, (sg + 1) * B.zseg) - sg * B.zseg;
+            }
+          mask.push_back(m);
+        }
+      B.zmask[ts] = upload(*plan->tables, mask);
+      B.zkeep[ts] = kept / ((double)mask.size() * N);
+    }
+  }
   B.nTv = (int64_t)B.prog.n_vb * L.Q * B.Npad;
   B.nTs = (int64_t)B.prog.n_sb * S * L.Qs * B.Npad;
   B.nWip = (int64_t)L.QG * B.prog.n_lt * nvar * B.npr;
@@ -688,8 +732,13 @@ int batch_cells(dpg_plan* plan, const BucketPlan& B, int ncells, int64_t per_cel
 //   panel update  A[k0:D, k0:k1] -= A[k0:D, kbeg:k0] A[k0:k1, kbeg:k0]^T   (MFMA)
 //   panel         POTRF of the diagonal block, X <- X L^-T below it
 // then the trailing block A[kend:D, kend:D] -= A[kend:D, kbeg:kend] (...)^T.
+// zeros: the matrix is a bucket's augmented [G; B; l; r] over the columns of G
+// (kbeg = 0, kend = N): the closing product of the B rows skips the k-segments
+// where the B rows' leading exact zeros leave every product of a tile zero
+// (zmask). (Splitting the panel updates at the zero runs as well was measured
+// slower: the extra, partly empty row tiles cost more than the skipped work.)
 void factor_columns(dpg_plan* plan, const BucketPlan& B, int nb, const Work& w, int kbeg, int kend, double beta_trailing,
-                    hipStream_t st) {
+                    hipStream_t st, bool zeros = false) {
   Profiler& P = plan->prof;
   const int64_t sA = B.nA, lda = B.lda;
   int32_t* info = w.info;
@@ -799,9 +848,13 @@ void factor_columns(dpg_plan* plan, const BucketPlan& B, int nb, const Work& w, 
     t.alpha = -1.0;
     t.beta = beta_trailing;
     t.ncells = nb;
+    if (zeros) {  // the B rows' leading zeros: per tile, the k-segments some product needs
+      t.kmask = B.zmask[B.tile[2] == 32 ? 1 : 0];
+      t.kseg = B.zseg;
+    }
     hipEvent_t e = P.start(st);
     launch_gemm(t, kGemmUpdate, st, B.tile[2]);
-    P.stop(e, st, DPG_STAGE_UPDATE, nb * (double)n * (n + 1.0) * t.k,
+    P.stop(e, st, DPG_STAGE_UPDATE, nb * (double)n * (n + 1.0) * t.k * (zeros ? B.zkeep[B.tile[2] == 32 ? 1 : 0] : 1.0),
            nb * ((double)n * t.k + n * (n + 1.0) / 2.0 * (beta_trailing != 0.0 ? 2.0 : 1.0)) * 8.0);
   }
 }
@@ -1003,7 +1056,7 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
   }
   launch_fill_i32(w.info, nb, 0, st);
   // the trailing block starts as 0 and is written (beta = 0) by the closing SYRK
-  factor_columns(plan, B, nb, w, 0, N, 0.0, st);
+  factor_columns(plan, B, nb, w, 0, N, 0.0, st, true);
 }
 
 // Stage 2: static condensation of the field dofs of K = -A[N:, N:]
@@ -1338,6 +1391,7 @@ int dpg_recover_fields(dpg_plan* plan, int bucket, int ncells, const double* K, 
     // the field block and one augmented row, in the bucket's workspace layout
     dpg::BucketPlan R = B;
     R.D = F + 1;
+    R.zmask[0] = R.zmask[1] = nullptr;  // not the augmented matrix of G: no known zeros
     int maxb = dpg::batch_cells(plan, B, ncells);
     dpg::prepare_workspace(plan, B, maxb);
     int bad = 0;

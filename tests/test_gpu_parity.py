@@ -1,6 +1,8 @@
 """GPU parity: the HIP path (through the C ABI) against the oracle on the same
 seeded inputs. FP64 tolerance: relative Frobenius error per cell <= 1e-12 for
 K and rhs (the bar of the plain FP64 form), looser only where stated."""
+import ctypes as ct
+
 import numpy as np
 import pytest
 
@@ -139,6 +141,51 @@ def test_update_tiles_give_identical_results(case):
     for k in ("K", "rhs", "schur", "schur_rhs"):
         print(f"{k:10s} fused panel solve vs 64 x 64 tiles {rel_fro(res[2][k].cpu().numpy(), res[0][k].cpu().numpy()):.2e}")
     A.close()
+
+
+def _coupled_everywhere(f):
+    """The form f plus a zero-coefficient term between every trial and test
+    variable that no term couples: the same B, with no pair left uncoupled."""
+    g = capi.FormDesc()
+    ct.memmove(ct.byref(g), ct.byref(f), ct.sizeof(f))
+    n = g.n_bterms
+    met = {(g.bterms[i].trial_var, g.bterms[i].test_var) for i in range(n)}
+    for tv in range(g.n_trial):
+        for sv in range(g.n_test):
+            if (tv, sv) in met:
+                continue
+            side = g.trial_kind[tv] in (capi.TRACE, capi.FLUX)
+            op = (capi.OP_VALUE if g.test_space[sv] == capi.HGRAD
+                  else (capi.OP_DOT_NORMAL if side else capi.OP_DIV))
+            g.bterms[n] = capi.BTerm(tv, 0, sv, op, capi.Coef(0.0, -1, 0, -1, 0))
+            n += 1
+    assert n <= capi.MAX_BTERMS
+    g.n_bterms = n
+    return g
+
+
+@pytest.mark.parametrize("update", [64, 128])
+@pytest.mark.parametrize("case", ["c4", "c5", "c2"])
+def test_skipping_the_exact_zeros_of_B_changes_no_bit(case, update):
+    """A B row is +0 at the test columns of the test variables its trial
+    variable shares no bilinear term with (their point weights are zeros), so
+    its solved row is +0 up to its first coupled column; the closing product
+    of the B rows skips the k-segments where every product of a tile is such a
+    zero. The same form with zero-coefficient terms coupling every pair has no
+    such zeros to skip: the outputs are the same bits."""
+    f, b, nodes, par, coefs, load = {"c4": lambda: poisson3d(2, 2, 1), "c5": lambda: cdr2d(6, 6),
+                                     "c2": lambda: stokes2d(3, 3)}[case]()
+    res = []
+    for form in (f, _coupled_everywhere(f)):
+        A = dpg.LocalAssembler(form, [b])
+        A.set_tiles(0, 64, 64, update, 0)
+        o, nbad = A.local_stiffness_and_rhs(0, dev(nodes), dev(par), dev(coefs), load, want_schur=True,
+                                            want_G=True)
+        assert nbad == 0
+        res.append({k: v.clone() for k, v in o.items()})
+        A.close()
+    for k in ("K", "rhs", "schur", "schur_rhs", "G"):
+        assert torch.equal(res[0][k], res[1][k]), k
 
 
 @pytest.mark.parametrize("case", ["c4", "hex_low", "c2", "c5", "c1"])
