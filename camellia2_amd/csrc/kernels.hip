@@ -127,16 +127,26 @@ __device__ inline double coef_value(const DSummand& s, const double* cc, const d
     double a = cc[s.slot];
     v *= (s.fn == DPG_FN_SQRT) ? sqrt(a) : (s.fn == DPG_FN_INV ? 1.0 / a : a);
   }
-  if (s.normal >= 0) v *= n ? n[s.normal] : 0.0;
+  if (s.normal >= 0) {
+    // the component selected among compile-time indices: a run-time index into
+    // the caller's normal would put it in scratch memory
+    double nv = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      if (d == s.normal && n) nv = n[d];
+    v *= nv;
+  }
   return v;
 }
 
-__device__ inline double load_value(const dpg_load_desc& L, int comp, const double* x, int dim) {
+template <int DIM>
+__device__ inline double load_value(const dpg_load_desc& L, int comp, const double* x) {
   if (comp >= L.ncomp) return 0.0;
   double f = L.constant[comp];
   for (int m = 0; m < L.nmodes[comp]; ++m) {
     double p = L.amp[comp][m];
-    for (int d = 0; d < dim; ++d) p *= sin(L.k[comp][m][d] * M_PI * x[d]);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) p *= sin(L.k[comp][m][d] * M_PI * x[d]);
     f += p;
   }
   return f;
@@ -211,14 +221,16 @@ __global__ __launch_bounds__(128) void point_kernel(int kind, const DevProgram* 
   double* Wp = W + ((size_t)c * npts + p) * slice;
   for (int e = 0; e < slice; ++e) Wp[e] = 0.0;
   double t[DIM + 1];
-  const double* nn = kind == 2 ? n : nullptr;
+  // n stays zero off the sides (normal components and DOT_NORMAL then give 0,
+  // as with no normal at all); always passing the array, rather than choosing
+  // between it and a null pointer, keeps it in registers
   for (int gi = 0; gi < ng; ++gi) {
     double gfac = fac;
     if (kind == 2 && prog->sb_flux[gi]) gfac *= (double)parities[(size_t)c * nsides + s];
     for (int si = gbeg[gi]; si < gbeg[gi + 1]; ++si) {
       const DSummand& sm = prog->s[si];
-      double cv = coef_value(sm, cc, nn) * gfac;
-      op_to_prims<DIM>(sm.op, g, nn, t);
+      double cv = coef_value(sm, cc, n) * gfac;
+      op_to_prims<DIM>(sm.op, g, n, t);
       double* w = Wp + ((size_t)gi * nvar + sm.var) * npr;
       for (int r = 0; r < npr; ++r) w[r] += cv * t[r];
     }
@@ -232,7 +244,7 @@ __global__ __launch_bounds__(128) void point_kernel(int kind, const DevProgram* 
     for (int e = 0; e < nvar * npr; ++e) Wr[e] = 0.0;
     for (int si = prog->rhs_beg[0]; si < prog->rhs_beg[1]; ++si) {
       const DSummand& sm = prog->s[si];
-      double cv = load_value(load, sm.aux, x, DIM) * sm.scale * fac;
+      double cv = load_value<DIM>(load, sm.aux, x) * sm.scale * fac;
       op_to_prims<DIM>(sm.op, g, nullptr, t);
       for (int r = 0; r < npr; ++r) Wr[sm.var * npr + r] += cv * t[r];
     }
@@ -856,15 +868,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void u
 //   G[(i1,i2),(j1,j2)] = sum_cls sum_q1 F_cfa(i1,q1) F_cfb(j1,q1) T2[cls][i2,j2][q1]
 // where cls groups the pairs by their direction-1 families. M_p is formed
 // from the point weights W in LDS first. Writes are unscaled (the DPOEQU
-// factors are taken from G's diagonal afterwards).
-template <int DIM>
+// factors are taken from G's diagonal afterwards). QN > 0: the points per
+// direction as a compile-time constant (the point sums unrolled, their loads
+// issued together; the same FMAs in the same order as with QN = 0, which takes
+// q at run time).
+template <int DIM, int QN>
 __global__ __launch_bounds__(256) void gram_sf_kernel(const SfBlockPair* __restrict__ bps,
                                                       const SfPair* __restrict__ pairs,
-                                                      const double* __restrict__ F, int nF, int q,
+                                                      const double* __restrict__ F, int nF, int q_rt,
                                                       const double* __restrict__ Wip, int n_lt, int na, int QG,
                                                       double* __restrict__ A, int64_t lda, int64_t sA) {
+  const int q = QN > 0 ? QN : q_rt;
   extern __shared__ double sm[];
-  const SfBlockPair bp = bps[blockIdx.x];
+  const SfBlockPair& bp = bps[blockIdx.x];  // read in place: a local copy, indexed at run time, would live in scratch
   const int c = blockIdx.y, t = threadIdx.x, NT = blockDim.x;
   const int np = bp.npairs, ncls = bp.ncls;
   const SfPair* pr = pairs + bp.pair0;
@@ -898,6 +914,7 @@ __global__ __launch_bounds__(256) void gram_sf_kernel(const SfBlockPair* __restr
           const double* fb = fam(pr[p].fb[2], j3);
           const double* m = Ms + p * QG + r;
           double acc = 0.0;
+#pragma unroll
           for (int q3 = 0; q3 < q; ++q3) acc = fma(fa[q3] * fb[q3], m[q3 * q2n], acc);
           T1[e] = acc;
         }
@@ -912,6 +929,7 @@ __global__ __launch_bounds__(256) void gram_sf_kernel(const SfBlockPair* __restr
             const double* fa = fam(pr[p].fa[1], i2);
             const double* fb = fam(pr[p].fb[1], j2);
             const double* tp = T1 + p * q2n + q1;
+#pragma unroll
             for (int q2 = 0; q2 < q; ++q2) acc = fma(fa[q2] * fb[q2], tp[q2 * q], acc);
           }
           T2[e] = acc;
@@ -928,6 +946,7 @@ __global__ __launch_bounds__(256) void gram_sf_kernel(const SfBlockPair* __restr
             const double* fa = fam(bp.clsfa[cl], i1);
             const double* fb = fam(bp.clsfb[cl], j1);
             const double* tp = T2 + (cl * n2 + i2 + bp.nA[1] * j2) * q;
+#pragma unroll
             for (int q1 = 0; q1 < q; ++q1) acc = fma(fa[q1] * fb[q1], tp[q1], acc);
           }
           Ac[(size_t)row + (size_t)col * lda] = acc;
@@ -946,6 +965,7 @@ __global__ __launch_bounds__(256) void gram_sf_kernel(const SfBlockPair* __restr
             const double* fa = fam(pr[p].fa[1], i2);
             const double* fb = fam(pr[p].fb[1], j2);
             const double* m = Ms + p * QG + q1;
+#pragma unroll
             for (int q2 = 0; q2 < q; ++q2) acc = fma(fa[q2] * fb[q2], m[q2 * q], acc);
           }
           T1[e] = acc;
@@ -960,6 +980,7 @@ __global__ __launch_bounds__(256) void gram_sf_kernel(const SfBlockPair* __restr
             const double* fa = fam(bp.clsfa[cl], i1);
             const double* fb = fam(bp.clsfb[cl], j1);
             const double* tp = T1 + cl * q;
+#pragma unroll
             for (int q1 = 0; q1 < q; ++q1) acc = fma(fa[q1] * fb[q1], tp[q1], acc);
           }
           Ac[(size_t)row + (size_t)col * lda] = acc;
@@ -2220,10 +2241,26 @@ void launch_gram_sf(int dim, const SfBlockPair* bps, int nbp, const SfPair* pair
   if (ncells <= 0 || nbp <= 0) return;
   if (lds_bytes > 64 * 1024) throw DeviceError("gram_sf: LDS request above 64 KiB");
   dim3 grid(nbp, ncells);
-  if (dim == 3)
-    gram_sf_kernel<3><<<grid, 256, lds_bytes, st>>>(bps, pairs, F, nF, q, Wip, n_lt, na, QG, A, lda, sA);
-  else
-    gram_sf_kernel<2><<<grid, 256, lds_bytes, st>>>(bps, pairs, F, nF, q, Wip, n_lt, na, QG, A, lda, sA);
+#define DPG_SF_CASE(D, QQ) \
+  case QQ:                 \
+    gram_sf_kernel<D, QQ><<<grid, 256, lds_bytes, st>>>(bps, pairs, F, nF, q, Wip, n_lt, na, QG, A, lda, sA); \
+    break;
+  if (dim == 3) {
+    switch (q) {
+      DPG_SF_CASE(3, 3) DPG_SF_CASE(3, 4) DPG_SF_CASE(3, 5) DPG_SF_CASE(3, 6) DPG_SF_CASE(3, 7) DPG_SF_CASE(3, 8)
+      DPG_SF_CASE(3, 9) DPG_SF_CASE(3, 10)
+      default:
+        gram_sf_kernel<3, 0><<<grid, 256, lds_bytes, st>>>(bps, pairs, F, nF, q, Wip, n_lt, na, QG, A, lda, sA);
+    }
+  } else {
+    switch (q) {
+      DPG_SF_CASE(2, 3) DPG_SF_CASE(2, 4) DPG_SF_CASE(2, 5) DPG_SF_CASE(2, 6) DPG_SF_CASE(2, 7) DPG_SF_CASE(2, 8)
+      DPG_SF_CASE(2, 9) DPG_SF_CASE(2, 10) DPG_SF_CASE(2, 11) DPG_SF_CASE(2, 12)
+      default:
+        gram_sf_kernel<2, 0><<<grid, 256, lds_bytes, st>>>(bps, pairs, F, nF, q, Wip, n_lt, na, QG, A, lda, sA);
+    }
+  }
+#undef DPG_SF_CASE
   check_launch("gram_sf_kernel");
 }
 
