@@ -45,7 +45,7 @@ struct DevProgram {
 // families per direction: L (closed Lagrange), L' (its derivative), O (open
 // Lagrange). A pair couples prim a of block A with prim b of block B:
 //   G_AB += sum_q M_ab(q) R_a(i, q) R_b(j, q),  M_ab(q) = sum_lt W[q][lt][a] W[q][lt][b].
-constexpr int kSfFamL = 0, kSfFamD = 1, kSfFamO = 2;
+constexpr int kSfFamL = 0, kSfFamD = 1, kSfFamO = 2, kSfFamFld = 3;  // Fld: trial field basis (B only)
 constexpr int kSfMaxCls = 9;
 struct SfPair {
   int16_t a, b;       // (var * npr + prim) of the two sides

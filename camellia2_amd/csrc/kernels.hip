@@ -872,12 +872,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void u
 // direction as a compile-time constant (the point sums unrolled, their loads
 // issued together; the same FMAs in the same order as with QN = 0, which takes
 // q at run time).
-template <int DIM, int QN>
-__global__ __launch_bounds__(256) void gram_sf_kernel(const SfBlockPair* __restrict__ bps,
-                                                      const SfPair* __restrict__ pairs,
-                                                      const double* __restrict__ F, int nF, int q_rt,
-                                                      const double* __restrict__ Wip, int n_lt, int na, int QG,
-                                                      double* __restrict__ A, int64_t lda, int64_t sA) {
+//
+// MODE 1 (the volume part of B, bmat_sf_kernel): side A is a trial field block
+// (rows N + its offset), side B a test block; the pair's weight is one point
+// weight of the stiffness program, M_p(q) = W[q][a] (a = (group * nvar + test
+// var) * npr + prim), and the stores carry the DPOEQU factor of the test column.
+template <int DIM, int QN, int MODE>
+__device__ inline void sf_body(const SfBlockPair* __restrict__ bps, const SfPair* __restrict__ pairs,
+                               const double* __restrict__ F, int nF, int nfam, int q_rt,
+                               const double* __restrict__ Wp, int n_lt, int na, int QG, double* __restrict__ A,
+                               int64_t lda, int64_t sA, const double* __restrict__ s, int64_t ss) {
   const int q = QN > 0 ? QN : q_rt;
   extern __shared__ double sm[];
   const SfBlockPair& bp = bps[blockIdx.x];  // read in place: a local copy, indexed at run time, would live in scratch
@@ -885,19 +889,28 @@ __global__ __launch_bounds__(256) void gram_sf_kernel(const SfBlockPair* __restr
   const int np = bp.npairs, ncls = bp.ncls;
   const SfPair* pr = pairs + bp.pair0;
   double* Ms = sm;                 // np * QG
-  double* Fs = Ms + np * QG;       // 3 * nF * q
-  double* T1 = Fs + 3 * nF * q;    // 3D: np*q*q   2D: ncls*q
+  double* Fs = Ms + np * QG;       // nfam * nF * q
+  double* T1 = Fs + nfam * nF * q;  // 3D: np*q*q   2D: ncls*q
   double* T2 = T1 + (DIM == 3 ? np * q * q : ncls * q);  // 3D: ncls * nA2*nB2 * q
-  const double* W = Wip + (size_t)c * QG * n_lt * na;
-  for (int e = t; e < np * QG; e += NT) {
-    int p = e / QG, qq = e - p * QG;
-    const double* w = W + (size_t)qq * n_lt * na;
-    int a = pr[p].a, b = pr[p].b;
-    double acc = 0.0;
-    for (int lt = 0; lt < n_lt; ++lt) acc = fma(w[lt * na + a], w[lt * na + b], acc);
-    Ms[e] = acc;
+  if constexpr (MODE == 0) {
+    const double* W = Wp + (size_t)c * QG * n_lt * na;
+    for (int e = t; e < np * QG; e += NT) {
+      int p = e / QG, qq = e - p * QG;
+      const double* w = W + (size_t)qq * n_lt * na;
+      int a = pr[p].a, b = pr[p].b;
+      double acc = 0.0;
+      for (int lt = 0; lt < n_lt; ++lt) acc = fma(w[lt * na + a], w[lt * na + b], acc);
+      Ms[e] = acc;
+    }
+  } else {
+    const double* W = Wp + (size_t)c * QG * na;
+    for (int e = t; e < np * QG; e += NT) {
+      int p = e / QG, qq = e - p * QG;
+      Ms[e] = W[(size_t)qq * na + pr[p].a];
+    }
   }
-  for (int e = t; e < 3 * nF * q; e += NT) Fs[e] = F[e];
+  const double* sc = MODE == 1 && s ? s + (size_t)c * ss : nullptr;
+  for (int e = t; e < nfam * nF * q; e += NT) Fs[e] = F[e];
   __syncthreads();
   double* Ac = A + (size_t)c * sA;
   auto fam = [&](int f, int i) { return Fs + (f * nF + i) * q; };
@@ -949,7 +962,7 @@ __global__ __launch_bounds__(256) void gram_sf_kernel(const SfBlockPair* __restr
 #pragma unroll
             for (int q1 = 0; q1 < q; ++q1) acc = fma(fa[q1] * fb[q1], tp[q1], acc);
           }
-          Ac[(size_t)row + (size_t)col * lda] = acc;
+          Ac[(size_t)row + (size_t)col * lda] = MODE == 1 && sc ? acc * sc[col] : acc;
         }
         __syncthreads();
       }
@@ -983,11 +996,29 @@ __global__ __launch_bounds__(256) void gram_sf_kernel(const SfBlockPair* __restr
 #pragma unroll
             for (int q1 = 0; q1 < q; ++q1) acc = fma(fa[q1] * fb[q1], tp[q1], acc);
           }
-          Ac[(size_t)row + (size_t)col * lda] = acc;
+          Ac[(size_t)row + (size_t)col * lda] = MODE == 1 && sc ? acc * sc[col] : acc;
         }
         __syncthreads();
       }
   }
+}
+
+template <int DIM, int QN>
+__global__ __launch_bounds__(256) void gram_sf_kernel(const SfBlockPair* __restrict__ bps,
+                                                      const SfPair* __restrict__ pairs,
+                                                      const double* __restrict__ F, int nF, int q_rt,
+                                                      const double* __restrict__ Wip, int n_lt, int na, int QG,
+                                                      double* __restrict__ A, int64_t lda, int64_t sA) {
+  sf_body<DIM, QN, 0>(bps, pairs, F, nF, 3, q_rt, Wip, n_lt, na, QG, A, lda, sA, nullptr, 0);
+}
+template <int DIM, int QN>
+__global__ __launch_bounds__(256) void bmat_sf_kernel(const SfBlockPair* __restrict__ bps,
+                                                      const SfPair* __restrict__ pairs,
+                                                      const double* __restrict__ F, int nF, int q_rt,
+                                                      const double* __restrict__ Wvol, int na, int Q,
+                                                      double* __restrict__ A, int64_t lda, int64_t sA,
+                                                      const double* __restrict__ s, int64_t ss) {
+  sf_body<DIM, QN, 1>(bps, pairs, F, nF, 4, q_rt, Wvol, 1, na, Q, A, lda, sA, s, ss);
 }
 
 // --------------------------------------------------- mixed-precision GEMM
@@ -2262,6 +2293,35 @@ void launch_gram_sf(int dim, const SfBlockPair* bps, int nbp, const SfPair* pair
   }
 #undef DPG_SF_CASE
   check_launch("gram_sf_kernel");
+}
+
+void launch_bmat_sf(int dim, const SfBlockPair* bps, int nbp, const SfPair* pairs, const double* F, int nF, int q,
+                    const double* Wvol, int na, int Q, double* A, int64_t lda, int64_t sA, const double* s,
+                    int64_t ss, int ncells, size_t lds_bytes, hipStream_t st) {
+  if (ncells <= 0 || nbp <= 0) return;
+  if (lds_bytes > 64 * 1024) throw DeviceError("bmat_sf: LDS request above 64 KiB");
+  dim3 grid(nbp, ncells);
+#define DPG_SFB_CASE(D, QQ) \
+  case QQ:                  \
+    bmat_sf_kernel<D, QQ><<<grid, 256, lds_bytes, st>>>(bps, pairs, F, nF, q, Wvol, na, Q, A, lda, sA, s, ss); \
+    break;
+  if (dim == 3) {
+    switch (q) {
+      DPG_SFB_CASE(3, 3) DPG_SFB_CASE(3, 4) DPG_SFB_CASE(3, 5) DPG_SFB_CASE(3, 6) DPG_SFB_CASE(3, 7)
+      DPG_SFB_CASE(3, 8) DPG_SFB_CASE(3, 9) DPG_SFB_CASE(3, 10)
+      default:
+        bmat_sf_kernel<3, 0><<<grid, 256, lds_bytes, st>>>(bps, pairs, F, nF, q, Wvol, na, Q, A, lda, sA, s, ss);
+    }
+  } else {
+    switch (q) {
+      DPG_SFB_CASE(2, 3) DPG_SFB_CASE(2, 4) DPG_SFB_CASE(2, 5) DPG_SFB_CASE(2, 6) DPG_SFB_CASE(2, 7)
+      DPG_SFB_CASE(2, 8) DPG_SFB_CASE(2, 9) DPG_SFB_CASE(2, 10) DPG_SFB_CASE(2, 11) DPG_SFB_CASE(2, 12)
+      default:
+        bmat_sf_kernel<2, 0><<<grid, 256, lds_bytes, st>>>(bps, pairs, F, nF, q, Wvol, na, Q, A, lda, sA, s, ss);
+    }
+  }
+#undef DPG_SFB_CASE
+  check_launch("bmat_sf_kernel");
 }
 
 // a kernel rather than hipMemsetAsync, so that captured graphs replay it reliably

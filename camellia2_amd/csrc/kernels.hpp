@@ -82,6 +82,13 @@ void launch_info_count(const int32_t* info, int n, int32_t* out, int32_t* count,
 void launch_gram_sf(int dim, const SfBlockPair* bps, int nbp, const SfPair* pairs, const double* F, int nF, int q,
                     const double* Wip, int n_lt, int na, int QG, double* A, int64_t lda, int64_t sA, int ncells,
                     size_t lds_bytes, hipStream_t st);
+// the volume part of B, sum-factorised: block pairs (trial field block, test
+// block) and their pairs, 1D tables of 4 families on the stiffness points
+// (q per direction), point weights Wvol (na per point), stores scaled by s
+// (the DPOEQU factors of the test columns; none when s is NULL)
+void launch_bmat_sf(int dim, const SfBlockPair* bps, int nbp, const SfPair* pairs, const double* F, int nF, int q,
+                    const double* Wvol, int na, int Q, double* A, int64_t lda, int64_t sA, const double* s,
+                    int64_t ss, int ncells, size_t lds_bytes, hipStream_t st);
 // the same contraction with FP64 operands converted on load to FP32 (precision
 // DPG_PREC_FP32) or to hi+lo BF16/FP16 pairs (DPG_PREC_BF16X3/FP16X3, products
 // hi*hi + hi*lo + lo*hi), FP32 MFMA accumulation, FP64 epilogue

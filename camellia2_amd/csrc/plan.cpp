@@ -242,6 +242,14 @@ struct BucketPlan {
   const SfBlockPair* sf_bps = nullptr;
   const SfPair* sf_pairs = nullptr;
   const double* sf_F = nullptr;
+  // the volume part of B sum-factorised along with G (sf_on): block pairs
+  // (trial field block, test block), 1D tables on the stiffness points
+  bool sfb_ok = false;
+  int sfb_nbp = 0, sfb_nF = 0, sfb_q = 0;
+  size_t sfb_lds = 0;
+  const SfBlockPair* sfb_bps = nullptr;
+  const SfPair* sfb_pairs = nullptr;
+  const double* sfb_F = nullptr;
 };
 
 // ---- static scratch of dpg_assemble_noalloc ---------------------------------
@@ -368,17 +376,14 @@ void compile_program(const dpg_form_desc& f, BucketPlan& B) {
   P.rhs_beg[1] = ns;
 }
 
-// Tensor-product structure of the test functions for the sum-factorised Gram
-// matrix: blocks (HGRAD var; HDIV var component), their prims' 1D families,
-// and the (block, block) pairs that some IP term couples.
-void build_sum_factorisation(dpg_plan* plan, BucketPlan& B) {
-  const dpg_form_desc& f = plan->form;
-  const ElementLayout& L = B.L;
-  if (L.topo != DPG_QUAD && L.topo != DPG_HEX) return;
+// Tensor-product test blocks of a quad or hex bucket (HGRAD var; HDIV var
+// component): 1D sizes per direction and, per prim, the 1D family per direction.
+struct SfTestBlock {
+  int var, off, n[3], nprim, prim[4], fam[4][3];
+};
+std::vector<SfTestBlock> sf_test_blocks(const dpg_form_desc& f, const ElementLayout& L) {
   const int dim = L.dim, npr = dim + 1, kt = L.kt;
-  struct Blk {
-    int var, off, n[3], nprim, prim[4], fam[4][3];
-  };
+  using Blk = SfTestBlock;
   std::vector<Blk> blocks;
   for (int v = 0; v < f.n_test; ++v) {
     if (f.test_space[v] == DPG_HGRAD) {
@@ -415,6 +420,19 @@ void build_sum_factorisation(dpg_plan* plan, BucketPlan& B) {
       }
     }
   }
+  return blocks;
+}
+
+// Tensor-product structure of the test functions for the sum-factorised Gram
+// matrix: blocks (HGRAD var; HDIV var component), their prims' 1D families,
+// and the (block, block) pairs that some IP term couples.
+void build_sum_factorisation(dpg_plan* plan, BucketPlan& B) {
+  const dpg_form_desc& f = plan->form;
+  const ElementLayout& L = B.L;
+  if (L.topo != DPG_QUAD && L.topo != DPG_HEX) return;
+  const int dim = L.dim, npr = dim + 1, kt = L.kt;
+  using Blk = SfTestBlock;
+  const std::vector<Blk> blocks = sf_test_blocks(f, L);
   // prims each IP term reaches (op_to_prims: grad -> all derivative prims, HDIV value -> all components)
   const int na = f.n_test * npr;
   std::vector<std::vector<char>> touch(f.n_ipterms, std::vector<char>(na, 0));
@@ -507,6 +525,107 @@ void build_sum_factorisation(dpg_plan* plan, BucketPlan& B) {
   B.sf_on = false;  // off until dpg_plan_set_tiles or dpg_plan_autotune switches it on (see dpg_tiles.sumfact)
 }
 
+// Reference prims (value, d/dxi_e; or components, divergence) a test operator
+// reaches through the point's geometry (op_to_prims): bit r set for prim r.
+unsigned prims_of_op(int op, int dim) {
+  if (op == DPG_OP_VALUE) return 1u;
+  if (op >= DPG_OP_DX && op < DPG_OP_DX + dim) return ((1u << dim) - 1u) << 1;
+  if (op >= DPG_OP_X && op < DPG_OP_X + dim) return (1u << dim) - 1u;
+  if (op == DPG_OP_DIV) return 1u << dim;
+  return 0u;
+}
+
+// The volume part of B the same way: B[f, j] = sum_q sum_r W[q][g][var j][r]
+// phi_f(q) R_r(j, q) for the trial field block g (a tensor-product scalar
+// basis, one 1D family) and the test block of j, its prims r the group's
+// summands reach. Taken together with the sum-factorised G (sf_on).
+void build_sum_factorisation_b(dpg_plan* plan, BucketPlan& B) {
+  const dpg_form_desc& f = plan->form;
+  const ElementLayout& L = B.L;
+  if (L.topo != DPG_QUAD && L.topo != DPG_HEX) return;
+  const int dim = L.dim, npr = dim + 1, kt = L.kt, nvar = f.n_test;
+  LagrangeLine Fld(gll_points(std::max(L.field_deg, 1)));
+  const int nfl = Fld.size();
+  if ((dim == 3 ? nfl * nfl * nfl : nfl * nfl) != L.nf) return;
+  const std::vector<SfTestBlock> blocks = sf_test_blocks(f, L);
+  const DevProgram& P = B.prog;
+  std::vector<SfBlockPair> bps;
+  std::vector<SfPair> pairs;
+  for (int g = 0; g < P.n_vb; ++g) {
+    const TrialBlock& blk = L.blocks[B.vb_block[g]];
+    if (blk.card != L.nf) return;
+    std::vector<unsigned> reach(nvar, 0u);  // prims of each test var the group's summands reach
+    for (int si = P.vb_beg[g]; si < P.vb_beg[g + 1]; ++si) reach[P.s[si].var] |= prims_of_op(P.s[si].op, dim);
+    for (const SfTestBlock& tb : blocks) {
+      SfBlockPair bp{};
+      bp.offA = L.N + blk.offset;
+      bp.offB = tb.off;
+      for (int d = 0; d < 3; ++d) {
+        bp.nA[d] = d < dim ? nfl : 1;
+        bp.nB[d] = tb.n[d];
+      }
+      bp.diag = 0;
+      bp.pair0 = (int)pairs.size();
+      for (int ri = 0; ri < tb.nprim; ++ri) {
+        const int r = tb.prim[ri];
+        if (!((reach[tb.var] >> r) & 1u)) continue;
+        SfPair p{};
+        p.a = p.b = (int16_t)((g * nvar + tb.var) * npr + r);
+        for (int d = 0; d < 3; ++d) {
+          p.fa[d] = (int8_t)kSfFamFld;
+          p.fb[d] = (int8_t)tb.fam[ri][d];
+        }
+        int cl = -1;
+        for (int k = 0; k < bp.ncls; ++k)
+          if (bp.clsfa[k] == p.fa[0] && bp.clsfb[k] == p.fb[0]) cl = k;
+        if (cl < 0) {
+          if (bp.ncls >= kSfMaxCls) throw std::logic_error("sum factorisation: too many classes");
+          cl = bp.ncls++;
+          bp.clsfa[cl] = p.fa[0];
+          bp.clsfb[cl] = p.fb[0];
+        }
+        p.cls = (int8_t)cl;
+        pairs.push_back(p);
+      }
+      bp.npairs = (int)pairs.size() - bp.pair0;
+      if (bp.npairs > 0) bps.push_back(bp);
+    }
+  }
+  // 1D tables on the stiffness points: L, L', O (padded with a zero row), Fld
+  const int q = L.Q1, nF = std::max(kt + 1, nfl);
+  Rule1D r = gauss_legendre(q);
+  LagrangeLine Cl(gll_points(kt)), Op(gauss_legendre(kt).x);
+  std::vector<double> F((size_t)4 * nF * q, 0.0);
+  std::vector<double> v(nF), dv(nF), o(nF), fl(nF);
+  for (int k = 0; k < q; ++k) {
+    Cl.eval(r.x[k], v.data(), dv.data());
+    Op.eval(r.x[k], o.data(), nullptr);
+    Fld.eval(r.x[k], fl.data(), nullptr);
+    for (int i = 0; i < nF; ++i) {
+      F[((size_t)kSfFamL * nF + i) * q + k] = i <= kt ? v[i] : 0.0;
+      F[((size_t)kSfFamD * nF + i) * q + k] = i <= kt ? dv[i] : 0.0;
+      F[((size_t)kSfFamO * nF + i) * q + k] = i < kt ? o[i] : 0.0;
+      F[((size_t)kSfFamFld * nF + i) * q + k] = i < nfl ? fl[i] : 0.0;
+    }
+  }
+  size_t lds = 0;
+  for (const SfBlockPair& bp : bps) {
+    size_t t1 = dim == 3 ? (size_t)bp.npairs * q * q : (size_t)bp.ncls * q;
+    size_t t2 = dim == 3 ? (size_t)bp.ncls * bp.nA[1] * bp.nB[1] * q : 0;
+    lds = std::max(lds, sizeof(double) * ((size_t)bp.npairs * L.Q + 4 * (size_t)nF * q + t1 + t2));
+  }
+  if (bps.empty() || lds > 64 * 1024) return;  // the dense contraction stays in use
+  auto& own = *plan->tables;
+  B.sfb_bps = upload(own, bps);
+  B.sfb_pairs = upload(own, pairs);
+  B.sfb_F = upload(own, F);
+  B.sfb_nbp = (int)bps.size();
+  B.sfb_nF = nF;
+  B.sfb_q = q;
+  B.sfb_lds = lds;
+  B.sfb_ok = true;
+}
+
 void build_bucket(dpg_plan* plan, const dpg_bucket_desc& desc, BucketPlan& B) {
   const dpg_form_desc& f = plan->form;
   B.desc = desc;
@@ -525,6 +644,7 @@ void build_bucket(dpg_plan* plan, const dpg_bucket_desc& desc, BucketPlan& B) {
   }
   compile_program(f, B);
   build_sum_factorisation(plan, B);
+  build_sum_factorisation_b(plan, B);
   RefTables T = make_tables(f, L);
   auto& own = *plan->tables;
   B.tab.R_vol = upload(own, T.vol.R);
@@ -883,6 +1003,7 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
   P.stop(e, st, DPG_STAGE_POINT, 0.0, nb * D8 * (B.nWip + B.nWvol + B.nWrhs + B.nWside + V * dim));
   const int prec = plan->opt.precision;
   const bool sf = B.sf_on && prec == DPG_PREC_FP64;
+  const bool sfb = sf && B.sfb_ok;  // the volume part of B sum-factorised too: no volume B operand
   // operands: weighted IP-term values (with the DPOEQU factors), B operands
   e = P.start(st);
   const int KG = B.prog.n_lt * B.QGp;
@@ -894,12 +1015,13 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
   else if (!sf)
     launch_ip_operand(dim, w.Wip, B.prog.n_lt, nvar, B.tab.R_ip, B.tab.var_of_dof, N, L.QG, B.QGp, nb, w.At, B.nAt,
                       B.Npad, w.s, B.nS, st);
-  launch_operand(dim, w.Wvol, B.prog.n_vb, nvar, B.tab.R_vol, B.tab.var_of_dof, N, L.Q, nb, w.Tv, B.nTv,
-                 (int64_t)L.Q * B.Npad, B.Npad, st);
+  if (!sfb)
+    launch_operand(dim, w.Wvol, B.prog.n_vb, nvar, B.tab.R_vol, B.tab.var_of_dof, N, L.Q, nb, w.Tv, B.nTv,
+                   (int64_t)L.Q * B.Npad, B.Npad, st);
   if (B.prog.n_sb > 0)
     launch_operand(dim, w.Wside, B.prog.n_sb, nvar, B.tab.R_side, B.tab.var_of_dof, N, S * L.Qs, nb, w.Ts, B.nTs,
                    (int64_t)S * L.Qs * B.Npad, B.Npad, st);
-  double opnd = (sf ? 0.0 : (double)KG) + (double)B.prog.n_vb * L.Q + (double)B.prog.n_sb * S * L.Qs;
+  double opnd = (sf ? 0.0 : (double)KG) + (sfb ? 0.0 : (double)B.prog.n_vb * L.Q) + (double)B.prog.n_sb * S * L.Qs;
   P.stop(e, st, DPG_STAGE_OPERAND, nb * opnd * N * 2.0 * B.npr, nb * opnd * N * D8);
   if (sf) {
     // sum-factorised G (unscaled), optional copy-out, then the DPOEQU factors
@@ -983,7 +1105,14 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
   // B^T blocks: rows N + trial offset, columns = test dofs
   e = P.start(st);
   double bfl = 0.0, bby = 0.0;
-  for (int vb = 0; vb < B.prog.n_vb; ++vb) {
+  if (sfb) {
+    launch_bmat_sf(dim, B.sfb_bps, B.sfb_nbp, B.sfb_pairs, B.sfb_F, B.sfb_nF, B.sfb_q, w.Wvol,
+                   B.prog.n_vb * nvar * B.npr, L.Q, w.A, lda, sA, s_ep, B.nS, nb, B.sfb_lds, st);
+    // counted as the dense contraction it replaces; bytes: weights in, B out
+    bfl += 2.0 * L.F * N * L.Q;
+    bby += ((double)B.nWvol + (double)L.F * N) * D8;
+  }
+  for (int vb = 0; vb < (sfb ? 0 : B.prog.n_vb); ++vb) {
     const TrialBlock& blk = L.blocks[B.vb_block[vb]];
     GemmArgs b{};
     b.X = B.tab.field_vol;

@@ -114,6 +114,17 @@ def test_sum_factorised_gram_accuracy(case):
     assert eg < 1e-13
 
 
+@pytest.mark.parametrize("case", ["c1", "c4", "c5", "hex_low"])
+def test_sum_factorised_contractions_meet_the_floor_rule(case):
+    """The sum-factorised path forced on computes G and the volume part of B
+    direction by direction (the B pairs' point weights are the stiffness
+    program's own, with per-cell coefficients on the CDR mesh): K, rhs and the
+    Schur complement meet the FP64 bar against the oracle, G within 1e-13."""
+    cases = {"c1": lambda: poisson2d(4, 4), "c4": lambda: poisson3d(2, 2, 1), "c5": lambda: cdr2d(6, 6),
+             "hex_low": lambda: poisson3d(2, 2, 2, k=2, kt=3)}
+    check_against_oracle(cases[case](), want_G=True, want_schur=True, sumfact=1)
+
+
 @pytest.mark.parametrize("k,kt", [(2, 3), (2, 4), (4, 6)])
 def test_poisson_2d_other_orders(k, kt):
     check_against_oracle(poisson2d(3, 3, k=k, kt=kt), want_G=True, want_schur=True)
