@@ -914,6 +914,38 @@ __device__ inline void sf_body(const SfBlockPair* __restrict__ bps, const SfPair
   __syncthreads();
   double* Ac = A + (size_t)c * sA;
   auto fam = [&](int f, int i) { return Fs + (f * nF + i) * q; };
+  // the last direction for a 2 x 2 block of outputs, rows r0 + i1 + {0, 1},
+  // columns c0 + j1 + {0, 1}: G[i, j] = sum_cls sum_q1 F_cfa(i1,q1) F_cfb(j1,q1)
+  // tp[cls * cstride + q1]. Each output sees the same FMAs in the same order as
+  // alone; the four share their loads of tp and of each family row. Outputs
+  // past the block's edge (odd sizes) or above a diagonal block's diagonal are
+  // computed but not stored.
+  auto final2x2 = [&](int i1, int j1, int r0, int c0, const double* tp0, int cstride) {
+    const int i1b = min(i1 + 1, bp.nA[0] - 1), j1b = min(j1 + 1, bp.nB[0] - 1);
+    if (bp.diag && r0 + i1b < c0 + j1) return;  // all four above the diagonal
+    double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+    for (int cl = 0; cl < ncls; ++cl) {
+      const double *fa0 = fam(bp.clsfa[cl], i1), *fa1 = fam(bp.clsfa[cl], i1b);
+      const double *fb0 = fam(bp.clsfb[cl], j1), *fb1 = fam(bp.clsfb[cl], j1b);
+      const double* tp = tp0 + cl * cstride;
+#pragma unroll
+      for (int q1 = 0; q1 < q; ++q1) {
+        const double tq = tp[q1];
+        acc[0][0] = fma(fa0[q1] * fb0[q1], tq, acc[0][0]);
+        acc[1][0] = fma(fa1[q1] * fb0[q1], tq, acc[1][0]);
+        acc[0][1] = fma(fa0[q1] * fb1[q1], tq, acc[0][1]);
+        acc[1][1] = fma(fa1[q1] * fb1[q1], tq, acc[1][1]);
+      }
+    }
+#pragma unroll
+    for (int da = 0; da < 2; ++da)
+#pragma unroll
+      for (int db = 0; db < 2; ++db) {
+        const int row = r0 + i1 + da, col = c0 + j1 + db;
+        if (i1 + da >= bp.nA[0] || j1 + db >= bp.nB[0] || (bp.diag && row < col)) continue;
+        Ac[(size_t)row + (size_t)col * lda] = MODE == 1 && sc ? acc[da][db] * sc[col] : acc[da][db];
+      }
+  };
   if constexpr (DIM == 3) {
     const int q2n = q * q;
     const int n2 = bp.nA[1] * bp.nB[1];
@@ -948,21 +980,16 @@ __device__ inline void sf_body(const SfBlockPair* __restrict__ bps, const SfPair
           T2[e] = acc;
         }
         __syncthreads();
-        for (int e = t; e < nIA * nJB; e += NT) {
-          int jl = e / nIA, il = e - jl * nIA;
-          int row = bp.offA + il + nIA * i3, col = bp.offB + jl + nJB * j3;
-          if (bp.diag && row < col) continue;
-          int i1 = il % bp.nA[0], i2 = il / bp.nA[0];
-          int j1 = jl % bp.nB[0], j2 = jl / bp.nB[0];
-          double acc = 0.0;
-          for (int cl = 0; cl < ncls; ++cl) {
-            const double* fa = fam(bp.clsfa[cl], i1);
-            const double* fb = fam(bp.clsfb[cl], j1);
-            const double* tp = T2 + (cl * n2 + i2 + bp.nA[1] * j2) * q;
-#pragma unroll
-            for (int q1 = 0; q1 < q; ++q1) acc = fma(fa[q1] * fb[q1], tp[q1], acc);
-          }
-          Ac[(size_t)row + (size_t)col * lda] = MODE == 1 && sc ? acc * sc[col] : acc;
+        const int hA = (bp.nA[0] + 1) >> 1, hB = (bp.nB[0] + 1) >> 1;
+        for (int e = t; e < hA * bp.nA[1] * hB * bp.nB[1]; e += NT) {
+          int r = e;
+          const int a = r % hA;
+          r /= hA;
+          const int i2 = r % bp.nA[1];
+          r /= bp.nA[1];
+          const int b = r % hB, j2 = r / hB;
+          final2x2(2 * a, 2 * b, bp.offA + bp.nA[0] * i2 + nIA * i3, bp.offB + bp.nB[0] * j2 + nJB * j3,
+                   T2 + (i2 + bp.nA[1] * j2) * q, n2 * q);
         }
         __syncthreads();
       }
@@ -984,19 +1011,10 @@ __device__ inline void sf_body(const SfBlockPair* __restrict__ bps, const SfPair
           T1[e] = acc;
         }
         __syncthreads();
-        for (int e = t; e < bp.nA[0] * bp.nB[0]; e += NT) {
-          int j1 = e / bp.nA[0], i1 = e - j1 * bp.nA[0];
-          int row = bp.offA + i1 + bp.nA[0] * i2, col = bp.offB + j1 + bp.nB[0] * j2;
-          if (bp.diag && row < col) continue;
-          double acc = 0.0;
-          for (int cl = 0; cl < ncls; ++cl) {
-            const double* fa = fam(bp.clsfa[cl], i1);
-            const double* fb = fam(bp.clsfb[cl], j1);
-            const double* tp = T1 + cl * q;
-#pragma unroll
-            for (int q1 = 0; q1 < q; ++q1) acc = fma(fa[q1] * fb[q1], tp[q1], acc);
-          }
-          Ac[(size_t)row + (size_t)col * lda] = MODE == 1 && sc ? acc * sc[col] : acc;
+        const int hA = (bp.nA[0] + 1) >> 1, hB = (bp.nB[0] + 1) >> 1;
+        for (int e = t; e < hA * hB; e += NT) {
+          const int b = e / hA, a = e - b * hA;
+          final2x2(2 * a, 2 * b, bp.offA + bp.nA[0] * i2, bp.offB + bp.nB[0] * j2, T1, q);
         }
         __syncthreads();
       }
