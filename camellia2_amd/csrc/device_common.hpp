@@ -53,8 +53,11 @@ struct SfPair {
   int8_t cls, pad;    // class of the direction-1 family pair
 };
 struct SfBlockPair {
-  int32_t offA, offB;  // first test dof of each block (offA >= offB)
+  int32_t offA, offB;  // first row (side A) and column (side B) of the block pair
   int32_t nA[3], nB[3];  // 1D sizes per direction
+  int32_t sB[3];       // column stride of each direction's index (side B)
+  int32_t foff;        // first entry of the block pair's 1D tables
+  int32_t woff;        // first point weight of the block pair (per cell)
   int32_t diag;        // A == B: lower triangle only
   int32_t npairs, pair0, ncls;
   int8_t clsfa[kSfMaxCls], clsfb[kSfMaxCls];  // direction-1 families of each class

@@ -884,16 +884,37 @@ __device__ inline void sf_body(const SfBlockPair* __restrict__ bps, const SfPair
                                int64_t lda, int64_t sA, const double* __restrict__ s, int64_t ss) {
   const int q = QN > 0 ? QN : q_rt;
   extern __shared__ double sm[];
-  const SfBlockPair& bp = bps[blockIdx.x];  // read in place: a local copy, indexed at run time, would live in scratch
-  const int c = blockIdx.y, t = threadIdx.x, NT = blockDim.x;
+  // workgroups are dealt round-robin to the XCDs in launch order (x fastest):
+  // remapped so that each XCD takes every block pair of a run of cells rather
+  // than a fixed few block pairs of every cell (which leaves XCDs unevenly
+  // loaded whenever 8 divides the number of block pairs, and splits a cell's
+  // point weights over eight L2s)
+  const int lin = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const int bpi = lin % gridDim.x, c = lin / gridDim.x;
+  const SfBlockPair& bp = bps[bpi];  // read in place: a local copy, indexed at run time, would live in scratch
+  const int t = threadIdx.x, NT = blockDim.x;
   const int np = bp.npairs, ncls = bp.ncls;
   const SfPair* pr = pairs + bp.pair0;
+  if (np == 0) {
+    // no term couples the two blocks: their entries are zeros (written, as the
+    // dense contraction writes them, over whatever the last call left there)
+    double* Az = A + (size_t)c * sA;
+    const int nAt = bp.nA[0] * bp.nA[1] * bp.nA[2], nB01 = bp.nB[0] * bp.nB[1];
+    for (int e = t; e < nAt * nB01 * bp.nB[2]; e += NT) {
+      const int ia = e % nAt, jb = e / nAt;
+      const int j1 = jb % bp.nB[0], j2 = (jb / bp.nB[0]) % bp.nB[1], j3 = jb / nB01;
+      const int row = bp.offA + ia, col = bp.offB + j1 * bp.sB[0] + j2 * bp.sB[1] + j3 * bp.sB[2];
+      if (bp.diag && row < col) continue;
+      Az[(size_t)row + (size_t)col * lda] = 0.0;
+    }
+    return;  // the whole workgroup: np is the block pair's
+  }
   double* Ms = sm;                 // np * QG
   double* Fs = Ms + np * QG;       // nfam * nF * q
   double* T1 = Fs + nfam * nF * q;  // 3D: np*q*q   2D: ncls*q
   double* T2 = T1 + (DIM == 3 ? np * q * q : ncls * q);  // 3D: ncls * nA2*nB2 * q
   if constexpr (MODE == 0) {
-    const double* W = Wp + (size_t)c * QG * n_lt * na;
+    const double* W = Wp + (size_t)c * QG * n_lt * na + bp.woff;
     for (int e = t; e < np * QG; e += NT) {
       int p = e / QG, qq = e - p * QG;
       const double* w = W + (size_t)qq * n_lt * na;
@@ -902,54 +923,81 @@ __device__ inline void sf_body(const SfBlockPair* __restrict__ bps, const SfPair
       for (int lt = 0; lt < n_lt; ++lt) acc = fma(w[lt * na + a], w[lt * na + b], acc);
       Ms[e] = acc;
     }
-  } else {
-    const double* W = Wp + (size_t)c * QG * na;
+  } else if constexpr (MODE == 1) {
+    const double* W = Wp + (size_t)c * QG * na + bp.woff;
     for (int e = t; e < np * QG; e += NT) {
       int p = e / QG, qq = e - p * QG;
       Ms[e] = W[(size_t)qq * na + pr[p].a];
     }
+  } else {
+    // a side: its points fill the first q^(DIM-1) of the q^DIM slots (the last
+    // direction is the normal one, one point); n_lt holds the sides' points per cell
+    const int qf = DIM == 3 ? q * q : q;
+    const double* W = Wp + (size_t)c * n_lt * na + bp.woff;
+    for (int e = t; e < np * QG; e += NT) {
+      int p = e / QG, qq = e - p * QG;
+      Ms[e] = qq < qf ? W[(size_t)qq * na + pr[p].a] : 0.0;
+    }
   }
-  const double* sc = MODE == 1 && s ? s + (size_t)c * ss : nullptr;
-  for (int e = t; e < nfam * nF * q; e += NT) Fs[e] = F[e];
+  const double* sc = MODE >= 1 && s ? s + (size_t)c * ss : nullptr;
+  for (int e = t; e < nfam * nF * q; e += NT) Fs[e] = F[bp.foff + e];
   __syncthreads();
   double* Ac = A + (size_t)c * sA;
   auto fam = [&](int f, int i) { return Fs + (f * nF + i) * q; };
-  // the last direction for a 2 x 2 block of outputs, rows r0 + i1 + {0, 1},
-  // columns c0 + j1 + {0, 1}: G[i, j] = sum_cls sum_q1 F_cfa(i1,q1) F_cfb(j1,q1)
-  // tp[cls * cstride + q1]. Each output sees the same FMAs in the same order as
-  // alone; the four share their loads of tp and of each family row. Outputs
-  // past the block's edge (odd sizes) or above a diagonal block's diagonal are
-  // computed but not stored.
-  auto final2x2 = [&](int i1, int j1, int r0, int c0, const double* tp0, int cstride) {
-    const int i1b = min(i1 + 1, bp.nA[0] - 1), j1b = min(j1 + 1, bp.nB[0] - 1);
-    if (bp.diag && r0 + i1b < c0 + j1) return;  // all four above the diagonal
-    double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+  // the last direction for a BI x BJ block of outputs, rows r0 + i1 + {0..BI),
+  // columns c0 + (j1 + {0..BJ)) * sB[0]: G[i, j] = sum_cls sum_q1 F_cfa(i1,q1)
+  // F_cfb(j1,q1) tp[cls * cstride + q1]. Each output sees the same FMAs in the
+  // same order as alone; a block shares its loads of tp and of each family row.
+  // Outputs past the block's edge (odd sizes) or above a diagonal block's
+  // diagonal are computed but not stored. 2 x 2 for G (LDS already limits its
+  // workgroups per CU), 1 x 1 for B, whose small workgroups are limited by
+  // registers.
+  auto final_block = [&](auto bi_c, auto bj_c, int i1, int j1, int r0, int c0, const double* tp0, int cstride) {
+    constexpr int BI = decltype(bi_c)::value, BJ = decltype(bj_c)::value;
+    const int cs = bp.sB[0];
+    int ii[BI], jj[BJ];
+#pragma unroll
+    for (int d = 0; d < BI; ++d) ii[d] = min(i1 + d, bp.nA[0] - 1);
+#pragma unroll
+    for (int d = 0; d < BJ; ++d) jj[d] = min(j1 + d, bp.nB[0] - 1);
+    if (bp.diag && r0 + ii[BI - 1] < c0 + j1 * cs) return;  // all above the diagonal
+    double acc[BI][BJ];
+#pragma unroll
+    for (int da = 0; da < BI; ++da)
+#pragma unroll
+      for (int db = 0; db < BJ; ++db) acc[da][db] = 0.0;
     for (int cl = 0; cl < ncls; ++cl) {
-      const double *fa0 = fam(bp.clsfa[cl], i1), *fa1 = fam(bp.clsfa[cl], i1b);
-      const double *fb0 = fam(bp.clsfb[cl], j1), *fb1 = fam(bp.clsfb[cl], j1b);
+      const double* fa[BI];
+      const double* fb[BJ];
+#pragma unroll
+      for (int d = 0; d < BI; ++d) fa[d] = fam(bp.clsfa[cl], ii[d]);
+#pragma unroll
+      for (int d = 0; d < BJ; ++d) fb[d] = fam(bp.clsfb[cl], jj[d]);
       const double* tp = tp0 + cl * cstride;
 #pragma unroll
       for (int q1 = 0; q1 < q; ++q1) {
         const double tq = tp[q1];
-        acc[0][0] = fma(fa0[q1] * fb0[q1], tq, acc[0][0]);
-        acc[1][0] = fma(fa1[q1] * fb0[q1], tq, acc[1][0]);
-        acc[0][1] = fma(fa0[q1] * fb1[q1], tq, acc[0][1]);
-        acc[1][1] = fma(fa1[q1] * fb1[q1], tq, acc[1][1]);
+#pragma unroll
+        for (int db = 0; db < BJ; ++db)
+#pragma unroll
+          for (int da = 0; da < BI; ++da) acc[da][db] = fma(fa[da][q1] * fb[db][q1], tq, acc[da][db]);
       }
     }
 #pragma unroll
-    for (int da = 0; da < 2; ++da)
+    for (int da = 0; da < BI; ++da)
 #pragma unroll
-      for (int db = 0; db < 2; ++db) {
-        const int row = r0 + i1 + da, col = c0 + j1 + db;
+      for (int db = 0; db < BJ; ++db) {
+        const int row = r0 + i1 + da, col = c0 + (j1 + db) * cs;
         if (i1 + da >= bp.nA[0] || j1 + db >= bp.nB[0] || (bp.diag && row < col)) continue;
-        Ac[(size_t)row + (size_t)col * lda] = MODE == 1 && sc ? acc[da][db] * sc[col] : acc[da][db];
+        Ac[(size_t)row + (size_t)col * lda] = MODE >= 1 && sc ? acc[da][db] * sc[col] : acc[da][db];
       }
   };
+  using one_c = std::integral_constant<int, 1>;
+  using two_c = std::integral_constant<int, 2>;
   if constexpr (DIM == 3) {
     const int q2n = q * q;
     const int n2 = bp.nA[1] * bp.nB[1];
-    const int nIA = bp.nA[0] * bp.nA[1], nJB = bp.nB[0] * bp.nB[1];
+    const int nIA = bp.nA[0] * bp.nA[1];
     for (int i3 = 0; i3 < bp.nA[2]; ++i3)
       for (int j3 = 0; j3 < bp.nB[2]; ++j3) {
         if (bp.diag && j3 > i3) continue;
@@ -981,15 +1029,28 @@ __device__ inline void sf_body(const SfBlockPair* __restrict__ bps, const SfPair
         }
         __syncthreads();
         const int hA = (bp.nA[0] + 1) >> 1, hB = (bp.nB[0] + 1) >> 1;
-        for (int e = t; e < hA * bp.nA[1] * hB * bp.nB[1]; e += NT) {
-          int r = e;
-          const int a = r % hA;
-          r /= hA;
-          const int i2 = r % bp.nA[1];
-          r /= bp.nA[1];
-          const int b = r % hB, j2 = r / hB;
-          final2x2(2 * a, 2 * b, bp.offA + bp.nA[0] * i2 + nIA * i3, bp.offB + bp.nB[0] * j2 + nJB * j3,
-                   T2 + (i2 + bp.nA[1] * j2) * q, n2 * q);
+        if constexpr (MODE == 0) {
+          for (int e = t; e < hA * bp.nA[1] * hB * bp.nB[1]; e += NT) {
+            int r = e;
+            const int a = r % hA;
+            r /= hA;
+            const int i2 = r % bp.nA[1];
+            r /= bp.nA[1];
+            const int b = r % hB, j2 = r / hB;
+            final_block(two_c{}, two_c{}, 2 * a, 2 * b, bp.offA + bp.nA[0] * i2 + nIA * i3,
+                        bp.offB + bp.sB[1] * j2 + bp.sB[2] * j3, T2 + (i2 + bp.nA[1] * j2) * q, n2 * q);
+          }
+        } else {
+          for (int e = t; e < nIA * bp.nB[0] * bp.nB[1]; e += NT) {
+            int r = e;
+            const int i1 = r % bp.nA[0];
+            r /= bp.nA[0];
+            const int i2 = r % bp.nA[1];
+            r /= bp.nA[1];
+            const int j1 = r % bp.nB[0], j2 = r / bp.nB[0];
+            final_block(one_c{}, one_c{}, i1, j1, bp.offA + bp.nA[0] * i2 + nIA * i3,
+                        bp.offB + bp.sB[1] * j2 + bp.sB[2] * j3, T2 + (i2 + bp.nA[1] * j2) * q, n2 * q);
+          }
         }
         __syncthreads();
       }
@@ -1012,9 +1073,16 @@ __device__ inline void sf_body(const SfBlockPair* __restrict__ bps, const SfPair
         }
         __syncthreads();
         const int hA = (bp.nA[0] + 1) >> 1, hB = (bp.nB[0] + 1) >> 1;
-        for (int e = t; e < hA * hB; e += NT) {
-          const int b = e / hA, a = e - b * hA;
-          final2x2(2 * a, 2 * b, bp.offA + bp.nA[0] * i2, bp.offB + bp.nB[0] * j2, T1, q);
+        if constexpr (MODE == 0) {
+          for (int e = t; e < hA * hB; e += NT) {
+            const int b = e / hA, a = e - b * hA;
+            final_block(two_c{}, two_c{}, 2 * a, 2 * b, bp.offA + bp.nA[0] * i2, bp.offB + bp.sB[1] * j2, T1, q);
+          }
+        } else {
+          for (int e = t; e < bp.nA[0] * bp.nB[0]; e += NT) {
+            const int j1 = e / bp.nA[0], i1 = e - j1 * bp.nA[0];
+            final_block(one_c{}, one_c{}, i1, j1, bp.offA + bp.nA[0] * i2, bp.offB + bp.sB[1] * j2, T1, q);
+          }
         }
         __syncthreads();
       }
@@ -1028,6 +1096,17 @@ __global__ __launch_bounds__(256) void gram_sf_kernel(const SfBlockPair* __restr
                                                       const double* __restrict__ Wip, int n_lt, int na, int QG,
                                                       double* __restrict__ A, int64_t lda, int64_t sA) {
   sf_body<DIM, QN, 0>(bps, pairs, F, nF, 3, q_rt, Wip, n_lt, na, QG, A, lda, sA, nullptr, 0);
+}
+template <int DIM, int QN>
+__global__ __launch_bounds__(256) void bmat_side_sf_kernel(const SfBlockPair* __restrict__ bps,
+                                                           const SfPair* __restrict__ pairs,
+                                                           const double* __restrict__ F, int nF, int nfam, int q_rt,
+                                                           const double* __restrict__ Wside, int npts, int na,
+                                                           double* __restrict__ A, int64_t lda, int64_t sA,
+                                                           const double* __restrict__ s, int64_t ss) {
+  const int q = QN > 0 ? QN : q_rt;
+  sf_body<DIM, QN, 2>(bps, pairs, F, nF, nfam, q_rt, Wside, npts, na, DIM == 3 ? q * q * q : q * q, A, lda, sA, s,
+                      ss);
 }
 template <int DIM, int QN>
 __global__ __launch_bounds__(256) void bmat_sf_kernel(const SfBlockPair* __restrict__ bps,
@@ -2340,6 +2419,38 @@ void launch_bmat_sf(int dim, const SfBlockPair* bps, int nbp, const SfPair* pair
   }
 #undef DPG_SFB_CASE
   check_launch("bmat_sf_kernel");
+}
+
+void launch_bmat_side_sf(int dim, const SfBlockPair* bps, int nbp, const SfPair* pairs, const double* F, int nF,
+                         int nfam, int q, const double* Wside, int npts, int na, double* A, int64_t lda, int64_t sA,
+                         const double* s, int64_t ss, int ncells, size_t lds_bytes, hipStream_t st) {
+  if (ncells <= 0 || nbp <= 0) return;
+  if (lds_bytes > 64 * 1024) throw DeviceError("bmat_side_sf: LDS request above 64 KiB");
+  dim3 grid(nbp, ncells);
+#define DPG_SFS_CASE(D, QQ)                                                                                     \
+  case QQ:                                                                                                      \
+    bmat_side_sf_kernel<D, QQ><<<grid, 256, lds_bytes, st>>>(bps, pairs, F, nF, nfam, q, Wside, npts, na, A, lda, \
+                                                              sA, s, ss);                                       \
+    break;
+  if (dim == 3) {
+    switch (q) {
+      DPG_SFS_CASE(3, 3) DPG_SFS_CASE(3, 4) DPG_SFS_CASE(3, 5) DPG_SFS_CASE(3, 6) DPG_SFS_CASE(3, 7)
+      DPG_SFS_CASE(3, 8)
+      default:
+        bmat_side_sf_kernel<3, 0><<<grid, 256, lds_bytes, st>>>(bps, pairs, F, nF, nfam, q, Wside, npts, na, A, lda,
+                                                                 sA, s, ss);
+    }
+  } else {
+    switch (q) {
+      DPG_SFS_CASE(2, 3) DPG_SFS_CASE(2, 4) DPG_SFS_CASE(2, 5) DPG_SFS_CASE(2, 6) DPG_SFS_CASE(2, 7)
+      DPG_SFS_CASE(2, 8) DPG_SFS_CASE(2, 9) DPG_SFS_CASE(2, 10)
+      default:
+        bmat_side_sf_kernel<2, 0><<<grid, 256, lds_bytes, st>>>(bps, pairs, F, nF, nfam, q, Wside, npts, na, A, lda,
+                                                                 sA, s, ss);
+    }
+  }
+#undef DPG_SFS_CASE
+  check_launch("bmat_side_sf_kernel");
 }
 
 // a kernel rather than hipMemsetAsync, so that captured graphs replay it reliably

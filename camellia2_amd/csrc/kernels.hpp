@@ -89,6 +89,14 @@ void launch_gram_sf(int dim, const SfBlockPair* bps, int nbp, const SfPair* pair
 void launch_bmat_sf(int dim, const SfBlockPair* bps, int nbp, const SfPair* pairs, const double* F, int nF, int q,
                     const double* Wvol, int na, int Q, double* A, int64_t lda, int64_t sA, const double* s,
                     int64_t ss, int ncells, size_t lds_bytes, hipStream_t st);
+// the side part of B, sum-factorised per side: side A a trace or flux block on
+// one side (rows N + its offset), side B a test block restricted to the side;
+// nfam 1D families per block pair from F + foff (tangential directions on the
+// side's points, the normal one at the side's coordinate), point weights Wside
+// (npts side points per cell, na per point, the pair's side from woff)
+void launch_bmat_side_sf(int dim, const SfBlockPair* bps, int nbp, const SfPair* pairs, const double* F, int nF,
+                         int nfam, int q, const double* Wside, int npts, int na, double* A, int64_t lda, int64_t sA,
+                         const double* s, int64_t ss, int ncells, size_t lds_bytes, hipStream_t st);
 // the same contraction with FP64 operands converted on load to FP32 (precision
 // DPG_PREC_FP32) or to hi+lo BF16/FP16 pairs (DPG_PREC_BF16X3/FP16X3, products
 // hi*hi + hi*lo + lo*hi), FP32 MFMA accumulation, FP64 epilogue

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -250,6 +252,14 @@ struct BucketPlan {
   const SfBlockPair* sfb_bps = nullptr;
   const SfPair* sfb_pairs = nullptr;
   const double* sfb_F = nullptr;
+  // ... and its side part (with sfb_ok): block pairs (trace or flux block on a
+  // side, test block), per-side 1D tables of 5 * dim families
+  bool sfs_ok = false;
+  int sfs_nbp = 0, sfs_nF = 0, sfs_nfam = 0, sfs_q = 0;
+  size_t sfs_lds = 0;
+  const SfBlockPair* sfs_bps = nullptr;
+  const SfPair* sfs_pairs = nullptr;
+  const double* sfs_F = nullptr;
 };
 
 // ---- static scratch of dpg_assemble_noalloc ---------------------------------
@@ -463,6 +473,9 @@ void build_sum_factorisation(dpg_plan* plan, BucketPlan& B) {
         bp.nA[d] = A.n[d];
         bp.nB[d] = Bb.n[d];
       }
+      bp.sB[0] = 1;
+      bp.sB[1] = Bb.n[0];
+      bp.sB[2] = Bb.n[0] * Bb.n[1];
       bp.diag = ia == ib;
       bp.pair0 = (int)pairs.size();
       for (int ra = 0; ra < A.nprim; ++ra)
@@ -489,7 +502,7 @@ void build_sum_factorisation(dpg_plan* plan, BucketPlan& B) {
           pairs.push_back(p);
         }
       bp.npairs = (int)pairs.size() - bp.pair0;
-      if (bp.npairs > 0) bps.push_back(bp);
+      bps.push_back(bp);  // without pairs too: it writes the block's zeros over the last call's factor
     }
   // 1D tables on the Gram points: L, L', O (O padded with a zero row)
   const int q = L.QG1, nF = kt + 1;
@@ -532,6 +545,7 @@ unsigned prims_of_op(int op, int dim) {
   if (op >= DPG_OP_DX && op < DPG_OP_DX + dim) return ((1u << dim) - 1u) << 1;
   if (op >= DPG_OP_X && op < DPG_OP_X + dim) return (1u << dim) - 1u;
   if (op == DPG_OP_DIV) return 1u << dim;
+  if (op == DPG_OP_DOT_NORMAL) return (1u << dim) - 1u;
   return 0u;
 }
 
@@ -564,6 +578,9 @@ void build_sum_factorisation_b(dpg_plan* plan, BucketPlan& B) {
         bp.nA[d] = d < dim ? nfl : 1;
         bp.nB[d] = tb.n[d];
       }
+      bp.sB[0] = 1;
+      bp.sB[1] = tb.n[0];
+      bp.sB[2] = tb.n[0] * tb.n[1];
       bp.diag = 0;
       bp.pair0 = (int)pairs.size();
       for (int ri = 0; ri < tb.nprim; ++ri) {
@@ -588,7 +605,7 @@ void build_sum_factorisation_b(dpg_plan* plan, BucketPlan& B) {
         pairs.push_back(p);
       }
       bp.npairs = (int)pairs.size() - bp.pair0;
-      if (bp.npairs > 0) bps.push_back(bp);
+      bps.push_back(bp);  // without pairs too: it writes the block's zeros (the dense GEMM writes every column)
     }
   }
   // 1D tables on the stiffness points: L, L', O (padded with a zero row), Fld
@@ -626,6 +643,158 @@ void build_sum_factorisation_b(dpg_plan* plan, BucketPlan& B) {
   B.sfb_ok = true;
 }
 
+// The side part of B the same way, one side at a time: B[t, j] = sum_q sum_r
+// W_s[q][g][var j][r] phi_t(q) R_r(j, x_s(q)) for the trace or flux block of
+// group g on side s (a tensor-product basis in the side's coordinates) and the
+// test block of j restricted to the side. The kernel's directions are the
+// side's coordinates (the reference directions they run along, with their
+// signs, from the affine side map) and last the normal direction, whose one
+// point is the side's reference coordinate; the test block's columns keep
+// their reference order through per-direction strides.
+void build_sum_factorisation_side(dpg_plan* plan, BucketPlan& B) {
+  const dpg_form_desc& f = plan->form;
+  const ElementLayout& L = B.L;
+  if (!B.sfb_ok) return;
+  const int dim = L.dim, npr = dim + 1, kt = L.kt, nvar = f.n_test, S = L.rc.nsides, q = L.Qs1;
+  const DevProgram& P = B.prog;
+  const int nfam = 5 * dim, na = P.n_sb * nvar * npr;
+  // family of base b along kernel direction k; b: 0 L, 1 L', 2 O, 3 the trace
+  // basis, 4 the flux basis (3 and 4 along the normal direction: the constant 1)
+  auto id = [](int k, int base) { return k * 5 + base; };
+  const std::vector<SfTestBlock> blocks = sf_test_blocks(f, L);
+  LagrangeLine Cl(gll_points(kt)), Op(gauss_legendre(kt).x);
+  std::unique_ptr<LagrangeLine> Tr, Fl;
+  if (L.trace_deg >= 1) Tr.reset(new LagrangeLine(gll_points(L.trace_deg)));
+  if (L.flux_deg >= 1) Fl.reset(new LagrangeLine(gll_points(L.flux_deg)));
+  int nF = kt + 1;
+  if (Tr) nF = std::max(nF, Tr->size());
+  if (Fl) nF = std::max(nF, Fl->size());
+  // side geometry: reference direction and sign of each side coordinate, the normal direction and its coordinate
+  std::vector<std::array<int, 3>> dirs(S);
+  std::vector<std::array<double, 3>> sign(S);
+  for (int sd = 0; sd < S; ++sd) {
+    const double* p = &L.rc.side_param[(size_t)sd * 3 * dim];
+    int used = 0;
+    for (int a = 0; a < dim - 1; ++a) {
+      int dd = -1;
+      for (int d = 0; d < dim; ++d)
+        if (p[(1 + a) * dim + d] != 0.0) {
+          if (dd >= 0 || std::fabs(p[(1 + a) * dim + d]) != 1.0 || p[d] != 0.0) return;  // not axis-aligned
+          dd = d;
+        }
+      if (dd < 0) return;
+      dirs[sd][a] = dd;
+      sign[sd][a] = p[(1 + a) * dim + dd];
+      used |= 1 << dd;
+    }
+    for (int d = 0; d < dim; ++d)
+      if (!((used >> d) & 1)) {
+        if (std::fabs(p[d]) != 1.0) return;
+        dirs[sd][dim - 1] = d;
+        sign[sd][dim - 1] = p[d];  // the normal direction's coordinate
+      }
+  }
+  // per side: nfam * nF * q table entries
+  Rule1D r = gauss_legendre(q);
+  std::vector<double> F((size_t)S * nfam * nF * q, 0.0);
+  std::vector<double> v(nF), dv(nF), o(nF), tb1(nF);
+  for (int sd = 0; sd < S; ++sd) {
+    double* Fs = &F[(size_t)sd * nfam * nF * q];
+    for (int k = 0; k < dim; ++k) {
+      const bool normal = k == dim - 1;
+      for (int j = 0; j < (normal ? 1 : q); ++j) {
+        const double xi = normal ? sign[sd][k] : sign[sd][k] * r.x[j];
+        Cl.eval(xi, v.data(), dv.data());
+        Op.eval(xi, o.data(), nullptr);
+        for (int i = 0; i < nF; ++i) {
+          Fs[((size_t)id(k, 0) * nF + i) * q + j] = i <= kt ? v[i] : 0.0;
+          Fs[((size_t)id(k, 1) * nF + i) * q + j] = i <= kt ? dv[i] : 0.0;
+          Fs[((size_t)id(k, 2) * nF + i) * q + j] = i < kt ? o[i] : 0.0;
+        }
+        for (int b = 3; b < 5; ++b) {
+          const LagrangeLine* T1d = b == 3 ? Tr.get() : Fl.get();
+          if (normal) {
+            Fs[((size_t)id(k, b) * nF + 0) * q + j] = 1.0;
+          } else if (T1d) {  // the trial bases along the side coordinate itself
+            T1d->eval(r.x[j], tb1.data(), nullptr);
+            for (int i = 0; i < T1d->size(); ++i) Fs[((size_t)id(k, b) * nF + i) * q + j] = tb1[i];
+          }
+        }
+      }
+    }
+  }
+  std::vector<SfBlockPair> bps;
+  std::vector<SfPair> pairs;
+  for (int g = 0; g < P.n_sb; ++g) {
+    std::vector<unsigned> reach(nvar, 0u);
+    for (int si = P.sb_beg[g]; si < P.sb_beg[g + 1]; ++si) reach[P.s[si].var] |= prims_of_op(P.s[si].op, dim);
+    for (const TrialBlock& blk : L.blocks) {
+      if (blk.var != B.sb_var[g] || blk.side < 0) continue;
+      const LagrangeLine* T1d = blk.kind == DPG_TRACE ? Tr.get() : Fl.get();
+      if (!T1d) return;
+      const int n1 = T1d->size();
+      if ((dim == 3 ? n1 * n1 : n1) != blk.card) return;
+      const int sd = blk.side, tbase = blk.kind == DPG_TRACE ? 3 : 4;
+      for (const SfTestBlock& tb : blocks) {
+        SfBlockPair bp{};
+        bp.offA = L.N + blk.offset;
+        bp.offB = tb.off;
+        int stride[3] = {1, tb.n[0], tb.n[0] * tb.n[1]};
+        for (int k = 0; k < 3; ++k) {
+          bp.nA[k] = k < dim - 1 ? n1 : 1;
+          bp.nB[k] = k < dim ? tb.n[dirs[sd][k]] : 1;
+          bp.sB[k] = k < dim ? stride[dirs[sd][k]] : 0;
+        }
+        bp.foff = sd * nfam * nF * q;
+        bp.woff = sd * L.Qs * na;
+        bp.diag = 0;
+        bp.pair0 = (int)pairs.size();
+        for (int ri = 0; ri < tb.nprim; ++ri) {
+          const int pr = tb.prim[ri];
+          if (!((reach[tb.var] >> pr) & 1u)) continue;
+          SfPair p{};
+          p.a = p.b = (int16_t)((g * nvar + tb.var) * npr + pr);
+          for (int k = 0; k < 3; ++k) {
+            p.fa[k] = (int8_t)(k < dim ? id(k, tbase) : 0);
+            p.fb[k] = (int8_t)(k < dim ? id(k, tb.fam[ri][dirs[sd][k]]) : 0);
+          }
+          int cl = -1;
+          for (int c2 = 0; c2 < bp.ncls; ++c2)
+            if (bp.clsfa[c2] == p.fa[0] && bp.clsfb[c2] == p.fb[0]) cl = c2;
+          if (cl < 0) {
+            if (bp.ncls >= kSfMaxCls) throw std::logic_error("sum factorisation: too many classes");
+            cl = bp.ncls++;
+            bp.clsfa[cl] = p.fa[0];
+            bp.clsfb[cl] = p.fb[0];
+          }
+          p.cls = (int8_t)cl;
+          pairs.push_back(p);
+        }
+        bp.npairs = (int)pairs.size() - bp.pair0;
+        bps.push_back(bp);  // without pairs too: it writes the block's zeros
+      }
+    }
+  }
+  const int QP = dim == 3 ? q * q * q : q * q;  // the kernel's point slots per pair
+  size_t lds = 0;
+  for (const SfBlockPair& bp : bps) {
+    size_t t1 = dim == 3 ? (size_t)bp.npairs * q * q : (size_t)bp.ncls * q;
+    size_t t2 = dim == 3 ? (size_t)bp.ncls * bp.nA[1] * bp.nB[1] * q : 0;
+    lds = std::max(lds, sizeof(double) * ((size_t)bp.npairs * QP + (size_t)nfam * nF * q + t1 + t2));
+  }
+  if (bps.empty() || lds > 64 * 1024) return;
+  auto& own = *plan->tables;
+  B.sfs_bps = upload(own, bps);
+  B.sfs_pairs = upload(own, pairs);
+  B.sfs_F = upload(own, F);
+  B.sfs_nbp = (int)bps.size();
+  B.sfs_nF = nF;
+  B.sfs_nfam = nfam;
+  B.sfs_q = q;
+  B.sfs_lds = lds;
+  B.sfs_ok = true;
+}
+
 void build_bucket(dpg_plan* plan, const dpg_bucket_desc& desc, BucketPlan& B) {
   const dpg_form_desc& f = plan->form;
   B.desc = desc;
@@ -645,6 +814,7 @@ void build_bucket(dpg_plan* plan, const dpg_bucket_desc& desc, BucketPlan& B) {
   compile_program(f, B);
   build_sum_factorisation(plan, B);
   build_sum_factorisation_b(plan, B);
+  build_sum_factorisation_side(plan, B);
   RefTables T = make_tables(f, L);
   auto& own = *plan->tables;
   B.tab.R_vol = upload(own, T.vol.R);
@@ -1004,6 +1174,7 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
   const int prec = plan->opt.precision;
   const bool sf = B.sf_on && prec == DPG_PREC_FP64;
   const bool sfb = sf && B.sfb_ok;  // the volume part of B sum-factorised too: no volume B operand
+  const bool sfs = sfb && B.sfs_ok;  // and its side part: no side B operand
   // operands: weighted IP-term values (with the DPOEQU factors), B operands
   e = P.start(st);
   const int KG = B.prog.n_lt * B.QGp;
@@ -1018,10 +1189,11 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
   if (!sfb)
     launch_operand(dim, w.Wvol, B.prog.n_vb, nvar, B.tab.R_vol, B.tab.var_of_dof, N, L.Q, nb, w.Tv, B.nTv,
                    (int64_t)L.Q * B.Npad, B.Npad, st);
-  if (B.prog.n_sb > 0)
+  if (B.prog.n_sb > 0 && !sfs)
     launch_operand(dim, w.Wside, B.prog.n_sb, nvar, B.tab.R_side, B.tab.var_of_dof, N, S * L.Qs, nb, w.Ts, B.nTs,
                    (int64_t)S * L.Qs * B.Npad, B.Npad, st);
-  double opnd = (sf ? 0.0 : (double)KG) + (sfb ? 0.0 : (double)B.prog.n_vb * L.Q) + (double)B.prog.n_sb * S * L.Qs;
+  double opnd = (sf ? 0.0 : (double)KG) + (sfb ? 0.0 : (double)B.prog.n_vb * L.Q) +
+                (sfs ? 0.0 : (double)B.prog.n_sb * S * L.Qs);
   P.stop(e, st, DPG_STAGE_OPERAND, nb * opnd * N * 2.0 * B.npr, nb * opnd * N * D8);
   if (sf) {
     // sum-factorised G (unscaled), optional copy-out, then the DPOEQU factors
@@ -1141,7 +1313,13 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
     bfl += 2.0 * b.m * b.n * b.k;
     bby += ((double)b.k * b.n + (double)b.m * b.n) * D8;
   }
-  for (int sb = 0; sb < B.prog.n_sb; ++sb) {
+  if (sfs) {
+    launch_bmat_side_sf(dim, B.sfs_bps, B.sfs_nbp, B.sfs_pairs, B.sfs_F, B.sfs_nF, B.sfs_nfam, B.sfs_q, w.Wside,
+                        S * L.Qs, B.prog.n_sb * nvar * B.npr, w.A, lda, sA, s_ep, B.nS, nb, B.sfs_lds, st);
+    bfl += 2.0 * L.T * N * L.Qs;  // counted as the side contractions it replaces
+    bby += ((double)B.nWside + (double)L.T * N) * D8;
+  }
+  for (int sb = 0; sb < (sfs ? 0 : B.prog.n_sb); ++sb) {
     int v = B.sb_var[sb];
     for (const TrialBlock& blk : L.blocks) {
       if (blk.var != v || blk.side < 0) continue;

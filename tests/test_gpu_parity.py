@@ -125,6 +125,36 @@ def test_sum_factorised_contractions_meet_the_floor_rule(case):
     check_against_oracle(cases[case](), want_G=True, want_schur=True, sumfact=1)
 
 
+@pytest.mark.parametrize("case", ["c4", "c5", "c1"])
+def test_sum_factorised_path_over_a_used_workspace(case):
+    """The sum-factorised G and B write every entry of their rows, the zeros
+    of uncoupled blocks included (the factorisation leaves other values
+    there): over a workspace that held another mesh's factorisation a call
+    gives the bits of a fresh plan."""
+    cases = {"c4": lambda s: poisson3d(2, 2, 1, seed=s), "c5": lambda s: cdr2d(6, 6, seed=s),
+             "c1": lambda s: poisson2d(4, 4, seed=s)}
+    other, case_ = cases[case](12345), cases[case](0xC0FFEE + 7)
+
+    def run(A, c):
+        f, b, nodes, par, coefs, load = c
+        o, nbad = A.local_stiffness_and_rhs(0, dev(nodes), dev(par), dev(coefs), load, want_schur=True)
+        assert nbad == 0
+        return {k: v.clone() for k, v in o.items()}
+
+    f, b = case_[0], case_[1]
+    A = dpg.LocalAssembler(f, [b])
+    A.set_tiles(0, 64, 64, 64, 1)
+    run(A, other)
+    used = run(A, case_)
+    A.close()
+    A = dpg.LocalAssembler(f, [b])
+    A.set_tiles(0, 64, 64, 64, 1)
+    fresh = run(A, case_)
+    A.close()
+    for k in ("K", "rhs", "schur", "schur_rhs"):
+        assert torch.equal(used[k], fresh[k]), k
+
+
 @pytest.mark.parametrize("k,kt", [(2, 3), (2, 4), (4, 6)])
 def test_poisson_2d_other_orders(k, kt):
     check_against_oracle(poisson2d(3, 3, k=k, kt=kt), want_G=True, want_schur=True)
