@@ -364,9 +364,10 @@ int dpg_adjoint(dpg_plan* plan, int bucket, const dpg_batch* batch, const double
  * other on the CPU (Solution.cpp:1162-1298); these serve the same loop. */
 
 /* FP64 contraction choices of one bucket: GEMM tile edge (64 or 32) per role,
- * and whether G comes from the sum-factorised kernel (quads and hexes only).
- * sumfact = 1 contracts G = sum_q M(q) R R direction by direction (about 60x
- * fewer flops than A^T A on the 32^3 hex workload). Its accuracy depends on the
+ * and whether G and B come from the sum-factorised kernels (quads and hexes
+ * only). sumfact = 1 contracts G = sum_q M(q) R R direction by direction
+ * (about 60x fewer flops than A^T A on the 32^3 hex workload), and B's volume
+ * and side parts the same way from the point weights (no B operand). Its accuracy depends on the
  * form and the mesh: on the 32^3 hexes K is 9e-11 from an 80-bit evaluation
  * (the dense G: 6.6e-10; far fewer roundings per entry), but where expanding
  * the graph norm's squares into separate metric terms cancels it is far worse
@@ -380,7 +381,7 @@ typedef struct {
                       and solved against it in one kernel on
                       128 x 64 tiles (the diagonal blocks and the
                       trailing update then take 64)                 */
-  int32_t sumfact; /* 1: sum-factorised G (see above)               */
+  int32_t sumfact; /* 1: sum-factorised G and B (see above)         */
 } dpg_tiles;
 int dpg_plan_get_tiles(const dpg_plan* plan, int bucket, dpg_tiles* out);
 int dpg_plan_set_tiles(dpg_plan* plan, int bucket, const dpg_tiles* tiles);
