@@ -1410,10 +1410,11 @@ void assemble_batch(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, in
 // second rounding of the dense G does (the same G with its equal IP terms summed
 // separately instead of merged, i.e. another FP64 summation order of the same
 // products). A form without mergeable terms has no second rounding (the two
-// dense K agree bit for bit), and then the sum-factorised G is not taken. On the 32^3-hex workload it passes (K 9e-11 from an 80-bit
-// evaluation against 6.6e-10 for the dense G: far fewer roundings per entry);
-// where expanding the graph norm's squares into separate metric terms cancels
-// (2D Stokes at h = 1/128: K 1e-1 off) it fails and G stays dense.
+// dense K agree bit for bit), and then the sum-factorised G is not taken. On
+// the 32^3-hex workload it passes (K 9e-11 from an 80-bit evaluation against
+// 6.6e-10 for the dense G: far fewer roundings per entry). The timing runs over
+// the whole sample: on the 2D Stokes mesh the sum-factorised path wins on a
+// few dozen cells and loses on thousands, and G stays dense there.
 void choose_gram_path(dpg_plan* plan, BucketPlan& B, const dpg_batch& bt, int ns, const Work& w, hipStream_t st) {
   const int N = B.L.N, M = B.L.M;
   const int64_t mm = (int64_t)M * M;
@@ -1424,14 +1425,17 @@ void choose_gram_path(dpg_plan* plan, BucketPlan& B, const dpg_batch& bt, int ns
   kx.ensure(sizeof(double) * ng * mm);
   nrm.ensure(sizeof(double) * 2 * ng);
   std::vector<double> h(2 * (size_t)ng);
+  // timed on the whole sample (a few dozen cells would favour the sum-factorised
+  // kernels, whose small workgroups fill the GPU sooner than the GEMM tiles);
+  // K of its first ng cells compared
   auto run = [&](double* K) {  // warm pass, timed pass; K of the timed pass
     double t = 0.0;
     for (int pass = 0; pass < 2; ++pass) {
       plan->prof.acc = dpg_timings{};
-      stage_factor(plan, B, bt, 0, ng, w, nullptr, nullptr, st);
+      stage_factor(plan, B, bt, 0, ns, w, nullptr, nullptr, st);
       launch_extract_sym(w.A, B.lda, B.nA, N, M, -1.0, K, ng, st);
-      stage_schur(plan, B, ng, w, st);
-      plan->prof.collect(st, ng);
+      stage_schur(plan, B, ns, w, st);
+      plan->prof.collect(st, ns);
       t = 0.0;
       for (int k = 0; k < DPG_NSTAGES; ++k) t += plan->prof.acc.ms[k];
     }

@@ -367,12 +367,11 @@ int dpg_adjoint(dpg_plan* plan, int bucket, const dpg_batch* batch, const double
  * and whether G and B come from the sum-factorised kernels (quads and hexes
  * only). sumfact = 1 contracts G = sum_q M(q) R R direction by direction
  * (about 60x fewer flops than A^T A on the 32^3 hex workload), and B's volume
- * and side parts the same way from the point weights (no B operand). Its accuracy depends on the
- * form and the mesh: on the 32^3 hexes K is 9e-11 from an 80-bit evaluation
- * (the dense G: 6.6e-10; far fewer roundings per entry), but where expanding
- * the graph norm's squares into separate metric terms cancels it is far worse
- * (2D Stokes at h = 1/128: K 1e-1 from the dense result). dpg_plan_autotune
- * switches it on only where it is faster and accurate (see there). */
+ * and side parts the same way from the point weights (no B operand). Its
+ * rounding differs from the dense path's: on the 32^3 hexes K is 9e-11 from an
+ * 80-bit evaluation (the dense G: 6.6e-10; far fewer roundings per entry), on
+ * the 2D Stokes mesh 1.5e-9 (dense: 2.4e-9). dpg_plan_autotune switches it on
+ * only where it is faster and passes a gate on K (see there). */
 typedef struct {
   int32_t gram;    /* G = A^T A (dense path)                        */
   int32_t bmat;    /* B blocks                                      */
