@@ -969,6 +969,9 @@ def main():
                    "tiles": {"gram": tiles[0], "bmat": tiles[1], "update": tiles[2],
                              "chosen_by": "--tiles" if args.tiles else f"autotuner ({t_tune} s, untimed)"}},
         "gflops": round(info.flops_per_cell * cells_per_s / 1e9, 1),
+        "gflops_note": "the dense algorithm's flop count per cell times cells/s: a rate of algorithmic work, not a "
+                       "utilisation (the Gram is computed as a lower triangle, and the sum-factorised G and B do far "
+                       "fewer flops than the contractions counted); utilisation is the roofline's",
         "algorithmic_flop_per_cell": info.flops_per_cell,
         "roofline": roofline,
         "stages": stages,
