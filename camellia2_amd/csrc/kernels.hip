@@ -873,6 +873,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void u
 // issued together; the same FMAs in the same order as with QN = 0, which takes
 // q at run time).
 //
+// MODE 3 (gram_sf_diag_kernel): G's diagonal only (the diagonal block pairs'
+// diagonal slabs and entries, each the same FMAs as in the full pass), so that
+// the DPOEQU factors exist before the full pass, which then stores S G S.
+//
 // MODE 1 (the volume part of B, bmat_sf_kernel): side A is a trial field block
 // (rows N + its offset), side B a test block; the pair's weight is one point
 // weight of the stiffness program, M_p(q) = W[q][a] (a = (group * nvar + test
@@ -913,7 +917,7 @@ __device__ inline void sf_body(const SfBlockPair* __restrict__ bps, const SfPair
   double* Fs = Ms + np * QG;       // nfam * nF * q
   double* T1 = Fs + nfam * nF * q;  // 3D: np*q*q   2D: ncls*q
   double* T2 = T1 + (DIM == 3 ? np * q * q : ncls * q);  // 3D: ncls * nA2*nB2 * q
-  if constexpr (MODE == 0) {
+  if constexpr (MODE == 0 || MODE == 3) {
     const double* W = Wp + (size_t)c * QG * n_lt * na + bp.woff;
     for (int e = t; e < np * QG; e += NT) {
       int p = e / QG, qq = e - p * QG;
@@ -939,7 +943,8 @@ __device__ inline void sf_body(const SfBlockPair* __restrict__ bps, const SfPair
       Ms[e] = qq < qf ? W[(size_t)qq * na + pr[p].a] : 0.0;
     }
   }
-  const double* sc = MODE >= 1 && s ? s + (size_t)c * ss : nullptr;
+  const double* sc = s ? s + (size_t)c * ss : nullptr;
+  if (MODE == 3 && !bp.diag) return;  // the whole workgroup
   for (int e = t; e < nfam * nF * q; e += NT) Fs[e] = F[bp.foff + e];
   __syncthreads();
   double* Ac = A + (size_t)c * sA;
@@ -989,7 +994,10 @@ __device__ inline void sf_body(const SfBlockPair* __restrict__ bps, const SfPair
       for (int db = 0; db < BJ; ++db) {
         const int row = r0 + i1 + da, col = c0 + (j1 + db) * cs;
         if (i1 + da >= bp.nA[0] || j1 + db >= bp.nB[0] || (bp.diag && row < col)) continue;
-        Ac[(size_t)row + (size_t)col * lda] = MODE >= 1 && sc ? acc[da][db] * sc[col] : acc[da][db];
+        // G: S G S as the scaling pass computes it ((g s_i) s_j); B: g s_j
+        double v = acc[da][db];
+        if (sc) v = MODE == 0 ? v * sc[row] * sc[col] : v * sc[col];
+        Ac[(size_t)row + (size_t)col * lda] = v;
       }
   };
   using one_c = std::integral_constant<int, 1>;
@@ -1000,7 +1008,7 @@ __device__ inline void sf_body(const SfBlockPair* __restrict__ bps, const SfPair
     const int nIA = bp.nA[0] * bp.nA[1];
     for (int i3 = 0; i3 < bp.nA[2]; ++i3)
       for (int j3 = 0; j3 < bp.nB[2]; ++j3) {
-        if (bp.diag && j3 > i3) continue;
+        if ((bp.diag && j3 > i3) || (MODE == 3 && j3 != i3)) continue;
         for (int e = t; e < np * q2n; e += NT) {
           int p = e / q2n, r = e - p * q2n;
           const double* fa = fam(pr[p].fa[2], i3);
@@ -1012,9 +1020,13 @@ __device__ inline void sf_body(const SfBlockPair* __restrict__ bps, const SfPair
           T1[e] = acc;
         }
         __syncthreads();
-        for (int e = t; e < ncls * n2 * q; e += NT) {
-          int cl = e / (n2 * q), r = e - cl * n2 * q;
+        // (the diagonal pass needs only the entries with j2 = i2)
+        const int n2e = MODE == 3 ? bp.nA[1] : n2;
+        for (int e2 = t; e2 < ncls * n2e * q; e2 += NT) {
+          int cl = e2 / (n2e * q), r = e2 - cl * n2e * q;
           int ij = r / q, q1 = r - ij * q;
+          if (MODE == 3) ij *= bp.nA[1] + 1;  // (i2, i2)
+          const int e = (cl * n2 + ij) * q + q1;
           int i2 = ij % bp.nA[1], j2 = ij / bp.nA[1];
           double acc = 0.0;
           for (int p = 0; p < np; ++p) {
@@ -1029,7 +1041,13 @@ __device__ inline void sf_body(const SfBlockPair* __restrict__ bps, const SfPair
         }
         __syncthreads();
         const int hA = (bp.nA[0] + 1) >> 1, hB = (bp.nB[0] + 1) >> 1;
-        if constexpr (MODE == 0) {
+        if constexpr (MODE == 3) {
+          for (int e = t; e < nIA; e += NT) {  // il = jl
+            const int i1 = e % bp.nA[0], i2 = e / bp.nA[0];
+            final_block(one_c{}, one_c{}, i1, i1, bp.offA + bp.nA[0] * i2 + nIA * i3,
+                        bp.offB + bp.sB[1] * i2 + bp.sB[2] * j3, T2 + (i2 + bp.nA[1] * i2) * q, n2 * q);
+          }
+        } else if constexpr (MODE == 0) {
           for (int e = t; e < hA * bp.nA[1] * hB * bp.nB[1]; e += NT) {
             int r = e;
             const int a = r % hA;
@@ -1057,7 +1075,7 @@ __device__ inline void sf_body(const SfBlockPair* __restrict__ bps, const SfPair
   } else {
     for (int i2 = 0; i2 < bp.nA[1]; ++i2)
       for (int j2 = 0; j2 < bp.nB[1]; ++j2) {
-        if (bp.diag && j2 > i2) continue;
+        if ((bp.diag && j2 > i2) || (MODE == 3 && j2 != i2)) continue;
         for (int e = t; e < ncls * q; e += NT) {
           int cl = e / q, q1 = e - cl * q;
           double acc = 0.0;
@@ -1073,7 +1091,10 @@ __device__ inline void sf_body(const SfBlockPair* __restrict__ bps, const SfPair
         }
         __syncthreads();
         const int hA = (bp.nA[0] + 1) >> 1, hB = (bp.nB[0] + 1) >> 1;
-        if constexpr (MODE == 0) {
+        if constexpr (MODE == 3) {
+          for (int e = t; e < bp.nA[0]; e += NT)
+            final_block(one_c{}, one_c{}, e, e, bp.offA + bp.nA[0] * i2, bp.offB + bp.sB[1] * j2, T1, q);
+        } else if constexpr (MODE == 0) {
           for (int e = t; e < hA * hB; e += NT) {
             const int b = e / hA, a = e - b * hA;
             final_block(two_c{}, two_c{}, 2 * a, 2 * b, bp.offA + bp.nA[0] * i2, bp.offB + bp.sB[1] * j2, T1, q);
@@ -1094,8 +1115,17 @@ __global__ __launch_bounds__(256) void gram_sf_kernel(const SfBlockPair* __restr
                                                       const SfPair* __restrict__ pairs,
                                                       const double* __restrict__ F, int nF, int q_rt,
                                                       const double* __restrict__ Wip, int n_lt, int na, int QG,
-                                                      double* __restrict__ A, int64_t lda, int64_t sA) {
-  sf_body<DIM, QN, 0>(bps, pairs, F, nF, 3, q_rt, Wip, n_lt, na, QG, A, lda, sA, nullptr, 0);
+                                                      double* __restrict__ A, int64_t lda, int64_t sA,
+                                                      const double* __restrict__ s, int64_t ss) {
+  sf_body<DIM, QN, 0>(bps, pairs, F, nF, 3, q_rt, Wip, n_lt, na, QG, A, lda, sA, s, ss);
+}
+template <int DIM, int QN>
+__global__ __launch_bounds__(256) void gram_sf_diag_kernel(const SfBlockPair* __restrict__ bps,
+                                                           const SfPair* __restrict__ pairs,
+                                                           const double* __restrict__ F, int nF, int q_rt,
+                                                           const double* __restrict__ Wip, int n_lt, int na, int QG,
+                                                           double* __restrict__ A, int64_t lda, int64_t sA) {
+  sf_body<DIM, QN, 3>(bps, pairs, F, nF, 3, q_rt, Wip, n_lt, na, QG, A, lda, sA, nullptr, 0);
 }
 template <int DIM, int QN>
 __global__ __launch_bounds__(256) void bmat_side_sf_kernel(const SfBlockPair* __restrict__ bps,
@@ -2365,27 +2395,36 @@ __global__ __launch_bounds__(256) void info_count(const int32_t* __restrict__ in
 
 void launch_gram_sf(int dim, const SfBlockPair* bps, int nbp, const SfPair* pairs, const double* F, int nF, int q,
                     const double* Wip, int n_lt, int na, int QG, double* A, int64_t lda, int64_t sA, int ncells,
-                    size_t lds_bytes, hipStream_t st) {
+                    size_t lds_bytes, hipStream_t st, const double* s, int64_t ss, bool diag_only) {
   if (ncells <= 0 || nbp <= 0) return;
   if (lds_bytes > 64 * 1024) throw DeviceError("gram_sf: LDS request above 64 KiB");
   dim3 grid(nbp, ncells);
-#define DPG_SF_CASE(D, QQ) \
-  case QQ:                 \
-    gram_sf_kernel<D, QQ><<<grid, 256, lds_bytes, st>>>(bps, pairs, F, nF, q, Wip, n_lt, na, QG, A, lda, sA); \
+#define DPG_SF_CASE(D, QQ)                                                                                         \
+  case QQ:                                                                                                         \
+    if (diag_only)                                                                                                 \
+      gram_sf_diag_kernel<D, QQ><<<grid, 256, lds_bytes, st>>>(bps, pairs, F, nF, q, Wip, n_lt, na, QG, A, lda, sA); \
+    else                                                                                                           \
+      gram_sf_kernel<D, QQ><<<grid, 256, lds_bytes, st>>>(bps, pairs, F, nF, q, Wip, n_lt, na, QG, A, lda, sA, s, ss); \
     break;
   if (dim == 3) {
     switch (q) {
       DPG_SF_CASE(3, 3) DPG_SF_CASE(3, 4) DPG_SF_CASE(3, 5) DPG_SF_CASE(3, 6) DPG_SF_CASE(3, 7) DPG_SF_CASE(3, 8)
       DPG_SF_CASE(3, 9) DPG_SF_CASE(3, 10)
       default:
-        gram_sf_kernel<3, 0><<<grid, 256, lds_bytes, st>>>(bps, pairs, F, nF, q, Wip, n_lt, na, QG, A, lda, sA);
+        if (diag_only)
+          gram_sf_diag_kernel<3, 0><<<grid, 256, lds_bytes, st>>>(bps, pairs, F, nF, q, Wip, n_lt, na, QG, A, lda, sA);
+        else
+          gram_sf_kernel<3, 0><<<grid, 256, lds_bytes, st>>>(bps, pairs, F, nF, q, Wip, n_lt, na, QG, A, lda, sA, s, ss);
     }
   } else {
     switch (q) {
       DPG_SF_CASE(2, 3) DPG_SF_CASE(2, 4) DPG_SF_CASE(2, 5) DPG_SF_CASE(2, 6) DPG_SF_CASE(2, 7) DPG_SF_CASE(2, 8)
       DPG_SF_CASE(2, 9) DPG_SF_CASE(2, 10) DPG_SF_CASE(2, 11) DPG_SF_CASE(2, 12)
       default:
-        gram_sf_kernel<2, 0><<<grid, 256, lds_bytes, st>>>(bps, pairs, F, nF, q, Wip, n_lt, na, QG, A, lda, sA);
+        if (diag_only)
+          gram_sf_diag_kernel<2, 0><<<grid, 256, lds_bytes, st>>>(bps, pairs, F, nF, q, Wip, n_lt, na, QG, A, lda, sA);
+        else
+          gram_sf_kernel<2, 0><<<grid, 256, lds_bytes, st>>>(bps, pairs, F, nF, q, Wip, n_lt, na, QG, A, lda, sA, s, ss);
     }
   }
 #undef DPG_SF_CASE
@@ -2478,6 +2517,12 @@ void launch_scaling(double* A, int64_t lda, int64_t sA, int r0, int D, int n, in
   check_launch("diag_scale_factors");
   apply_scaling<<<dim3((ncol + 3) / 4, ncells), 256, 0, st>>>(A, lda, sA, r0, D, n, ncol, s, ss, sign);
   check_launch("apply_scaling");
+}
+
+void launch_scale_factors(const double* A, int64_t lda, int64_t sA, int r0, int n, double* s, int64_t ss, int ncells,
+                          hipStream_t st) {
+  diag_scale_factors<<<dim3((n + 255) / 256, ncells), 256, 0, st>>>(A, lda, sA, r0, n, s, ss, 1);
+  check_launch("diag_scale_factors");
 }
 
 void launch_residual_row(double* A, int64_t lda, int64_t sA, int N, int M, int r0, int row_l, int row_r,

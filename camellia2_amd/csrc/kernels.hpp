@@ -78,10 +78,15 @@ void launch_gemm(GemmArgs g, int role, hipStream_t st, int tile = 64);
 void launch_fill_i32(int32_t* p, int n, int32_t v, hipStream_t st);
 // copies info (n) to out (optional) and writes the count of non-zero entries to *count
 void launch_info_count(const int32_t* info, int n, int32_t* out, int32_t* count, hipStream_t st);
-// sum-factorised Gram matrix, unscaled, lower triangle of A's first N rows/columns
+// sum-factorised Gram matrix, lower triangle of A's first N rows/columns: unscaled, or S G S
+// with the DPOEQU factors s; diag_only: G's diagonal alone (what s is computed from)
 void launch_gram_sf(int dim, const SfBlockPair* bps, int nbp, const SfPair* pairs, const double* F, int nF, int q,
                     const double* Wip, int n_lt, int na, int QG, double* A, int64_t lda, int64_t sA, int ncells,
-                    size_t lds_bytes, hipStream_t st);
+                    size_t lds_bytes, hipStream_t st, const double* s = nullptr, int64_t ss = 0,
+                    bool diag_only = false);
+// s[i] = 1 / sqrt(A[r0 + i][r0 + i]), i < n (the DPOEQU factors alone)
+void launch_scale_factors(const double* A, int64_t lda, int64_t sA, int r0, int n, double* s, int64_t ss, int ncells,
+                          hipStream_t st);
 // the volume part of B, sum-factorised: block pairs (trial field block, test
 // block) and their pairs, 1D tables of 4 families on the stiffness points
 // (q per direction), point weights Wvol (na per point), stores scaled by s

@@ -1195,18 +1195,31 @@ void stage_factor(dpg_plan* plan, const BucketPlan& B, const dpg_batch& bt, int 
   double opnd = (sf ? 0.0 : (double)KG) + (sfb ? 0.0 : (double)B.prog.n_vb * L.Q) +
                 (sfs ? 0.0 : (double)B.prog.n_sb * S * L.Qs);
   P.stop(e, st, DPG_STAGE_OPERAND, nb * opnd * N * 2.0 * B.npr, nb * opnd * N * D8);
-  if (sf) {
-    // sum-factorised G (unscaled), optional copy-out, then the DPOEQU factors
-    // from its diagonal and S G S in place; B and l are scaled in their epilogues
+  if (sf && !Gout) {
+    // G's diagonal first (the same sums as the full pass), the DPOEQU factors
+    // from it, then the full sum-factorised G written as S G S (the products of
+    // the scaling pass, in its order); B and l are scaled in their epilogues
+    e = P.start(st);
+    launch_gram_sf(dim, B.sf_bps, B.sf_nbp, B.sf_pairs, B.sf_F, B.sf_nF, B.sf_q, w.Wip, B.prog.n_lt,
+                   nvar * B.npr, L.QG, w.A, lda, sA, nb, B.sf_lds, st, nullptr, 0, true);
+    P.stop(e, st, DPG_STAGE_GRAM, 0.0, nb * (double)B.nWip * D8);
+    e = P.start(st);
+    launch_scale_factors(w.A, lda, sA, 0, N, w.s, B.nS, nb, st);
+    P.stop(e, st, DPG_STAGE_SCALE, 0.0, nb * (double)N * 2.0 * D8);
+    e = P.start(st);
+    launch_gram_sf(dim, B.sf_bps, B.sf_nbp, B.sf_pairs, B.sf_F, B.sf_nF, B.sf_q, w.Wip, B.prog.n_lt,
+                   nvar * B.npr, L.QG, w.A, lda, sA, nb, B.sf_lds, st, w.s, B.nS);
+    P.stop(e, st, DPG_STAGE_GRAM, nb * B.gram_flops, nb * ((double)B.nWip + N * (N + 1.0) / 2.0) * D8);
+  } else if (sf) {
+    // G an output too: sum-factorised G (unscaled), its copy-out, then the
+    // DPOEQU factors from its diagonal and S G S in place
     e = P.start(st);
     launch_gram_sf(dim, B.sf_bps, B.sf_nbp, B.sf_pairs, B.sf_F, B.sf_nF, B.sf_q, w.Wip, B.prog.n_lt,
                    nvar * B.npr, L.QG, w.A, lda, sA, nb, B.sf_lds, st);
     P.stop(e, st, DPG_STAGE_GRAM, nb * B.gram_flops, nb * ((double)B.nWip + N * (N + 1.0) / 2.0) * D8);
-    if (Gout) {
-      e = P.start(st);
-      launch_extract_sym(w.A, lda, sA, 0, N, 1.0, Gout + (size_t)c0 * N * N, nb, st);
-      P.stop(e, st, DPG_STAGE_EXTRACT, 0.0, nb * (double)N * N * 2.0 * D8);
-    }
+    e = P.start(st);
+    launch_extract_sym(w.A, lda, sA, 0, N, 1.0, Gout + (size_t)c0 * N * N, nb, st);
+    P.stop(e, st, DPG_STAGE_EXTRACT, 0.0, nb * (double)N * N * 2.0 * D8);
     e = P.start(st);
     launch_scaling(w.A, lda, sA, 0, N, N, N, w.s, B.nS, nb, 1.0, st);
     P.stop(e, st, DPG_STAGE_SCALE, 0.0, nb * (double)N * (N + 1) / 2.0 * 16.0);

@@ -125,6 +125,26 @@ def test_sum_factorised_contractions_meet_the_floor_rule(case):
     check_against_oracle(cases[case](), want_G=True, want_schur=True, sumfact=1)
 
 
+@pytest.mark.parametrize("case", ["c4", "c1"])
+def test_sum_factorised_gram_scaled_as_written(case):
+    """Without G as an output the sum-factorised G is written already scaled
+    (its diagonal first, the DPOEQU factors from it, then S G S in the
+    scaling pass's order); with G as an output it is written, copied out and
+    scaled in place. The two give the same bits."""
+    f, b, nodes, par, coefs, load = {"c4": lambda: poisson3d(2, 2, 1), "c1": lambda: poisson2d(4, 4)}[case]()
+    res = []
+    for want_G in (False, True):
+        A = dpg.LocalAssembler(f, [b])
+        A.set_tiles(0, 64, 64, 64, 1)
+        o, nbad = A.local_stiffness_and_rhs(0, dev(nodes), dev(par), dev(coefs), load, want_schur=True,
+                                            want_G=want_G)
+        assert nbad == 0
+        res.append({k: v.clone() for k, v in o.items()})
+        A.close()
+    for k in ("K", "rhs", "schur", "schur_rhs"):
+        assert torch.equal(res[0][k], res[1][k]), k
+
+
 @pytest.mark.parametrize("case", ["c4", "c5", "c1"])
 def test_sum_factorised_path_over_a_used_workspace(case):
     """The sum-factorised G and B write every entry of their rows, the zeros
