@@ -867,8 +867,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void u
 //   T2[cls][i2,j2][q1] = sum_{p in cls} sum_q2 F_fa2(i2,q2) F_fb2(j2,q2) T1[p][q2][q1]
 //   G[(i1,i2),(j1,j2)] = sum_cls sum_q1 F_cfa(i1,q1) F_cfb(j1,q1) T2[cls][i2,j2][q1]
 // where cls groups the pairs by their direction-1 families. M_p is formed
-// from the point weights W in LDS first. Writes are unscaled (the DPOEQU
-// factors are taken from G's diagonal afterwards). QN > 0: the points per
+// from the point weights W in LDS first. Writes are unscaled, or S G S when
+// the DPOEQU factors s are given (the products of the scaling pass, in its
+// order: (g s_i) s_j). A block pair with no pairs writes its zeros. QN > 0: the points per
 // direction as a compile-time constant (the point sums unrolled, their loads
 // issued together; the same FMAs in the same order as with QN = 0, which takes
 // q at run time).
@@ -881,6 +882,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void u
 // (rows N + its offset), side B a test block; the pair's weight is one point
 // weight of the stiffness program, M_p(q) = W[q][a] (a = (group * nvar + test
 // var) * npr + prim), and the stores carry the DPOEQU factor of the test column.
+//
+// MODE 2 (the side part of B, bmat_side_sf_kernel): side A is a trace or flux
+// block on one side, side B a test block restricted to that side; the kernel's
+// directions are the side's coordinates and last the normal one, whose single
+// point fills the first q^(DIM-1) of the q^DIM point slots (the others zero);
+// each block pair brings its side's 1D tables (foff) and point weights (woff),
+// and its columns' strides (sB) keep the test block's reference order.
 template <int DIM, int QN, int MODE>
 __device__ inline void sf_body(const SfBlockPair* __restrict__ bps, const SfPair* __restrict__ pairs,
                                const double* __restrict__ F, int nF, int nfam, int q_rt,
