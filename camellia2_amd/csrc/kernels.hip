@@ -618,7 +618,10 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
 // per wave and each X row is staged once for the whole panel. Chunks of 8 rows
 // rather than the square tile's 16 keep the kernel within 128 VGPRs, 4 waves
 // per SIMD (3% faster on the tall panels than 16-row chunks at 3 waves, which
-// hold the operands of twice the k-rows in registers). Same staging scheme as above (fixed columns per thread, clamped
+// hold the operands of twice the k-rows in registers). A panel of at most 32
+// columns (the short last panel of a factorisation) takes the 128 x 32 form of
+// the same tile (TN = 32, 2 x 2 MFMA blocks per wave) rather than half-empty
+// blocks. Same staging scheme as above (fixed columns per thread, clamped
 // addresses, zeroing at LDS write, two chunks in flight) and the same k order
 // per output element, so the results are those of the square tiles bit for
 // bit. No k-segment masks, no lower-triangle tile walk (dense m x n).
@@ -626,7 +629,7 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
 // SOLVE (rows below a panel's diagonal block): the updated tile is then solved
 // in registers against the diagonal factor, X = C L^-T, before it is stored,
 // so the panel is written once instead of written, read and written again.
-// The solve is blocked by the four 16-column MFMA blocks: inside a block the
+// The solve is blocked by the 16-column MFMA blocks: inside a block the
 // substitution runs column by column as the reference DTRSM does (x_s = c_s /
 // L_ss, then c_m -= L_ms x_s for the later columns m of the block), with x_s
 // moved between the four lane groups that hold a row's columns by two row
@@ -634,13 +637,13 @@ __device__ inline void gemm_tn_f64_body(const GemmArgs& g) {
 // (a rank-4 update per k-slice, the B operand being the solved accumulator
 // element itself). L stays in LDS (the staging buffers, reused).
 constexpr int WKC = 8;  // k-rows per staged chunk of the 128 x 64 tile
-template <bool SOLVE>
+template <bool SOLVE, int TN>
 __device__ inline void gemm_tn_f64_wide_body(const GemmArgs& g) {
-  constexpr int TM = 128, TN = 64, NT = 256;
+  constexpr int TM = 128, NT = 256, NB = TN / 16;  // NB column blocks of 16
   constexpr int PX = WKC * TM / NT, RX = NT / TM;  // X: 4 elements per thread, k-rows t/128 + 2r
-  constexpr int PY = WKC * TN / NT, RY = NT / TN;  // Y: 2 elements per thread, k-rows t/64 + 4r
+  constexpr int PY = WKC * TN / NT, RY = NT / TN;  // Y: 2 (TN 64) or 1 elements per thread, k-rows t/TN + RY r
   constexpr int LXS = TM + 4, LYS = TN + 4, LP = 68;  // LDS row pitches (L: one column of the factor per row)
-  constexpr int NSTAGE = 2 * WKC * (LXS + LYS), NL = SOLVE ? 64 * LP : 0;  // two staging buffers
+  constexpr int NSTAGE = 2 * WKC * (LXS + LYS), NL = SOLVE ? TN * LP : 0;  // two staging buffers; L's first TN columns
   int b = xcd_remap(blockIdx.x, gridDim.x);
   int cell = b / g.ntiles, tile = b % g.ntiles;
   if (cell >= g.ncells) return;
@@ -651,9 +654,9 @@ __device__ inline void gemm_tn_f64_wide_body(const GemmArgs& g) {
   double (*Xs)[WKC][LXS] = reinterpret_cast<double (*)[WKC][LXS]>(smem);
   double (*Ys)[WKC][LYS] = reinterpret_cast<double (*)[WKC][LYS]>(smem + 2 * WKC * LXS);
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  d4 acc[4][2];
+  d4 acc[NB][2];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < NB; ++a)
 #pragma unroll
     for (int bb = 0; bb < 2; ++bb) acc[a][bb] = d4{0.0, 0.0, 0.0, 0.0};
   const int kend = g.k;
@@ -685,13 +688,13 @@ __device__ inline void gemm_tn_f64_wide_body(const GemmArgs& g) {
 #pragma unroll
     for (int k4 = 0; k4 < WKC; k4 += 4) {
       const int kr = k4 + (lane >> 4), cc = lane & 15;
-      double xa[2], yb[4];
+      double xa[2], yb[NB];
       xa[0] = Xs[buf][kr][w * 32 + cc];
       xa[1] = Xs[buf][kr][w * 32 + 16 + cc];
 #pragma unroll
-      for (int bj = 0; bj < 4; ++bj) yb[bj] = Ys[buf][kr][bj * 16 + cc];
+      for (int bj = 0; bj < NB; ++bj) yb[bj] = Ys[buf][kr][bj * 16 + cc];
 #pragma unroll
-      for (int bj = 0; bj < 4; ++bj)
+      for (int bj = 0; bj < NB; ++bj)
 #pragma unroll
         for (int bi = 0; bi < 2; ++bi)
           acc[bj][bi] = __builtin_amdgcn_mfma_f64_16x16x4f64(yb[bj], xa[bi], acc[bj][bi], 0, 0, 0);
@@ -727,7 +730,7 @@ __device__ inline void gemm_tn_f64_wide_body(const GemmArgs& g) {
       kf = kh;
     }
   }
-  // epilogue as in the square tile, in four column quarters (bj)
+  // epilogue as in the square tile, by 16-column blocks (bj)
   int ir[2];
   bool iv[2];
   double sr[2] = {1.0, 1.0};
@@ -741,7 +744,7 @@ __device__ inline void gemm_tn_f64_wide_body(const GemmArgs& g) {
     // the updated tile C = beta C + alpha acc (0 outside the matrix), in place
     // of the accumulators
 #pragma unroll
-    for (int bj = 0; bj < 4; ++bj) {
+    for (int bj = 0; bj < NB; ++bj) {
       int jc[4];
       bool jv[4];
       double cold[2][4];
@@ -767,11 +770,11 @@ __device__ inline void gemm_tn_f64_wide_body(const GemmArgs& g) {
     __syncthreads();
     double* Ls = smem;
     const double* __restrict__ Lc = g.Lt + (size_t)cell * 64 * 64;
-    for (int e = t; e < 64 * 64; e += NT) Ls[(e >> 6) * LP + (e & 63)] = Lc[e];
+    for (int e = t; e < TN * 64; e += NT) Ls[(e >> 6) * LP + (e & 63)] = Lc[e];
     __syncthreads();
     const int q = lane >> 4, rowl = lane & 15;
 #pragma unroll
-    for (int bj0 = 0; bj0 < 4; ++bj0) {
+    for (int bj0 = 0; bj0 < NB; ++bj0) {
       // substitution over the 16 columns of block bj0; column s of the block is
       // element s/4 of lane group s%4
 #pragma unroll
@@ -792,7 +795,7 @@ __device__ inline void gemm_tn_f64_wide_body(const GemmArgs& g) {
       // the later blocks: C[:, bj1] -= X[:, bj0] L[bj1, bj0]^T, a rank-4 MFMA
       // per k-slice v0 (columns bj0*16 + 4 v0 + k, k = lane group)
 #pragma unroll
-      for (int bj1 = bj0 + 1; bj1 < 4; ++bj1)
+      for (int bj1 = bj0 + 1; bj1 < NB; ++bj1)
 #pragma unroll
         for (int v0 = 0; v0 < 4; ++v0) {
           const double la = -Ls[(bj0 * 16 + 4 * v0 + q) * LP + bj1 * 16 + rowl];
@@ -802,7 +805,7 @@ __device__ inline void gemm_tn_f64_wide_body(const GemmArgs& g) {
         }
     }
 #pragma unroll
-    for (int bj = 0; bj < 4; ++bj)
+    for (int bj = 0; bj < NB; ++bj)
 #pragma unroll
       for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
@@ -813,7 +816,7 @@ __device__ inline void gemm_tn_f64_wide_body(const GemmArgs& g) {
     return;
   }
 #pragma unroll
-  for (int bj = 0; bj < 4; ++bj) {
+  for (int bj = 0; bj < NB; ++bj) {
     int jc[4];
     bool jv[4];
     double sc[4] = {1.0, 1.0, 1.0, 1.0}, cold[2][4];
@@ -855,10 +858,14 @@ __global__ __launch_bounds__(64) void gemm_gram_f64_t32(GemmArgs g) { gemm_tn_f6
 __global__ __launch_bounds__(64) void gemm_bmat_f64_t32(GemmArgs g) { gemm_tn_f64_body<32>(g); }
 __global__ __launch_bounds__(64) void gemm_update_f64_t32(GemmArgs g) { gemm_tn_f64_body<32>(g); }
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void gemm_update_f64_w128(GemmArgs g) {
-  gemm_tn_f64_wide_body<false>(g);
+  gemm_tn_f64_wide_body<false, 64>(g);
 }
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void update_solve_f64(GemmArgs g) {
-  gemm_tn_f64_wide_body<true>(g);
+  gemm_tn_f64_wide_body<true, 64>(g);
+}
+// panels of at most 32 columns (the short last panel): half the MFMA blocks
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void update_solve_f64_n32(GemmArgs g) {
+  gemm_tn_f64_wide_body<true, 32>(g);
 }
 
 // ------------------------------------------------- sum-factorised Gram
@@ -2361,6 +2368,11 @@ void launch_update_solve(GemmArgs g, hipStream_t st) {
   g.ntiles = g.tiles_m;
   int64_t nb = (int64_t)g.ntiles * g.ncells;
   if (nb > 0x7fffffff) throw DeviceError("gemm grid too large");
+  if (g.n <= 32) {
+    update_solve_f64_n32<<<dim3((unsigned)nb), 256, 0, st>>>(g);
+    check_launch("update_solve_f64_n32");
+    return;
+  }
   update_solve_f64<<<dim3((unsigned)nb), 256, 0, st>>>(g);
   check_launch("update_solve_f64");
 }
