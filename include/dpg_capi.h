@@ -378,7 +378,8 @@ typedef struct {
   int32_t update;  /* Cholesky panel/trailing updates; also 128: the
                       rows below each diagonal block are updated
                       and solved against it in one kernel on
-                      128 x 64 tiles (the diagonal blocks and the
+                      128 x 64 tiles (128 x 32 for a panel of at
+                      most 32 columns; the diagonal blocks and the
                       trailing update then take 64)                 */
   int32_t sumfact; /* 1: sum-factorised G and B (see above)         */
 } dpg_tiles;
