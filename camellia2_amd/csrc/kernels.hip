@@ -1553,8 +1553,9 @@ constexpr int PNB = 64;
 // Diagonal block (kb <= NB <= 64 columns of a 64-column panel) of one cell per
 // wavefront (4 cells per workgroup), in registers: lane i holds row i of the
 // block. Right-looking Cholesky with the pivot and the scaled column broadcast
-// across lanes (no LDS, no barriers). Writes the factor to Ld[(k0+r)*64 + c]
-// and, for the panel solve, Lt[j][m] = L[m][j] (m > j), Lt[j][j] = 1 / L[j][j],
+// across lanes (no LDS, no barriers). Writes the factor by columns (coalesced),
+// Ld[(k0+c)*64 + r] = L[k0+r][k0+c] for r, c < kb (r < c: zero), and, for the
+// panel solve, Lt[j][m] = L[m][j] (m > j), Lt[j][j] = 1 / L[j][j],
 // zero outside the block. NB is the smallest of 8, 16, 32, 64 that holds kb,
 // so short last panels cost little.
 // A non-positive pivot sets info to its 1-based column (first failure only) and
@@ -1604,10 +1605,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void p
   asm volatile("v_mov_b32 %0, %1" : "=v"(ls) : "v"(lane));
   if (ls < kb) {
 #pragma unroll
-    for (int cc = 0; cc < PNB; ++cc) {  // cc is a constant after unrolling: a[] stays in registers
-      if constexpr (NB == PNB) Lc[(size_t)(k0 + ls) * PNB + cc] = (cc <= ls && cc < kb) ? a[cc < NB ? cc : 0] : 0.0;
-      else Lc[(size_t)(k0 + ls) * PNB + cc] = (cc < NB && cc <= ls && cc < kb) ? a[cc < NB ? cc : 0] : 0.0;
-    }
+    for (int cc = 0; cc < NB; ++cc)  // cc is a constant after unrolling: a[] stays in registers
+      if (cc < kb) Lc[(size_t)(k0 + cc) * PNB + ls] = cc <= ls ? a[cc] : 0.0;  // column cc: lanes contiguous
   }
   if (lane == 0 && fail >= 0 && fail < kb && info) atomicCAS(&info[c], 0, info_base + k0 + fail + 1);
   // Lt, column j of L contiguous (lane = row): coalesced stores; the diagonal
@@ -1798,7 +1797,7 @@ __global__ __launch_bounds__(256) void backsolve_kernel(const double* __restrict
     if (t == 0) z[j] = zj;
     // y_i -= L[j][i] z_j for i < j
     for (int i = t; i < j; i += blockDim.x) {
-      double lji = (i >= kj) ? Lc[(size_t)j * 64 + (i - kj)] : Ac[(size_t)j + (size_t)i * lda];
+      double lji = (i >= kj) ? Lc[(size_t)i * 64 + (j - kj)] : Ac[(size_t)j + (size_t)i * lda];
       z[i] = fma(-lji, zj, z[i]);
     }
   }
@@ -1878,7 +1877,7 @@ __global__ __launch_bounds__(256) void extract_factor(const double* __restrict__
     double v = 0.0;
     if (i < R && j < N && i >= j) {
       const int kb = i & ~63;
-      v = (i < N && j >= kb) ? Lc[(size_t)i * 64 + (j - kb)] : Ac[(size_t)i + (size_t)j * lda];
+      v = (i < N && j >= kb) ? Lc[(size_t)j * 64 + (i - kb)] : Ac[(size_t)i + (size_t)j * lda];
     }
     tile[tx][yy] = v;
   }
